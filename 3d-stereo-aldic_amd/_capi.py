@@ -1,0 +1,87 @@
+"""ctypes bindings of libaldic.so (include/aldic.h).
+
+The library must exist: there is no fallback path.  Import raises
+:class:`ImportError` naming the build command when it is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libaldic.so")
+
+OK, EINVAL, EDEVICE, ENOMEM = 0, 1, 2, 3
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="F_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="F_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="F_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="F_CONTIGUOUS")
+_i64 = C.c_int64
+_vp = C.c_void_p
+_pp = C.POINTER(C.c_void_p)
+
+# Every symbol of include/aldic.h with (restype, argtypes).
+SIGNATURES = {
+    "aldic_last_error": (C.c_char_p, []),
+    "aldic_ctx_create": (C.c_int, [C.c_int, _pp]),
+    "aldic_ctx_destroy": (C.c_int, [_vp]),
+    "aldic_ctx_stream": (_vp, [_vp]),
+    "aldic_ctx_synchronize": (C.c_int, [_vp]),
+    "aldic_interp_spline_f64": (C.c_int, [_vp, _dp, _i64, _i64, _i64, _dp, _dp, _i64, _dp]),
+    "aldic_interp_spline_f32": (C.c_int, [_vp, _fp, _i64, _i64, _i64, _fp, _fp, _i64, _fp]),
+    "aldic_img_gradient": (C.c_int, [_vp, _dp, _vp, _i64, _i64, _dp, _dp]),
+    "aldic_images_create": (C.c_int, [_vp, _dp, _dp, _vp, _vp, _vp, _i64, _i64, _pp]),
+    "aldic_images_create_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _pp]),
+    "aldic_images_destroy": (C.c_int, [_vp]),
+    "aldic_local_icgn": (C.c_int, [_vp, _vp, _dp, _dp, _i64, _i32, C.c_int32, C.c_double,
+                                   _dp, _dp, _vp, _i32, _vp, _vp]),
+    "aldic_subpb1": (C.c_int, [_vp, _vp, _dp, _dp, _dp, _dp, _dp, _i64, _i32, C.c_double,
+                               C.c_double, _dp, _i32]),
+    "aldic_plan_create": (C.c_int, [_vp, _i32, _i64, C.c_int32, C.c_int32, _pp]),
+    "aldic_plan_destroy": (C.c_int, [_vp]),
+    "aldic_local_icgn_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp, _vp,
+                                       _vp, _vp, _vp]),
+    "aldic_subpb1_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
+                                   C.c_double, _vp, _vp]),
+    "aldic_integer_search": (C.c_int, [_vp, _dp, _dp, _i64, _i64, C.POINTER(_i64),
+                                       C.POINTER(_i64), C.c_int32, C.c_int32, C.POINTER(_i64),
+                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i64),
+                                       C.POINTER(_i64)]),
+    "aldic_integer_search_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.POINTER(_i64),
+                                           C.POINTER(_i64), C.c_int32, C.c_int32,
+                                           C.POINTER(_i64), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+class AldicError(RuntimeError):
+    """A non-OK return of the C ABI (bad argument, HIP error, out of memory)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[aldic error {code}] {msg}")
+        self.code = code
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError(f"{LIBPATH} is missing: build it with "
+                              f"`python 3d-stereo-aldic_amd/build.py` (hipcc, gfx950)")
+        L = C.CDLL(LIBPATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != OK:
+        msg = lib().aldic_last_error()
+        raise AldicError(rc, msg.decode() if msg else "")

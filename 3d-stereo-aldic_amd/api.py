@@ -1,0 +1,575 @@
+"""Host-side mirror of the reference's MATLAB interface for the local matching
+path, over the C ABI of libaldic.so (HIP on gfx950).
+
+Names, argument order, argument meaning and outputs follow the reference:
+
+* ``ba_interp2_spline(F, X, Y)``                         ba_interp2_spline.cpp:418-435
+* ``funImgGradient(ImgRef, ImgDef, ImgRefMask)``           func/funImgGradient.m:1-87
+* ``IntegerSearchQuadtree(ImgRef, ImgDef, file_name, DICpara, ImgSeqNum)``
+                                                          func_quadtree/IntegerSearchQuadtree.m:1-228
+* ``funIntegerSearch(f, g, ...)`` (whole field)            func/funIntegerSearch.m:2-98
+* ``LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod, tol,
+  shapeFunctionOrder)``                                   func_quadtree/LocalICGNQuadtree.m:1-349
+* ``Subpb1Quadtree(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, beta,
+  HPar, ALSolveStep, DICpara, ICGNmethod, tol)``          func_quadtree/Subpb1Quadtree.m:1-276
+
+Arrays use MATLAB's layout: an image is ``(M, N)`` with the first axis along
+image x, node coordinates are 1-based, vectors are flat.  ``Df`` and
+``DICpara`` are dicts with the reference's field names.
+
+Differences that a caller sees, all stated where they occur:
+* the natural-neighbour NaN fill after the bad-point rules
+  (LocalICGNQuadtree.m:208-325, Subpb1Quadtree.m:190-254) is not reproduced:
+  bad points come back as NaN (``fill_nans`` is reserved for it);
+* ``cc["qfactors"]`` (PCE/PPE) is not computed: it is unused downstream of
+  IntegerSearchQuadtree with qDICOrNot = 0 (IntegerSearchQuadtree.m:159);
+* funRemoveOutliers' median test and inpaint_nans are applied on the host
+  only where they change something (see :func:`remove_outliers`).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _capi
+from ._capi import check, lib
+
+
+def _F(a, dt=np.float64):
+    return np.asfortranarray(np.asarray(a, dtype=dt))
+
+
+class Context:
+    """One HIP device + stream + reusable staging buffers (aldic_ctx)."""
+
+    def __init__(self, device: int = 0):
+        p = C.c_void_p()
+        check(lib().aldic_ctx_create(int(device), C.byref(p)))
+        self.ptr = p
+        self.device = device
+
+    @property
+    def stream(self) -> int:
+        return lib().aldic_ctx_stream(self.ptr) or 0
+
+    def synchronize(self):
+        check(lib().aldic_ctx_synchronize(self.ptr))
+
+    def close(self):
+        if self.ptr:
+            lib().aldic_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+# ---------------------------------------------------------------------------
+# ba_interp2_spline
+# ---------------------------------------------------------------------------
+def ba_interp2_spline(F, X, Y, method=None, ctx: Context | None = None):
+    """Z = ba_interp2_spline(F, X, Y, [method]).
+
+    F is M x N (x P); X (dim-2 coordinate) and Y (dim-1 coordinate) have the
+    same shape; all three double or all three single (else an error, as
+    ba_interp2_spline.cpp:427-434).  ``method`` is ignored like the MEX's."""
+    ctx = ctx or default_context()
+    F = np.asarray(F)
+    X = np.asarray(X)
+    Y = np.asarray(Y)
+    if X.shape != Y.shape:
+        raise ValueError("X, Y should have the same size")
+    kinds = {F.dtype, X.dtype, Y.dtype}
+    if kinds == {np.dtype(np.float64)}:
+        dt, fn = np.float64, lib().aldic_interp_spline_f64
+    elif kinds == {np.dtype(np.float32)}:
+        dt, fn = np.float32, lib().aldic_interp_spline_f32
+    else:
+        raise TypeError("ba_interp2 takes only double or single arguments for IMG,X,Y, "
+                        "and all must have the same class")
+    F = _F(F, dt)
+    M, N = F.shape[0], F.shape[1]
+    P = int(np.prod(F.shape[2:])) if F.ndim > 2 else 1
+    ND = X.size
+    Z = np.zeros((ND, P), dtype=dt, order="F")
+    check(fn(ctx.ptr, F, M, N, P, _F(X, dt).reshape(-1, order="F"),
+             _F(Y, dt).reshape(-1, order="F"), ND, Z))
+    return Z.reshape(X.shape + (F.shape[2:] if F.ndim > 2 else ()), order="F")
+
+
+# ---------------------------------------------------------------------------
+# funImgGradient / funNormalizeImg
+# ---------------------------------------------------------------------------
+def funImgGradient(ImgRef, ImgDef=None, ImgRefMask=None, ctx: Context | None = None):
+    """Df = funImgGradient(ImgRef, ImgDef, ImgRefMask) (finite-difference branch)."""
+    ctx = ctx or default_context()
+    ImgRef = _F(ImgRef)
+    M, N = ImgRef.shape
+    gx = np.zeros((M - 6, N - 6), order="F")
+    gy = np.zeros((M - 6, N - 6), order="F")
+    mask = None if ImgRefMask is None else _F(ImgRefMask)
+    check(lib().aldic_img_gradient(ctx.ptr, ImgRef, None if mask is None else mask.ctypes.data,
+                                   M, N, gx, gy))
+    m = np.ones((M, N)) if mask is None else mask
+    return {"DfDx": gx, "DfDy": gy, "DfCropWidth": 3, "imgSize": (M, N),
+            "DfAxis": [4, M - 3, 4, N - 3], "ImgRefMask": m}
+
+
+def funNormalizeImg(Img, gridxyROIRange):
+    """[ImgNormalized, gridxyROIRange] = funNormalizeImg(Img, gridxyROIRange)
+    (func/funNormalizeImg.m:22-40): (Img - mean_ROI) / std_ROI per image."""
+    gx = list(gridxyROIRange["gridx"])
+    gy = list(gridxyROIRange["gridy"])
+    M, N = np.asarray(Img[0]).shape
+    gx[0] = max(gx[0], 1)
+    gx[1] = min(gx[1], M)
+    gy[0] = max(gy[0], 1)
+    gy[1] = min(gy[1], N)
+    out = []
+    for im in Img:
+        im = np.asarray(im, dtype=np.float64)
+        roi = im[gx[0] - 1:gx[1], gy[0] - 1:gy[1]]
+        out.append(_F((im - roi.mean()) / roi.std(ddof=1)))
+    return out, {"gridx": gx, "gridy": gy}
+
+
+# ---------------------------------------------------------------------------
+# resident images
+# ---------------------------------------------------------------------------
+class Images:
+    """The (Df, ImgRef, ImgDef) state of the solvers, resident on the GPU."""
+
+    def __init__(self, ImgRef, ImgDef, ImgRefMask=None, Df=None, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        ImgRef = _F(ImgRef)
+        ImgDef = _F(ImgDef)
+        M, N = ImgRef.shape
+        if ImgDef.shape != (M, N):
+            raise ValueError("ImgRef and ImgDef must have the same size")
+        mask = None if ImgRefMask is None else _F(ImgRefMask)
+        gx = gy = None
+        if Df is not None:
+            gx = _F(Df["DfDx"])
+            gy = _F(Df["DfDy"])
+            if gx.shape != (M - 6, N - 6):
+                raise ValueError("Df.DfDx must be (M-6) x (N-6) (DfCropWidth = 3)")
+        self._keep = (ImgRef, ImgDef, mask, gx, gy)
+        p = C.c_void_p()
+        check(lib().aldic_images_create(
+            self.ctx.ptr, ImgRef, ImgDef, None if mask is None else mask.ctypes.data,
+            None if gx is None else gx.ctypes.data, None if gy is None else gy.ctypes.data,
+            M, N, C.byref(p)))
+        self._keep = None
+        self.ptr = p
+        self.M, self.N = M, N
+
+    @classmethod
+    def from_device(cls, fm_ptr, g_ptr, M, N, mask_ptr=None, fx_ptr=None, fy_ptr=None,
+                    ctx: Context | None = None):
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        p = C.c_void_p()
+        check(lib().aldic_images_create_dev(self.ctx.ptr, fm_ptr, g_ptr, mask_ptr, fx_ptr, fy_ptr,
+                                            M, N, C.byref(p)))
+        self.ptr = p
+        self.M, self.N = M, N
+        return self
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().aldic_images_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _images_for(Df, ImgRef, ImgDef, ctx):
+    if isinstance(ImgRef, Images):
+        return ImgRef
+    mask = None if Df is None else Df.get("ImgRefMask")
+    return Images(ImgRef, ImgDef, mask, Df, ctx=ctx)
+
+
+# ---------------------------------------------------------------------------
+# per-node solvers (raw outputs, no bad-point rules)
+# ---------------------------------------------------------------------------
+def local_icgn_raw(images: Images, coordinatesFEM, U0, winsize, tol):
+    """One funICGNQuadtree per node (LocalICGNQuadtree.m:67-163).
+
+    Returns U (2n), F (4n), HtempPar (n x 21), ConvItPerEle (n),
+    markCoordHoleStrainOrNot (n), zncc (n)."""
+    coords = _F(coordinatesFEM)[:, :2]
+    coords = _F(coords)
+    n = coords.shape[0]
+    ws = np.asarray(winsize, dtype=np.int32).reshape(-1)
+    per = 1 if ws.size == n and n > 1 else 0
+    if not per and ws.size != 1:
+        raise ValueError("winsize must be a scalar or one value per node")
+    U = np.zeros(2 * n, order="F")
+    Fo = np.zeros(4 * n, order="F")
+    H = np.zeros((n, 21), order="F")
+    conv = np.zeros(n, dtype=np.int32, order="F")
+    hole = np.zeros(n, dtype=np.uint8, order="F")
+    z = np.zeros(n, order="F")
+    check(lib().aldic_local_icgn(images.ctx.ptr, images.ptr, coords, _F(U0).reshape(-1), n,
+                                 _F(ws, np.int32), per, float(tol), U, Fo, H.ctypes.data, conv,
+                                 hole.ctypes.data, z.ctypes.data))
+    return U, Fo, H, conv, hole, z
+
+
+def subpb1_raw(images: Images, coordinatesFEM, UOld, FOld, udual, vdual, winsize_List, mu, tol):
+    """One funICGN_Subpb1quadtree per node (Subpb1Quadtree.m:71-165).
+
+    Returns U (2n) and ConvItPerEle (n)."""
+    coords = _F(_F(coordinatesFEM)[:, :2])
+    n = coords.shape[0]
+    wl = _F(np.asarray(winsize_List, dtype=np.int32).reshape(n, 2), np.int32)
+    U = np.zeros(2 * n, order="F")
+    conv = np.zeros(n, dtype=np.int32, order="F")
+    check(lib().aldic_subpb1(images.ctx.ptr, images.ptr, coords, _F(UOld).reshape(-1),
+                             _F(FOld).reshape(-1), _F(udual).reshape(-1), _F(vdual).reshape(-1),
+                             n, wl, float(mu), float(tol), U, conv))
+    return U, conv
+
+
+# ---------------------------------------------------------------------------
+# bad-point rules of the drivers (host statistics)
+# ---------------------------------------------------------------------------
+def icgn_bad_points(ConvItPerEle, maxIterNum=100):
+    """LocalICGNQuadtree.m:167-192.  Returns (bad index array, LocalICGNBadPtNum)."""
+    c = np.asarray(ConvItPerEle, dtype=np.float64)
+    row12 = np.union1d(np.flatnonzero(c < 0), np.flatnonzero(c > maxIterNum - 1))
+    row3 = np.flatnonzero(c == maxIterNum + 2)
+    badnum = len(row12) - len(row3)
+    good = np.setdiff1d(np.arange(c.size), row12)
+    if good.size >= 3:
+        mu = c[good].mean()
+        sigma = c[good].std(ddof=1)  # fitdist 'Normal' sigma is the unbiased std
+    else:
+        mu, sigma = np.inf, 0.0
+    row4 = np.flatnonzero(c > max(mu + sigma, 6))
+    return np.union1d(row12, row4), badnum
+
+
+def subpb1_bad_points(ConvItPerEle, maxIterNum=100):
+    """Subpb1Quadtree.m:167-182.  Returns (bad index array, bad count)."""
+    c = np.asarray(ConvItPerEle, dtype=np.float64)
+    row12 = np.union1d(np.flatnonzero(c < 0), np.flatnonzero(c > maxIterNum - 1))
+    row3 = np.flatnonzero(c == maxIterNum + 2)
+    badnum = len(row12) - len(row3)
+    good = np.setdiff1d(np.arange(c.size), row12)
+    if good.size > 0:
+        m = c[good].mean()
+        s = c[good].std(ddof=1) if good.size > 1 else 0.0
+        thr = np.nanmax([m + 0.25 * s, 10.0])
+    else:
+        thr = 10.0  # max([NaN, 10]) = 10
+    row4 = np.flatnonzero(c > thr)
+    return np.union1d(row12, row4), badnum
+
+
+# ---------------------------------------------------------------------------
+# MATLAB-level drivers
+# ---------------------------------------------------------------------------
+def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="GaussNewton",
+                      tol=1e-3, shapeFunctionOrder=1, ctx: Context | None = None,
+                      fill_nans=False):
+    """[U,F,HtempPar,LocalTime,ConvItPerEle,LocalICGNBadPtNum,markCoordHoleStrain] =
+    LocalICGNQuadtree(U0,coordinatesFEM,Df,ImgRef,ImgDef,DICpara,ICGNmethod,tol,
+    shapeFunctionOrder)
+
+    ``DICpara["winsize"]`` may also be an array with one winsize per node.
+    markCoordHoleStrain is 0-based (numpy indexing)."""
+    if shapeFunctionOrder != 1:
+        raise NotImplementedError("only first-order IC-GN (shapeFunctionOrder = 1) is built")
+    if ICGNmethod != "GaussNewton":
+        raise NotImplementedError("only ICGNmethod = 'GaussNewton' is used by the reference")
+    if fill_nans:
+        raise NotImplementedError("the natural-neighbour NaN fill is not reproduced")
+    images = _images_for(Df, ImgRef, ImgDef, ctx)
+    t0 = time.perf_counter()
+    U, F, H, conv, hole, _z = local_icgn_raw(images, coordinatesFEM, U0, DICpara["winsize"], tol)
+    LocalTime = time.perf_counter() - t0
+    bad, badnum = icgn_bad_points(conv)
+    U = U.copy()
+    U[2 * bad] = np.nan
+    U[2 * bad + 1] = np.nan
+    return U, F, H, LocalTime, conv.astype(np.float64), badnum, np.flatnonzero(hole == 1)
+
+
+def Subpb1Quadtree(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, beta, HPar,
+                   ALSolveStep, DICpara, ICGNmethod="GaussNewton", tol=1e-2,
+                   ctx: Context | None = None, fill_nans=False):
+    """[U,HPar,ALSub1Time,ConvItPerEle,LocalICGNBadPtNum] = Subpb1Quadtree(UOld,FOld,udual,
+    vdual,coordinatesFEM,Df,ImgRef,ImgDef,mu,beta,HPar,ALSolveStep,DICpara,ICGNmethod,tol)
+
+    ``beta`` and ``HPar`` are accepted and unused, as in the reference's
+    quadtree variant (the local H is rebuilt inside each subset).  The
+    ``ALSolveStep == 1`` branch of the reference reads an undefined variable
+    (Subpb1Quadtree.m:101-107) and is rejected."""
+    if ALSolveStep == 1:
+        raise ValueError("ALSolveStep == 1 is not reachable in the reference (undefined Htemp)")
+    if fill_nans:
+        raise NotImplementedError("the natural-neighbour NaN fill is not reproduced")
+    images = _images_for(Df, ImgRef, ImgDef, ctx)
+    n = np.asarray(coordinatesFEM).shape[0]
+    wl = DICpara.get("winsize_List")
+    if wl is None:
+        wl = np.full((n, 2), int(DICpara["winsize"]), dtype=np.int32)
+    t0 = time.perf_counter()
+    U, conv = subpb1_raw(images, coordinatesFEM, UOld, FOld, udual, vdual, wl, mu, tol)
+    ALSub1Time = time.perf_counter() - t0
+    bad, badnum = subpb1_bad_points(conv)
+    U = U.copy()
+    U[2 * bad] = np.nan
+    U[2 * bad + 1] = np.nan
+    return U, HPar, ALSub1Time, conv.astype(np.float64), badnum
+
+
+def funIntegerSearch(f, g, tempSizeOfSearchRegion, gridx, gridy, winsize, winstepsize,
+                     tempNoOfInitPt=0, ctx: Context | None = None, return_peaks=False):
+    """[x,y,u,v,cc] = funIntegerSearch(f,g,tempSizeOfSearchRegion,gridx,gridy,winsize,
+    winstepsize,0) -- whole-field branch (func/funIntegerSearch.m:7-46, :100-203).
+
+    Grids are nY x nX.  cc = {"max": ..., "A": []}; q-factors are not computed."""
+    if tempNoOfInitPt != 0:
+        raise NotImplementedError("only the whole-field search (tempNoOfInitPt = 0) is built")
+    ctx = ctx or default_context()
+    f = _F(f)
+    g = _F(g)
+    M, N = f.shape
+    r = np.asarray(tempSizeOfSearchRegion, dtype=np.int64).reshape(-1)
+    if r.size == 1:
+        r = np.array([r[0], r[0]])
+    region = np.concatenate([r, r])[:4]  # repmat([sx sy],1,2) -> [L R U D] = [sx sy sx sy]
+    ws = int(np.asarray(winsize).reshape(-1)[0])
+    st = int(np.asarray(winstepsize).reshape(-1)[0])
+    gx = (C.c_int64 * 2)(int(gridx[0]), int(gridx[-1]))
+    gy = (C.c_int64 * 2)(int(gridy[0]), int(gridy[-1]))
+    rg = (C.c_int64 * 4)(*[int(v) for v in region])
+    nX, nY = C.c_int64(), C.c_int64()
+    L = lib()
+    check(L.aldic_integer_search(ctx.ptr, f, g, M, N, gx, gy, ws, st, rg, None, None, None, None,
+                                 None, None, None, C.byref(nX), C.byref(nY)))
+    shp = (nY.value, nX.value)
+    out = {k: np.zeros(shp, order="F") for k in ("x", "y", "u", "v", "max")}
+    pr = np.zeros(shp, dtype=np.int32, order="F")
+    pc = np.zeros(shp, dtype=np.int32, order="F")
+    if shp[0] * shp[1] > 0:
+        check(L.aldic_integer_search(ctx.ptr, f, g, M, N, gx, gy, ws, st, rg,
+                                     out["x"].ctypes.data, out["y"].ctypes.data,
+                                     out["u"].ctypes.data, out["v"].ctypes.data,
+                                     out["max"].ctypes.data, pr.ctypes.data, pc.ctypes.data,
+                                     C.byref(nX), C.byref(nY)))
+    cc = {"max": out["max"], "A": []}
+    if return_peaks:
+        return out["x"], out["y"], out["u"], out["v"], cc, pr, pc
+    return out["x"], out["y"], out["u"], out["v"], cc
+
+
+def IntegerSearchQuadtree(ImgRef, ImgDef, file_name, DICpara, ImgSeqNum,
+                          ctx: Context | None = None):
+    """[DICpara,x0,y0,u,v,cc] = IntegerSearchQuadtree(ImgRef,ImgDef,file_name,DICpara,ImgSeqNum)
+    with InitFFTSearchMethod = 1 (IntegerSearchQuadtree.m:47-160).  The search
+    radius is chosen as :54-71 (stereo: StereoSearchDistance; temporal:
+    TemporalSearchDistance; else NewFFTSearchDistance); there is no
+    interactive prompt, a missing radius raises.  Points whose rounded
+    centre is masked become NaN (:106-115); funRemoveOutliers(u,v,cc,0,100)
+    (:158-160) runs on the host."""
+    method = DICpara.get("InitFFTSearchMethod", 1)
+    if method != 1:
+        raise NotImplementedError("only InitFFTSearchMethod = 1 (whole field) is built")
+    r = None
+    if ImgSeqNum == -1:
+        r = DICpara.get("StereoSearchDistance") or DICpara.get("NewFFTSearchDistance")
+    elif ImgSeqNum >= 2:
+        r = DICpara.get("TemporalSearchDistance") or DICpara.get("NewFFTSearchDistance")
+    if r is None or (np.size(r) == 0):
+        raise ValueError("search distance not set (the reference would prompt with input())")
+    r = np.asarray(r).reshape(-1)
+    if r.size == 1:
+        r = np.array([r[0], r[0]])
+    roi = DICpara["gridxyROIRange"]
+    x0, y0, u, v, cc = funIntegerSearch(ImgRef, ImgDef, r, roi["gridx"], roi["gridy"],
+                                        DICpara["winsize"], DICpara["winstepsize"], 0, ctx=ctx)
+    mask = DICpara.get("ImgRefMask")
+    if mask is not None:
+        mask = np.asarray(mask)
+        xi = np.round(x0).astype(np.int64) - 1
+        yi = np.round(y0).astype(np.int64) - 1
+        hole = mask[xi, yi] == 0
+        u = np.where(hole, np.nan, u)
+        v = np.where(hole, np.nan, v)
+    cc["ccThreshold"] = 1.25
+    u, v, cc = remove_outliers(u, v, cc, 0, 100)
+    DICpara = dict(DICpara)
+    DICpara["InitFFTSearchMethod"] = 1
+    DICpara["SizeOfFFTSearchRegion"] = r
+    return DICpara, x0, y0, u, v, cc
+
+
+# ---------------------------------------------------------------------------
+# funRemoveOutliers (host post-step of the search)
+# ---------------------------------------------------------------------------
+def _medfilt2(V0):
+    """medFilt2(V0,[2 2],'symmetric',2) (func/funRemoveOutliers.m:393-444):
+    pad one row and one column after the data by mirroring, take the 2x2
+    window anchored at each point, drop its second element (the right
+    neighbour) and return the median of the remaining three -- the point,
+    the one below it along dim 1 and the diagonal one -- in single
+    precision."""
+    V = np.asarray(V0, dtype=np.float32)
+    P = np.pad(V, ((0, 1), (0, 1)), mode="symmetric")
+    a = P[:-1, :-1]
+    b = P[1:, :-1]
+    c = P[1:, 1:]
+    return np.median(np.stack([a, b, c], axis=2), axis=2).astype(np.float32)
+
+
+def _med_fluct(u, epsilon=0.1):
+    """funMedRemoveOutliers (func/funRemoveOutliers.m:374-390): normalised
+    fluctuation |u - med| / (med(|u - med|) + epsilon), single precision as
+    the reference's mixed double/single arithmetic yields.  NaNs are filled
+    first (inpaint_nans(.,3) in the reference; the spring fill here)."""
+    uf = inpaint_springs(np.asarray(u, dtype=np.float64))
+    med = _medfilt2(uf)
+    fluct = (uf.astype(np.float32) - med).astype(np.float32)
+    res = _medfilt2(np.abs(fluct))
+    return med, np.abs(fluct / (res + np.float32(epsilon)))
+
+
+def remove_outliers(u, v, cc, qDICOrNot=0, Thr0=100):
+    """funRemoveOutliers(u,v,cc,0,Thr0) restricted to what changes the grid:
+    the median test with threshold Thr0 and, where it (or a NaN) asks for it,
+    the fill of the flagged points.  With no NaN and no outlier the grid is
+    returned unchanged, as in the reference.  The fill itself is
+    inpaint_nans(.,4) in the reference (spring model); here it is the same
+    spring model solved by least squares on the host (parity unpinned: the
+    reference's inpaint_nans is MATLAB code this build does not run)."""
+    u = np.array(u, dtype=np.float64)
+    v = np.array(v, dtype=np.float64)
+    hole = np.isnan(u)
+    _, fu = _med_fluct(u)
+    _, fv = _med_fluct(v)
+    mag = np.sqrt(np.nan_to_num(fu) ** 2 + np.nan_to_num(fv) ** 2)
+    out = mag > Thr0
+    if not out.any() and not hole.any():
+        return u, v, cc
+    u2 = u.copy()
+    v2 = v.copy()
+    u2[out] = np.nan
+    v2[out] = np.nan
+    u2 = inpaint_springs(u2)
+    v2 = inpaint_springs(v2)
+    u2[hole] = np.nan
+    v2[hole] = np.nan
+    return u2, v2, cc
+
+
+def inpaint_springs(a):
+    """Spring-model fill of NaNs (the method of inpaint_nans(.,4)): minimise
+    the sum of squared differences across every horizontal, vertical and
+    diagonal neighbour pair with the known values fixed."""
+    a = np.array(a, dtype=np.float64)
+    nanm = np.isnan(a)
+    if not nanm.any() or nanm.all():
+        return a
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    M, N = a.shape
+    idx = np.arange(M * N).reshape(M, N, order="F")
+    rows, cols, vals = [], [], []
+    r = 0
+    for di, dj in ((1, 0), (0, 1), (1, 1), (1, -1)):
+        i0, i1 = max(0, -di), M - max(0, di)
+        j0, j1 = max(0, -dj), N - max(0, dj)
+        p = idx[i0:i1, j0:j1].reshape(-1)
+        q = idx[i0 + di:i1 + di, j0 + dj:j1 + dj].reshape(-1)
+        k = np.arange(p.size) + r
+        rows += [k, k]
+        cols += [p, q]
+        vals += [np.ones(p.size), -np.ones(p.size)]
+        r += p.size
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(r, M * N))
+    flat = a.reshape(-1, order="F")
+    unk = np.flatnonzero(np.isnan(flat))
+    kn = np.flatnonzero(~np.isnan(flat))
+    rhs = -A[:, kn] @ flat[kn]
+    Au = A[:, unk]
+    sol = spl.lsqr(Au, rhs, atol=1e-12, btol=1e-12)[0]
+    flat = flat.copy()
+    flat[unk] = sol
+    return flat.reshape(M, N, order="F")
+
+
+# ---------------------------------------------------------------------------
+# device-resident entry points (pointers are device addresses as ints, e.g.
+# torch tensor.data_ptr(); torch is only the allocator here)
+# ---------------------------------------------------------------------------
+class Plan:
+    """Size-class node lists + hand-on scratch for repeated device solves."""
+
+    def __init__(self, winsize, n, rect=False, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        ws = np.asarray(winsize, dtype=np.int32)
+        per = 1 if ws.size > (2 if rect else 1) else 0
+        ws = _F(ws.reshape(-1, order="F"), np.int32)
+        p = C.c_void_p()
+        check(lib().aldic_plan_create(self.ctx.ptr, ws, int(n), per, 1 if rect else 0,
+                                      C.byref(p)))
+        self.ptr = p
+        self.n = int(n)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().aldic_plan_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def local_icgn_dev(images: Images, plan: Plan, d_coords, d_U0, d_winsize, tol, d_U, d_F=None,
+                   d_H21=None, d_conv=None, d_hole=None, d_zncc=None):
+    check(lib().aldic_local_icgn_dev(images.ctx.ptr, images.ptr, plan.ptr, d_coords, d_U0,
+                                     d_winsize, float(tol), d_U, d_F, d_H21, d_conv, d_hole,
+                                     d_zncc))
+
+
+def subpb1_dev(images: Images, plan: Plan, d_coords, d_UOld, d_FOld, d_vdual, d_winsize_xy, mu,
+               tol, d_U, d_conv):
+    check(lib().aldic_subpb1_dev(images.ctx.ptr, images.ptr, plan.ptr, d_coords, d_UOld, d_FOld,
+                                 d_vdual, d_winsize_xy, float(mu), float(tol), d_U, d_conv))
+
+
+def integer_search_dev(ctx: Context, d_f, d_g, M, N, gridx, gridy, winsize, winstepsize, region,
+                       d_x, d_y, d_u, d_v, d_ccmax, d_ipr, d_ipc):
+    gx = (C.c_int64 * 2)(int(gridx[0]), int(gridx[-1]))
+    gy = (C.c_int64 * 2)(int(gridy[0]), int(gridy[-1]))
+    rg = (C.c_int64 * 4)(*[int(v) for v in region])
+    check(lib().aldic_integer_search_dev(ctx.ptr, d_f, d_g, int(M), int(N), gx, gy, int(winsize),
+                                         int(winstepsize), rg, d_x, d_y, d_u, d_v, d_ccmax,
+                                         d_ipr, d_ipc))

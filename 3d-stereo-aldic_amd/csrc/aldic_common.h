@@ -1,0 +1,55 @@
+// aldic_common.h -- shared device helpers for the gfx950 kernels of the ALDIC
+// local matching path.  Wave64 everywhere: reductions use DPP row operations
+// and the GFX9 row broadcasts, never 32-lane warp idioms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ALDIC_WAVE 64
+
+// Status codes shared with the reference (funICGNQuadtree.m:425-444).
+#define ST_ERR (-1)      // MATLAB would throw inside the subset
+#define ST_NOCONV 101    // not converged / NaN / warped out of the image
+#define ST_MASKED 102    // >= 40% of the subset masked
+#define ST_EMPTY 103     // sum |f| < 1e-6
+#define ST_DEFER (-100)  // internal: fast kernel handed the subset on
+
+// Sum over the 64 lanes of a wave; every lane receives the total.
+__device__ __forceinline__ float wave_sum(float v) {
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x124, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x128, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x142, 0xA, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x143, 0xC, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), 63));
+}
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x124, 0xF, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+// True on every lane if the predicate holds on any lane of the wave.
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// MATLAB round(): half away from zero.
+__device__ __forceinline__ int64_t mround(double v) { return (int64_t)round(v); }
+
+// Separable form of the reference's local bicubic kernel: the MEX's 16
+// coefficients a_ij = rot90(BC*Q*BC',2) (ba_interp2_spline.cpp:310-325,
+// func/BicubicBsplineBuild.m:13-16) factor as sum_rc Q[r][c] w_r(dx) w_c(dy)
+// with 56*w_k(t) the cubics below (taps -1, 0, +1, +2).
+__device__ __forceinline__ void spline_weights(float t, float w[4]) {
+  const float s = 1.0f / 56.0f;
+  w[0] = t * (-33.0f + t * (57.0f - 24.0f * t)) * s;
+  w[1] = (56.0f + t * (-3.0f + t * (-117.0f + 64.0f * t))) * s;
+  w[2] = t * (45.0f + t * (75.0f - 64.0f * t)) * s;
+  w[3] = t * (-9.0f + t * (-15.0f + 24.0f * t)) * s;
+}

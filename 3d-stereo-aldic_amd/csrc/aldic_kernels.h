@@ -1,0 +1,76 @@
+// aldic_kernels.h -- internal interface between the C-ABI layer (capi.hip) and
+// the kernels.  Not part of the public header.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Arguments of the per-subset solvers (IC-GN and Subpb1).  Images are M x N
+// column-major FP32 planes with the first dimension along image x, all of
+// the same shape: fm = ImgRef .* mask, fx/fy = Df.DfDx/Df.DfDy placed on the
+// full grid (pixel (i,j) holds DfDx(i-3,j-3), zero on the 3-pixel border),
+// g = ImgDef, mask = ImgRefMask.
+struct SolveArgs {
+  const float* fm;
+  const float* fx;
+  const float* fy;
+  const float* g;
+  const float* mask;
+  int M, N;
+  int n;                // number of nodes (stride of H21)
+  const int* list;      // node indices handled by this launch
+  int cnt;              // length of list
+  const double* coords; // n x 2 column-major (coordinatesFEM), 1-based
+  const int* wsx;       // winsize (IC-GN) or winsize_List(:,1) (Subpb1)
+  const int* wsy;       // winsize_List(:,2) for Subpb1, nullptr for IC-GN
+  int ws_per_node;      // 0: wsx[0] (and wsy[0]) for every node
+  const float* U0;      // IC-GN: 2n interleaved initial guess
+  const float* UOld;    // Subpb1: 2n
+  const float* FOld;    // Subpb1: 4n
+  const float* vdual;   // Subpb1: 2n
+  float mu;
+  float tol;
+  float* U;             // 2n
+  float* F;             // 4n (IC-GN), may be null
+  float* H21;           // n x 21 column-major (IC-GN), may be null
+  int* conv;            // n
+  float* zncc;          // n (IC-GN), may be null
+  uint8_t* hole;        // n (IC-GN markCoordHoleStrain flags), may be null
+  int* defer_cnt;       // subsets handed from the fast to the generic kernel
+  int* defer_list;
+};
+
+namespace aldic {
+int size_class(int S1, int S2);
+hipError_t launch_fast(int mode, int cls, const SolveArgs& a, hipStream_t s);
+hipError_t launch_generic(int mode, const SolveArgs& a, const int* list, const int* count,
+                          int fixed_count, int grid, hipStream_t s);
+int generic_max_pixels();
+
+// image preparation (prep.hip)
+hipError_t launch_prep_planes(const double* ref, const double* def, const double* mask,
+                              int64_t M, int64_t N, float* fm, float* g, float* maskf,
+                              hipStream_t s);
+hipError_t launch_gradient(const double* ref, const double* mask, int64_t M, int64_t N,
+                           float* fx, float* fy, hipStream_t s);
+hipError_t launch_place_df(const double* DfDx, const double* DfDy, int64_t M, int64_t N,
+                           float* fx, float* fy, hipStream_t s);
+hipError_t launch_gradient_f32(const float* ref, const float* mask, int64_t M, int64_t N,
+                               float* fx, float* fy, hipStream_t s);
+hipError_t launch_gradient_out_f64(const double* ref, const double* mask, int64_t M, int64_t N,
+                                   double* DfDx, double* DfDy, hipStream_t s);
+
+// integer search (search.hip)
+size_t search_smem_bytes(int ws);
+hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, int nY,
+                         const int gridx[2], const int gridy[2], int ws, int step,
+                         const int reg[4], float* gmap, int64_t kmax, double* x, double* y,
+                         double* u, double* v, double* ccmax, int* ipr, int* ipc,
+                         hipStream_t s);
+
+// interpolation (interp.hip)
+hipError_t launch_interp_f64(const double* F, int64_t M, int64_t N, int64_t P, const double* X,
+                             const double* Y, int64_t ND, double* Z, hipStream_t s);
+hipError_t launch_interp_f32(const float* F, int64_t M, int64_t N, int64_t P, const float* X,
+                             const float* Y, int64_t ND, float* Z, hipStream_t s);
+}  // namespace aldic

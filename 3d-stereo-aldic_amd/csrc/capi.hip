@@ -1,0 +1,733 @@
+// capi.hip -- the C ABI of include/aldic.h: contexts, resident image planes,
+// plans (size-class node lists + hand-on scratch) and the host-pointer
+// drop-in entry points that copy in, launch and copy out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aldic.h"
+#include "aldic_kernels.h"
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? ALDIC_ENOMEM : ALDIC_EDEVICE,            \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+extern "C" const char* aldic_last_error(void) { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// objects
+// ---------------------------------------------------------------------------
+struct aldic_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // reusable staging buffers of the host-pointer entry points
+  std::vector<void*> buf;
+  std::vector<size_t> cap;
+};
+
+struct aldic_images {
+  aldic_ctx* ctx = nullptr;
+  int64_t M = 0, N = 0;
+  float* fm = nullptr;
+  float* g = nullptr;
+  float* mask = nullptr;
+  float* fx = nullptr;
+  float* fy = nullptr;
+};
+
+struct aldic_plan {
+  aldic_ctx* ctx = nullptr;
+  int64_t n = 0;
+  int rect = 0;
+  int* lists[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int cnt[5] = {0, 0, 0, 0, 0};
+  int* defer_cnt = nullptr;
+  int* defer_list = nullptr;
+};
+
+namespace {
+
+__global__ void zero_ints(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+int set_device(aldic_ctx* ctx) {
+  HIP_TRY(hipSetDevice(ctx->device));
+  return ALDIC_OK;
+}
+
+// staging buffer `slot` of at least `bytes`
+int staging(aldic_ctx* ctx, int slot, size_t bytes, void** out) {
+  if ((int)ctx->buf.size() <= slot) {
+    ctx->buf.resize(slot + 1, nullptr);
+    ctx->cap.resize(slot + 1, 0);
+  }
+  if (ctx->cap[slot] < bytes) {
+    if (ctx->buf[slot]) HIP_TRY(hipFree(ctx->buf[slot]));
+    ctx->buf[slot] = nullptr;
+    ctx->cap[slot] = 0;
+    const size_t want = std::max<size_t>(bytes, 256);
+    HIP_TRY(hipMalloc(&ctx->buf[slot], want));
+    ctx->cap[slot] = want;
+  }
+  *out = ctx->buf[slot];
+  return ALDIC_OK;
+}
+
+template <typename T>
+int stage(aldic_ctx* ctx, int slot, size_t count, T** out) {
+  void* p = nullptr;
+  const int rc = staging(ctx, slot, count * sizeof(T), &p);
+  *out = static_cast<T*>(p);
+  return rc;
+}
+
+#define RC_TRY(expr)            \
+  do {                          \
+    const int rc_ = (expr);     \
+    if (rc_ != ALDIC_OK) return rc_; \
+  } while (0)
+
+int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, int rect,
+               aldic_plan* p) {
+  p->ctx = ctx;
+  p->n = n;
+  p->rect = rect;
+  std::vector<int> lists[5];
+  for (int64_t i = 0; i < n; i++) {
+    int wx, wy;
+    if (rect) {
+      wx = per_node ? winsize[i] : winsize[0];
+      wy = per_node ? winsize[i + n] : winsize[1];
+    } else {
+      wx = wy = per_node ? winsize[i] : winsize[0];
+    }
+    if (wx < 0 || wy < 0 || (wx % 2) || (wy % 2))
+      return fail(ALDIC_EINVAL, "winsize must be even and non-negative (validateDICpara.m:22-28)");
+    lists[aldic::size_class(wx + 1, wy + 1)].push_back((int)i);
+  }
+  for (int c = 0; c < 5; c++) {
+    p->cnt[c] = (int)lists[c].size();
+    if (p->cnt[c] > 0) {
+      HIP_TRY(hipMalloc(&p->lists[c], sizeof(int) * lists[c].size()));
+      HIP_TRY(hipMemcpy(p->lists[c], lists[c].data(), sizeof(int) * lists[c].size(),
+                        hipMemcpyHostToDevice));
+    }
+  }
+  HIP_TRY(hipMalloc(&p->defer_cnt, sizeof(int)));
+  HIP_TRY(hipMalloc(&p->defer_list, sizeof(int) * std::max<int64_t>(n, 1)));
+  return ALDIC_OK;
+}
+
+void free_plan(aldic_plan* p) {
+  for (int c = 0; c < 5; c++)
+    if (p->lists[c]) (void)hipFree(p->lists[c]);
+  if (p->defer_cnt) (void)hipFree(p->defer_cnt);
+  if (p->defer_list) (void)hipFree(p->defer_list);
+}
+
+// Launch sequence shared by IC-GN (mode 0) and Subpb1 (mode 1).
+int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
+  hipStream_t s = ctx->stream;
+  a.n = (int)p->n;
+  a.defer_cnt = p->defer_cnt;
+  a.defer_list = p->defer_list;
+  zero_ints<<<1, 64, 0, s>>>(p->defer_cnt, 1);
+  HIP_TRY(hipGetLastError());
+  for (int c = 0; c < 4; c++) {
+    if (p->cnt[c] == 0) continue;
+    a.list = p->lists[c];
+    a.cnt = p->cnt[c];
+    HIP_TRY(aldic::launch_fast(mode, c, a, s));
+  }
+  if (p->cnt[4] > 0) {
+    const int grid = std::min(p->cnt[4], 2048);
+    HIP_TRY(aldic::launch_generic(mode, a, p->lists[4], nullptr, p->cnt[4], grid, s));
+  }
+  // subsets handed on by the fast kernels; the grid-stride loop reads the
+  // count on the device, so no host round trip is needed
+  const int grid = (int)std::min<int64_t>(std::max<int64_t>(p->n, 1), 1024);
+  HIP_TRY(aldic::launch_generic(mode, a, p->defer_list, p->defer_cnt, 0, grid, s));
+  return ALDIC_OK;
+}
+
+bool images_ok(const aldic_images* im) {
+  return im && im->fm && im->g && im->mask && im->fx && im->fy && im->M >= 7 && im->N >= 7;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int aldic_ctx_create(int device, aldic_ctx** out) {
+  if (!out) return fail(ALDIC_EINVAL, "aldic_ctx_create: out is NULL");
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(ALDIC_EINVAL, "aldic_ctx_create: no such device");
+  auto* c = new aldic_ctx();
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(ALDIC_EDEVICE, std::string("aldic_ctx_create: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_ctx_destroy(aldic_ctx* ctx) {
+  if (!ctx) return ALDIC_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : ctx->buf)
+    if (p) (void)hipFree(p);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return ALDIC_OK;
+}
+
+extern "C" void* aldic_ctx_stream(aldic_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+extern "C" int aldic_ctx_synchronize(aldic_ctx* ctx) {
+  if (!ctx) return fail(ALDIC_EINVAL, "NULL context");
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return ALDIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// ba_interp2_spline
+// ---------------------------------------------------------------------------
+template <typename T>
+static int interp_host(aldic_ctx* ctx, const T* F, int64_t M, int64_t N, int64_t P, const T* X,
+                       const T* Y, int64_t ND, T* Z) {
+  if (!ctx || !F || !X || !Y || !Z) return fail(ALDIC_EINVAL, "ba_interp2_spline: NULL argument");
+  if (M < 1 || N < 1 || P < 1 || ND < 0)
+    return fail(ALDIC_EINVAL, "ba_interp2_spline: bad dimensions");
+  if (ND == 0) return ALDIC_OK;
+  RC_TRY(set_device(ctx));
+  T *dF, *dX, *dY, *dZ;
+  RC_TRY(stage(ctx, 0, (size_t)(M * N * P), &dF));
+  RC_TRY(stage(ctx, 1, (size_t)ND, &dX));
+  RC_TRY(stage(ctx, 2, (size_t)ND, &dY));
+  RC_TRY(stage(ctx, 3, (size_t)(ND * P), &dZ));
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(dF, F, sizeof(T) * M * N * P, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dX, X, sizeof(T) * ND, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dY, Y, sizeof(T) * ND, hipMemcpyHostToDevice, s));
+  if constexpr (sizeof(T) == 8)
+    HIP_TRY(aldic::launch_interp_f64(dF, M, N, P, dX, dY, ND, dZ, s));
+  else
+    HIP_TRY(aldic::launch_interp_f32(dF, M, N, P, dX, dY, ND, dZ, s));
+  HIP_TRY(hipMemcpyAsync(Z, dZ, sizeof(T) * ND * P, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_interp_spline_f64(aldic_ctx* ctx, const double* F, int64_t M, int64_t N,
+                                       int64_t P, const double* X, const double* Y, int64_t ND,
+                                       double* Z) {
+  return interp_host<double>(ctx, F, M, N, P, X, Y, ND, Z);
+}
+
+extern "C" int aldic_interp_spline_f32(aldic_ctx* ctx, const float* F, int64_t M, int64_t N,
+                                       int64_t P, const float* X, const float* Y, int64_t ND,
+                                       float* Z) {
+  return interp_host<float>(ctx, F, M, N, P, X, Y, ND, Z);
+}
+
+// ---------------------------------------------------------------------------
+// funImgGradient
+// ---------------------------------------------------------------------------
+extern "C" int aldic_img_gradient(aldic_ctx* ctx, const double* ImgRef, const double* mask,
+                                  int64_t M, int64_t N, double* DfDx, double* DfDy) {
+  if (!ctx || !ImgRef || !DfDx || !DfDy) return fail(ALDIC_EINVAL, "funImgGradient: NULL argument");
+  if (M < 7 || N < 7) return fail(ALDIC_EINVAL, "funImgGradient: image smaller than 7 x 7");
+  RC_TRY(set_device(ctx));
+  double *dI, *dm = nullptr, *dx, *dy;
+  const int64_t MN = M * N, MC = (M - 6) * (N - 6);
+  RC_TRY(stage(ctx, 0, (size_t)MN, &dI));
+  if (mask) RC_TRY(stage(ctx, 1, (size_t)MN, &dm));
+  RC_TRY(stage(ctx, 2, (size_t)MC, &dx));
+  RC_TRY(stage(ctx, 3, (size_t)MC, &dy));
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(dI, ImgRef, sizeof(double) * MN, hipMemcpyHostToDevice, s));
+  if (mask) HIP_TRY(hipMemcpyAsync(dm, mask, sizeof(double) * MN, hipMemcpyHostToDevice, s));
+  HIP_TRY(aldic::launch_gradient_out_f64(dI, dm, M, N, dx, dy, s));
+  HIP_TRY(hipMemcpyAsync(DfDx, dx, sizeof(double) * MC, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(DfDy, dy, sizeof(double) * MC, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return ALDIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// images
+// ---------------------------------------------------------------------------
+static int alloc_planes(aldic_images* im) {
+  const size_t bytes = sizeof(float) * (size_t)(im->M * im->N);
+  HIP_TRY(hipMalloc(&im->fm, bytes));
+  HIP_TRY(hipMalloc(&im->g, bytes));
+  HIP_TRY(hipMalloc(&im->mask, bytes));
+  HIP_TRY(hipMalloc(&im->fx, bytes));
+  HIP_TRY(hipMalloc(&im->fy, bytes));
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_images_destroy(aldic_images* im) {
+  if (!im) return ALDIC_OK;
+  if (im->ctx) (void)hipSetDevice(im->ctx->device);
+  float* ps[5] = {im->fm, im->g, im->mask, im->fx, im->fy};
+  for (float* p : ps)
+    if (p) (void)hipFree(p);
+  delete im;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_images_create(aldic_ctx* ctx, const double* ImgRef, const double* ImgDef,
+                                   const double* mask, const double* DfDx, const double* DfDy,
+                                   int64_t M, int64_t N, aldic_images** out) {
+  if (!ctx || !ImgRef || !ImgDef || !out) return fail(ALDIC_EINVAL, "images: NULL argument");
+  if (M < 7 || N < 7) return fail(ALDIC_EINVAL, "images: smaller than 7 x 7");
+  if ((DfDx == nullptr) != (DfDy == nullptr))
+    return fail(ALDIC_EINVAL, "images: give both DfDx and DfDy or neither");
+  RC_TRY(set_device(ctx));
+  auto* im = new aldic_images();
+  im->ctx = ctx;
+  im->M = M;
+  im->N = N;
+  int rc = alloc_planes(im);
+  if (rc != ALDIC_OK) {
+    aldic_images_destroy(im);
+    return rc;
+  }
+  const int64_t MN = M * N;
+  double *dr, *dd, *dm = nullptr;
+  hipStream_t s = ctx->stream;
+  auto body = [&]() -> int {
+    RC_TRY(stage(ctx, 0, (size_t)MN, &dr));
+    RC_TRY(stage(ctx, 1, (size_t)MN, &dd));
+    if (mask) RC_TRY(stage(ctx, 2, (size_t)MN, &dm));
+    HIP_TRY(hipMemcpyAsync(dr, ImgRef, sizeof(double) * MN, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dd, ImgDef, sizeof(double) * MN, hipMemcpyHostToDevice, s));
+    if (mask) HIP_TRY(hipMemcpyAsync(dm, mask, sizeof(double) * MN, hipMemcpyHostToDevice, s));
+    HIP_TRY(aldic::launch_prep_planes(dr, dd, dm, M, N, im->fm, im->g, im->mask, s));
+    if (DfDx) {
+      const int64_t MC = (M - 6) * (N - 6);
+      double *dgx, *dgy;
+      RC_TRY(stage(ctx, 3, (size_t)MC, &dgx));
+      RC_TRY(stage(ctx, 4, (size_t)MC, &dgy));
+      HIP_TRY(hipMemcpyAsync(dgx, DfDx, sizeof(double) * MC, hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemcpyAsync(dgy, DfDy, sizeof(double) * MC, hipMemcpyHostToDevice, s));
+      HIP_TRY(aldic::launch_place_df(dgx, dgy, M, N, im->fx, im->fy, s));
+    } else {
+      HIP_TRY(aldic::launch_gradient(dr, dm, M, N, im->fx, im->fy, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return ALDIC_OK;
+  };
+  rc = body();
+  if (rc != ALDIC_OK) {
+    aldic_images_destroy(im);
+    return rc;
+  }
+  *out = im;
+  return ALDIC_OK;
+}
+
+namespace {
+__global__ void fill_ones(float* p, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 1.0f;
+}
+}  // namespace
+
+extern "C" int aldic_images_create_dev(aldic_ctx* ctx, const float* fm, const float* g,
+                                       const float* mask, const float* fx, const float* fy,
+                                       int64_t M, int64_t N, aldic_images** out) {
+  if (!ctx || !fm || !g || !out) return fail(ALDIC_EINVAL, "images_dev: NULL argument");
+  if (M < 7 || N < 7) return fail(ALDIC_EINVAL, "images_dev: smaller than 7 x 7");
+  if ((fx == nullptr) != (fy == nullptr))
+    return fail(ALDIC_EINVAL, "images_dev: give both fx and fy or neither");
+  RC_TRY(set_device(ctx));
+  auto* im = new aldic_images();
+  im->ctx = ctx;
+  im->M = M;
+  im->N = N;
+  int rc = alloc_planes(im);
+  if (rc != ALDIC_OK) {
+    aldic_images_destroy(im);
+    return rc;
+  }
+  const size_t bytes = sizeof(float) * (size_t)(M * N);
+  hipStream_t s = ctx->stream;
+  auto body = [&]() -> int {
+    HIP_TRY(hipMemcpyAsync(im->fm, fm, bytes, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(im->g, g, bytes, hipMemcpyDeviceToDevice, s));
+    if (mask) {
+      HIP_TRY(hipMemcpyAsync(im->mask, mask, bytes, hipMemcpyDeviceToDevice, s));
+    } else {
+      fill_ones<<<1024, 256, 0, s>>>(im->mask, M * N);
+      HIP_TRY(hipGetLastError());
+    }
+    if (fx) {
+      HIP_TRY(hipMemcpyAsync(im->fx, fx, bytes, hipMemcpyDeviceToDevice, s));
+      HIP_TRY(hipMemcpyAsync(im->fy, fy, bytes, hipMemcpyDeviceToDevice, s));
+    } else {
+      HIP_TRY(aldic::launch_gradient_f32(im->fm, nullptr, M, N, im->fx, im->fy, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return ALDIC_OK;
+  };
+  rc = body();
+  if (rc != ALDIC_OK) {
+    aldic_images_destroy(im);
+    return rc;
+  }
+  *out = im;
+  return ALDIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------
+extern "C" int aldic_plan_create(aldic_ctx* ctx, const int32_t* winsize, int64_t n,
+                                 int32_t winsize_per_node, int32_t rect, aldic_plan** out) {
+  if (!ctx || !winsize || !out || n < 0) return fail(ALDIC_EINVAL, "plan: bad argument");
+  if (n > 0x7fffffff) return fail(ALDIC_EINVAL, "plan: more than 2^31-1 nodes");
+  RC_TRY(set_device(ctx));
+  auto* p = new aldic_plan();
+  const int rc = build_plan(ctx, winsize, n, winsize_per_node, rect, p);
+  if (rc != ALDIC_OK) {
+    free_plan(p);
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_plan_destroy(aldic_plan* plan) {
+  if (!plan) return ALDIC_OK;
+  if (plan->ctx) (void)hipSetDevice(plan->ctx->device);
+  free_plan(plan);
+  delete plan;
+  return ALDIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// LocalICGN / Subpb1, device-resident
+// ---------------------------------------------------------------------------
+extern "C" int aldic_local_icgn_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
+                                    const double* d_coords, const float* d_U0,
+                                    const int32_t* d_winsize, double tol, float* d_U, float* d_F,
+                                    float* d_H21, int32_t* d_convIt, uint8_t* d_hole,
+                                    float* d_zncc) {
+  if (!ctx || !images_ok(im) || !plan || plan->rect)
+    return fail(ALDIC_EINVAL, "local_icgn_dev: bad context/images/plan");
+  if (plan->n == 0) return ALDIC_OK;
+  if (!d_coords || !d_U0 || !d_winsize || !d_U || !d_convIt)
+    return fail(ALDIC_EINVAL, "local_icgn_dev: NULL array");
+  RC_TRY(set_device(ctx));
+  SolveArgs a{};
+  a.fm = im->fm;
+  a.fx = im->fx;
+  a.fy = im->fy;
+  a.g = im->g;
+  a.mask = im->mask;
+  a.M = (int)im->M;
+  a.N = (int)im->N;
+  a.coords = d_coords;
+  a.wsx = d_winsize;
+  a.wsy = nullptr;
+  // a plan over one winsize value still carries a per-node flag of 0
+  a.ws_per_node = 1;
+  a.U0 = d_U0;
+  a.tol = (float)tol;
+  a.U = d_U;
+  a.F = d_F;
+  a.H21 = d_H21;
+  a.conv = d_convIt;
+  a.zncc = d_zncc;
+  a.hole = d_hole;
+  return run_solver(ctx, 0, a, plan);
+}
+
+extern "C" int aldic_subpb1_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
+                                const double* d_coords, const float* d_UOld, const float* d_FOld,
+                                const float* d_vdual, const int32_t* d_winsize_xy, double mu,
+                                double tol, float* d_U, int32_t* d_convIt) {
+  if (!ctx || !images_ok(im) || !plan || !plan->rect)
+    return fail(ALDIC_EINVAL, "subpb1_dev: bad context/images/plan (plan must be rect)");
+  if (plan->n == 0) return ALDIC_OK;
+  if (!d_coords || !d_UOld || !d_FOld || !d_vdual || !d_winsize_xy || !d_U || !d_convIt)
+    return fail(ALDIC_EINVAL, "subpb1_dev: NULL array");
+  RC_TRY(set_device(ctx));
+  SolveArgs a{};
+  a.fm = im->fm;
+  a.fx = im->fx;
+  a.fy = im->fy;
+  a.g = im->g;
+  a.mask = im->mask;
+  a.M = (int)im->M;
+  a.N = (int)im->N;
+  a.coords = d_coords;
+  a.wsx = d_winsize_xy;
+  a.wsy = d_winsize_xy + plan->n;
+  a.ws_per_node = 1;
+  a.UOld = d_UOld;
+  a.FOld = d_FOld;
+  a.vdual = d_vdual;
+  a.mu = (float)mu;
+  a.tol = (float)tol;
+  a.U = d_U;
+  a.conv = d_convIt;
+  return run_solver(ctx, 1, a, plan);
+}
+
+// ---------------------------------------------------------------------------
+// LocalICGN / Subpb1, host pointers
+// ---------------------------------------------------------------------------
+extern "C" int aldic_local_icgn(aldic_ctx* ctx, const aldic_images* im, const double* coords,
+                                const double* U0, int64_t n, const int32_t* winsize,
+                                int32_t winsize_per_node, double tol, double* U, double* F,
+                                double* H21, int32_t* convIt, uint8_t* markHole, double* zncc) {
+  if (!ctx || !images_ok(im) || !coords || !U0 || !winsize || !U || !F || !convIt)
+    return fail(ALDIC_EINVAL, "LocalICGNQuadtree: NULL argument");
+  if (n < 0) return fail(ALDIC_EINVAL, "LocalICGNQuadtree: n < 0");
+  if (n == 0) return ALDIC_OK;
+  RC_TRY(set_device(ctx));
+  // per-node winsize array on both sides
+  std::vector<int32_t> ws((size_t)n);
+  for (int64_t i = 0; i < n; i++) ws[i] = winsize_per_node ? winsize[i] : winsize[0];
+  aldic_plan plan;
+  int rc = build_plan(ctx, ws.data(), n, 1, 0, &plan);
+  if (rc != ALDIC_OK) {
+    free_plan(&plan);
+    return rc;
+  }
+  std::vector<float> U0f((size_t)(2 * n));
+  for (int64_t i = 0; i < 2 * n; i++) U0f[i] = (float)U0[i];
+  auto body = [&]() -> int {
+    double* dc;
+    float *dU0, *dU, *dF, *dH, *dz;
+    int32_t *dws, *dconv;
+    uint8_t* dhole;
+    RC_TRY(stage(ctx, 0, (size_t)(2 * n), &dc));
+    RC_TRY(stage(ctx, 1, (size_t)(2 * n), &dU0));
+    RC_TRY(stage(ctx, 2, (size_t)n, &dws));
+    RC_TRY(stage(ctx, 3, (size_t)(2 * n), &dU));
+    RC_TRY(stage(ctx, 4, (size_t)(4 * n), &dF));
+    RC_TRY(stage(ctx, 5, (size_t)(21 * n), &dH));
+    RC_TRY(stage(ctx, 6, (size_t)n, &dconv));
+    RC_TRY(stage(ctx, 7, (size_t)n, &dhole));
+    RC_TRY(stage(ctx, 8, (size_t)n, &dz));
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(dc, coords, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dU0, U0f.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dws, ws.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    RC_TRY(aldic_local_icgn_dev(ctx, im, &plan, dc, dU0, dws, tol, dU, dF, dH, dconv, dhole, dz));
+    std::vector<float> hU((size_t)(2 * n)), hF((size_t)(4 * n)), hH((size_t)(21 * n)),
+        hz((size_t)n);
+    HIP_TRY(hipMemcpyAsync(hU.data(), dU, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hF.data(), dF, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hH.data(), dH, sizeof(float) * 21 * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hz.data(), dz, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(convIt, dconv, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    std::vector<uint8_t> hh((size_t)n);
+    HIP_TRY(hipMemcpyAsync(hh.data(), dhole, n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < 2 * n; i++) U[i] = hU[i];
+    for (int64_t i = 0; i < 4 * n; i++) F[i] = hF[i];
+    if (H21)
+      for (int64_t i = 0; i < 21 * n; i++) H21[i] = hH[i];
+    if (zncc)
+      for (int64_t i = 0; i < n; i++) zncc[i] = hz[i];
+    if (markHole) std::memcpy(markHole, hh.data(), (size_t)n);
+    return ALDIC_OK;
+  };
+  rc = body();
+  (void)hipStreamSynchronize(ctx->stream);
+  free_plan(&plan);
+  return rc;
+}
+
+extern "C" int aldic_subpb1(aldic_ctx* ctx, const aldic_images* im, const double* coords,
+                            const double* UOld, const double* FOld, const double* udual,
+                            const double* vdual, int64_t n, const int32_t* winsize_xy, double mu,
+                            double tol, double* U, int32_t* convIt) {
+  (void)udual;  // read by the reference, unused by its first-order update
+  if (!ctx || !images_ok(im) || !coords || !UOld || !FOld || !vdual || !winsize_xy || !U ||
+      !convIt)
+    return fail(ALDIC_EINVAL, "Subpb1Quadtree: NULL argument");
+  if (n < 0) return fail(ALDIC_EINVAL, "Subpb1Quadtree: n < 0");
+  if (n == 0) return ALDIC_OK;
+  RC_TRY(set_device(ctx));
+  aldic_plan plan;
+  int rc = build_plan(ctx, winsize_xy, n, 1, 1, &plan);
+  if (rc != ALDIC_OK) {
+    free_plan(&plan);
+    return rc;
+  }
+  std::vector<float> hUO((size_t)(2 * n)), hFO((size_t)(4 * n)), hvd((size_t)(2 * n));
+  for (int64_t i = 0; i < 2 * n; i++) {
+    hUO[i] = (float)UOld[i];
+    hvd[i] = (float)vdual[i];
+  }
+  for (int64_t i = 0; i < 4 * n; i++) hFO[i] = (float)FOld[i];
+  auto body = [&]() -> int {
+    double* dc;
+    float *dUO, *dFO, *dvd, *dU;
+    int32_t *dws, *dconv;
+    RC_TRY(stage(ctx, 0, (size_t)(2 * n), &dc));
+    RC_TRY(stage(ctx, 1, (size_t)(2 * n), &dUO));
+    RC_TRY(stage(ctx, 2, (size_t)(4 * n), &dFO));
+    RC_TRY(stage(ctx, 3, (size_t)(2 * n), &dvd));
+    RC_TRY(stage(ctx, 4, (size_t)(2 * n), &dws));
+    RC_TRY(stage(ctx, 5, (size_t)(2 * n), &dU));
+    RC_TRY(stage(ctx, 6, (size_t)n, &dconv));
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(dc, coords, sizeof(double) * 2 * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dUO, hUO.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dFO, hFO.data(), sizeof(float) * 4 * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dvd, hvd.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dws, winsize_xy, sizeof(int32_t) * 2 * n, hipMemcpyHostToDevice, s));
+    RC_TRY(aldic_subpb1_dev(ctx, im, &plan, dc, dUO, dFO, dvd, dws, mu, tol, dU, dconv));
+    std::vector<float> hU((size_t)(2 * n));
+    HIP_TRY(hipMemcpyAsync(hU.data(), dU, sizeof(float) * 2 * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(convIt, dconv, sizeof(int32_t) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int64_t i = 0; i < 2 * n; i++) U[i] = hU[i];
+    return ALDIC_OK;
+  };
+  rc = body();
+  (void)hipStreamSynchronize(ctx->stream);
+  free_plan(&plan);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// integer search
+// ---------------------------------------------------------------------------
+namespace {
+
+// XList = gridx(1):step:gridx(2)-ws-step (funIntegerSearch.m:106-107)
+int64_t grid_count(int64_t a, int64_t b, int64_t ws, int64_t step) {
+  const int64_t last = b - ws - step;
+  if (step <= 0 || last < a) return 0;
+  return (last - a) / step + 1;
+}
+
+constexpr int SLOT_GMAP = 20;
+
+int search_checks(int64_t M, int64_t N, const int64_t gridx[2], const int64_t gridy[2],
+                  int32_t ws, int32_t step, const int64_t region[4]) {
+  if (!gridx || !gridy || !region) return fail(ALDIC_EINVAL, "integer search: NULL argument");
+  if (ws < 0 || (ws % 2)) return fail(ALDIC_EINVAL, "integer search: winsize must be even");
+  if (step <= 0) return fail(ALDIC_EINVAL, "integer search: winstepsize must be positive");
+  for (int k = 0; k < 4; k++)
+    if (region[k] < 0) return fail(ALDIC_EINVAL, "integer search: negative search region");
+  if (gridx[0] < 1 || gridy[0] < 1 || gridx[1] > M || gridy[1] > N)
+    return fail(ALDIC_EINVAL, "integer search: ROI outside the image (funNormalizeImg.m:24-28)");
+  if (M > 0x7fffffff || N > 0x7fffffff) return fail(ALDIC_EINVAL, "integer search: image too large");
+  if (aldic::search_smem_bytes(ws) > 160 * 1024)
+    return fail(ALDIC_EINVAL, "integer search: winsize too large for one workgroup's LDS");
+  return ALDIC_OK;
+}
+
+}  // namespace
+
+extern "C" int aldic_integer_search_dev(aldic_ctx* ctx, const float* d_f, const float* d_g,
+                                        int64_t M, int64_t N, const int64_t gridx[2],
+                                        const int64_t gridy[2], int32_t winsize,
+                                        int32_t winstepsize, const int64_t region[4], double* d_x,
+                                        double* d_y, double* d_u, double* d_v, double* d_ccmax,
+                                        int32_t* d_ipr, int32_t* d_ipc) {
+  if (!ctx || !d_f || !d_g) return fail(ALDIC_EINVAL, "integer search: NULL argument");
+  RC_TRY(search_checks(M, N, gridx, gridy, winsize, winstepsize, region));
+  const int64_t nX = grid_count(gridx[0], gridx[1], winsize, winstepsize);
+  const int64_t nY = grid_count(gridy[0], gridy[1], winsize, winstepsize);
+  if (nX * nY == 0) return ALDIC_OK;
+  if (!d_x || !d_y || !d_u || !d_v || !d_ccmax || !d_ipr || !d_ipc)
+    return fail(ALDIC_EINVAL, "integer search: NULL output");
+  RC_TRY(set_device(ctx));
+  const int64_t kmax = (region[0] + region[1] + 3) * (region[2] + region[3] + 3);
+  float* gmap;
+  RC_TRY(stage(ctx, SLOT_GMAP, (size_t)(kmax * nX * nY), &gmap));
+  const int gx[2] = {(int)gridx[0], (int)gridx[1]}, gy[2] = {(int)gridy[0], (int)gridy[1]};
+  const int rg[4] = {(int)region[0], (int)region[1], (int)region[2], (int)region[3]};
+  HIP_TRY(aldic::launch_search(d_f, d_g, (int)M, (int)N, (int)nX, (int)nY, gx, gy, winsize,
+                               winstepsize, rg, gmap, kmax, d_x, d_y, d_u, d_v, d_ccmax, d_ipr,
+                               d_ipc, ctx->stream));
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_integer_search(aldic_ctx* ctx, const double* f, const double* g, int64_t M,
+                                    int64_t N, const int64_t gridx[2], const int64_t gridy[2],
+                                    int32_t winsize, int32_t winstepsize, const int64_t region[4],
+                                    double* x, double* y, double* u, double* v, double* ccmax,
+                                    int32_t* ipr, int32_t* ipc, int64_t* nX_out,
+                                    int64_t* nY_out) {
+  if (!ctx) return fail(ALDIC_EINVAL, "integer search: NULL context");
+  RC_TRY(search_checks(M, N, gridx, gridy, winsize, winstepsize, region));
+  const int64_t nX = grid_count(gridx[0], gridx[1], winsize, winstepsize);
+  const int64_t nY = grid_count(gridy[0], gridy[1], winsize, winstepsize);
+  if (nX_out) *nX_out = nX;
+  if (nY_out) *nY_out = nY;
+  if (!x && !y && !u && !v) return ALDIC_OK;  // size query
+  if (!f || !g || !x || !y || !u || !v || !ccmax)
+    return fail(ALDIC_EINVAL, "integer search: NULL argument");
+  const int64_t np = nX * nY;
+  if (np == 0) return ALDIC_OK;
+  RC_TRY(set_device(ctx));
+  const int64_t MN = M * N;
+  double *df, *dg, *dx, *dy, *du, *dv, *dc;
+  float *ff, *fg;
+  int32_t *dpr, *dpc;
+  RC_TRY(stage(ctx, 0, (size_t)MN, &df));
+  RC_TRY(stage(ctx, 1, (size_t)MN, &dg));
+  RC_TRY(stage(ctx, 2, (size_t)MN, &ff));
+  RC_TRY(stage(ctx, 3, (size_t)MN, &fg));
+  RC_TRY(stage(ctx, 4, (size_t)np, &dx));
+  RC_TRY(stage(ctx, 5, (size_t)np, &dy));
+  RC_TRY(stage(ctx, 6, (size_t)np, &du));
+  RC_TRY(stage(ctx, 7, (size_t)np, &dv));
+  RC_TRY(stage(ctx, 8, (size_t)np, &dc));
+  RC_TRY(stage(ctx, 9, (size_t)np, &dpr));
+  RC_TRY(stage(ctx, 10, (size_t)np, &dpc));
+  hipStream_t s = ctx->stream;
+  HIP_TRY(hipMemcpyAsync(df, f, sizeof(double) * MN, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dg, g, sizeof(double) * MN, hipMemcpyHostToDevice, s));
+  HIP_TRY(aldic::launch_prep_planes(df, dg, nullptr, M, N, ff, fg, nullptr, s));
+  RC_TRY(aldic_integer_search_dev(ctx, ff, fg, M, N, gridx, gridy, winsize, winstepsize, region,
+                                  dx, dy, du, dv, dc, dpr, dpc));
+  HIP_TRY(hipMemcpyAsync(x, dx, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(y, dy, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(u, du, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(v, dv, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(ccmax, dc, sizeof(double) * np, hipMemcpyDeviceToHost, s));
+  if (ipr) HIP_TRY(hipMemcpyAsync(ipr, dpr, sizeof(int32_t) * np, hipMemcpyDeviceToHost, s));
+  if (ipc) HIP_TRY(hipMemcpyAsync(ipc, dpc, sizeof(int32_t) * np, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  return ALDIC_OK;
+}

@@ -1,0 +1,1176 @@
+// icgn.hip -- gfx950 kernels for the per-subset solvers of ALDIC's local
+// matching path:
+//   * first-order IC-GN  (func_quadtree/funICGNQuadtree.m:1-454)
+//   * ALM subproblem 1   (func_quadtree/funICGN_Subpb1quadtree.m:1-476)
+//
+// Two kernels per solver share one body each:
+//   fast_kernel<MODE,NW,PPL,TS>   NW waves per subset (NW=1: one wave per
+//       subset, 4 subsets per workgroup), PPL subset pixels per lane held in
+//       VGPRs (f, DfDx, DfDy, offsets), the deformed image staged in an LDS
+//       tile of TS x TS floats that is reloaded only when the warped subset
+//       leaves it, 4x4 separable spline gathers from that tile, all sums by
+//       wave (DPP) reductions in FP32, 6x6 Cholesky in registers.  It runs
+//       the reference's unmasked branch.  The moment a zero shows up in f
+//       or g (the reference then takes its mask branch, :68-88 and
+//       :166-226) or the warp outgrows the tile, it hands the subset to
+//   generic_kernel<MODE>   one 256-thread workgroup per subset, any size,
+//       full mask semantics (window growth, bwselect flood fills in LDS,
+//       per-iteration re-sizing), gathers straight from the global image.
+// Both restart nothing the other has written: a handed-on subset is solved
+// from scratch by the generic kernel, so the result is the reference's
+// whatever path produced it.
+#include "aldic_common.h"
+#include "aldic_kernels.h"
+
+namespace {
+
+constexpr int MODE_ICGN = 0;
+constexpr int MODE_SUBPB1 = 1;
+
+// ---------------------------------------------------------------------------
+// Small dense algebra (uniform across the lanes of a subset)
+// ---------------------------------------------------------------------------
+
+// Packed lower-triangular Cholesky of a 6x6 SPD matrix stored as the 21
+// entries H[r][c], r >= c, column by column (the reference's HGlobal order).
+// Returns false when a pivot is not positive.
+__device__ __forceinline__ int tri(int r, int c) {
+  // index of (r, c), r >= c, in column-major lower-triangular packing
+  return c * 6 - (c * (c - 1)) / 2 + (r - c);
+}
+
+__device__ __forceinline__ bool chol6(const float h[21], float L[21]) {
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    float d = h[tri(j, j)];
+#pragma unroll
+    for (int k = 0; k < j; k++) d -= L[tri(j, k)] * L[tri(j, k)];
+    if (!(d > 0.0f)) return false;
+    const float ljj = sqrtf(d);
+    L[tri(j, j)] = ljj;
+    const float inv = 1.0f / ljj;
+#pragma unroll
+    for (int i = j + 1; i < 6; i++) {
+      float s = h[tri(i, j)];
+#pragma unroll
+      for (int k = 0; k < j; k++) s -= L[tri(i, k)] * L[tri(j, k)];
+      L[tri(i, j)] = s * inv;
+    }
+  }
+  return true;
+}
+
+__device__ __forceinline__ void chol6_solve(const float L[21], const float b[6], float x[6]) {
+  float y[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    float s = b[i];
+#pragma unroll
+    for (int k = 0; k < i; k++) s -= L[tri(i, k)] * y[k];
+    y[i] = s / L[tri(i, i)];
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; i--) {
+    float s = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; k++) s -= L[tri(k, i)] * x[k];
+    x[i] = s / L[tri(i, i)];
+  }
+}
+
+// LU with partial pivoting for the rare indefinite H (MATLAB's backslash
+// falls back to it when Cholesky fails).
+// Every index is static (the pivot swap is a select), so the matrix stays in
+// registers.
+__device__ __forceinline__ void lu6_solve(const float h[21], const float b[6], float x[6]) {
+  float A[6][6], bb[6];
+#pragma unroll
+  for (int r = 0; r < 6; r++) {
+    bb[r] = b[r];
+#pragma unroll
+    for (int c = 0; c < 6; c++) A[r][c] = (r >= c) ? h[tri(r, c)] : h[tri(c, r)];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    int p = k;
+    float mx = fabsf(A[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < 6; i++) {
+      const bool take = fabsf(A[i][k]) > mx;
+      mx = take ? fabsf(A[i][k]) : mx;
+      p = take ? i : p;
+    }
+#pragma unroll
+    for (int i = k + 1; i < 6; i++) {
+      const bool sw = (i == p);
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        const float t = A[k][j];
+        A[k][j] = sw ? A[i][j] : t;
+        A[i][j] = sw ? t : A[i][j];
+      }
+      const float t = bb[k];
+      bb[k] = sw ? bb[i] : t;
+      bb[i] = sw ? t : bb[i];
+    }
+#pragma unroll
+    for (int i = k + 1; i < 6; i++) {
+      const float f = A[i][k] / A[k][k];
+#pragma unroll
+      for (int j = k; j < 6; j++) A[i][j] -= f * A[k][j];
+      bb[i] -= f * bb[k];
+    }
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; i--) {
+    float s = bb[i];
+#pragma unroll
+    for (int j = i + 1; j < 6; j++) s -= A[i][j] * x[j];
+    x[i] = s / A[i][i];
+  }
+}
+
+// 2x2 solve of the Subpb1 step (funICGN_Subpb1quadtree.m:393-396).
+__device__ __forceinline__ void solve2(const float H[4], float b0, float b1, float* x0,
+                                       float* x1) {
+  const float a = H[0], b = H[1], c = H[2], d = H[3];
+  if (a > 0.0f) {
+    const float l11 = sqrtf(a), l21 = c / l11, t = d - l21 * l21;
+    if (t > 0.0f) {
+      const float l22 = sqrtf(t);
+      const float y0 = b0 / l11, y1 = (b1 - l21 * y0) / l22;
+      *x1 = y1 / l22;
+      *x0 = (y0 - l21 * *x1) / l11;
+      return;
+    }
+  }
+  const float det = a * d - b * c;
+  *x0 = (d * b0 - b * b1) / det;
+  *x1 = (a * b1 - c * b0) / det;
+}
+
+// Inverse-compositional update W(P) o W(dP)^-1 (funICGNQuadtree.m:396-414).
+// Returns false when det(I + dP_F) == 0.
+__device__ __forceinline__ bool ic_update(float P[6], const float d[6]) {
+  const float det = (1.0f + d[0]) * (1.0f + d[3]) - d[1] * d[2];
+  if (det == 0.0f) return false;
+  const float id = 1.0f / det;
+  const float t1 = (-d[0] - d[0] * d[3] + d[1] * d[2]) * id;
+  const float t2 = -d[1] * id;
+  const float t3 = -d[2] * id;
+  const float t4 = (-d[3] - d[0] * d[3] + d[1] * d[2]) * id;
+  const float t5 = (-d[4] - d[3] * d[4] + d[2] * d[5]) * id;
+  const float t6 = (-d[5] - d[0] * d[5] + d[1] * d[4]) * id;
+  const float m11 = 1.0f + P[0], m12 = P[2], m13 = P[4];
+  const float m21 = P[1], m22 = 1.0f + P[3], m23 = P[5];
+  P[0] = (m11 * (1.0f + t1) + m12 * t2) - 1.0f;
+  P[1] = m21 * (1.0f + t1) + m22 * t2;
+  P[2] = m11 * t3 + m12 * (1.0f + t4);
+  P[3] = (m21 * t3 + m22 * (1.0f + t4)) - 1.0f;
+  P[4] = m11 * t5 + m12 * t6 + m13;
+  P[5] = m21 * t5 + m22 * t6 + m23;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Group reductions: NW waves cooperate on one subset.
+// ---------------------------------------------------------------------------
+template <int NW, int K>
+__device__ __forceinline__ void group_sum(float (&v)[K], float* red) {
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+  if constexpr (NW > 1) {
+    const int w = threadIdx.x / ALDIC_WAVE, l = threadIdx.x % ALDIC_WAVE;
+    __syncthreads();
+    if (l == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) red[w * K + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      float s = 0.0f;
+#pragma unroll
+      for (int q = 0; q < NW; q++) s += red[q * K + k];
+      v[k] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-subset output writers
+// ---------------------------------------------------------------------------
+// The Hessian is passed by reference with a flag (never as a pointer that
+// may be null) so the private array keeps living in registers.
+__device__ __forceinline__ void write_icgn(const SolveArgs& a, int node, int st, const float (&P)[6],
+                                           const float (&h21)[21], bool has_h, float zn) {
+  a.conv[node] = st;
+  const bool err = (st == ST_ERR);
+  const float nan = __builtin_nanf("");
+  a.U[2 * node] = err ? nan : P[4];
+  a.U[2 * node + 1] = err ? nan : P[5];
+  if (a.F) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) a.F[4 * node + k] = err ? nan : P[k];
+  }
+  if (a.H21) {
+#pragma unroll
+    for (int k = 0; k < 21; k++) a.H21[(int64_t)k * a.n + node] = (err || !has_h) ? 0.0f : h21[k];
+  }
+  if (a.zncc) a.zncc[node] = zn;
+}
+
+__device__ __forceinline__ void write_icgn_noh(const SolveArgs& a, int node, int st,
+                                               const float (&P)[6], float zn) {
+  const float z[21] = {0};
+  write_icgn(a, node, st, P, z, false, zn);
+}
+
+__device__ __forceinline__ void write_subpb1(const SolveArgs& a, int node, int st,
+                                             const float P[6]) {
+  // A throw leaves ConvItPerEle at its initial 0 (Subpb1Quadtree.m:96-99).
+  a.conv[node] = (st == ST_ERR) ? 0 : st;
+  const float nan = __builtin_nanf("");
+  a.U[2 * node] = (st == ST_ERR) ? nan : P[4];
+  a.U[2 * node + 1] = (st == ST_ERR) ? nan : P[5];
+}
+
+__device__ __forceinline__ void node_window(const SolveArgs& a, int node, int* x0, int* y0,
+                                            int* wx, int* wy) {
+  *x0 = (int)mround(a.coords[node]);
+  *y0 = (int)mround(a.coords[node + a.n]);
+  if (a.ws_per_node) {
+    *wx = a.wsx[node];
+    *wy = a.wsy ? a.wsy[node] : a.wsx[node];
+  } else {
+    *wx = a.wsx[0];
+    *wy = a.wsy ? a.wsy[0] : a.wsx[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fast kernel (unmasked branch)
+// ---------------------------------------------------------------------------
+template <int MODE, int NW, int PPL, int TS>
+__global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
+  constexpr int NT = ALDIC_WAVE * NW;
+  constexpr int SUBS = 256 / NT;
+  __shared__ float tiles[SUBS][TS * TS];
+  __shared__ float red[NW * 32];
+  const int sub = threadIdx.x / NT;
+  const int tid = threadIdx.x % NT;
+  const int slot = blockIdx.x * SUBS + sub;
+  if (slot >= a.cnt) return;  // uniform per subset group (SUBS == 1 when NW > 1)
+  const int node = a.list[slot];
+  float* tile = tiles[sub];
+  const int M = a.M, N = a.N;
+  if (M < TS || N < TS) {  // the tile would not fit in the image
+    if (tid == 0) {
+      const int q = atomicAdd(a.defer_cnt, 1);
+      a.defer_list[q] = node;
+    }
+    return;
+  }
+
+  int x0, y0, wx, wy;
+  node_window(a, node, &x0, &y0, &wx, &wy);
+  const int hx = wx / 2, hy = wy / 2;
+  const int S1 = wx + 1, S2 = wy + 1;
+  const int NS = S1 * S2;
+
+  float P[6];
+  if (MODE == MODE_ICGN) {
+    P[0] = P[1] = P[2] = P[3] = 0.0f;
+    P[4] = a.U0[2 * node];
+    P[5] = a.U0[2 * node + 1];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; k++) P[k] = a.FOld[4 * node + k];
+    P[4] = a.UOld[2 * node];
+    P[5] = a.UOld[2 * node + 1];
+  }
+  // Window inside the image (ImgRef rows 1..M) and inside Df.DfDx (rows
+  // 4..M-3); outside the latter the reference throws.
+  const bool in_img = x0 - hx >= 1 && x0 + hx <= M && y0 - hy >= 1 && y0 + hy <= N;
+  const bool in_df = x0 - hx >= 4 && x0 + hx <= M - 3 && y0 - hy >= 4 && y0 + hy <= N - 3;
+
+  // ---- load the reference subset into registers ----
+  float pf[PPL], pgx[PPL], pgy[PPL], pdx[PPL], pdy[PPL], pg[PPL];
+  uint32_t valid = 0;
+  int zf = 0, nanf = 0;
+#pragma unroll
+  for (int k = 0; k < PPL; k++) {
+    const int p = tid + NT * k;
+    const int aa = p % S1, bb = p / S1;
+    pdx[k] = (float)(aa - hx);
+    pdy[k] = (float)(bb - hy);
+    pf[k] = pgx[k] = pgy[k] = 0.0f;
+    if (p < NS && in_img) {
+      valid |= (1u << k);
+      const int64_t idx = (int64_t)(x0 - hx - 1 + aa) + (int64_t)(y0 - hy - 1 + bb) * M;
+      pf[k] = a.fm[idx];
+      if (in_df) {
+        pgx[k] = a.fx[idx];
+        pgy[k] = a.fy[idx];
+      }
+      zf += (pf[k] == 0.0f);
+      nanf |= (pf[k] != pf[k]);
+    }
+  }
+  {
+    float v[2] = {(float)zf, (float)nanf};
+    group_sum<NW, 2>(v, red);
+    zf = (int)v[0];
+    nanf = (int)v[1];
+  }
+  // markCoordHoleStrainOrNot (LocalICGNQuadtree.m:75-86)
+  if (MODE == MODE_ICGN && a.hole && tid == 0)
+    a.hole[node] = (!in_img || nanf || (float)zf > 0.6f * (float)S1 * (float)S2) ? 1 : 0;
+  if (!in_df) {
+    if (tid == 0) {
+      if (MODE == MODE_ICGN)
+        write_icgn_noh(a, node, ST_ERR, P, __builtin_nanf(""));
+      else
+        write_subpb1(a, node, ST_ERR, P);
+    }
+    return;
+  }
+  if (nanf) {  // logical(NaN) throws
+    if (tid == 0) {
+      if (MODE == MODE_ICGN)
+        write_icgn_noh(a, node, ST_ERR, P, __builtin_nanf(""));
+      else
+        write_subpb1(a, node, ST_ERR, P);
+    }
+    return;
+  }
+  if ((float)zf >= 0.4f * (float)S1 * (float)S2) {
+    // >= 40% masked: status 102, P stays P0, H = 0 (:440-444)
+    if (tid == 0) {
+      if (MODE == MODE_ICGN) {
+        float Pz[6] = {0, 0, 0, 0, P[4], P[5]};
+        write_icgn_noh(a, node, ST_MASKED, Pz, __builtin_nanf(""));
+      } else {
+        write_subpb1(a, node, ST_MASKED, P);
+      }
+    }
+    return;
+  }
+  if (zf > 0) {  // the reference's mask branch: hand on
+    if (tid == 0) {
+      const int q = atomicAdd(a.defer_cnt, 1);
+      a.defer_list[q] = node;
+    }
+    return;
+  }
+
+  // ---- moments of f over |f| > 1e-10 (:119-120) and sum |f| (:434) ----
+  float meanf, bottomf, sabs;
+  {
+    float v[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < PPL; k++) {
+      const bool use = ((valid >> k) & 1u) && fabsf(pf[k]) > 1e-10f;
+      v[0] += use ? 1.0f : 0.0f;
+      v[1] += use ? pf[k] : 0.0f;
+      v[2] += fabsf(pf[k]);
+    }
+    group_sum<NW, 3>(v, red);
+    meanf = v[1] / v[0];
+    sabs = v[2];
+    const float nf = v[0];
+    float w[1] = {0.0f};
+#pragma unroll
+    for (int k = 0; k < PPL; k++) {
+      const bool use = ((valid >> k) & 1u) && fabsf(pf[k]) > 1e-10f;
+      const float d = pf[k] - meanf;
+      w[0] += use ? d * d : 0.0f;
+    }
+    group_sum<NW, 1>(w, red);
+    bottomf = (nf > 1.5f) ? sqrtf(w[0]) : (nf > 0.5f ? 0.0f : __builtin_nanf(""));
+  }
+  // fc = (f - meanf) / bottomf, the reference half of the residual
+#pragma unroll
+  for (int k = 0; k < PPL; k++) pf[k] = ((valid >> k) & 1u) ? (pf[k] - meanf) / bottomf : 0.0f;
+
+  // ---- Hessian ----
+  float h21[21];
+  float L[21];
+  bool spd = true;
+  float H2[4] = {0, 0, 0, 0};
+  if (MODE == MODE_ICGN) {
+#pragma unroll
+    for (int k = 0; k < 21; k++) h21[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PPL; k++) {
+      const float sd[6] = {pdx[k] * pgx[k], pdx[k] * pgy[k], pdy[k] * pgx[k],
+                           pdy[k] * pgy[k], pgx[k], pgy[k]};
+      int t = 0;
+#pragma unroll
+      for (int c = 0; c < 6; c++)
+#pragma unroll
+        for (int r = c; r < 6; r++) h21[t++] += sd[r] * sd[c];
+    }
+    group_sum<NW, 21>(h21, red);
+    spd = chol6(h21, L);
+  } else {
+    float v[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < PPL; k++) {
+      v[0] += pgx[k] * pgx[k];
+      v[1] += pgx[k] * pgy[k];
+      v[2] += pgy[k] * pgy[k];
+    }
+    group_sum<NW, 3>(v, red);
+    const float s = 2.0f / (bottomf * bottomf);
+    H2[0] = v[0] * s + a.mu;
+    H2[1] = v[1] * s;
+    H2[2] = v[1] * s;
+    H2[3] = v[2] * s + a.mu;
+  }
+
+  // ---- Gauss-Newton loop (:135-422) ----
+  const float tol = a.tol;
+  float normNew = 1.0f, normAbs = 1.0f, normInit = 0.0f;
+  int step = 0;
+  int tx0 = -1000000, ty0 = -1000000;  // tile origin (0-based image index)
+  float last_zncc = __builtin_nanf("");
+  bool err = false;
+  // corner offsets of the subset for the tile bounding box
+  const float cdx[2] = {(float)(-hx), (float)(hx)};
+  const float cdy[2] = {(float)(-hy), (float)(hy)};
+  const float lo_u = 3.0f - (float)x0, hi_u = (float)(M - 2 - x0);
+  const float lo_v = 3.0f - (float)y0, hi_v = (float)(N - 2 - y0);
+
+  while (step <= 100 && normNew > tol && normAbs > tol) {
+    step++;
+    const float A11 = 1.0f + P[0], A12 = P[2], A21 = P[1], A22 = 1.0f + P[3];
+    // warp (:149-152) in displacement-relative form.  The warp is affine,
+    // so its extremes over the subset are at the four corners: the bounds
+    // test of :154-157 and the tile's bounding box need only the corners.
+    float umin = 1e30f, umax = -1e30f, vmin = 1e30f, vmax = -1e30f;
+    bool nanw = false;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const float cu = (A11 * cdx[i] + A12 * cdy[j]) + P[4];
+        const float cv = (A21 * cdx[i] + A22 * cdy[j]) + P[5];
+        nanw |= (cu != cu) | (cv != cv);
+        umin = fminf(umin, cu);
+        umax = fmaxf(umax, cu);
+        vmin = fminf(vmin, cv);
+        vmax = fmaxf(vmax, cv);
+      }
+    if (!nanw && (umin < lo_u || umax > hi_u || vmin < lo_v || vmax > hi_v)) {
+      normNew = 1e6f;
+      break;
+    }
+    if (nanw) {  // interpolating at NaN gives NaN g and logical(NaN) throws
+      err = true;
+      break;
+    }
+    const int lox = (int)floorf((float)(x0 - 1) + umin) - 2, hix = (int)floorf((float)(x0 - 1) + umax) + 3;
+    const int loy = (int)floorf((float)(y0 - 1) + vmin) - 2, hiy = (int)floorf((float)(y0 - 1) + vmax) + 3;
+    if (hix - lox + 1 > TS || hiy - loy + 1 > TS) {
+      // the warp outgrew the tile: the generic kernel takes the subset
+      if (tid == 0) {
+        const int q = atomicAdd(a.defer_cnt, 1);
+        a.defer_list[q] = node;
+      }
+      return;
+    }
+    if (lox < tx0 || hix > tx0 + TS - 1 || loy < ty0 || hiy > ty0 + TS - 1) {
+      tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
+      ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
+      if constexpr (NW > 1) __syncthreads();
+      for (int e = tid; e < TS * TS; e += NT) {
+        const int rr = e % TS, cc = e / TS;
+        tile[e] = a.g[(int64_t)(tx0 + rr) + (int64_t)(ty0 + cc) * M];
+      }
+      if constexpr (NW > 1)
+        __syncthreads();
+      else
+        __builtin_amdgcn_wave_barrier();
+    }
+    // gather + interpolate
+    const float ox = (float)(x0 - 1 - tx0), oy = (float)(y0 - 1 - ty0);
+    int zg = 0;
+    float s1[3] = {0.0f, 0.0f, 0.0f};  // count, sum, sum of squares over |g| > 1e-10
+#pragma unroll
+    for (int k = 0; k < PPL; k++) {
+      const float tx = ox + ((A11 * pdx[k] + A12 * pdy[k]) + P[4]);
+      const float ty = oy + ((A21 * pdx[k] + A22 * pdy[k]) + P[5]);
+      const float fxl = floorf(tx), fyl = floorf(ty);
+      const float dxf = tx - fxl, dyf = ty - fyl;
+      int ix = (int)fxl - 1, iy = (int)fyl - 1;
+      ix = min(max(ix, 0), TS - 4);
+      iy = min(max(iy, 0), TS - 4);
+      float wr[4], wc[4];
+      spline_weights(dxf, wr);
+      spline_weights(dyf, wc);
+      const float* base = tile + iy * TS + ix;
+      float acc = 0.0f;
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        const float* row = base + c * TS;
+        const float rsum = wr[0] * row[0] + wr[1] * row[1] + wr[2] * row[2] + wr[3] * row[3];
+        acc += wc[c] * rsum;
+      }
+      pg[k] = acc;
+      if ((valid >> k) & 1u) {
+        zg += (acc == 0.0f);
+        const bool use = fabsf(acc) > 1e-10f;
+        s1[0] += use ? 1.0f : 0.0f;
+        s1[1] += use ? acc : 0.0f;
+        s1[2] += use ? acc * acc : 0.0f;
+      }
+    }
+    {
+      float v[4] = {s1[0], s1[1], s1[2], (float)zg};
+      group_sum<NW, 4>(v, red);
+      s1[0] = v[0];
+      s1[1] = v[1];
+      s1[2] = v[2];
+      zg = (int)v[3];
+    }
+    if (zg > 0) {  // mask branch of the reference (:170): hand on
+      if (tid == 0) {
+        const int q = atomicAdd(a.defer_cnt, 1);
+        a.defer_list[q] = node;
+      }
+      return;
+    }
+    const float meang = s1[1] / s1[0];
+    const float varsum = fmaxf(s1[2] - s1[0] * meang * meang, 0.0f);
+    const float bottomg = (s1[0] > 1.5f) ? sqrtf(varsum) : (s1[0] > 0.5f ? 0.0f : __builtin_nanf(""));
+    const float ig = 1.0f / bottomg;
+    // residual and b (:365-374)
+    float bnorm;
+    float bvec[6];
+    if (MODE == MODE_ICGN) {
+      float v[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int k = 0; k < PPL; k++) {
+        if ((valid >> k) & 1u) {
+          const float r = pf[k] - (pg[k] - meang) * ig;
+          const float rx = r * pgx[k], ry = r * pgy[k];
+          v[0] += pdx[k] * rx;
+          v[1] += pdx[k] * ry;
+          v[2] += pdy[k] * rx;
+          v[3] += pdy[k] * ry;
+          v[4] += rx;
+          v[5] += ry;
+          v[6] += r * r;
+        }
+      }
+      group_sum<NW, 7>(v, red);
+      float nb = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        bvec[k] = bottomf * v[k];
+        nb += bvec[k] * bvec[k];
+      }
+      bnorm = sqrtf(nb);
+      last_zncc = 1.0f - 0.5f * v[6];
+    } else {
+      float v[3] = {0, 0, 0};
+#pragma unroll
+      for (int k = 0; k < PPL; k++) {
+        if ((valid >> k) & 1u) {
+          const float r = pf[k] - (pg[k] - meang) * ig;
+          v[0] += r * pgx[k];
+          v[1] += r * pgy[k];
+          v[2] += r * r;
+        }
+      }
+      group_sum<NW, 3>(v, red);
+      const float s = 2.0f / (bottomf * bottomf);
+      bvec[0] = bottomf * v[0] * s + a.mu * (P[4] - a.UOld[2 * node] - a.vdual[2 * node]);
+      bvec[1] = bottomf * v[1] * s + a.mu * (P[5] - a.UOld[2 * node + 1] - a.vdual[2 * node + 1]);
+      bnorm = sqrtf(bvec[0] * bvec[0] + bvec[1] * bvec[1]);
+      last_zncc = 1.0f - 0.5f * v[2];
+    }
+    // convergence rule (:376-389)
+    normNew = bnorm;
+    normAbs = bnorm;
+    if (step == 1) normInit = bnorm;
+    normNew = (normInit > tol) ? normNew / normInit : 0.0f;
+    if (normNew < tol || normAbs < tol) break;
+    float dP[6];
+    if (MODE == MODE_ICGN) {
+      float nb[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) nb[k] = -bvec[k];
+      if (spd)
+        chol6_solve(L, nb, dP);
+      else
+        lu6_solve(h21, nb, dP);
+    } else {
+      const float dmax = fmaxf(H2[0], H2[3]);
+      const float tH[4] = {H2[0] + 1e-3f * dmax, H2[1], H2[2], H2[3] + 1e-3f * dmax};
+      dP[0] = dP[1] = dP[2] = dP[3] = 0.0f;
+      solve2(tH, -bvec[0], -bvec[1], &dP[4], &dP[5]);
+    }
+    if (!ic_update(P, dP)) break;
+  }
+  int st;
+  if (err) {
+    st = ST_ERR;
+  } else {
+    st = step;
+    if (!(normNew < tol || normAbs < tol)) st = 101;
+    if (normNew != normNew) st = 101;
+    if (sabs < 1e-6f) st = 103;
+  }
+  if (tid == 0) {
+    if (MODE == MODE_ICGN)
+      write_icgn(a, node, st, P, h21, true, last_zncc);
+    else
+      write_subpb1(a, node, st, P);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Generic kernel: any size, full mask semantics.  One workgroup per subset.
+// ---------------------------------------------------------------------------
+constexpr int GEN_THREADS = 256;
+constexpr int GEN_MAXPIX = 8192;  // largest (grown) window, e.g. 90 x 90
+
+template <int K>
+__device__ __forceinline__ void block_sum(float (&v)[K], float* red) {
+  group_sum<GEN_THREADS / ALDIC_WAVE, K>(v, red);
+}
+
+// bwselect(BW, c, r, conn) on the S1 x S2 window held in LDS: out = the
+// conn-connected component of BW containing (r, c) (1-based), by monotone
+// label propagation until a full sweep changes nothing.
+__device__ void lds_bwselect(const uint8_t* bw, uint8_t* out, int S1, int S2, int r, int c,
+                             int conn) {
+  const int NS = S1 * S2;
+  for (int p = threadIdx.x; p < NS; p += GEN_THREADS) out[p] = 0;
+  __syncthreads();
+  if (threadIdx.x == 0 && r >= 1 && r <= S1 && c >= 1 && c <= S2) {
+    const int s = (r - 1) + (c - 1) * S1;
+    out[s] = bw[s] ? 1 : 0;
+  }
+  __syncthreads();
+  bool again = true;
+  while (again) {
+    int changed = 0;
+    for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+      if (!bw[p] || out[p]) continue;
+      const int pr = p % S1, pc = p / S1;
+      bool hit = false;
+      for (int dc = -1; dc <= 1 && !hit; dc++)
+        for (int dr = -1; dr <= 1; dr++) {
+          if ((dr == 0 && dc == 0) || (conn == 4 && dr != 0 && dc != 0)) continue;
+          const int qr = pr + dr, qc = pc + dc;
+          if (qr < 0 || qr >= S1 || qc < 0 || qc >= S2) continue;
+          if (out[qr + qc * S1]) {
+            hit = true;
+            break;
+          }
+        }
+      if (hit) {
+        out[p] = 1;
+        changed = 1;
+      }
+    }
+    again = __syncthreads_or(changed) != 0;
+  }
+}
+
+__device__ __forceinline__ float interp_global(const float* g, int M, float u22, float v22) {
+  // u22, v22 are 1-based (dim-1, dim-2) coordinates inside [3, M-2] x [3, N-2]
+  const float fu = floorf(u22), fv = floorf(v22);
+  float wr[4], wc[4];
+  spline_weights(u22 - fu, wr);
+  spline_weights(v22 - fv, wc);
+  const int ix = (int)fu - 2, iy = (int)fv - 2;  // 0-based first tap
+  float acc = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const float* row = g + (int64_t)(iy + c) * M + ix;
+    acc += wc[c] * (wr[0] * row[0] + wr[1] * row[1] + wr[2] * row[2] + wr[3] * row[3]);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ int grow_ws(float A, float B, int w) {
+  // 2*max(ceil(sqrt(A/B)*w/2)); -1 for a non-finite size (colon throws)
+  const float v = 2.0f * ceilf(sqrtf(A / B) * (float)w * 0.5f);
+  if (!(v >= 0.0f) || v > 1e5f) return -1;
+  return (int)v;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const int* list,
+                                                              const int* count, int fixed_count) {
+  __shared__ float sg[GEN_MAXPIX];
+  __shared__ uint8_t sbwf[GEN_MAXPIX];  // bwselect of the f mask (setup)
+  __shared__ uint8_t sbwg[GEN_MAXPIX];  // current g mask
+  __shared__ uint8_t stmp[GEN_MAXPIX];  // logical image fed to bwselect
+  __shared__ uint8_t sfsel[GEN_MAXPIX]; // Subpb1: f selection after its mask block
+  __shared__ float red[4 * 32];
+  const int total = count ? *count : fixed_count;
+  const int M = a.M, N = a.N;
+  const float nan = __builtin_nanf("");
+  for (int it = blockIdx.x; it < total; it += gridDim.x) {
+    const int node = list[it];
+    int x0, y0, wx, wy;
+    node_window(a, node, &x0, &y0, &wx, &wy);
+    const int wx0 = wx;
+    float P[6];
+    float UOld0 = 0.0f, UOld1 = 0.0f, vd0 = 0.0f, vd1 = 0.0f;
+    if (MODE == MODE_ICGN) {
+      P[0] = P[1] = P[2] = P[3] = 0.0f;
+      P[4] = a.U0[2 * node];
+      P[5] = a.U0[2 * node + 1];
+    } else {
+      for (int k = 0; k < 4; k++) P[k] = a.FOld[4 * node + k];
+      UOld0 = a.UOld[2 * node];
+      UOld1 = a.UOld[2 * node + 1];
+      vd0 = a.vdual[2 * node];
+      vd1 = a.vdual[2 * node + 1];
+      P[4] = UOld0;
+      P[5] = UOld1;
+    }
+    float h21[21];
+    for (int k = 0; k < 21; k++) h21[k] = 0.0f;
+    int st = ST_ERR;
+    float last_zncc = nan;
+    // window helpers
+    auto win_ok = [&](int ax, int ay) {
+      return x0 - ax / 2 >= 4 && x0 + ax / 2 <= M - 3 && y0 - ay / 2 >= 4 && y0 + ay / 2 <= N - 3;
+    };
+    if (MODE == MODE_ICGN && a.hole) {
+      // markCoordHoleStrainOrNot on the nominal window (LocalICGNQuadtree.m:75-86)
+      const bool in_img = x0 - wx / 2 >= 1 && x0 + wx / 2 <= M && y0 - wy / 2 >= 1 && y0 + wy / 2 <= N;
+      float v[2] = {0.0f, 0.0f};
+      if (in_img) {
+        const int S1 = wx + 1, NS = (wx + 1) * (wy + 1);
+        for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+          const float f = a.fm[(int64_t)(x0 - wx / 2 - 1 + p % S1) + (int64_t)(y0 - wy / 2 - 1 + p / S1) * M];
+          v[0] += (f == 0.0f) ? 1.0f : 0.0f;
+          v[1] += (f != f) ? 1.0f : 0.0f;
+        }
+      }
+      block_sum<2>(v, red);
+      if (threadIdx.x == 0)
+        a.hole[node] = (!in_img || v[1] > 0.0f || v[0] > 0.6f * (float)(wx + 1) * (float)(wy + 1)) ? 1 : 0;
+    }
+    do {
+      if (!win_ok(wx, wy)) break;
+      int S1 = wx + 1, S2 = wy + 1;
+      int NS = S1 * S2;
+      if (NS > GEN_MAXPIX) break;  // outside what this build supports: reported as a throw
+      // zeros and NaN in tempf = ImgRef .* mask
+      float zc = 0.0f, nanc = 0.0f;
+      for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+        const int i = x0 - wx / 2 - 1 + p % S1, j = y0 - wy / 2 - 1 + p / S1;
+        const float v = a.fm[(int64_t)i + (int64_t)j * M];
+        zc += (v == 0.0f);
+        nanc += (v != v);
+      }
+      {
+        float v[2] = {zc, nanc};
+        block_sum<2>(v, red);
+        zc = v[0];
+        nanc = v[1];
+      }
+      if (nanc > 0.0f) break;
+      const float c0 = zc;
+      const float area0 = (float)S1 * (float)S2;
+      if (c0 >= 0.4f * area0) {
+        st = ST_MASKED;
+        if (MODE == MODE_ICGN) P[0] = P[1] = P[2] = P[3] = 0.0f;
+        break;
+      }
+      if (c0 > 0.0f) {
+        if (MODE == MODE_ICGN) {
+          const int wn = grow_ws(area0, area0 - c0, wx);
+          if (wn < 0) break;
+          wx = wy = wn;
+        } else {
+          const int wxn = grow_ws(area0, area0 - c0, wx);
+          if (wxn < 0) break;
+          const float area1 = (float)(wxn + 1) * (float)(wy + 1);
+          const int wyn = grow_ws(area1, area1 - c0, wy);
+          if (wyn < 0) break;
+          wx = wxn;
+          wy = wyn;
+        }
+        if (!win_ok(wx, wy)) break;
+        S1 = wx + 1;
+        S2 = wy + 1;
+        NS = S1 * S2;
+        if (NS > GEN_MAXPIX) break;
+      }
+      const int hx = wx / 2, hy = wy / 2;
+      const int ix0 = x0 - hx - 1, iy0 = y0 - hy - 1;  // 0-based window origin
+      // seeds: ICGN (f, f); Subpb1 row from winsize_y, column from winsize_x
+      const int seed_r = (MODE == MODE_ICGN) ? (wx + 1) / 2 : (wy + 1) / 2;
+      const int seed_c = (wx + 1) / 2;
+      // bwselect(logical(tempfImgMask), seed, 4)
+      for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+        const int64_t idx = (int64_t)(ix0 + p % S1) + (int64_t)(iy0 + p / S1) * M;
+        stmp[p] = a.mask[idx] != 0.0f;
+      }
+      __syncthreads();
+      lds_bwselect(stmp, sbwf, S1, S2, seed_r, seed_c, 4);
+      __syncthreads();
+      // f statistics and H over the selected pixels
+      auto fval = [&](int p, const uint8_t* sel) {
+        const int64_t idx = (int64_t)(ix0 + p % S1) + (int64_t)(iy0 + p / S1) * M;
+        return sel[p] ? a.fm[idx] : 0.0f;
+      };
+      float meanf, bottomf, sabs;
+      auto f_moments = [&](const uint8_t* sel) {
+        float v[3] = {0, 0, 0};
+        for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+          const float f = fval(p, sel);
+          const bool use = fabsf(f) > 1e-10f;
+          v[0] += use ? 1.0f : 0.0f;
+          v[1] += use ? f : 0.0f;
+          v[2] += fabsf(f);
+        }
+        block_sum<3>(v, red);
+        meanf = v[1] / v[0];
+        sabs = v[2];
+        float w[1] = {0.0f};
+        for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+          const float f = fval(p, sel);
+          const float d = f - meanf;
+          w[0] += (fabsf(f) > 1e-10f) ? d * d : 0.0f;
+        }
+        block_sum<1>(w, red);
+        bottomf = (v[0] > 1.5f) ? sqrtf(w[0]) : (v[0] > 0.5f ? 0.0f : nan);
+      };
+      float H2[4];
+      auto hessian = [&](const uint8_t* sel) {
+        if (MODE == MODE_ICGN) {
+          for (int k = 0; k < 21; k++) h21[k] = 0.0f;
+          for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+            const int64_t idx = (int64_t)(ix0 + p % S1) + (int64_t)(iy0 + p / S1) * M;
+            const float m = sel[p] ? 1.0f : 0.0f;
+            const float gx = a.fx[idx] * m, gy = a.fy[idx] * m;
+            const float dxm = (float)(p % S1 - hx), dym = (float)(p / S1 - hy);
+            const float sd[6] = {dxm * gx, dxm * gy, dym * gx, dym * gy, gx, gy};
+            int t = 0;
+            for (int c = 0; c < 6; c++)
+              for (int r = c; r < 6; r++) h21[t++] += sd[r] * sd[c];
+          }
+          block_sum<21>(h21, red);
+        } else {
+          float v[3] = {0, 0, 0};
+          for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+            const int64_t idx = (int64_t)(ix0 + p % S1) + (int64_t)(iy0 + p / S1) * M;
+            const float m = sel[p] ? 1.0f : 0.0f;
+            const float gx = a.fx[idx] * m, gy = a.fy[idx] * m;
+            v[0] += gx * gx;
+            v[1] += gx * gy;
+            v[2] += gy * gy;
+          }
+          block_sum<3>(v, red);
+          const float s = 2.0f / (bottomf * bottomf);
+          H2[0] = v[0] * s + a.mu;
+          H2[1] = H2[2] = v[1] * s;
+          H2[3] = v[2] * s + a.mu;
+        }
+      };
+      f_moments(sbwf);
+      hessian(sbwf);
+      float L[21];
+      bool spd = (MODE == MODE_ICGN) ? chol6(h21, L) : true;
+      // the f selection used in the residual: sbwf (ICGN always; Subpb1 until
+      // its mask block replaces it with the g mask)
+      const uint8_t* fsel = sbwf;
+      const float tol = a.tol;
+      float normNew = 1.0f, normAbs = 1.0f, normInit = 0.0f;
+      int step = 0;
+      bool have_bwg = false, err = false, defer_oob = false;
+      float sum_bwg = 0.0f;
+      const int ws0 = wx0;
+      int cur_w = wx;
+      while (step <= 100 && normNew > tol && normAbs > tol) {
+        step++;
+        if (MODE == MODE_ICGN && step > 1 && c0 > 0.0f) {
+          // :139-145 -- a change of size throws at the next elementwise op
+          if (!have_bwg || !(sum_bwg > 0.0f)) {
+            err = true;
+            break;
+          }
+          const int wn = grow_ws((float)(ws0 + 1) * (float)(ws0 + 1), sum_bwg, cur_w);
+          if (wn < 0) {
+            err = true;
+            break;
+          }
+          if (wn != cur_w) {
+            const int h2 = wn / 2;
+            if (x0 - h2 < 1 || x0 + h2 > M || y0 - h2 < 1 || y0 + h2 > N) {
+              err = true;
+              break;
+            }
+            float o = 0.0f;
+            const int n2 = (wn + 1) * (wn + 1);
+            for (int p = threadIdx.x; p < n2; p += GEN_THREADS) {
+              const float dxm = (float)(p % (wn + 1) - h2), dym = (float)(p / (wn + 1) - h2);
+              const float u = ((1.0f + P[0]) * dxm + P[2] * dym) + P[4];
+              const float v = (P[1] * dxm + (1.0f + P[3]) * dym) + P[5];
+              o += ((u < 3.0f - (float)x0) | (u > (float)(M - 2 - x0)) | (v < 3.0f - (float)y0) |
+                    (v > (float)(N - 2 - y0)))
+                       ? 1.0f
+                       : 0.0f;
+            }
+            float vv[1] = {o};
+            block_sum<1>(vv, red);
+            if (vv[0] > 0.0f) {
+              defer_oob = true;  // break with status 101
+              normNew = 1e6f;
+              break;
+            }
+            err = true;
+            break;
+          }
+        }
+        // warp and bounds
+        float oob = 0.0f, nanw = 0.0f, zg = 0.0f;
+        for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+          const float dxm = (float)(p % S1 - hx), dym = (float)(p / S1 - hy);
+          const float u = ((1.0f + P[0]) * dxm + P[2] * dym) + P[4];
+          const float v = (P[1] * dxm + (1.0f + P[3]) * dym) + P[5];
+          const bool o = (u < 3.0f - (float)x0) | (u > (float)(M - 2 - x0)) |
+                         (v < 3.0f - (float)y0) | (v > (float)(N - 2 - y0));
+          const bool nn = (u != u) | (v != v);
+          oob += o ? 1.0f : 0.0f;
+          nanw += nn ? 1.0f : 0.0f;
+          float gv = 0.0f;
+          if (!o && !nn) gv = interp_global(a.g, M, (float)x0 + u, (float)y0 + v);
+          sg[p] = gv;
+          zg += (gv == 0.0f) ? 1.0f : 0.0f;
+        }
+        {
+          float v[3] = {oob, nanw, zg};
+          block_sum<3>(v, red);
+          oob = v[0];
+          nanw = v[1];
+          zg = v[2];
+        }
+        if (oob > 0.0f) {
+          normNew = 1e6f;
+          break;
+        }
+        if (nanw > 0.0f) {
+          err = true;
+          break;
+        }
+        if (c0 > 0.0f || zg > 0.0f) {
+          // :170-214 (ICGN) / :158-192 (Subpb1)
+          for (int p = threadIdx.x; p < NS; p += GEN_THREADS) stmp[p] = sg[p] != 0.0f;
+          __syncthreads();
+          lds_bwselect(stmp, sbwg, S1, S2, seed_r, seed_c, MODE == MODE_ICGN ? 8 : 4);
+          __syncthreads();
+          float nz = 0.0f;
+          for (int p = threadIdx.x; p < NS; p += GEN_THREADS) nz += sbwg[p] ? 0.0f : 1.0f;
+          float vv[1] = {nz};
+          block_sum<1>(vv, red);
+          if (vv[0] == 0.0f) {
+            for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+              const int64_t idx = (int64_t)(ix0 + p % S1) + (int64_t)(iy0 + p / S1) * M;
+              stmp[p] = a.mask[idx] != 0.0f;
+            }
+            __syncthreads();
+            lds_bwselect(stmp, sbwg, S1, S2, seed_r, seed_c, 8);
+            __syncthreads();
+          }
+          float sb = 0.0f;
+          for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+            sg[p] = sbwg[p] ? sg[p] : 0.0f;
+            sb += sbwg[p] ? 1.0f : 0.0f;
+          }
+          float sv[1] = {sb};
+          block_sum<1>(sv, red);
+          sum_bwg = sv[0];
+          have_bwg = true;
+          if (MODE == MODE_SUBPB1) {
+            // f, DfDx, DfDy and H are rebuilt on the g mask (:173-192);
+            // tempf keeps the image mask as well
+            for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+              const int64_t idx = (int64_t)(ix0 + p % S1) + (int64_t)(iy0 + p / S1) * M;
+              sfsel[p] = sbwg[p] && (a.mask[idx] != 0.0f);
+            }
+            __syncthreads();
+            f_moments(sfsel);
+            hessian(sbwg);
+            fsel = sfsel;
+          } else {
+            f_moments(sbwf);
+          }
+          __syncthreads();
+        }
+        // g moments over |g| > 1e-10 (two passes as in the reference)
+        float gm[2] = {0, 0};
+        for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+          const bool use = fabsf(sg[p]) > 1e-10f;
+          gm[0] += use ? 1.0f : 0.0f;
+          gm[1] += use ? sg[p] : 0.0f;
+        }
+        block_sum<2>(gm, red);
+        const float meang = gm[1] / gm[0];
+        float gv[1] = {0.0f};
+        for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+          const float d = sg[p] - meang;
+          gv[0] += (fabsf(sg[p]) > 1e-10f) ? d * d : 0.0f;
+        }
+        block_sum<1>(gv, red);
+        const float bottomg = (gm[0] > 1.5f) ? sqrtf(gv[0]) : (gm[0] > 0.5f ? 0.0f : nan);
+        // b
+        float bvec[6];
+        float bnorm;
+        const uint8_t* dsel = (MODE == MODE_SUBPB1 && have_bwg) ? sbwg : sbwf;
+        if (MODE == MODE_ICGN) {
+          float v[7] = {0, 0, 0, 0, 0, 0, 0};
+          for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+            const int64_t idx = (int64_t)(ix0 + p % S1) + (int64_t)(iy0 + p / S1) * M;
+            const float m = dsel[p] ? 1.0f : 0.0f;
+            const float f = fval(p, fsel);
+            const float gx = a.fx[idx] * m, gy = a.fy[idx] * m;
+            const float r = (f - meanf) / bottomf - (sg[p] - meang) / bottomg;
+            const float dxm = (float)(p % S1 - hx), dym = (float)(p / S1 - hy);
+            v[0] += dxm * gx * r;
+            v[1] += dxm * gy * r;
+            v[2] += dym * gx * r;
+            v[3] += dym * gy * r;
+            v[4] += gx * r;
+            v[5] += gy * r;
+            v[6] += r * r;
+          }
+          block_sum<7>(v, red);
+          float nb = 0.0f;
+          for (int k = 0; k < 6; k++) {
+            bvec[k] = bottomf * v[k];
+            nb += bvec[k] * bvec[k];
+          }
+          bnorm = sqrtf(nb);
+          last_zncc = 1.0f - 0.5f * v[6];
+        } else {
+          float v[3] = {0, 0, 0};
+          for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+            const int64_t idx = (int64_t)(ix0 + p % S1) + (int64_t)(iy0 + p / S1) * M;
+            const float m = dsel[p] ? 1.0f : 0.0f;
+            const float f = fval(p, fsel);
+            const float gx = a.fx[idx] * m, gy = a.fy[idx] * m;
+            const float r = (f - meanf) / bottomf - (sg[p] - meang) / bottomg;
+            v[0] += gx * r;
+            v[1] += gy * r;
+            v[2] += r * r;
+          }
+          block_sum<3>(v, red);
+          const float s = 2.0f / (bottomf * bottomf);
+          bvec[0] = bottomf * v[0] * s + a.mu * (P[4] - UOld0 - vd0);
+          bvec[1] = bottomf * v[1] * s + a.mu * (P[5] - UOld1 - vd1);
+          bnorm = sqrtf(bvec[0] * bvec[0] + bvec[1] * bvec[1]);
+          last_zncc = 1.0f - 0.5f * v[2];
+        }
+        normNew = bnorm;
+        normAbs = bnorm;
+        if (step == 1) normInit = bnorm;
+        normNew = (normInit > tol) ? normNew / normInit : 0.0f;
+        if (normNew < tol || normAbs < tol) break;
+        float dP[6];
+        if (MODE == MODE_ICGN) {
+          float nb[6];
+          for (int k = 0; k < 6; k++) nb[k] = -bvec[k];
+          if (spd)
+            chol6_solve(L, nb, dP);
+          else
+            lu6_solve(h21, nb, dP);
+        } else {
+          const float dmax = (H2[0] != H2[0]) ? H2[3] : ((H2[3] != H2[3]) ? H2[0] : fmaxf(H2[0], H2[3]));
+          const float tH[4] = {H2[0] + 1e-3f * dmax, H2[1], H2[2], H2[3] + 1e-3f * dmax};
+          dP[0] = dP[1] = dP[2] = dP[3] = 0.0f;
+          solve2(tH, -bvec[0], -bvec[1], &dP[4], &dP[5]);
+        }
+        if (!ic_update(P, dP)) break;
+        __syncthreads();
+      }
+      (void)defer_oob;
+      if (err) {
+        st = ST_ERR;
+        break;
+      }
+      st = step;
+      if (!(normNew < tol || normAbs < tol)) st = 101;
+      if (normNew != normNew) st = 101;
+      // sum |tempf| with the current f selection
+      float sa[1] = {0.0f};
+      for (int p = threadIdx.x; p < NS; p += GEN_THREADS) sa[0] += fabsf(fval(p, fsel));
+      block_sum<1>(sa, red);
+      if (sa[0] < 1e-6f) st = 103;
+      (void)sabs;
+    } while (0);
+    if (threadIdx.x == 0) {
+      if (MODE == MODE_ICGN)
+        write_icgn(a, node, st, P, h21, st != ST_MASKED, st == ST_MASKED ? nan : last_zncc);
+      else
+        write_subpb1(a, node, st, P);
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (declared in aldic_kernels.h)
+// ---------------------------------------------------------------------------
+namespace aldic {
+
+int size_class(int S1, int S2) {
+  const int s = S1 > S2 ? S1 : S2;
+  if (s <= 11) return 0;
+  if (s <= 17) return 1;
+  if (s <= 33) return 2;
+  if (s <= 65) return 3;
+  return 4;  // generic kernel only
+}
+
+template <int MODE>
+static hipError_t launch_fast_mode(int cls, const SolveArgs& a, hipStream_t s) {
+  if (a.cnt <= 0) return hipSuccess;
+  switch (cls) {
+    case 0:
+      fast_kernel<MODE, 1, 2, 24><<<(a.cnt + 3) / 4, 256, 0, s>>>(a);
+      break;
+    case 1:
+      fast_kernel<MODE, 1, 5, 32><<<(a.cnt + 3) / 4, 256, 0, s>>>(a);
+      break;
+    case 2:
+      fast_kernel<MODE, 1, 17, 48><<<(a.cnt + 3) / 4, 256, 0, s>>>(a);
+      break;
+    case 3:
+      fast_kernel<MODE, 4, 17, 84><<<a.cnt, 256, 0, s>>>(a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fast(int mode, int cls, const SolveArgs& a, hipStream_t s) {
+  return mode == 0 ? launch_fast_mode<MODE_ICGN>(cls, a, s) : launch_fast_mode<MODE_SUBPB1>(cls, a, s);
+}
+
+hipError_t launch_generic(int mode, const SolveArgs& a, const int* list, const int* count,
+                          int fixed_count, int grid, hipStream_t s) {
+  if (mode == 0)
+    generic_kernel<MODE_ICGN><<<grid, GEN_THREADS, 0, s>>>(a, list, count, fixed_count);
+  else
+    generic_kernel<MODE_SUBPB1><<<grid, GEN_THREADS, 0, s>>>(a, list, count, fixed_count);
+  return hipGetLastError();
+}
+
+int generic_max_pixels() { return GEN_MAXPIX; }
+
+}  // namespace aldic

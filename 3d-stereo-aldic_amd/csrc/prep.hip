@@ -1,0 +1,125 @@
+// prep.hip -- image planes consumed by the solvers.
+//   * FP64 MATLAB arrays -> FP32 planes f.*mask, g, mask
+//   * Df (funImgGradient.m:49-78): 7-tap 6th-order central differences
+//     [-1/60 3/20 -3/4 0 3/4 -3/20 1/60] along dim 1 (DfDx) and dim 2 (DfDy)
+//     times the mask, evaluated in FP64 and stored either on the full grid
+//     (FP32, zero on the 3-pixel border, for the solvers) or as the
+//     reference's cropped (M-6) x (N-6) FP64 arrays.
+// Threads run along dim 1 (image x), the contiguous axis, so every load of
+// the stencil is coalesced.
+#include "aldic_common.h"
+#include "aldic_kernels.h"
+
+namespace {
+
+__global__ __launch_bounds__(256) void planes_kernel(const double* __restrict__ ref,
+                                                     const double* __restrict__ def,
+                                                     const double* __restrict__ mask, int64_t MN,
+                                                     float* __restrict__ fm, float* __restrict__ g,
+                                                     float* __restrict__ maskf) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < MN;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const double m = mask ? mask[k] : 1.0;
+    if (fm) fm[k] = (float)(ref[k] * m);
+    if (g) g[k] = (float)def[k];
+    if (maskf) maskf[k] = (float)m;
+  }
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void gradient_kernel(const TI* __restrict__ img,
+                                                       const TI* __restrict__ mask, int64_t M,
+                                                       int64_t N, TO* __restrict__ gx,
+                                                       TO* __restrict__ gy, int full_grid) {
+  // full_grid: outputs are M x N with zero border; else (M-6) x (N-6)
+  const int64_t total = full_grid ? M * N : (M - 6) * (N - 6);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    int64_t i, j;
+    if (full_grid) {
+      i = k % M;
+      j = k / M;
+      if (i < 3 || i >= M - 3 || j < 3 || j >= N - 3) {
+        gx[k] = TO(0);
+        gy[k] = TO(0);
+        continue;
+      }
+    } else {
+      i = k % (M - 6) + 3;
+      j = k / (M - 6) + 3;
+    }
+    const TI* c = img + i + j * M;
+    const double dxv = (-1.0 / 60) * (double)c[-3] + (3.0 / 20) * (double)c[-2] +
+                       (-3.0 / 4) * (double)c[-1] + (3.0 / 4) * (double)c[1] +
+                       (-3.0 / 20) * (double)c[2] + (1.0 / 60) * (double)c[3];
+    const double dyv = (-1.0 / 60) * (double)c[-3 * M] + (3.0 / 20) * (double)c[-2 * M] +
+                       (-3.0 / 4) * (double)c[-M] + (3.0 / 4) * (double)c[M] +
+                       (-3.0 / 20) * (double)c[2 * M] + (1.0 / 60) * (double)c[3 * M];
+    const double m = mask ? (double)mask[i + j * M] : 1.0;
+    gx[k] = (TO)(dxv * m);
+    gy[k] = (TO)(dyv * m);
+  }
+}
+
+__global__ __launch_bounds__(256) void place_df_kernel(const double* __restrict__ DfDx,
+                                                       const double* __restrict__ DfDy, int64_t M,
+                                                       int64_t N, float* __restrict__ fx,
+                                                       float* __restrict__ fy) {
+  const int64_t MN = M * N;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < MN;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = k % M, j = k / M;
+    if (i < 3 || i >= M - 3 || j < 3 || j >= N - 3) {
+      fx[k] = 0.0f;
+      fy[k] = 0.0f;
+    } else {
+      const int64_t c = (i - 3) + (j - 3) * (M - 6);
+      fx[k] = (float)DfDx[c];
+      fy[k] = (float)DfDy[c];
+    }
+  }
+}
+
+int grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 16384) g = 16384;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace
+
+namespace aldic {
+
+hipError_t launch_prep_planes(const double* ref, const double* def, const double* mask, int64_t M,
+                              int64_t N, float* fm, float* g, float* maskf, hipStream_t s) {
+  planes_kernel<<<grid_for(M * N), 256, 0, s>>>(ref, def, mask, M * N, fm, g, maskf);
+  return hipGetLastError();
+}
+
+hipError_t launch_gradient(const double* ref, const double* mask, int64_t M, int64_t N, float* fx,
+                           float* fy, hipStream_t s) {
+  gradient_kernel<double, float><<<grid_for(M * N), 256, 0, s>>>(ref, mask, M, N, fx, fy, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_gradient_f32(const float* ref, const float* mask, int64_t M, int64_t N, float* fx,
+                               float* fy, hipStream_t s) {
+  gradient_kernel<float, float><<<grid_for(M * N), 256, 0, s>>>(ref, mask, M, N, fx, fy, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_gradient_out_f64(const double* ref, const double* mask, int64_t M, int64_t N,
+                                   double* DfDx, double* DfDy, hipStream_t s) {
+  gradient_kernel<double, double><<<grid_for((M - 6) * (N - 6)), 256, 0, s>>>(ref, mask, M, N, DfDx,
+                                                                              DfDy, 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_place_df(const double* DfDx, const double* DfDy, int64_t M, int64_t N, float* fx,
+                           float* fy, hipStream_t s) {
+  place_df_kernel<<<grid_for(M * N), 256, 0, s>>>(DfDx, DfDy, M, N, fx, fy);
+  return hipGetLastError();
+}
+
+}  // namespace aldic

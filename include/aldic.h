@@ -1,0 +1,154 @@
+/*
+ * aldic.h -- C ABI of the MI355X (gfx950) implementation of ALDIC's local
+ * matching path.  Plain pointers and sizes only; every entry point returns
+ * ALDIC_OK or an error code, with a message from aldic_last_error().
+ *
+ * Data layout is the reference's MATLAB layout: images are M x N
+ * column-major double arrays whose first dimension is image x (the
+ * reference loads images transposed, gui/loadStereoImages.m:79); node
+ * coordinates are 1-based; U is interleaved [u1 v1 u2 v2 ...];
+ * F is [F11 F21 F12 F22 ...] per node (LocalICGNQuadtree.m:159-163).
+ *
+ * Entry points with host pointers copy in and out (the MATLAB/MEX-facing
+ * drop-in).  Entry points with the _dev suffix take device pointers that
+ * are already resident in HBM and run asynchronously on the context's
+ * stream.
+ *
+ * Per-subset failures are not errors: they are reported through convIt
+ * exactly as the reference reports them (-1 where MATLAB would throw,
+ * 101 not converged, 102 >= 40% masked, 103 empty subset).
+ */
+#ifndef ALDIC_H
+#define ALDIC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ALDIC_OK 0
+#define ALDIC_EINVAL 1  /* bad argument */
+#define ALDIC_EDEVICE 2 /* HIP runtime error */
+#define ALDIC_ENOMEM 3  /* device allocation failed */
+
+typedef struct aldic_ctx aldic_ctx;
+typedef struct aldic_images aldic_images;
+typedef struct aldic_plan aldic_plan;
+
+/* Message of the last error raised on this thread. */
+const char* aldic_last_error(void);
+
+/* A context owns a HIP stream (default priority) and reusable device
+ * buffers for the host-pointer entry points.  Contexts are independent;
+ * one context must not be used from two threads at once.  Replaces the
+ * MEX build/load step (func/ensureMexCompiled.m:1-88). */
+int aldic_ctx_create(int device, aldic_ctx** out);
+int aldic_ctx_destroy(aldic_ctx* ctx);
+/* hipStream_t of the context, for callers that order their own work. */
+void* aldic_ctx_stream(aldic_ctx* ctx);
+int aldic_ctx_synchronize(aldic_ctx* ctx);
+
+/* Z = ba_interp2_spline(F, X, Y): F is M x N x P, X (dim-2 coordinate) and
+ * Y (dim-1 coordinate) hold ND points, Z is ND x P.  Same class for all
+ * arrays.  Replaces mexFunction / interpolate<REAL>,
+ * ba_interp2_spline.cpp:359-435. */
+int aldic_interp_spline_f64(aldic_ctx* ctx, const double* F, int64_t M, int64_t N, int64_t P,
+                            const double* X, const double* Y, int64_t ND, double* Z);
+int aldic_interp_spline_f32(aldic_ctx* ctx, const float* F, int64_t M, int64_t N, int64_t P,
+                            const float* X, const float* Y, int64_t ND, float* Z);
+
+/* Df = funImgGradient(ImgRef, ImgDef, mask): DfDx, DfDy are (M-6) x (N-6)
+ * (Df.DfCropWidth = 3).  mask may be NULL (all ones).  Replaces
+ * func/funImgGradient.m:1-87 (finite-difference branch). */
+int aldic_img_gradient(aldic_ctx* ctx, const double* ImgRef, const double* mask, int64_t M,
+                       int64_t N, double* DfDx, double* DfDy);
+
+/* Upload the normalised reference and deformed images, the reference mask
+ * (Df.ImgRefMask, NULL = all ones) and Df.DfDx/Df.DfDy ((M-6) x (N-6), NULL
+ * = computed on the device by the funImgGradient rule) as resident FP32
+ * planes.  This is the state that LocalICGNQuadtree, Subpb1Quadtree and
+ * IntegerSearchQuadtree receive as (Df, ImgRef, ImgDef). */
+int aldic_images_create(aldic_ctx* ctx, const double* ImgRef, const double* ImgDef,
+                        const double* mask, const double* DfDx, const double* DfDy, int64_t M,
+                        int64_t N, aldic_images** out);
+/* Same, from FP32 planes already on the device (copied into the handle):
+ * fm = ImgRef .* mask, g = ImgDef, mask (NULL = ones), fx/fy NULL = computed
+ * from fm on the device. */
+int aldic_images_create_dev(aldic_ctx* ctx, const float* fm, const float* g, const float* mask,
+                            const float* fx, const float* fy, int64_t M, int64_t N,
+                            aldic_images** out);
+int aldic_images_destroy(aldic_images* im);
+
+/* Per-node part of LocalICGNQuadtree (func_quadtree/LocalICGNQuadtree.m:
+ * 67-163, one funICGNQuadtree per node, func_quadtree/funICGNQuadtree.m:
+ * 1-454), first-order, 'GaussNewton'.  coords is n x 2 column-major
+ * (coordinatesFEM), U0 is 2n; winsize holds one value (winsize_per_node=0,
+ * DICpara.winsize) or one per node.  Outputs: U (2n), F (4n), H21 (n x 21
+ * column-major, HtempPar; may be NULL), convIt (n, ConvItPerEle), markHole
+ * (n; may be NULL), zncc (n, ZNCC of the last evaluated warp, no
+ * counterpart in the reference; may be NULL).  The bad-point statistics and
+ * the NaN fill of LocalICGNQuadtree.m:167-325 are the caller's (host). */
+int aldic_local_icgn(aldic_ctx* ctx, const aldic_images* im, const double* coords,
+                     const double* U0, int64_t n, const int32_t* winsize,
+                     int32_t winsize_per_node, double tol, double* U, double* F, double* H21,
+                     int32_t* convIt, uint8_t* markHole, double* zncc);
+
+/* Per-node part of Subpb1Quadtree (func_quadtree/Subpb1Quadtree.m:71-165,
+ * funICGN_Subpb1quadtree.m:1-476).  winsize_xy is n x 2 column-major
+ * (DICpara.winsize_List).  udual (4n) is read by the reference but does
+ * not enter the update.  Outputs U (2n) and convIt (n; 0 where MATLAB
+ * would throw, as the reference leaves it). */
+int aldic_subpb1(aldic_ctx* ctx, const aldic_images* im, const double* coords,
+                 const double* UOld, const double* FOld, const double* udual,
+                 const double* vdual, int64_t n, const int32_t* winsize_xy, double mu,
+                 double tol, double* U, int32_t* convIt);
+
+/* A plan fixes the node set of repeated device-resident solves: it sorts
+ * nodes into size classes (one wave per subset up to 33 px, four waves up
+ * to 65 px, a workgroup beyond) and owns the scratch for subsets handed
+ * from the fast to the generic kernel.  winsize is a host array (one value
+ * or n values), or n x 2 (winsize_List) when rect is nonzero. */
+int aldic_plan_create(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int32_t winsize_per_node,
+                      int32_t rect, aldic_plan** out);
+int aldic_plan_destroy(aldic_plan* plan);
+
+/* Device-resident LocalICGN: d_coords n x 2 (double), d_U0 2n (float);
+ * outputs float/int device arrays as aldic_local_icgn (d_F, d_H21, d_hole,
+ * d_zncc may be NULL).  Asynchronous on the context stream. */
+int aldic_local_icgn_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
+                         const double* d_coords, const float* d_U0, const int32_t* d_winsize,
+                         double tol, float* d_U, float* d_F, float* d_H21, int32_t* d_convIt,
+                         uint8_t* d_hole, float* d_zncc);
+
+/* Device-resident Subpb1. */
+int aldic_subpb1_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
+                     const double* d_coords, const float* d_UOld, const float* d_FOld,
+                     const float* d_vdual, const int32_t* d_winsize_xy, double mu, double tol,
+                     float* d_U, int32_t* d_convIt);
+
+/* funIntegerSearchWholeField (func/funIntegerSearch.m:100-203) as called by
+ * IntegerSearchQuadtree (func_quadtree/IntegerSearchQuadtree.m:93) with
+ * InitFFTSearchMethod = 1.  region = [L R U D] after the repmat of
+ * funIntegerSearch.m:12.  Grid outputs are nY x nX column-major; pass NULL
+ * outputs to query nX, nY.  ipeak_row/col are the 1-based integer peak in
+ * the cropped lag window (0 where normxcorr2 would throw, u = v = NaN). */
+int aldic_integer_search(aldic_ctx* ctx, const double* f, const double* g, int64_t M, int64_t N,
+                         const int64_t gridx[2], const int64_t gridy[2], int32_t winsize,
+                         int32_t winstepsize, const int64_t region[4], double* x, double* y,
+                         double* u, double* v, double* ccmax, int32_t* ipeak_row,
+                         int32_t* ipeak_col, int64_t* nX, int64_t* nY);
+
+/* Device-resident integer search on FP32 planes d_f, d_g (M x N); outputs
+ * are device arrays of nX*nY entries (nY x nX column-major).  The per-point
+ * ZNCC lag maps live in a context-owned scratch buffer. */
+int aldic_integer_search_dev(aldic_ctx* ctx, const float* d_f, const float* d_g, int64_t M,
+                             int64_t N, const int64_t gridx[2], const int64_t gridy[2],
+                             int32_t winsize, int32_t winstepsize, const int64_t region[4],
+                             double* d_x, double* d_y, double* d_u, double* d_v,
+                             double* d_ccmax, int32_t* d_ipeak_row, int32_t* d_ipeak_col);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

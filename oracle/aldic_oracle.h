@@ -1,0 +1,123 @@
+/*
+ * aldic_oracle.h -- FP64 CPU restatement of ALDIC's local matching path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load this library, and only as the
+ * checker or as the timed CPU baseline.  The product path (the HIP library
+ * under 3d-stereo-aldic_amd/) never links or calls it.
+ *
+ * Conventions follow the reference's MATLAB data layout: an image is an
+ * M x N column-major array whose first dimension is image x (images are
+ * loaded transposed, gui/loadStereoImages.m:79), element (i,j) 1-based lives
+ * at [(i-1) + (j-1)*M].  Node coordinates are 1-based.
+ *
+ * Parity status of each function (what pins it) is listed in the header of
+ * aldic_oracle.c.
+ */
+#ifndef ALDIC_ORACLE_H
+#define ALDIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-subset status codes, identical to funICGNQuadtree.m:425-444. */
+#define ORC_ERR (-1)       /* MATLAB error inside the subset -> caller writes NaN */
+#define ORC_NOCONV 101     /* not converged / NaN / out of image */
+#define ORC_MASKED 102     /* >= 40% of the subset masked */
+#define ORC_EMPTY 103      /* sum |f| < 1e-6 */
+
+/* ba_interp2_spline.cpp:235-334 (double and single instantiations). */
+void orc_interp_spline_d(const double* F, int64_t M, int64_t N, int64_t P,
+                         const double* X, const double* Y, int64_t ND, double* Z);
+void orc_interp_spline_f(const float* F, int64_t M, int64_t N, int64_t P,
+                         const float* X, const float* Y, int64_t ND, float* Z);
+
+/* funImgGradient.m:49-78: 7-tap gradients, cropped by 3, times the mask.
+ * DfDx, DfDy are (M-6) x (N-6) column-major.  mask may be NULL (all ones). */
+void orc_img_gradient(const double* img, const double* mask, int64_t M, int64_t N,
+                      double* DfDx, double* DfDy);
+
+/* funNormalizeImg.m:36-40 over ROI [gx0,gx1] x [gy0,gy1] (1-based, inclusive). */
+void orc_normalize(const double* img, int64_t M, int64_t N, int64_t gx0, int64_t gx1,
+                   int64_t gy0, int64_t gy1, double* out);
+
+typedef struct {
+  const double* ImgRef;  /* M x N normalised reference image */
+  const double* ImgDef;  /* M x N normalised deformed image */
+  const double* mask;    /* M x N reference mask (Df.ImgRefMask) */
+  const double* DfDx;    /* (M-6) x (N-6) */
+  const double* DfDy;    /* (M-6) x (N-6) */
+  int64_t M, N;
+} orc_images;
+
+/* funICGNQuadtree.m:1-454 for one node (x0, y0 already rounded).
+ * Returns the status (stepwithinwhile, or ORC_ERR when MATLAB would throw).
+ * out_H21 follows HGlobal = H([1:6 8:12 15:18 22:24 29:30 36]).
+ * out_zncc (new output, no counterpart in the reference) is the ZNCC of the
+ * last evaluated warp. */
+int orc_icgn_subset(const orc_images* im, int64_t x0, int64_t y0, int winsize,
+                    double tol, const double U0[2], double U[2], double F[4],
+                    double H21[21], double* zncc);
+
+/* funICGN_Subpb1quadtree.m:1-476 for one node.  Returns the status (ORC_ERR on
+ * a MATLAB error). */
+int orc_subpb1_subset(const orc_images* im, int64_t x0, int64_t y0, int winsize_x,
+                      int winsize_y, double mu, const double udual[4],
+                      const double vdual[2], const double UOld[2],
+                      const double FOld[4], double tol, double U[2]);
+
+/* LocalICGNQuadtree.m:40-192 loop + status handling over n nodes (OpenMP
+ * parallel over nodes when nthreads > 1).  coords is n x 2 column-major
+ * (coordinatesFEM), U0 interleaved 2n, winsize per node (or one value if
+ * winsize_per_node == 0).  Outputs U 2n, F 4n, H n x 21 (column-major),
+ * convIt n, markHole n, zncc n.  No bad-point rule and no NaN fill. */
+void orc_local_icgn(const orc_images* im, const double* coords, const double* U0,
+                    int64_t n, const int32_t* winsize, int winsize_per_node,
+                    double tol, int nthreads, double* U, double* F, double* H,
+                    int32_t* convIt, uint8_t* markHole, double* zncc);
+
+/* Subpb1Quadtree.m:43-165 loop over n nodes.  winsize_xy is n x 2
+ * column-major (DICpara.winsize_List). */
+void orc_subpb1(const orc_images* im, const double* coords, const double* UOld,
+                const double* FOld, const double* udual, const double* vdual,
+                int64_t n, const int32_t* winsize_xy, double mu, double tol,
+                int nthreads, double* U, int32_t* convIt);
+
+/* Textbook normxcorr2 (Lewis 1995) restricted to the lag window that
+ * funIntegerSearch.m:166 crops: lags -1 .. W-s+1 along each dimension.
+ * T: s1 x s2, A: W1 x W2 (column-major).  out: (W1-s1+3) x (W2-s2+3).
+ * Returns 0, or -1 if normxcorr2 would throw (constant template). */
+int orc_normxcorr2_crop(const double* T, int64_t s1, int64_t s2, const double* A,
+                        int64_t W1, int64_t W2, double* out);
+
+/* findpeak.m:1-67 with subpixel=1.  f: m x n column-major.
+ * xpeak is the column (dim 2) and ypeak the row (dim 1), both 1-based. */
+void orc_findpeak(const double* f, int64_t m, int64_t n, double* xpeak,
+                  double* ypeak, double* max_f, int64_t* ipeak_row,
+                  int64_t* ipeak_col);
+
+/* funIntegerSearchWholeField (funIntegerSearch.m:100-203) with the search
+ * region already expanded to [L R U D] (funIntegerSearch.m:12).  Grid
+ * outputs are nY x nX column-major as in the reference.  Also returns the
+ * integer peak (row, col) per grid point (1-based in the cropped window),
+ * 0 where normxcorr2 failed. */
+int orc_integer_search(const double* f, const double* g, int64_t M, int64_t N,
+                       const int64_t gridx[2], const int64_t gridy[2],
+                       int winsize, int winstepsize, const int64_t region[4],
+                       int nthreads, double* x, double* y, double* u, double* v,
+                       double* ccmax, int32_t* ipeak_row, int32_t* ipeak_col,
+                       int64_t* nX_out, int64_t* nY_out);
+
+/* bwselect(BW, c, r, n) on a m1 x m2 column-major logical image: the
+ * n-connected (4 or 8) component of BW containing pixel (r, c) (1-based).
+ * All-false if that pixel is false. */
+void orc_bwselect(const uint8_t* BW, int64_t m1, int64_t m2, int64_t r, int64_t c,
+                  int conn, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,338 @@
+"""CPU tests that pin the FP64 oracle to what the mathematics and the
+reference fix (the reference itself ships no golden vectors for this path
+and its MEX needs MATLAB's mex.h, so it cannot be run here)."""
+import numpy as np
+import pytest
+
+from _cases import c1_grid, c1_pair, hole_mask, small_pair
+
+# func/BicubicBsplineBuild.m:9-10 (commented definition of B and C)
+B = np.array([[-1, 3, -3, 1], [3, -6, 3, 0], [-3, 0, 3, 0], [1, 4, 1, 0]]) / 6.0
+Cm = np.array([[71, -19, 5, -1], [-19, 95, -25, 5], [5, -25, 95, -19], [-1, 5, -19, 71]]) / 56.0
+BC = B @ Cm
+
+
+def closed_form(Q, dx, dy):
+    """Value of A = rot90(BC*Q*BC',2) polynomial, A[i][j] dx^i dy^j."""
+    A = np.rot90(BC @ Q @ BC.T, 2)
+    return sum(A[i, j] * dx ** i * dy ** j for i in range(4) for j in range(4))
+
+
+def test_bc_matches_reference_table():
+    # the numeric BC the reference hard-codes (func/BicubicBsplineBuild.m:13-16)
+    ref = np.array([[-0.428571428571429, 1.14285714285714, -1.14285714285714, 0.428571428571429],
+                    [1.01785714285714, -2.08928571428571, 1.33928571428571, -0.267857142857143],
+                    [-0.589285714285714, -0.0535714285714286, 0.803571428571429, -0.160714285714286],
+                    [0, 1, 0, 0]])
+    assert np.allclose(BC, ref, atol=1e-13)
+
+
+def test_interp_matches_closed_form(orc):
+    rng = np.random.default_rng(0)
+    F = np.asfortranarray(rng.normal(size=(40, 37)))
+    n = 500
+    X = rng.uniform(3, 35, n)  # dim-2 coordinate
+    Y = rng.uniform(3, 38, n)  # dim-1 coordinate
+    Z = orc.interp_spline(F, X, Y)
+    for k in range(n):
+        xb, yb = int(np.floor(X[k])) - 1, int(np.floor(Y[k])) - 1  # 0-based base col/row
+        Q = F[yb - 1:yb + 3, xb - 1:xb + 3]  # Q[r][c], r along dim 1
+        ref = closed_form(Q, Y[k] - np.floor(Y[k]), X[k] - np.floor(X[k]))
+        # the MEX's float-rounded literal scales differ from 1/3136 by < 5e-8
+        assert abs(Z[k] - ref) < 1e-7 * (1 + np.abs(Q).sum())
+
+
+def test_interp_float_literals_are_used(orc):
+    # with the MEX's single-precision scales the double result is NOT the
+    # exact rational value: a sample where only a33 (scale 1.0f/49) matters
+    F = np.zeros((12, 12))
+    F[5, 5] = 1.0
+    Z = orc.interp_spline(F, np.array([6.5]), np.array([6.5]))[0]
+    exact = closed_form(F[4:8, 4:8], 0.5, 0.5)
+    assert Z != exact
+    assert abs(Z - exact) < 1e-7
+
+
+def test_interp_integer_nodes_and_constant(orc):
+    rng = np.random.default_rng(1)
+    F = np.asfortranarray(rng.normal(size=(30, 30)))
+    j, i = np.meshgrid(np.arange(1, 31), np.arange(1, 31))
+    Z = orc.interp_spline(F, j.astype(float), i.astype(float))
+    assert np.array_equal(Z, F)  # a00 = Q11 exactly
+    c = np.full((20, 20), 3.25)
+    Zc = orc.interp_spline(c, rng.uniform(1, 20, 50), rng.uniform(1, 20, 50))
+    assert np.allclose(Zc, 3.25, atol=1e-12)  # partition of unity
+
+
+def test_interp_ramp_bias(orc):
+    i = np.arange(1, 41, dtype=float)
+    F = np.asfortranarray(np.repeat(i[:, None], 40, axis=1))  # F = dim-1 coordinate
+    t = np.linspace(0.0, 0.95, 20)
+    Z = orc.interp_spline(F, np.full(20, 20.0), 20.0 + t)
+    e = t * (2 * t - 1) * (t - 1) / 14.0
+    assert np.allclose(Z - (20.0 + t), e, atol=1e-7)
+
+
+def test_interp_clamps_like_edge_padding(orc):
+    rng = np.random.default_rng(2)
+    F = np.asfortranarray(rng.normal(size=(16, 18)))
+    X = rng.uniform(-3, 22, 300)
+    Y = rng.uniform(-3, 20, 300)
+    Z = orc.interp_spline(F, X, Y)
+    pad = np.pad(F, 8, mode="edge")
+    Zp = orc.interp_spline(pad, X + 8, Y + 8)
+    assert np.allclose(Z, Zp, atol=1e-12)
+
+
+def test_interp_single(orc):
+    rng = np.random.default_rng(3)
+    F = np.asfortranarray(rng.normal(size=(30, 30)))
+    X = rng.uniform(3, 27, 200)
+    Y = rng.uniform(3, 27, 200)
+    Zd = orc.interp_spline(F, X, Y)
+    Zs = orc.interp_spline(F.astype(np.float32), X.astype(np.float32), Y.astype(np.float32),
+                           single=True)
+    assert Zs.dtype == np.float32
+    assert np.allclose(Zs, Zd, atol=2e-5)
+
+
+def test_gradient_exact_on_polynomials(orc):
+    M, N = 30, 26
+    x = np.arange(1, M + 1, dtype=float)[:, None]
+    y = np.arange(1, N + 1, dtype=float)[None, :]
+    img = np.asfortranarray(0.001 * x ** 5 + 0.01 * (y ** 3) * x - 0.5 * y ** 2)
+    gx, gy = orc.img_gradient(img)
+    X, Y = x[3:-3], y[:, 3:-3]
+    assert np.allclose(gx, 0.005 * X ** 4 + 0.01 * Y ** 3, rtol=1e-10, atol=1e-9)
+    assert np.allclose(gy, 0.03 * (Y ** 2) * X - Y, rtol=1e-10, atol=1e-9)
+    mask = np.ones((M, N))
+    mask[10, 12] = 0
+    gxm, _ = orc.img_gradient(img, mask)
+    assert gxm[7, 9] == 0 and gxm[7, 10] == gx[7, 10]
+
+
+def _brute_ncc(T, A):
+    s1, s2 = T.shape
+    W1, W2 = A.shape
+    Ap = np.zeros((W1 + 2, W2 + 2))
+    Ap[1:-1, 1:-1] = A
+    out = np.zeros((W1 - s1 + 3, W2 - s2 + 3))
+    Tc = T - T.mean()
+    for p in range(out.shape[0]):
+        for q in range(out.shape[1]):
+            w = Ap[p:p + s1, q:q + s2]
+            wc = w - w.mean()
+            d = np.sqrt((Tc ** 2).sum() * (wc ** 2).sum())
+            out[p, q] = (Tc * w).sum() / d if d > 0 else 0.0
+    return out
+
+
+def test_normxcorr2_crop_bruteforce(orc):
+    rng = np.random.default_rng(4)
+    T = rng.normal(size=(7, 7))
+    A = rng.normal(size=(15, 13))
+    c = orc.normxcorr2_crop(T, A)
+    assert np.allclose(c, _brute_ncc(T, A), atol=1e-12)
+    # invariance to A -> a*A + b (Pearson correlation)
+    c2 = orc.normxcorr2_crop(T, 3.0 * A + 7.0)
+    assert np.allclose(c2[1:-1, 1:-1], c[1:-1, 1:-1], atol=1e-12)
+    with pytest.raises(ValueError):
+        orc.normxcorr2_crop(np.ones((5, 5)), A)
+
+
+def test_findpeak_quadratic(orc):
+    x0, y0 = 0.3137, -0.2249
+    r = np.arange(-5, 6, dtype=float)
+    Y, X = np.meshgrid(r, r, indexing="ij")  # rows = y, columns = x
+    f = 2.0 - 0.7 * (X - x0) ** 2 - 0.4 * (Y - y0) ** 2 + 0.1 * (X - x0) * (Y - y0)
+    xp, yp, mf, pr, pc = orc.findpeak(f)
+    assert (pr, pc) == (6, 6)
+    assert xp == pytest.approx(6 + round(1000 * x0) / 1000, abs=1e-12)
+    assert yp == pytest.approx(6 + round(1000 * y0) / 1000, abs=1e-12)
+    # on the edge: integer peak
+    g = np.zeros((5, 5))
+    g[0, 2] = 1.0
+    xp, yp, mf, pr, pc = orc.findpeak(g)
+    assert (xp, yp, mf) == (3.0, 1.0, 1.0)
+    # ties: first in column-major order
+    h = np.zeros((5, 5))
+    h[3, 1] = h[1, 3] = 1.0
+    assert orc.findpeak(h)[3:] == (4, 2)
+
+
+def test_integer_search_recovers_integer_shift(orc):
+    rng = np.random.default_rng(5)
+    base = rng.normal(size=(140, 130))
+    from scipy.ndimage import gaussian_filter
+    base = gaussian_filter(base, 1.5)
+    f = np.asfortranarray(base[10:-10, 10:-10])
+    g = np.asfortranarray(np.roll(np.roll(base, -3, axis=0), 2, axis=1)[10:-10, 10:-10])
+    out = orc.integer_search(f, g, [1, 120], [1, 110], 20, 16, [6, 6, 6, 6])
+    # g(i, j) = f(i - 3, j + 2): the match of f at i sits at i - 3 in g
+    # (u = -3) and at j + 2 (v = +2).  The first grid row has ii = 1, its
+    # left search reach is clamped to 0 (funIntegerSearch.m:140-142) and it
+    # can only report the edge lag u = -1.
+    inner = out["x"] > 11.0
+    assert np.all(out["u"][~inner] == -1.0)
+    assert np.all(np.round(out["u"][inner]) == -3.0)
+    assert np.all(np.round(out["v"]) == 2.0)
+    assert np.all(np.abs(out["u"][inner] + 3.0) < 0.1) and np.all(np.abs(out["v"] - 2.0) < 0.1)
+    # the integer peak sits at lag index u + 2 + L (1-based), i.e. 5 for L = 6
+    assert np.all(out["ipeak_row"][inner] == 5)
+    assert out["x"][0, 0] == 11.0 and out["y"][0, 0] == 11.0
+
+
+def test_icgn_zero_motion_and_translation(orc):
+    f, _, _, _ = small_pair()
+    coords = np.asfortranarray(np.array([[60.0, 60.0], [80.0, 70.0], [100.0, 90.0]]))
+    im = orc.Images(f, f)
+    U, F, H, conv, hole, z = orc.local_icgn(im, coords, np.zeros(6), 20, 1e-3)
+    assert np.all(conv == 1) and np.all(U == 0) and np.all(F == 0)
+    assert np.allclose(z, 1.0)
+    # integer translation: g(x) = f(x - t) exactly, start from the truth
+    g = np.asfortranarray(np.roll(np.roll(f, 2, axis=0), -1, axis=1))
+    im2 = orc.Images(f, g)
+    U, F, H, conv, hole, z = orc.local_icgn(im2, coords, np.tile([2.0, -1.0], 3), 20, 1e-3)
+    assert np.all(conv == 1) and np.allclose(U, np.tile([2.0, -1.0], 3), atol=1e-12)
+    # and from a sub-pixel-off start it converges back to it
+    U, F, H, conv, hole, z = orc.local_icgn(im2, coords, np.tile([2.4, -0.6], 3), 20, 1e-3)
+    assert np.all((conv > 1) & (conv < 101))
+    assert np.allclose(U, np.tile([2.0, -1.0], 3), atol=2e-3)
+    # H symmetric positive definite
+    for k in range(3):
+        Hm = np.zeros((6, 6))
+        t = 0
+        for c in range(6):
+            for r in range(c, 6):
+                Hm[r, c] = Hm[c, r] = H[k, t]
+                t += 1
+        assert np.all(np.linalg.eigvalsh(Hm) > 0)
+
+
+def test_icgn_affine_intensity_invariance(orc):
+    f, g, _, _ = small_pair(shift=(1.3, -0.7))
+    coords = np.asfortranarray(np.array([[60.0, 60.0], [90.0, 80.0]]))
+    U0 = np.tile([1.0, -1.0], 2)
+    a = orc.local_icgn(orc.Images(f, g), coords, U0, 20, 1e-3)
+    b = orc.local_icgn(orc.Images(f, np.asfortranarray(2.5 * g + 0.75)), coords, U0, 20, 1e-3)
+    assert np.array_equal(a[3], b[3])
+    assert np.allclose(a[0], b[0], atol=1e-9)
+
+
+def test_icgn_status_codes(orc):
+    f, g, _, _ = small_pair(shift=(1.0, 0.5))
+    M, N = f.shape
+    mask = np.ones((M, N))
+    mask[40:80, 40:80] = 0  # a masked block
+    im = orc.Images(f, g, mask)
+    coords = np.asfortranarray(np.array([
+        [50.0, 50.0],    # inside the masked block: >= 40% masked -> 102
+        [10.0, 60.0],    # x0 - 10 < 4 -> Df index error -> -1
+        [120.0, 100.0],  # fine
+        [100.0, 60.0],   # U0 far out of the image -> 101
+    ]))
+    U0 = np.array([1.0, 0.5, 1.0, 0.5, 1.0, 0.5, 500.0, 0.0])
+    U, F, H, conv, hole, z = orc.local_icgn(im, coords, U0, 20, 1e-3)
+    assert conv[0] == 102 and np.array_equal(U[0:2], U0[0:2]) and np.all(F[0:4] == 0)
+    assert np.all(H[0] == 0) and hole[0] == 1
+    assert conv[1] == -1 and np.all(np.isnan(U[2:4]))
+    assert 1 <= conv[2] <= 20 and hole[2] == 0
+    assert conv[3] == 101 and np.array_equal(U[6:8], U0[6:8])
+    # NaN initial guess -> -1 (logical(NaN) throws)
+    U, F, H, conv, hole, z = orc.local_icgn(im, coords[2:3], np.array([np.nan, 0.0]), 20, 1e-3)
+    assert conv[0] == -1
+
+
+def test_icgn_partly_masked_subset_runs_mask_branch(orc):
+    f, g, _, _ = small_pair(shift=(0.6, -0.4))
+    M, N = f.shape
+    mask = np.ones((M, N))
+    mask[:, 78:] = 0  # node at y0 = 70 with winsize 20: columns 60..80 partly masked
+    im = orc.Images(f, g, mask)
+    U, F, H, conv, hole, z = orc.local_icgn(im, np.asfortranarray([[80.0, 70.0]]),
+                                            np.array([1.0, 0.0]), 20, 1e-3)
+    # window grows, masked pixels are excluded; the subset still converges
+    assert conv[0] == -1 or (1 <= conv[0] <= 100)
+
+
+def test_bwselect_bruteforce(orc):
+    from scipy.ndimage import label
+    rng = np.random.default_rng(6)
+    for trial in range(30):
+        BW = rng.random((11, 9)) < 0.55
+        r, c = rng.integers(1, 12), rng.integers(1, 10)
+        for conn, st in ((4, None), (8, np.ones((3, 3)))):
+            lab, _ = label(BW, structure=st)
+            want = (lab == lab[r - 1, c - 1]) & BW[r - 1, c - 1]
+            got = orc.bwselect(BW, r, c, conn)
+            assert np.array_equal(got, want)
+
+
+def test_subpb1_limits(orc):
+    f, g, _, _ = small_pair(shift=(1.2, -0.3))
+    coords = np.asfortranarray(np.array([[70.0, 70.0], [90.0, 80.0]]))
+    UOld = np.tile([1.0, 0.0], 2)
+    FOld = np.zeros(8)
+    vdual = np.tile([0.05, -0.02], 2)
+    ws = np.asfortranarray(np.full((2, 2), 20, dtype=np.int32))
+    im = orc.Images(f, g)
+    # tiny mu: the data term wins, U goes to the true translation
+    U0, conv = orc.subpb1(im, coords, UOld, FOld, np.zeros(8), vdual, ws, 1e-12, 1e-6)
+    assert np.allclose(U0, np.tile([1.2, -0.3], 2), atol=0.02)
+    # Fixed point with a penalty: the step solves (Hd + mu I) dP = -tempb,
+    # tempb = 2 b / bottomf^2 + mu (P - UOld - vdual), and the inverse-
+    # compositional update moves P by -dP (funICGN_Subpb1quadtree.m:381-440).
+    # Near the data optimum P0, 2 b / bottomf^2 ~ -Hd (P - P0) with
+    # Hd = 2 H / bottomf^2, so the iteration stops where
+    # (Hd - mu I) P = Hd P0 - mu T: the penalty, with the reference's sign,
+    # pushes P away from T = UOld + vdual.  Hd from the subset's own
+    # gradients and moments, computed independently here.
+    gx, gy = orc.img_gradient(f)
+    for k in range(2):
+        x0, y0 = int(coords[k, 0]), int(coords[k, 1])
+        sl = (slice(x0 - 11, x0 + 10), slice(y0 - 11, y0 + 10))
+        dsl = (slice(x0 - 14, x0 + 7), slice(y0 - 14, y0 + 7))
+        fx, fy = gx[dsl], gy[dsl]
+        fs = f[sl]
+        bf2 = ((fs - fs.mean()) ** 2).sum()
+        Hd = 2.0 / bf2 * np.array([[(fx * fx).sum(), (fx * fy).sum()],
+                                   [(fx * fy).sum(), (fy * fy).sum()]])
+        mu = 0.05 * np.trace(Hd) / 2
+        T = UOld[2 * k:2 * k + 2] + vdual[2 * k:2 * k + 2]
+        Uk, ck = orc.subpb1(im, coords[k:k + 1], UOld[2 * k:2 * k + 2], FOld[:4], np.zeros(4),
+                            vdual[2 * k:2 * k + 2], ws[k:k + 1], mu, 1e-8)
+        P0 = U0[2 * k:2 * k + 2]
+        want = np.linalg.solve(Hd - mu * np.eye(2), Hd @ P0 - mu * T)
+        assert np.allclose(Uk, want, atol=2e-3)
+        # a dropped penalty or a flipped sign would land elsewhere by
+        assert np.linalg.norm(want - P0) > 5e-3
+
+
+def test_c1_reference_configuration(orc):
+    """C1: 900 subsets; the first grid row/column throws (DfDx index < 1)."""
+    f, g, uf, vf = c1_pair()
+    coords, nX, nY = c1_grid()
+    assert (nX, nY) == (30, 30)
+    s = orc.integer_search(f, g, [1, 512], [1, 512], 30, 16, [10, 10, 10, 10], nthreads=4)
+    assert np.nanmax(np.abs(s["u"] - uf(s["x"], s["y"]))) < 0.6
+    U0 = np.zeros(2 * 900)
+    U0[0::2] = s["u"].T.reshape(-1)
+    U0[1::2] = s["v"].T.reshape(-1)
+    U, F, H, conv, hole, z = orc.local_icgn(orc.Images(f, g), coords, U0, 30, 1e-3, nthreads=4)
+    assert np.sum(conv == -1) == 59
+    ok = (conv > 0) & (conv <= 100)
+    assert ok.sum() == 841
+    eu = U[0::2][ok] - uf(coords[ok, 0], coords[ok, 1])
+    assert np.sqrt(np.mean(eu ** 2)) < 0.03  # first-order bias of the sine term
+
+
+def test_bad_point_rules(pkg):
+    conv = np.array([3, 4, 5, 4, 3, 101, -1, 102, 20, 4])
+    bad, num = pkg.icgn_bad_points(conv)
+    good = np.array([3, 4, 5, 4, 3, 20, 4], dtype=float)
+    thr = max(good.mean() + good.std(ddof=1), 6)
+    assert num == 3 - 1
+    assert set(bad) == {5, 6, 7} | set(np.flatnonzero(conv > thr))
+    bad1, num1 = pkg.subpb1_bad_points(conv)
+    thr1 = max(good.mean() + 0.25 * good.std(ddof=1), 10)
+    assert set(bad1) == {5, 6, 7} | set(np.flatnonzero(conv > thr1))
