@@ -30,6 +30,14 @@ SIGNATURES = {
     "aldic_ctx_destroy": (C.c_int, [_vp]),
     "aldic_ctx_stream": (_vp, [_vp]),
     "aldic_ctx_synchronize": (C.c_int, [_vp]),
+    "aldic_dev_malloc": (C.c_int, [_vp, _i64, _pp]),
+    "aldic_dev_free": (C.c_int, [_vp, _vp]),
+    "aldic_memcpy_htod": (C.c_int, [_vp, _vp, _vp, _i64]),
+    "aldic_memcpy_dtoh": (C.c_int, [_vp, _vp, _vp, _i64]),
+    "aldic_ctx_set_timing": (C.c_int, [_vp, C.c_int32]),
+    "aldic_ctx_stage_times": (C.c_int, [_vp, _vp, _vp, C.c_int32]),
+    "aldic_grid_to_nodes_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.c_double, C.c_double,
+                                          C.c_double, _vp, _i64, _vp]),
     "aldic_interp_spline_f64": (C.c_int, [_vp, _dp, _i64, _i64, _i64, _dp, _dp, _i64, _dp]),
     "aldic_interp_spline_f32": (C.c_int, [_vp, _fp, _i64, _i64, _i64, _fp, _fp, _i64, _fp]),
     "aldic_img_gradient": (C.c_int, [_vp, _dp, _vp, _i64, _i64, _dp, _dp]),

@@ -573,3 +573,62 @@ def integer_search_dev(ctx: Context, d_f, d_g, M, N, gridx, gridy, winsize, wins
     check(lib().aldic_integer_search_dev(ctx.ptr, d_f, d_g, int(M), int(N), gx, gy, int(winsize),
                                          int(winstepsize), rg, d_x, d_y, d_u, d_v, d_ccmax,
                                          d_ipr, d_ipc))
+
+
+def grid_to_nodes_dev(ctx: Context, d_u, d_v, nX, nY, x0, y0, step, d_coords, n, d_U0):
+    check(lib().aldic_grid_to_nodes_dev(ctx.ptr, d_u, d_v, int(nX), int(nY), float(x0), float(y0),
+                                        float(step), d_coords, int(n), d_U0))
+
+
+STAGES = {0: "search", 1: "grid_to_nodes", 2: "icgn_c0", 3: "icgn_c1", 4: "icgn_c2",
+          5: "icgn_c3", 6: "icgn_generic", 7: "icgn_handed_on", 8: "subpb1_c0", 9: "subpb1_c1",
+          10: "subpb1_c2", 11: "subpb1_c3", 12: "subpb1_generic", 13: "subpb1_handed_on"}
+NSTAGES = 16
+
+
+def set_timing(ctx: Context, enable: bool):
+    check(lib().aldic_ctx_set_timing(ctx.ptr, 1 if enable else 0))
+
+
+def stage_times(ctx: Context, reset=True):
+    """{stage name: (total ms, launches)} since the last reset."""
+    ms = np.zeros(NSTAGES)
+    cnt = np.zeros(NSTAGES, dtype=np.int32)
+    check(lib().aldic_ctx_stage_times(ctx.ptr, ms.ctypes.data, cnt.ctypes.data, 1 if reset else 0))
+    return {STAGES.get(k, f"stage{k}"): (float(ms[k]), int(cnt[k])) for k in range(NSTAGES)
+            if cnt[k] > 0}
+
+
+class DeviceArray:
+    """A flat device buffer owned through the C ABI (aldic_dev_malloc)."""
+
+    def __init__(self, ctx: Context, count, dtype):
+        self.ctx = ctx
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        p = C.c_void_p()
+        check(lib().aldic_dev_malloc(ctx.ptr, self.count * self.dtype.itemsize, C.byref(p)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx: Context, a, dtype=None):
+        a = np.ascontiguousarray(np.asarray(a, dtype=dtype).reshape(-1, order="F"))
+        d = cls(ctx, a.size, a.dtype)
+        check(lib().aldic_memcpy_htod(ctx.ptr, d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    def to_host(self):
+        out = np.empty(self.count, dtype=self.dtype)
+        check(lib().aldic_memcpy_dtoh(self.ctx.ptr, out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().aldic_dev_free(self.ctx.ptr, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

@@ -68,6 +68,11 @@ hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, i
                          double* u, double* v, double* ccmax, int* ipr, int* ipc,
                          hipStream_t s);
 
+// search grid -> node initial guess (prep.hip)
+hipError_t launch_grid_to_nodes(const double* gu, const double* gv, int nX, int nY, double x0,
+                                double y0, double step, const double* coords, int n, float* U0,
+                                hipStream_t s);
+
 // interpolation (interp.hip)
 hipError_t launch_interp_f64(const double* F, int64_t M, int64_t N, int64_t P, const double* X,
                              const double* Y, int64_t ND, double* Z, hipStream_t s);

@@ -41,6 +41,12 @@ struct aldic_ctx {
   // reusable staging buffers of the host-pointer entry points
   std::vector<void*> buf;
   std::vector<size_t> cap;
+  // per-stage kernel timing (HIP events on the context stream)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  double stage_ms[ALDIC_NSTAGES] = {0};
+  int stage_cnt[ALDIC_NSTAGES] = {0};
 };
 
 struct aldic_images {
@@ -74,6 +80,36 @@ int set_device(aldic_ctx* ctx) {
   HIP_TRY(hipSetDevice(ctx->device));
   return ALDIC_OK;
 }
+
+// Stage timing: a pair of events around each launch of a stage, resolved
+// when the caller asks for the totals.
+hipEvent_t take_event(aldic_ctx* ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct StageTimer {
+  aldic_ctx* ctx;
+  int id;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(aldic_ctx* c, int stage) : ctx(c), id(stage) {
+    if (!ctx->timing) return;
+    a = take_event(ctx);
+    b = take_event(ctx);
+    if (a) (void)hipEventRecord(a, ctx->stream);
+  }
+  ~StageTimer() {
+    if (!ctx->timing || !a || !b) return;
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->pending.push_back({id, {a, b}});
+  }
+};
 
 // staging buffer `slot` of at least `bytes`
 int staging(aldic_ctx* ctx, int slot, size_t bytes, void** out) {
@@ -151,21 +187,25 @@ int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
   a.n = (int)p->n;
   a.defer_cnt = p->defer_cnt;
   a.defer_list = p->defer_list;
+  const int base = mode == 0 ? ALDIC_STAGE_ICGN_C0 : ALDIC_STAGE_SUBPB1_C0;
   zero_ints<<<1, 64, 0, s>>>(p->defer_cnt, 1);
   HIP_TRY(hipGetLastError());
   for (int c = 0; c < 4; c++) {
     if (p->cnt[c] == 0) continue;
     a.list = p->lists[c];
     a.cnt = p->cnt[c];
+    StageTimer t(ctx, base + c);
     HIP_TRY(aldic::launch_fast(mode, c, a, s));
   }
   if (p->cnt[4] > 0) {
     const int grid = std::min(p->cnt[4], 2048);
+    StageTimer t(ctx, base + 4);
     HIP_TRY(aldic::launch_generic(mode, a, p->lists[4], nullptr, p->cnt[4], grid, s));
   }
   // subsets handed on by the fast kernels; the grid-stride loop reads the
   // count on the device, so no host round trip is needed
   const int grid = (int)std::min<int64_t>(std::max<int64_t>(p->n, 1), 1024);
+  StageTimer t(ctx, base + 5);
   HIP_TRY(aldic::launch_generic(mode, a, p->defer_list, p->defer_cnt, 0, grid, s));
   return ALDIC_OK;
 }
@@ -196,10 +236,46 @@ extern "C" int aldic_ctx_create(int device, aldic_ctx** out) {
   return ALDIC_OK;
 }
 
+extern "C" int aldic_ctx_set_timing(aldic_ctx* ctx, int32_t enable) {
+  if (!ctx) return fail(ALDIC_EINVAL, "NULL context");
+  ctx->timing = enable != 0;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_ctx_stage_times(aldic_ctx* ctx, double* ms, int32_t* launches, int32_t reset) {
+  if (!ctx) return fail(ALDIC_EINVAL, "NULL context");
+  RC_TRY(set_device(ctx));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (auto& pr : ctx->pending) {
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, pr.second.first, pr.second.second));
+    ctx->stage_ms[pr.first] += t;
+    ctx->stage_cnt[pr.first] += 1;
+    ctx->ev_pool.push_back(pr.second.first);
+    ctx->ev_pool.push_back(pr.second.second);
+  }
+  ctx->pending.clear();
+  for (int k = 0; k < ALDIC_NSTAGES; k++) {
+    if (ms) ms[k] = ctx->stage_ms[k];
+    if (launches) launches[k] = ctx->stage_cnt[k];
+    if (reset) {
+      ctx->stage_ms[k] = 0.0;
+      ctx->stage_cnt[k] = 0;
+    }
+  }
+  return ALDIC_OK;
+}
+
 extern "C" int aldic_ctx_destroy(aldic_ctx* ctx) {
   if (!ctx) return ALDIC_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  for (auto& pr : ctx->pending) {
+    ctx->ev_pool.push_back(pr.second.first);
+    ctx->ev_pool.push_back(pr.second.second);
+  }
+  for (hipEvent_t e : ctx->ev_pool)
+    if (e) (void)hipEventDestroy(e);
   for (void* p : ctx->buf)
     if (p) (void)hipFree(p);
   (void)hipStreamDestroy(ctx->stream);
@@ -211,6 +287,46 @@ extern "C" void* aldic_ctx_stream(aldic_ctx* ctx) { return ctx ? (void*)ctx->str
 
 extern "C" int aldic_ctx_synchronize(aldic_ctx* ctx) {
   if (!ctx) return fail(ALDIC_EINVAL, "NULL context");
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return ALDIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device memory
+// ---------------------------------------------------------------------------
+extern "C" int aldic_dev_malloc(aldic_ctx* ctx, int64_t bytes, void** out) {
+  if (!ctx || !out || bytes < 0) return fail(ALDIC_EINVAL, "dev_malloc: bad argument");
+  RC_TRY(set_device(ctx));
+  *out = nullptr;
+  HIP_TRY(hipMalloc(out, (size_t)std::max<int64_t>(bytes, 1)));
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_dev_free(aldic_ctx* ctx, void* p) {
+  if (!ctx) return fail(ALDIC_EINVAL, "dev_free: NULL context");
+  if (!p) return ALDIC_OK;
+  RC_TRY(set_device(ctx));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipFree(p));
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_memcpy_htod(aldic_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes) || bytes < 0)
+    return fail(ALDIC_EINVAL, "memcpy_htod: bad argument");
+  if (bytes == 0) return ALDIC_OK;
+  RC_TRY(set_device(ctx));
+  HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_memcpy_dtoh(aldic_ctx* ctx, void* dst, const void* src, int64_t bytes) {
+  if (!ctx || (!dst && bytes) || (!src && bytes) || bytes < 0)
+    return fail(ALDIC_EINVAL, "memcpy_dtoh: bad argument");
+  if (bytes == 0) return ALDIC_OK;
+  RC_TRY(set_device(ctx));
+  HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   return ALDIC_OK;
 }
@@ -676,9 +792,24 @@ extern "C" int aldic_integer_search_dev(aldic_ctx* ctx, const float* d_f, const 
   RC_TRY(stage(ctx, SLOT_GMAP, (size_t)(kmax * nX * nY), &gmap));
   const int gx[2] = {(int)gridx[0], (int)gridx[1]}, gy[2] = {(int)gridy[0], (int)gridy[1]};
   const int rg[4] = {(int)region[0], (int)region[1], (int)region[2], (int)region[3]};
+  StageTimer t(ctx, ALDIC_STAGE_SEARCH);
   HIP_TRY(aldic::launch_search(d_f, d_g, (int)M, (int)N, (int)nX, (int)nY, gx, gy, winsize,
                                winstepsize, rg, gmap, kmax, d_x, d_y, d_u, d_v, d_ccmax, d_ipr,
                                d_ipc, ctx->stream));
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_grid_to_nodes_dev(aldic_ctx* ctx, const double* d_u, const double* d_v,
+                                       int64_t nX, int64_t nY, double x0, double y0, double step,
+                                       const double* d_coords, int64_t n, float* d_U0) {
+  if (!ctx || !d_u || !d_v || !d_coords || !d_U0)
+    return fail(ALDIC_EINVAL, "grid_to_nodes: NULL argument");
+  if (nX < 1 || nY < 1 || n < 0 || step <= 0 || n > 0x7fffffff)
+    return fail(ALDIC_EINVAL, "grid_to_nodes: bad sizes");
+  RC_TRY(set_device(ctx));
+  StageTimer t(ctx, ALDIC_STAGE_GRID2NODES);
+  HIP_TRY(aldic::launch_grid_to_nodes(d_u, d_v, (int)nX, (int)nY, x0, y0, step, d_coords, (int)n,
+                                      d_U0, ctx->stream));
   return ALDIC_OK;
 }
 

@@ -1127,13 +1127,33 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
 // ---------------------------------------------------------------------------
 namespace aldic {
 
+// Size classes of the fast kernel: largest subset side, waves per subset,
+// pixels per lane, LDS tile side.  Every class must hold a full square
+// subset of its largest side (NW * 64 * PPL >= side^2) and a tile with room
+// for that side, the 4x4 cell and a few pixels of warp.
+constexpr int CLS_SIDE[4] = {11, 17, 33, 65};
+constexpr int CLS_NW[4] = {1, 1, 1, 4};
+constexpr int CLS_PPL[4] = {2, 5, 18, 17};
+constexpr int CLS_TS[4] = {24, 32, 48, 84};
+static_assert(CLS_NW[0] * 64 * CLS_PPL[0] >= CLS_SIDE[0] * CLS_SIDE[0], "class 0 capacity");
+static_assert(CLS_NW[1] * 64 * CLS_PPL[1] >= CLS_SIDE[1] * CLS_SIDE[1], "class 1 capacity");
+static_assert(CLS_NW[2] * 64 * CLS_PPL[2] >= CLS_SIDE[2] * CLS_SIDE[2], "class 2 capacity");
+static_assert(CLS_NW[3] * 64 * CLS_PPL[3] >= CLS_SIDE[3] * CLS_SIDE[3], "class 3 capacity");
+static_assert(CLS_TS[0] >= CLS_SIDE[0] + 8 && CLS_TS[1] >= CLS_SIDE[1] + 8 &&
+                  CLS_TS[2] >= CLS_SIDE[2] + 8 && CLS_TS[3] >= CLS_SIDE[3] + 8,
+              "tile sides");
+
 int size_class(int S1, int S2) {
   const int s = S1 > S2 ? S1 : S2;
-  if (s <= 11) return 0;
-  if (s <= 17) return 1;
-  if (s <= 33) return 2;
-  if (s <= 65) return 3;
+  for (int c = 0; c < 4; c++)
+    if (s <= CLS_SIDE[c]) return c;
   return 4;  // generic kernel only
+}
+
+template <int MODE, int C>
+static void launch_class(const SolveArgs& a, hipStream_t s) {
+  constexpr int subs = 4 / CLS_NW[C];
+  fast_kernel<MODE, CLS_NW[C], CLS_PPL[C], CLS_TS[C]><<<(a.cnt + subs - 1) / subs, 256, 0, s>>>(a);
 }
 
 template <int MODE>
@@ -1141,16 +1161,16 @@ static hipError_t launch_fast_mode(int cls, const SolveArgs& a, hipStream_t s) {
   if (a.cnt <= 0) return hipSuccess;
   switch (cls) {
     case 0:
-      fast_kernel<MODE, 1, 2, 24><<<(a.cnt + 3) / 4, 256, 0, s>>>(a);
+      launch_class<MODE, 0>(a, s);
       break;
     case 1:
-      fast_kernel<MODE, 1, 5, 32><<<(a.cnt + 3) / 4, 256, 0, s>>>(a);
+      launch_class<MODE, 1>(a, s);
       break;
     case 2:
-      fast_kernel<MODE, 1, 17, 48><<<(a.cnt + 3) / 4, 256, 0, s>>>(a);
+      launch_class<MODE, 2>(a, s);
       break;
     case 3:
-      fast_kernel<MODE, 4, 17, 84><<<a.cnt, 256, 0, s>>>(a);
+      launch_class<MODE, 3>(a, s);
       break;
     default:
       return hipErrorInvalidValue;

@@ -123,3 +123,50 @@ hipError_t launch_place_df(const double* DfDx, const double* DfDy, int64_t M, in
 }
 
 }  // namespace aldic
+
+// ---------------------------------------------------------------------------
+// Search grid -> node initial guess.  The integer search yields u, v on the
+// nY x nX grid of node centres x(ci) = x0 + ci*step (funIntegerSearch.m:
+// 175-176); a node list that is not that grid (quadtree nodes, per-node
+// subset sizes) takes the bilinear interpolant of the grid, held constant
+// beyond its outer nodes.  Output is the interleaved U0 (FP32) the solvers
+// read.
+// ---------------------------------------------------------------------------
+namespace {
+__global__ __launch_bounds__(256) void grid_to_nodes_kernel(const double* __restrict__ gu,
+                                                            const double* __restrict__ gv,
+                                                            int nX, int nY, double x0, double y0,
+                                                            double step,
+                                                            const double* __restrict__ coords,
+                                                            int n, float* __restrict__ U0) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  double fx = (coords[k] - x0) / step, fy = (coords[k + n] - y0) / step;
+  fx = fmin(fmax(fx, 0.0), (double)(nX - 1));
+  fy = fmin(fmax(fy, 0.0), (double)(nY - 1));
+  int ix = (int)floor(fx), iy = (int)floor(fy);
+  ix = min(ix, max(nX - 2, 0));
+  iy = min(iy, max(nY - 2, 0));
+  const double tx = fx - ix, ty = fy - iy;
+  const int ix1 = min(ix + 1, nX - 1), iy1 = min(iy + 1, nY - 1);
+  // grids are nY x nX column-major: (cj, ci) -> cj + ci*nY
+  auto at = [&](const double* a, int ci, int cj) { return a[cj + (int64_t)ci * nY]; };
+  const double u = (1 - tx) * (1 - ty) * at(gu, ix, iy) + tx * (1 - ty) * at(gu, ix1, iy) +
+                   (1 - tx) * ty * at(gu, ix, iy1) + tx * ty * at(gu, ix1, iy1);
+  const double v = (1 - tx) * (1 - ty) * at(gv, ix, iy) + tx * (1 - ty) * at(gv, ix1, iy) +
+                   (1 - tx) * ty * at(gv, ix, iy1) + tx * ty * at(gv, ix1, iy1);
+  U0[2 * k] = (float)u;
+  U0[2 * k + 1] = (float)v;
+}
+}  // namespace
+
+namespace aldic {
+hipError_t launch_grid_to_nodes(const double* gu, const double* gv, int nX, int nY, double x0,
+                                double y0, double step, const double* coords, int n, float* U0,
+                                hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  grid_to_nodes_kernel<<<(n + 255) / 256, 256, 0, s>>>(gu, gv, nX, nY, x0, y0, step, coords, n,
+                                                       U0);
+  return hipGetLastError();
+}
+}  // namespace aldic

@@ -49,6 +49,30 @@ int aldic_ctx_destroy(aldic_ctx* ctx);
 void* aldic_ctx_stream(aldic_ctx* ctx);
 int aldic_ctx_synchronize(aldic_ctx* ctx);
 
+/* Per-stage kernel timing with HIP events on the context stream, the
+ * counterpart of the reference's per-stage tic/toc (LocalICGNQuadtree.m:
+ * 69/107, Subpb1Quadtree.m:73/110, funIntegerSearch.m:130/182).  When
+ * enabled, every launch of a stage is bracketed by two events;
+ * aldic_ctx_stage_times synchronises the stream and returns, per stage,
+ * the summed milliseconds and the number of launches (arrays of
+ * ALDIC_NSTAGES entries, either may be NULL), optionally resetting them. */
+#define ALDIC_STAGE_SEARCH 0
+#define ALDIC_STAGE_GRID2NODES 1
+#define ALDIC_STAGE_ICGN_C0 2      /* +0..3: fast kernel per size class, */
+                                   /* +4: generic (> 65 px), +5: handed-on subsets */
+#define ALDIC_STAGE_SUBPB1_C0 8    /* same layout for Subpb1 */
+#define ALDIC_NSTAGES 16
+int aldic_ctx_set_timing(aldic_ctx* ctx, int32_t enable);
+int aldic_ctx_stage_times(aldic_ctx* ctx, double* ms, int32_t* launches, int32_t reset);
+
+/* Device memory for the _dev entry points (plumbing with no counterpart in
+ * the reference, whose MATLAB workers own every array).  Copies are
+ * synchronous on the context stream. */
+int aldic_dev_malloc(aldic_ctx* ctx, int64_t bytes, void** out);
+int aldic_dev_free(aldic_ctx* ctx, void* p);
+int aldic_memcpy_htod(aldic_ctx* ctx, void* dst, const void* src, int64_t bytes);
+int aldic_memcpy_dtoh(aldic_ctx* ctx, void* dst, const void* src, int64_t bytes);
+
 /* Z = ba_interp2_spline(F, X, Y): F is M x N x P, X (dim-2 coordinate) and
  * Y (dim-1 coordinate) hold ND points, Z is ND x P.  Same class for all
  * arrays.  Replaces mexFunction / interpolate<REAL>,
@@ -138,6 +162,15 @@ int aldic_integer_search(aldic_ctx* ctx, const double* f, const double* g, int64
                          int32_t winstepsize, const int64_t region[4], double* x, double* y,
                          double* u, double* v, double* ccmax, int32_t* ipeak_row,
                          int32_t* ipeak_col, int64_t* nX, int64_t* nY);
+
+/* Initial guess of arbitrary nodes from the search grid (as the reference
+ * carries U0 from the search grid onto quadtree nodes,
+ * func/StereoMatch_STAQ.m:20 and :59): bilinear interpolation of the nY x nX
+ * grids d_u, d_v (node centres x0 + ci*step, y0 + cj*step), constant
+ * beyond the outer nodes.  d_coords n x 2 (double), d_U0 2n (float). */
+int aldic_grid_to_nodes_dev(aldic_ctx* ctx, const double* d_u, const double* d_v, int64_t nX,
+                            int64_t nY, double x0, double y0, double step,
+                            const double* d_coords, int64_t n, float* d_U0);
 
 /* Device-resident integer search on FP32 planes d_f, d_g (M x N); outputs
  * are device arrays of nX*nY entries (nY x nX column-major).  The per-point
