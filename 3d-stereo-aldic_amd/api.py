@@ -632,3 +632,14 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def local_icgn_autotune_dev(images: Images, plan: Plan, d_coords, d_U0, d_winsize, tol, d_U, d_F,
+                            d_conv, d_zncc, reps=3):
+    """Pick the fastest launch shape per size class on these nodes; returns
+    the chosen shape index per class."""
+    chosen = np.zeros(4, dtype=np.int32)
+    check(lib().aldic_local_icgn_autotune_dev(images.ctx.ptr, images.ptr, plan.ptr, d_coords,
+                                              d_U0, d_winsize, float(tol), d_U, d_F, d_conv,
+                                              d_zncc, int(reps), chosen.ctypes.data))
+    return chosen

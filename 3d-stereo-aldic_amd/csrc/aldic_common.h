@@ -36,6 +36,13 @@ __device__ __forceinline__ int wave_sum_i(int v) {
   return __builtin_amdgcn_readlane(v, 63);
 }
 
+// A value that is the same on every lane, moved to a scalar register so
+// wave-uniform state (warp parameters, Cholesky factor) does not occupy
+// vector registers.
+__device__ __forceinline__ float uni(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
 // True on every lane if the predicate holds on any lane of the wave.
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
@@ -46,6 +53,16 @@ __device__ __forceinline__ int64_t mround(double v) { return (int64_t)round(v); 
 // coefficients a_ij = rot90(BC*Q*BC',2) (ba_interp2_spline.cpp:310-325,
 // func/BicubicBsplineBuild.m:13-16) factor as sum_rc Q[r][c] w_r(dx) w_c(dy)
 // with 56*w_k(t) the cubics below (taps -1, 0, +1, +2).
+// The same weights times 56 (three FMA-class operations each).  A value
+// built from them is 3136 times the interpolant, which the zero-normalised
+// residual does not see.
+__device__ __forceinline__ void spline_weights56(float t, float w[4]) {
+  w[0] = t * fmaf(t, fmaf(-24.0f, t, 57.0f), -33.0f);
+  w[1] = fmaf(t, fmaf(t, fmaf(64.0f, t, -117.0f), -3.0f), 56.0f);
+  w[2] = t * fmaf(t, fmaf(-64.0f, t, 75.0f), 45.0f);
+  w[3] = t * fmaf(t, fmaf(24.0f, t, -15.0f), -9.0f);
+}
+
 __device__ __forceinline__ void spline_weights(float t, float w[4]) {
   const float s = 1.0f / 56.0f;
   w[0] = t * (-33.0f + t * (57.0f - 24.0f * t)) * s;

@@ -42,7 +42,8 @@ struct SolveArgs {
 
 namespace aldic {
 int size_class(int S1, int S2);
-hipError_t launch_fast(int mode, int cls, const SolveArgs& a, hipStream_t s);
+hipError_t launch_fast(int mode, int cls, int var, const SolveArgs& a, hipStream_t s);
+int num_variants();
 hipError_t launch_generic(int mode, const SolveArgs& a, const int* list, const int* count,
                           int fixed_count, int grid, hipStream_t s);
 int generic_max_pixels();

@@ -67,6 +67,8 @@ struct aldic_plan {
   int cnt[5] = {0, 0, 0, 0, 0};
   int* defer_cnt = nullptr;
   int* defer_list = nullptr;
+  // launch shape per size class and solver (0 IC-GN, 1 Subpb1)
+  int var[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 };
 
 namespace {
@@ -195,7 +197,7 @@ int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
     a.list = p->lists[c];
     a.cnt = p->cnt[c];
     StageTimer t(ctx, base + c);
-    HIP_TRY(aldic::launch_fast(mode, c, a, s));
+    HIP_TRY(aldic::launch_fast(mode, c, p->var[mode][c], a, s));
   }
   if (p->cnt[4] > 0) {
     const int grid = std::min(p->cnt[4], 2048);
@@ -587,6 +589,94 @@ extern "C" int aldic_local_icgn_dev(aldic_ctx* ctx, const aldic_images* im, cons
   a.zncc = d_zncc;
   a.hole = d_hole;
   return run_solver(ctx, 0, a, plan);
+}
+
+// Autotuner: time every launch shape of every size class on the caller's
+// own nodes and keep the fastest in the plan.  The solve it runs is a real
+// one (outputs are written); subsets handed on are not part of the timing.
+extern "C" int aldic_local_icgn_autotune_dev(aldic_ctx* ctx, const aldic_images* im,
+                                             aldic_plan* plan, const double* d_coords,
+                                             const float* d_U0, const int32_t* d_winsize,
+                                             double tol, float* d_U, float* d_F,
+                                             int32_t* d_convIt, float* d_zncc, int32_t reps,
+                                             int32_t* chosen) {
+  if (!ctx || !images_ok(im) || !plan || plan->rect || reps < 1)
+    return fail(ALDIC_EINVAL, "autotune: bad context/images/plan/reps");
+  if (!d_coords || !d_U0 || !d_winsize || !d_U || !d_convIt)
+    return fail(ALDIC_EINVAL, "autotune: NULL array");
+  RC_TRY(set_device(ctx));
+  SolveArgs a{};
+  a.fm = im->fm;
+  a.fx = im->fx;
+  a.fy = im->fy;
+  a.g = im->g;
+  a.mask = im->mask;
+  a.M = (int)im->M;
+  a.N = (int)im->N;
+  a.n = (int)plan->n;
+  a.coords = d_coords;
+  a.wsx = d_winsize;
+  a.ws_per_node = 1;
+  a.U0 = d_U0;
+  a.tol = (float)tol;
+  a.U = d_U;
+  a.F = d_F;
+  a.conv = d_convIt;
+  a.zncc = d_zncc;
+  a.defer_cnt = plan->defer_cnt;
+  a.defer_list = plan->defer_list;
+  hipStream_t s = ctx->stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  int rc = ALDIC_OK;
+  for (int c = 0; c < 4 && rc == ALDIC_OK; c++) {
+    if (plan->cnt[c] == 0) continue;
+    a.list = plan->lists[c];
+    a.cnt = plan->cnt[c];
+    float best = 1e30f;
+    int bestv = 0;
+    for (int v = 0; v < aldic::num_variants(); v++) {
+      // warm once, then time reps launches
+      zero_ints<<<1, 64, 0, s>>>(plan->defer_cnt, 1);
+      if (aldic::launch_fast(0, c, v, a, s) != hipSuccess) {
+        rc = fail(ALDIC_EDEVICE, "autotune: launch failed");
+        break;
+      }
+      (void)hipEventRecord(e0, s);
+      for (int r = 0; r < reps; r++) {
+        zero_ints<<<1, 64, 0, s>>>(plan->defer_cnt, 1);
+        (void)aldic::launch_fast(0, c, v, a, s);
+      }
+      (void)hipEventRecord(e1, s);
+      if (hipEventSynchronize(e1) != hipSuccess) {
+        rc = fail(ALDIC_EDEVICE, "autotune: synchronize failed");
+        break;
+      }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best) {
+        best = ms;
+        bestv = v;
+      }
+    }
+    plan->var[0][c] = bestv;
+    plan->var[1][c] = bestv;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (chosen)
+    for (int c = 0; c < 4; c++) chosen[c] = plan->var[0][c];
+  return rc;
+}
+
+extern "C" int aldic_plan_set_variant(aldic_plan* plan, int32_t solver, int32_t cls,
+                                      int32_t variant) {
+  if (!plan || solver < 0 || solver > 1 || cls < 0 || cls > 3 || variant < 0 ||
+      variant >= aldic::num_variants())
+    return fail(ALDIC_EINVAL, "plan_set_variant: out of range");
+  plan->var[solver][cls] = variant;
+  return ALDIC_OK;
 }
 
 extern "C" int aldic_subpb1_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,

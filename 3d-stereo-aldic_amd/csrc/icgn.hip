@@ -60,21 +60,25 @@ __device__ __forceinline__ bool chol6(const float h[21], float L[21]) {
   return true;
 }
 
+// Forward and back substitution; the diagonal is used through its
+// hardware reciprocal (v_rcp_f32, 1 ulp), computed once per solve.
 __device__ __forceinline__ void chol6_solve(const float L[21], const float b[6], float x[6]) {
-  float y[6];
+  float y[6], id[6];
+#pragma unroll
+  for (int i = 0; i < 6; i++) id[i] = __builtin_amdgcn_rcpf(L[tri(i, i)]);
 #pragma unroll
   for (int i = 0; i < 6; i++) {
     float s = b[i];
 #pragma unroll
     for (int k = 0; k < i; k++) s -= L[tri(i, k)] * y[k];
-    y[i] = s / L[tri(i, i)];
+    y[i] = s * id[i];
   }
 #pragma unroll
   for (int i = 5; i >= 0; i--) {
     float s = y[i];
 #pragma unroll
     for (int k = i + 1; k < 6; k++) s -= L[tri(k, i)] * x[k];
-    x[i] = s / L[tri(i, i)];
+    x[i] = s * id[i];
   }
 }
 
@@ -251,17 +255,27 @@ __device__ __forceinline__ void node_window(const SolveArgs& a, int node, int* x
 // ---------------------------------------------------------------------------
 // Fast kernel (unmasked branch)
 // ---------------------------------------------------------------------------
+// Block shape: one wave per subset packs 4 subsets in a 256-thread block;
+// several waves per subset take the whole block (barriers stay inside one
+// subset's control flow).
+template <int NW>
+constexpr int fast_block() {
+  return NW == 1 ? 256 : ALDIC_WAVE * NW;
+}
+
 template <int MODE, int NW, int PPL, int TS>
-__global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
+__global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
   constexpr int NT = ALDIC_WAVE * NW;
-  constexpr int SUBS = 256 / NT;
+  constexpr int SUBS = fast_block<NW>() / NT;
+  static_assert(NW == 1 || SUBS == 1, "multi-wave subsets own their block");
   __shared__ float tiles[SUBS][TS * TS];
   __shared__ float red[NW * 32];
   const int sub = threadIdx.x / NT;
   const int tid = threadIdx.x % NT;
   const int slot = blockIdx.x * SUBS + sub;
   if (slot >= a.cnt) return;  // uniform per subset group (SUBS == 1 when NW > 1)
-  const int node = a.list[slot];
+  // every value of one subset is uniform across its waves: keep it scalar
+  const int node = __builtin_amdgcn_readfirstlane(a.list[slot]);
   float* tile = tiles[sub];
   const int M = a.M, N = a.N;
   if (M < TS || N < TS) {  // the tile would not fit in the image
@@ -274,6 +288,10 @@ __global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
 
   int x0, y0, wx, wy;
   node_window(a, node, &x0, &y0, &wx, &wy);
+  x0 = __builtin_amdgcn_readfirstlane(x0);
+  y0 = __builtin_amdgcn_readfirstlane(y0);
+  wx = __builtin_amdgcn_readfirstlane(wx);
+  wy = __builtin_amdgcn_readfirstlane(wy);
   const int hx = wx / 2, hy = wy / 2;
   const int S1 = wx + 1, S2 = wy + 1;
   const int NS = S1 * S2;
@@ -281,8 +299,8 @@ __global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
   float P[6];
   if (MODE == MODE_ICGN) {
     P[0] = P[1] = P[2] = P[3] = 0.0f;
-    P[4] = a.U0[2 * node];
-    P[5] = a.U0[2 * node + 1];
+    P[4] = uni(a.U0[2 * node]);
+    P[5] = uni(a.U0[2 * node + 1]);
   } else {
 #pragma unroll
     for (int k = 0; k < 4; k++) P[k] = a.FOld[4 * node + k];
@@ -295,18 +313,22 @@ __global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
   const bool in_df = x0 - hx >= 4 && x0 + hx <= M - 3 && y0 - hy >= 4 && y0 + hy <= N - 3;
 
   // ---- load the reference subset into registers ----
+  // Pixel p = tid + NT*k.  Slots k < kfull hold a pixel on every lane, so
+  // their sums need no lane mask; the slots of the last, partial round are
+  // masked and their offsets set to the centre so every gather stays inside
+  // the tile.
+  const int kfull = NS / NT;
   float pf[PPL], pgx[PPL], pgy[PPL], pdx[PPL], pdy[PPL], pg[PPL];
-  uint32_t valid = 0;
-  int zf = 0, nanf = 0;
+  int zf = 0, nanf = 0, tinyf = 0;
 #pragma unroll
   for (int k = 0; k < PPL; k++) {
     const int p = tid + NT * k;
     const int aa = p % S1, bb = p / S1;
-    pdx[k] = (float)(aa - hx);
-    pdy[k] = (float)(bb - hy);
+    const bool v = p < NS;
+    pdx[k] = v ? (float)(aa - hx) : 0.0f;
+    pdy[k] = v ? (float)(bb - hy) : 0.0f;
     pf[k] = pgx[k] = pgy[k] = 0.0f;
-    if (p < NS && in_img) {
-      valid |= (1u << k);
+    if (v && in_img) {
       const int64_t idx = (int64_t)(x0 - hx - 1 + aa) + (int64_t)(y0 - hy - 1 + bb) * M;
       pf[k] = a.fm[idx];
       if (in_df) {
@@ -314,14 +336,16 @@ __global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
         pgy[k] = a.fy[idx];
       }
       zf += (pf[k] == 0.0f);
+      tinyf |= (fabsf(pf[k]) <= 1e-10f);
       nanf |= (pf[k] != pf[k]);
     }
   }
   {
-    float v[2] = {(float)zf, (float)nanf};
-    group_sum<NW, 2>(v, red);
+    float v[3] = {(float)zf, (float)nanf, (float)tinyf};
+    group_sum<NW, 3>(v, red);
     zf = (int)v[0];
     nanf = (int)v[1];
+    tinyf = (int)v[2];
   }
   // markCoordHoleStrainOrNot (LocalICGNQuadtree.m:75-86)
   if (MODE == MODE_ICGN && a.hole && tid == 0)
@@ -356,7 +380,10 @@ __global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
     }
     return;
   }
-  if (zf > 0) {  // the reference's mask branch: hand on
+  if (tinyf) {
+    // a zero in f is the reference's mask branch; a value of magnitude
+    // <= 1e-10 leaves the moments' pixel set (:119-120): either way the
+    // generic kernel, which carries both rules, takes the subset
     if (tid == 0) {
       const int q = atomicAdd(a.defer_cnt, 1);
       a.defer_list[q] = node;
@@ -364,34 +391,39 @@ __global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
     return;
   }
 
-  // ---- moments of f over |f| > 1e-10 (:119-120) and sum |f| (:434) ----
+  // ---- moments of f (:119-120) and sum |f| (:434); every pixel counts ----
   float meanf, bottomf, sabs;
   {
-    float v[3] = {0.0f, 0.0f, 0.0f};
+    float v[2] = {0.0f, 0.0f};
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-      const bool use = ((valid >> k) & 1u) && fabsf(pf[k]) > 1e-10f;
-      v[0] += use ? 1.0f : 0.0f;
-      v[1] += use ? pf[k] : 0.0f;
-      v[2] += fabsf(pf[k]);
+      v[0] += pf[k];  // empty slots hold 0
+      v[1] += fabsf(pf[k]);
     }
-    group_sum<NW, 3>(v, red);
-    meanf = v[1] / v[0];
-    sabs = v[2];
-    const float nf = v[0];
+    group_sum<NW, 2>(v, red);
+    meanf = v[0] / (float)NS;
+    sabs = v[1];
     float w[1] = {0.0f};
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-      const bool use = ((valid >> k) & 1u) && fabsf(pf[k]) > 1e-10f;
       const float d = pf[k] - meanf;
-      w[0] += use ? d * d : 0.0f;
+      if (k < kfull)
+        w[0] += d * d;
+      else
+        w[0] += (tid + NT * k < NS) ? d * d : 0.0f;
     }
     group_sum<NW, 1>(w, red);
-    bottomf = (nf > 1.5f) ? sqrtf(w[0]) : (nf > 0.5f ? 0.0f : __builtin_nanf(""));
+    bottomf = (NS > 1) ? sqrtf(w[0]) : 0.0f;
   }
   // fc = (f - meanf) / bottomf, the reference half of the residual
+  {
+    const float ibf = 1.0f / bottomf;
 #pragma unroll
-  for (int k = 0; k < PPL; k++) pf[k] = ((valid >> k) & 1u) ? (pf[k] - meanf) / bottomf : 0.0f;
+    for (int k = 0; k < PPL; k++) {
+      const float fc = (pf[k] - meanf) * ibf;
+      pf[k] = (k < kfull || tid + NT * k < NS) ? fc : 0.0f;
+    }
+  }
 
   // ---- Hessian ----
   float h21[21];
@@ -412,7 +444,11 @@ __global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
         for (int r = c; r < 6; r++) h21[t++] += sd[r] * sd[c];
     }
     group_sum<NW, 21>(h21, red);
+#pragma unroll
+    for (int k = 0; k < 21; k++) h21[k] = uni(h21[k]);
     spd = chol6(h21, L);
+#pragma unroll
+    for (int k = 0; k < 21; k++) L[k] = uni(L[k]);
   } else {
     float v[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -493,76 +529,91 @@ __global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
       else
         __builtin_amdgcn_wave_barrier();
     }
-    // gather + interpolate
-    const float ox = (float)(x0 - 1 - tx0), oy = (float)(y0 - 1 - ty0);
-    int zg = 0;
-    float s1[3] = {0.0f, 0.0f, 0.0f};  // count, sum, sum of squares over |g| > 1e-10
+    // gather + interpolate.  The value is kept 3136 times the interpolant
+    // (unscaled 56x weights on both axes): the residual below only sees
+    // (g - mean g) / sqrt(sum (g - mean g)^2), which is scale free, and the
+    // magnitude test of the reference (|g| <= 1e-10, :272-273) is scaled
+    // with it.
+    constexpr float TINY = 1e-10f * 3136.0f;
+    const float cx = (float)(x0 - 1 - tx0) + P[4], cy = (float)(y0 - 1 - ty0) + P[5];
+    int tiny = 0;
+    float s1[2] = {0.0f, 0.0f};  // sum, sum of squares
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-      const float tx = ox + ((A11 * pdx[k] + A12 * pdy[k]) + P[4]);
-      const float ty = oy + ((A21 * pdx[k] + A22 * pdy[k]) + P[5]);
+      const float tx = fmaf(A11, pdx[k], fmaf(A12, pdy[k], cx));
+      const float ty = fmaf(A21, pdx[k], fmaf(A22, pdy[k], cy));
       const float fxl = floorf(tx), fyl = floorf(ty);
       const float dxf = tx - fxl, dyf = ty - fyl;
-      int ix = (int)fxl - 1, iy = (int)fyl - 1;
-      ix = min(max(ix, 0), TS - 4);
-      iy = min(max(iy, 0), TS - 4);
+      // inside the tile by the bounding-box test above (one pixel to spare)
+      const int ix = (int)fxl - 1, iy = (int)fyl - 1;
       float wr[4], wc[4];
-      spline_weights(dxf, wr);
-      spline_weights(dyf, wc);
-      const float* base = tile + iy * TS + ix;
+      spline_weights56(dxf, wr);
+      spline_weights56(dyf, wc);
+      const float* base = tile + __umul24(iy, TS) + ix;
       float acc = 0.0f;
 #pragma unroll
       for (int c = 0; c < 4; c++) {
         const float* row = base + c * TS;
         const float rsum = wr[0] * row[0] + wr[1] * row[1] + wr[2] * row[2] + wr[3] * row[3];
-        acc += wc[c] * rsum;
+        acc = fmaf(wc[c], rsum, acc);
       }
       pg[k] = acc;
-      if ((valid >> k) & 1u) {
-        zg += (acc == 0.0f);
-        const bool use = fabsf(acc) > 1e-10f;
-        s1[0] += use ? 1.0f : 0.0f;
-        s1[1] += use ? acc : 0.0f;
-        s1[2] += use ? acc * acc : 0.0f;
+      // k < kfull is uniform (a scalar branch); the partial slots mask by
+      // selects, never by a divergent branch
+      if (k < kfull) {
+        s1[0] += acc;
+        s1[1] = fmaf(acc, acc, s1[1]);
+        tiny |= (fabsf(acc) <= TINY);
+      } else {
+        const bool in = tid + NT * k < NS;
+        const float av = in ? acc : 0.0f;
+        s1[0] += av;
+        s1[1] = fmaf(av, av, s1[1]);
+        tiny |= in & (fabsf(acc) <= TINY);
       }
     }
     {
-      float v[4] = {s1[0], s1[1], s1[2], (float)zg};
-      group_sum<NW, 4>(v, red);
+      float v[3] = {s1[0], s1[1], (float)tiny};
+      group_sum<NW, 3>(v, red);
       s1[0] = v[0];
       s1[1] = v[1];
-      s1[2] = v[2];
-      zg = (int)v[3];
+      tiny = (int)v[2];
     }
-    if (zg > 0) {  // mask branch of the reference (:170): hand on
+    if (tiny) {
+      // a zero in g is the reference's mask branch (:166-170); a tiny value
+      // leaves the moments' pixel set: the generic kernel takes the subset
       if (tid == 0) {
         const int q = atomicAdd(a.defer_cnt, 1);
         a.defer_list[q] = node;
       }
       return;
     }
-    const float meang = s1[1] / s1[0];
-    const float varsum = fmaxf(s1[2] - s1[0] * meang * meang, 0.0f);
-    const float bottomg = (s1[0] > 1.5f) ? sqrtf(varsum) : (s1[0] > 0.5f ? 0.0f : __builtin_nanf(""));
+    const float meang = s1[0] / (float)NS;
+    const float varsum = fmaxf(fmaf(-(float)NS * meang, meang, s1[1]), 0.0f);
+    const float bottomg = (NS > 1) ? sqrtf(varsum) : 0.0f;
     const float ig = 1.0f / bottomg;
-    // residual and b (:365-374)
+    const float c2 = meang * ig;
+    // residual r = (f - mean f)/bottom f - (g - mean g)/bottom g and b
+    // (:365-374); empty slots carry zero gradients and contribute to b
+    // nothing, only their r^2 is masked
     float bnorm;
     float bvec[6];
     if (MODE == MODE_ICGN) {
       float v[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
       for (int k = 0; k < PPL; k++) {
-        if ((valid >> k) & 1u) {
-          const float r = pf[k] - (pg[k] - meang) * ig;
-          const float rx = r * pgx[k], ry = r * pgy[k];
-          v[0] += pdx[k] * rx;
-          v[1] += pdx[k] * ry;
-          v[2] += pdy[k] * rx;
-          v[3] += pdy[k] * ry;
-          v[4] += rx;
-          v[5] += ry;
-          v[6] += r * r;
-        }
+        const float r = fmaf(-ig, pg[k], pf[k] + c2);
+        const float rx = r * pgx[k], ry = r * pgy[k];
+        v[0] = fmaf(pdx[k], rx, v[0]);
+        v[1] = fmaf(pdx[k], ry, v[1]);
+        v[2] = fmaf(pdy[k], rx, v[2]);
+        v[3] = fmaf(pdy[k], ry, v[3]);
+        v[4] += rx;
+        v[5] += ry;
+        if (k < kfull)
+          v[6] = fmaf(r, r, v[6]);
+        else
+          v[6] += (tid + NT * k < NS) ? r * r : 0.0f;
       }
       group_sum<NW, 7>(v, red);
       float nb = 0.0f;
@@ -577,12 +628,13 @@ __global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
       float v[3] = {0, 0, 0};
 #pragma unroll
       for (int k = 0; k < PPL; k++) {
-        if ((valid >> k) & 1u) {
-          const float r = pf[k] - (pg[k] - meang) * ig;
-          v[0] += r * pgx[k];
-          v[1] += r * pgy[k];
-          v[2] += r * r;
-        }
+        const float r = fmaf(-ig, pg[k], pf[k] + c2);
+        v[0] = fmaf(r, pgx[k], v[0]);
+        v[1] = fmaf(r, pgy[k], v[1]);
+        if (k < kfull)
+          v[2] = fmaf(r, r, v[2]);
+        else
+          v[2] += (tid + NT * k < NS) ? r * r : 0.0f;
       }
       group_sum<NW, 3>(v, red);
       const float s = 2.0f / (bottomf * bottomf);
@@ -613,6 +665,8 @@ __global__ __launch_bounds__(256) void fast_kernel(SolveArgs a) {
       solve2(tH, -bvec[0], -bvec[1], &dP[4], &dP[5]);
     }
     if (!ic_update(P, dP)) break;
+#pragma unroll
+    for (int k = 0; k < 6; k++) P[k] = uni(P[k]);
   }
   int st;
   if (err) {
@@ -1127,21 +1181,26 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
 // ---------------------------------------------------------------------------
 namespace aldic {
 
-// Size classes of the fast kernel: largest subset side, waves per subset,
-// pixels per lane, LDS tile side.  Every class must hold a full square
-// subset of its largest side (NW * 64 * PPL >= side^2) and a tile with room
-// for that side, the 4x4 cell and a few pixels of warp.
+// Size classes of the fast kernel (largest subset side, LDS tile side) and,
+// per class, the launch shapes the autotuner may choose from (waves per
+// subset NW, pixels per lane PPL).  Every shape must hold a full square
+// subset of its class's largest side (NW * 64 * PPL >= side^2); every tile
+// has room for that side, the 4x4 cell and a few pixels of warp.
 constexpr int CLS_SIDE[4] = {11, 17, 33, 65};
-constexpr int CLS_NW[4] = {1, 1, 1, 4};
-constexpr int CLS_PPL[4] = {2, 5, 18, 17};
 constexpr int CLS_TS[4] = {24, 32, 48, 84};
-static_assert(CLS_NW[0] * 64 * CLS_PPL[0] >= CLS_SIDE[0] * CLS_SIDE[0], "class 0 capacity");
-static_assert(CLS_NW[1] * 64 * CLS_PPL[1] >= CLS_SIDE[1] * CLS_SIDE[1], "class 1 capacity");
-static_assert(CLS_NW[2] * 64 * CLS_PPL[2] >= CLS_SIDE[2] * CLS_SIDE[2], "class 2 capacity");
-static_assert(CLS_NW[3] * 64 * CLS_PPL[3] >= CLS_SIDE[3] * CLS_SIDE[3], "class 3 capacity");
-static_assert(CLS_TS[0] >= CLS_SIDE[0] + 8 && CLS_TS[1] >= CLS_SIDE[1] + 8 &&
-                  CLS_TS[2] >= CLS_SIDE[2] + 8 && CLS_TS[3] >= CLS_SIDE[3] + 8,
-              "tile sides");
+constexpr int NVAR = 3;
+constexpr int VAR_NW[4][NVAR] = {{1, 1, 1}, {1, 2, 2}, {1, 2, 4}, {4, 8, 5}};
+constexpr int VAR_PPL[4][NVAR] = {{2, 2, 2}, {5, 3, 3}, {18, 9, 5}, {17, 9, 14}};
+
+constexpr bool shapes_hold() {
+  for (int c = 0; c < 4; c++) {
+    if (CLS_TS[c] < CLS_SIDE[c] + 8) return false;
+    for (int v = 0; v < NVAR; v++)
+      if (VAR_NW[c][v] * 64 * VAR_PPL[c][v] < CLS_SIDE[c] * CLS_SIDE[c]) return false;
+  }
+  return true;
+}
+static_assert(shapes_hold(), "a launch shape cannot hold its class's largest subset");
 
 int size_class(int S1, int S2) {
   const int s = S1 > S2 ? S1 : S2;
@@ -1150,27 +1209,46 @@ int size_class(int S1, int S2) {
   return 4;  // generic kernel only
 }
 
+int num_variants() { return NVAR; }
+
+template <int MODE, int C, int V>
+static void launch_shape(const SolveArgs& a, hipStream_t s) {
+  constexpr int NW = VAR_NW[C][V];
+  constexpr int block = fast_block<NW>();
+  constexpr int subs = block / (ALDIC_WAVE * NW);
+  fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C]><<<(a.cnt + subs - 1) / subs, block, 0, s>>>(a);
+}
+
 template <int MODE, int C>
-static void launch_class(const SolveArgs& a, hipStream_t s) {
-  constexpr int subs = 4 / CLS_NW[C];
-  fast_kernel<MODE, CLS_NW[C], CLS_PPL[C], CLS_TS[C]><<<(a.cnt + subs - 1) / subs, 256, 0, s>>>(a);
+static void launch_class(int var, const SolveArgs& a, hipStream_t s) {
+  switch (var) {
+    case 1:
+      launch_shape<MODE, C, 1>(a, s);
+      break;
+    case 2:
+      launch_shape<MODE, C, 2>(a, s);
+      break;
+    default:
+      launch_shape<MODE, C, 0>(a, s);
+      break;
+  }
 }
 
 template <int MODE>
-static hipError_t launch_fast_mode(int cls, const SolveArgs& a, hipStream_t s) {
+static hipError_t launch_fast_mode(int cls, int var, const SolveArgs& a, hipStream_t s) {
   if (a.cnt <= 0) return hipSuccess;
   switch (cls) {
     case 0:
-      launch_class<MODE, 0>(a, s);
+      launch_class<MODE, 0>(var, a, s);
       break;
     case 1:
-      launch_class<MODE, 1>(a, s);
+      launch_class<MODE, 1>(var, a, s);
       break;
     case 2:
-      launch_class<MODE, 2>(a, s);
+      launch_class<MODE, 2>(var, a, s);
       break;
     case 3:
-      launch_class<MODE, 3>(a, s);
+      launch_class<MODE, 3>(var, a, s);
       break;
     default:
       return hipErrorInvalidValue;
@@ -1178,8 +1256,9 @@ static hipError_t launch_fast_mode(int cls, const SolveArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_fast(int mode, int cls, const SolveArgs& a, hipStream_t s) {
-  return mode == 0 ? launch_fast_mode<MODE_ICGN>(cls, a, s) : launch_fast_mode<MODE_SUBPB1>(cls, a, s);
+hipError_t launch_fast(int mode, int cls, int var, const SolveArgs& a, hipStream_t s) {
+  return mode == 0 ? launch_fast_mode<MODE_ICGN>(cls, var, a, s)
+                   : launch_fast_mode<MODE_SUBPB1>(cls, var, a, s);
 }
 
 hipError_t launch_generic(int mode, const SolveArgs& a, const int* list, const int* count,

@@ -168,6 +168,13 @@ def main():
         api.local_icgn_dev(im, plan, d_coords.ptr, d_U0.ptr, d_ws.ptr, 1e-3, d_U.ptr, d_F.ptr,
                            None, d_conv.ptr, None, d_z.ptr)
 
+    # plan-time autotune of the IC-GN launch shapes on this node set (not timed)
+    api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst, [r, r, r, r],
+                           *[o.ptr for o in s_out], *[p.ptr for p in s_pk])
+    api.grid_to_nodes_dev(ctx, s_out[2].ptr, s_out[3].ptr, nX, nY, x0, x0, sst, d_coords.ptr, n,
+                          d_U0.ptr)
+    chosen = api.local_icgn_autotune_dev(im, plan, d_coords.ptr, d_U0.ptr, d_ws.ptr, 1e-3, d_U.ptr,
+                                         d_F.ptr, d_conv.ptr, d_z.ptr, reps=3)
     for _ in range(args.warmup):
         step()
     ctx.synchronize()
@@ -247,6 +254,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel_ms": round(dom_ms, 4), "algorithmic_bytes": dom_bytes},
         "gen_s": round(t_gen, 1),
+        "icgn_launch_shapes": [int(c) for c in chosen],
     }
     if not args.no_cpu:
         diag, line["cpu_baseline"] = cpu_baseline(W, args.cpu_sample, args.cpu_threads, U)

@@ -145,6 +145,19 @@ int aldic_local_icgn_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_pla
                          double tol, float* d_U, float* d_F, float* d_H21, int32_t* d_convIt,
                          uint8_t* d_hole, float* d_zncc);
 
+/* Autotuner of the fast IC-GN kernel: per size class, times each launch
+ * shape (waves per subset x pixels per lane; ALDIC_NVARIANTS of them) over
+ * reps launches on the given nodes and stores the fastest in the plan (used
+ * by both solvers).  chosen (4 entries, may be NULL) receives the picks.
+ * The timed launches are real solves: outputs are overwritten. */
+#define ALDIC_NVARIANTS 3
+int aldic_local_icgn_autotune_dev(aldic_ctx* ctx, const aldic_images* im, aldic_plan* plan,
+                                  const double* d_coords, const float* d_U0,
+                                  const int32_t* d_winsize, double tol, float* d_U, float* d_F,
+                                  int32_t* d_convIt, float* d_zncc, int32_t reps, int32_t* chosen);
+/* Fix the launch shape of one size class (solver 0: IC-GN, 1: Subpb1). */
+int aldic_plan_set_variant(aldic_plan* plan, int32_t solver, int32_t cls, int32_t variant);
+
 /* Device-resident Subpb1. */
 int aldic_subpb1_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
                      const double* d_coords, const float* d_UOld, const float* d_FOld,
