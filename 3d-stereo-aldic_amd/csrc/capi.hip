@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/aldic.h"
+#include "aldic_internal.h"
 #include "aldic_kernels.h"
 
 // ---------------------------------------------------------------------------
@@ -17,59 +17,16 @@
 // ---------------------------------------------------------------------------
 static thread_local std::string g_last_error;
 
-static int fail(int code, const std::string& msg) {
+namespace aldic {
+int set_error(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
 }
+}  // namespace aldic
 
-#define HIP_TRY(expr)                                                                  \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess)                                                              \
-      return fail(e_ == hipErrorOutOfMemory ? ALDIC_ENOMEM : ALDIC_EDEVICE,            \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));                  \
-  } while (0)
+static int fail(int code, const std::string& msg) { return aldic::set_error(code, msg); }
 
 extern "C" const char* aldic_last_error(void) { return g_last_error.c_str(); }
-
-// ---------------------------------------------------------------------------
-// objects
-// ---------------------------------------------------------------------------
-struct aldic_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  // reusable staging buffers of the host-pointer entry points
-  std::vector<void*> buf;
-  std::vector<size_t> cap;
-  // per-stage kernel timing (HIP events on the context stream)
-  bool timing = false;
-  std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
-  double stage_ms[ALDIC_NSTAGES] = {0};
-  int stage_cnt[ALDIC_NSTAGES] = {0};
-};
-
-struct aldic_images {
-  aldic_ctx* ctx = nullptr;
-  int64_t M = 0, N = 0;
-  float* fm = nullptr;
-  float* g = nullptr;
-  float* mask = nullptr;
-  float* fx = nullptr;
-  float* fy = nullptr;
-};
-
-struct aldic_plan {
-  aldic_ctx* ctx = nullptr;
-  int64_t n = 0;
-  int rect = 0;
-  int* lists[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  int cnt[5] = {0, 0, 0, 0, 0};
-  int* defer_cnt = nullptr;
-  int* defer_list = nullptr;
-  // launch shape per size class and solver (0 IC-GN, 1 Subpb1)
-  int var[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-};
 
 namespace {
 
@@ -139,11 +96,6 @@ int stage(aldic_ctx* ctx, int slot, size_t count, T** out) {
   return rc;
 }
 
-#define RC_TRY(expr)            \
-  do {                          \
-    const int rc_ = (expr);     \
-    if (rc_ != ALDIC_OK) return rc_; \
-  } while (0)
 
 int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, int rect,
                aldic_plan* p) {
