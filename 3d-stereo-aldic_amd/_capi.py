@@ -64,6 +64,29 @@ SIGNATURES = {
     "aldic_integer_search_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.POINTER(_i64),
                                            C.POINTER(_i64), C.c_int32, C.c_int32,
                                            C.POINTER(_i64), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "aldic_local_icgn_f0_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double, _vp,
+                                          _vp, _vp, _vp, _vp, _vp]),
+    "aldic_cnn_create": (C.c_int, [_vp, C.c_int32, _pp]),
+    "aldic_cnn_destroy": (C.c_int, [_vp]),
+    "aldic_cnn_num_params": (C.c_int, [_vp, C.POINTER(_i64)]),
+    "aldic_cnn_layout": (C.c_int, [_vp, C.c_int32, _vp]),
+    "aldic_cnn_scales": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "aldic_cnn_set_params": (C.c_int, [_vp, _vp, _i64]),
+    "aldic_cnn_get_params": (C.c_int, [_vp, _vp, _i64]),
+    "aldic_cnn_get_grads": (C.c_int, [_vp, _vp, _i64]),
+    "aldic_cnn_set_loss_weights": (C.c_int, [_vp, _vp]),
+    "aldic_cnn_forward_dev": (C.c_int, [_vp, _vp, _i64, _vp]),
+    "aldic_cnn_train_step_dev": (C.c_int, [_vp, _vp, _vp, _i64, C.c_float, C.c_int32, _vp]),
+    "aldic_cnn_patches_dev": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "aldic_cnn_predict_dev": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "aldic_speckle_mosaic_dev": (C.c_int, [_vp, _i64, _i64, C.c_int32, _vp, C.c_uint32,
+                                           C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, C.c_double, _vp]),
+    "aldic_gate_dev": (C.c_int, [_vp, _i64, _vp, _vp, C.c_int32, C.c_double, _vp, _vp]),
+    "aldic_search_nodes_dev": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, C.c_int32,
+                                         C.POINTER(_i64), _vp, _vp]),
+    "aldic_local_icgn_list_dev": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64,
+                                            C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -580,9 +580,42 @@ def grid_to_nodes_dev(ctx: Context, d_u, d_v, nX, nY, x0, y0, step, d_coords, n,
                                         float(step), d_coords, int(n), d_U0))
 
 
+def local_icgn_f0_dev(images: Images, plan: Plan, d_coords, d_U0, d_F0, d_winsize, tol, d_U,
+                      d_F=None, d_H21=None, d_conv=None, d_hole=None, d_zncc=None):
+    """IC-GN from P0 = [F0 U0] (d_F0 None: zeros, as local_icgn_dev)."""
+    check(lib().aldic_local_icgn_f0_dev(images.ctx.ptr, images.ptr, plan.ptr, d_coords, d_U0,
+                                        d_F0, d_winsize, float(tol), d_U, d_F, d_H21, d_conv,
+                                        d_hole, d_zncc))
+
+
+def gate_dev(ctx: Context, n, d_conv, d_zncc, max_iter, min_zncc, d_list, d_count):
+    """List the nodes whose IC-GN took more than max_iter evaluations, did
+    not converge, or ended below min_zncc (count on the device)."""
+    check(lib().aldic_gate_dev(ctx.ptr, int(n), d_conv, d_zncc, int(max_iter), float(min_zncc),
+                               d_list, d_count))
+
+
+def search_nodes_dev(images: Images, d_coords, n, d_list, d_count, max_count, winsize, region,
+                     d_U0, d_F0=None):
+    """Integer search centred on each listed node -> U0 (and F0 = 0)."""
+    rg = (C.c_int64 * 4)(*[int(v) for v in region])
+    check(lib().aldic_search_nodes_dev(images.ctx.ptr, images.ptr, d_coords, int(n), d_list,
+                                       d_count, int(max_count), int(winsize), rg, d_U0, d_F0))
+
+
+def local_icgn_list_dev(images: Images, d_coords, n, d_U0, d_F0, d_winsize, d_list, d_count,
+                        max_count, tol, d_U, d_F=None, d_H21=None, d_conv=None, d_hole=None,
+                        d_zncc=None):
+    """IC-GN of the listed nodes only (outputs of the others untouched)."""
+    check(lib().aldic_local_icgn_list_dev(images.ctx.ptr, images.ptr, d_coords, int(n), d_U0,
+                                          d_F0, d_winsize, d_list, d_count, int(max_count),
+                                          float(tol), d_U, d_F, d_H21, d_conv, d_hole, d_zncc))
+
+
 STAGES = {0: "search", 1: "grid_to_nodes", 2: "icgn_c0", 3: "icgn_c1", 4: "icgn_c2",
           5: "icgn_c3", 6: "icgn_generic", 7: "icgn_handed_on", 8: "subpb1_c0", 9: "subpb1_c1",
-          10: "subpb1_c2", 11: "subpb1_c3", 12: "subpb1_generic", 13: "subpb1_handed_on"}
+          10: "subpb1_c2", 11: "subpb1_c3", 12: "subpb1_generic", 13: "subpb1_handed_on",
+          14: "cnn", 15: "fallback"}
 NSTAGES = 16
 
 

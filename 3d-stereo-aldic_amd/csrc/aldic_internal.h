@@ -42,8 +42,10 @@ struct aldic_ctx {
   int stage_cnt[ALDIC_NSTAGES] = {0};
 };
 
+// Handles record their device so they can be destroyed after their context.
 struct aldic_images {
   aldic_ctx* ctx = nullptr;
+  int device = 0;
   int64_t M = 0, N = 0;
   float* fm = nullptr;
   float* g = nullptr;
@@ -54,6 +56,7 @@ struct aldic_images {
 
 struct aldic_plan {
   aldic_ctx* ctx = nullptr;
+  int device = 0;
   int64_t n = 0;
   int rect = 0;
   int* lists[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -62,4 +65,34 @@ struct aldic_plan {
   int* defer_list = nullptr;
   // launch shape per size class and solver (0 IC-GN, 1 Subpb1)
   int var[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+};
+
+// Stage timing: a pair of events around each launch of a stage on the
+// context stream, resolved when the caller asks for the totals.
+inline hipEvent_t take_event(aldic_ctx* ctx) {
+  if (!ctx->ev_pool.empty()) {
+    hipEvent_t e = ctx->ev_pool.back();
+    ctx->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct StageTimer {
+  aldic_ctx* ctx;
+  int id;
+  hipEvent_t a = nullptr, b = nullptr;
+  StageTimer(aldic_ctx* c, int stage) : ctx(c), id(stage) {
+    if (!ctx->timing) return;
+    a = take_event(ctx);
+    b = take_event(ctx);
+    if (a) (void)hipEventRecord(a, ctx->stream);
+  }
+  ~StageTimer() {
+    if (!ctx->timing || !a || !b) return;
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->pending.push_back({id, {a, b}});
+  }
 };

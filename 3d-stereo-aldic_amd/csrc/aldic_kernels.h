@@ -69,6 +69,11 @@ hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, i
                          double* u, double* v, double* ccmax, int* ipr, int* ipc,
                          hipStream_t s);
 
+hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
+                               const double* coords, int n, const int* list, const int* count,
+                               int grid, int ws, const int reg[4], float* gmap, int64_t kmax,
+                               float* U0, float* F0, hipStream_t s);
+
 // search grid -> node initial guess (prep.hip)
 hipError_t launch_grid_to_nodes(const double* gu, const double* gv, int nX, int nY, double x0,
                                 double y0, double step, const double* coords, int n, float* U0,
@@ -79,4 +84,38 @@ hipError_t launch_interp_f64(const double* F, int64_t M, int64_t N, int64_t P, c
                              const double* Y, int64_t ND, double* Z, hipStream_t s);
 hipError_t launch_interp_f32(const float* F, int64_t M, int64_t N, int64_t P, const float* X,
                              const float* Y, int64_t ND, float* Z, hipStream_t s);
+
+// learned initial guess (cnn.hip); activations and weights are bf16 (void*)
+hipError_t cnn_gemm(const void* A, int lda, const void* Bt, int ldb, int M, int N, int K,
+                    int splits, float* C, int ldc, const float* bias, int relu, void* Cb, int ldcb,
+                    hipStream_t s);
+hipError_t cnn_conv(const void* x, int B, int H, int W, int Cin, int stride, int Ho, int Wo,
+                    const void* Wt, int Kp, int Cout, const float* bias, int relu, void* y,
+                    float* yf, hipStream_t s);
+hipError_t cnn_splitk_reduce(const float* P, int splits, int M, int N, int ldp, const float* bias,
+                             int relu, float* C, int ldc, void* Cb, int ldcb, hipStream_t s);
+hipError_t cnn_im2col(const void* x, int B, int H, int W, int Cin, int stride, int Ho, int Wo,
+                      int Kp, void* col, hipStream_t s);
+hipError_t cnn_col2im(const float* dcol, int B, int H, int W, int Cin, int stride, int Ho, int Wo,
+                      int Kp, float* dx, hipStream_t s);
+hipError_t cnn_transpose_bf16(const void* in, int R, int Cc, int ldi, void* out, int ldo,
+                              hipStream_t s);
+hipError_t cnn_transpose_f32(const float* in, int R, int Cc, int ldi, void* out, int ldo,
+                             hipStream_t s);
+hipError_t cnn_relu_back(const float* g, const void* y, int64_t n, void* gb, float* gf,
+                         hipStream_t s);
+hipError_t cnn_colsum(const float* g, int M, int N, int ldg, float* out, hipStream_t s);
+hipError_t cnn_to_bf16(const float* in, int64_t n, void* out, hipStream_t s);
+hipError_t cnn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                    float b2, float eps, float c1, float c2, float wd, const float* mask,
+                    void* wb, hipStream_t s);
+hipError_t cnn_mse(const float* pred, int ldp, const float* lab, const float* scale, int B,
+                   float* d, int ldd, float* loss, hipStream_t s);
+hipError_t cnn_patches(const float* f, const float* g, int M, int N, const double* coords, int n,
+                       int first, int count, const float* prior, void* out, hipStream_t s);
+hipError_t cnn_predict(const float* out, int ldo, int first, int count, const float* prior,
+                       float sf, float su, float* U0, float* F0, hipStream_t s);
+hipError_t cnn_mosaic(int M, int N, int tile, const float* warps, int ntx, uint32_t seed,
+                      uint32_t noise_seed, float density, float radius, float scale, float noise,
+                      float top, float* img, hipStream_t s);
 }  // namespace aldic

@@ -40,36 +40,6 @@ int set_device(aldic_ctx* ctx) {
   return ALDIC_OK;
 }
 
-// Stage timing: a pair of events around each launch of a stage, resolved
-// when the caller asks for the totals.
-hipEvent_t take_event(aldic_ctx* ctx) {
-  if (!ctx->ev_pool.empty()) {
-    hipEvent_t e = ctx->ev_pool.back();
-    ctx->ev_pool.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
-  return e;
-}
-
-struct StageTimer {
-  aldic_ctx* ctx;
-  int id;
-  hipEvent_t a = nullptr, b = nullptr;
-  StageTimer(aldic_ctx* c, int stage) : ctx(c), id(stage) {
-    if (!ctx->timing) return;
-    a = take_event(ctx);
-    b = take_event(ctx);
-    if (a) (void)hipEventRecord(a, ctx->stream);
-  }
-  ~StageTimer() {
-    if (!ctx->timing || !a || !b) return;
-    (void)hipEventRecord(b, ctx->stream);
-    ctx->pending.push_back({id, {a, b}});
-  }
-};
-
 // staging buffer `slot` of at least `bytes`
 int staging(aldic_ctx* ctx, int slot, size_t bytes, void** out) {
   if ((int)ctx->buf.size() <= slot) {
@@ -96,6 +66,10 @@ int stage(aldic_ctx* ctx, int slot, size_t count, T** out) {
   return rc;
 }
 
+
+// staging slots of the device-resident search entry points
+constexpr int SLOT_GMAP = 20;
+constexpr int SLOT_GMAP_NODES = 21;
 
 int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, int rect,
                aldic_plan* p) {
@@ -365,7 +339,7 @@ static int alloc_planes(aldic_images* im) {
 
 extern "C" int aldic_images_destroy(aldic_images* im) {
   if (!im) return ALDIC_OK;
-  if (im->ctx) (void)hipSetDevice(im->ctx->device);
+  (void)hipSetDevice(im->device);
   float* ps[5] = {im->fm, im->g, im->mask, im->fx, im->fy};
   for (float* p : ps)
     if (p) (void)hipFree(p);
@@ -383,6 +357,7 @@ extern "C" int aldic_images_create(aldic_ctx* ctx, const double* ImgRef, const d
   RC_TRY(set_device(ctx));
   auto* im = new aldic_images();
   im->ctx = ctx;
+  im->device = ctx->device;
   im->M = M;
   im->N = N;
   int rc = alloc_planes(im);
@@ -442,6 +417,7 @@ extern "C" int aldic_images_create_dev(aldic_ctx* ctx, const float* fm, const fl
   RC_TRY(set_device(ctx));
   auto* im = new aldic_images();
   im->ctx = ctx;
+  im->device = ctx->device;
   im->M = M;
   im->N = N;
   int rc = alloc_planes(im);
@@ -487,6 +463,7 @@ extern "C" int aldic_plan_create(aldic_ctx* ctx, const int32_t* winsize, int64_t
   if (n > 0x7fffffff) return fail(ALDIC_EINVAL, "plan: more than 2^31-1 nodes");
   RC_TRY(set_device(ctx));
   auto* p = new aldic_plan();
+  p->device = ctx->device;
   const int rc = build_plan(ctx, winsize, n, winsize_per_node, rect, p);
   if (rc != ALDIC_OK) {
     free_plan(p);
@@ -499,7 +476,7 @@ extern "C" int aldic_plan_create(aldic_ctx* ctx, const int32_t* winsize, int64_t
 
 extern "C" int aldic_plan_destroy(aldic_plan* plan) {
   if (!plan) return ALDIC_OK;
-  if (plan->ctx) (void)hipSetDevice(plan->ctx->device);
+  (void)hipSetDevice(plan->device);
   free_plan(plan);
   delete plan;
   return ALDIC_OK;
@@ -508,6 +485,131 @@ extern "C" int aldic_plan_destroy(aldic_plan* plan) {
 // ---------------------------------------------------------------------------
 // LocalICGN / Subpb1, device-resident
 // ---------------------------------------------------------------------------
+namespace {
+
+SolveArgs icgn_args(const aldic_images* im, int64_t n, const double* d_coords, const float* d_U0,
+                    const float* d_F0, const int32_t* d_winsize, double tol, float* d_U,
+                    float* d_F, float* d_H21, int32_t* d_convIt, uint8_t* d_hole, float* d_zncc) {
+  SolveArgs a{};
+  a.fm = im->fm;
+  a.fx = im->fx;
+  a.fy = im->fy;
+  a.g = im->g;
+  a.mask = im->mask;
+  a.M = (int)im->M;
+  a.N = (int)im->N;
+  a.n = (int)n;
+  a.coords = d_coords;
+  a.wsx = d_winsize;
+  a.wsy = nullptr;
+  a.ws_per_node = 1;
+  a.U0 = d_U0;
+  a.FOld = d_F0;
+  a.tol = (float)tol;
+  a.U = d_U;
+  a.F = d_F;
+  a.H21 = d_H21;
+  a.conv = d_convIt;
+  a.zncc = d_zncc;
+  a.hole = d_hole;
+  return a;
+}
+
+// Gate of the learned starting point: nodes whose IC-GN did not converge
+// within max_iter evaluations or ends below min_zncc are appended to list
+__global__ void gate_kernel(int n, const int32_t* conv, const float* zncc, int max_iter,
+                            float min_zncc, int* list, int* count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int st = conv[i];
+  const float z = zncc ? zncc[i] : 1.0f;
+  // NaN ZNCC fails the gate too
+  if (st < 1 || st > max_iter || !(z >= min_zncc)) list[atomicAdd(count, 1)] = i;
+}
+
+}  // namespace
+
+extern "C" int aldic_local_icgn_f0_dev(aldic_ctx* ctx, const aldic_images* im,
+                                       const aldic_plan* plan, const double* d_coords,
+                                       const float* d_U0, const float* d_F0,
+                                       const int32_t* d_winsize, double tol, float* d_U,
+                                       float* d_F, float* d_H21, int32_t* d_convIt,
+                                       uint8_t* d_hole, float* d_zncc) {
+  if (!ctx || !images_ok(im) || !plan || plan->rect)
+    return fail(ALDIC_EINVAL, "local_icgn_dev: bad context/images/plan");
+  if (plan->n == 0) return ALDIC_OK;
+  if (!d_coords || !d_U0 || !d_winsize || !d_U || !d_convIt)
+    return fail(ALDIC_EINVAL, "local_icgn_dev: NULL array");
+  RC_TRY(set_device(ctx));
+  SolveArgs a = icgn_args(im, plan->n, d_coords, d_U0, d_F0, d_winsize, tol, d_U, d_F, d_H21,
+                          d_convIt, d_hole, d_zncc);
+  return run_solver(ctx, 0, a, plan);
+}
+
+extern "C" int aldic_gate_dev(aldic_ctx* ctx, int64_t n, const int32_t* d_convIt,
+                              const float* d_zncc, int32_t max_iter, double min_zncc,
+                              int32_t* d_list, int32_t* d_count) {
+  if (!ctx || !d_convIt || !d_list || !d_count || n < 0)
+    return fail(ALDIC_EINVAL, "gate_dev: NULL argument");
+  RC_TRY(set_device(ctx));
+  hipStream_t s = ctx->stream;
+  StageTimer t(ctx, ALDIC_STAGE_FALLBACK);
+  zero_ints<<<1, 64, 0, s>>>(d_count, 1);
+  HIP_TRY(hipGetLastError());
+  if (n == 0) return ALDIC_OK;
+  gate_kernel<<<(int)((n + 255) / 256), 256, 0, s>>>((int)n, d_convIt, d_zncc, max_iter,
+                                                      (float)min_zncc, d_list, d_count);
+  HIP_TRY(hipGetLastError());
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_search_nodes_dev(aldic_ctx* ctx, const aldic_images* im,
+                                      const double* d_coords, int64_t n, const int32_t* d_list,
+                                      const int32_t* d_count, int64_t max_count, int32_t winsize,
+                                      const int64_t region[4], float* d_U0, float* d_F0) {
+  if (!ctx || !images_ok(im) || !d_coords || !d_list || !d_count || !d_U0 || !region)
+    return fail(ALDIC_EINVAL, "search_nodes_dev: NULL argument");
+  if (winsize < 0 || (winsize % 2)) return fail(ALDIC_EINVAL, "search_nodes_dev: winsize must be even");
+  for (int k = 0; k < 4; k++)
+    if (region[k] < 0) return fail(ALDIC_EINVAL, "search_nodes_dev: negative search region");
+  if (aldic::search_smem_bytes(winsize) > 160 * 1024)
+    return fail(ALDIC_EINVAL, "search_nodes_dev: winsize too large for one workgroup's LDS");
+  if (max_count <= 0 || n == 0) return ALDIC_OK;
+  RC_TRY(set_device(ctx));
+  const int grid = (int)std::min<int64_t>(max_count, 1024);
+  const int64_t kmax = (region[0] + region[1] + 3) * (region[2] + region[3] + 3);
+  float* gmap;
+  RC_TRY(stage(ctx, SLOT_GMAP_NODES, (size_t)(kmax * grid), &gmap));
+  const int rg[4] = {(int)region[0], (int)region[1], (int)region[2], (int)region[3]};
+  StageTimer t(ctx, ALDIC_STAGE_FALLBACK);
+  HIP_TRY(aldic::launch_search_nodes(im->fm, im->g, (int)im->M, (int)im->N, d_coords, (int)n,
+                                     d_list, d_count, grid, winsize, rg, gmap, kmax, d_U0, d_F0,
+                                     ctx->stream));
+  return ALDIC_OK;
+}
+
+// IC-GN over the flagged nodes d_list[0 .. *d_count) with the workgroup
+// kernel (any subset size); the count is read on the device
+extern "C" int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im,
+                                         const double* d_coords, int64_t n, const float* d_U0,
+                                         const float* d_F0, const int32_t* d_winsize,
+                                         const int32_t* d_list, const int32_t* d_count,
+                                         int64_t max_count, double tol, float* d_U, float* d_F,
+                                         float* d_H21, int32_t* d_convIt, uint8_t* d_hole,
+                                         float* d_zncc) {
+  if (!ctx || !images_ok(im)) return fail(ALDIC_EINVAL, "local_icgn_list_dev: bad context/images");
+  if (!d_coords || !d_U0 || !d_winsize || !d_list || !d_count || !d_U || !d_convIt)
+    return fail(ALDIC_EINVAL, "local_icgn_list_dev: NULL array");
+  if (max_count <= 0 || n == 0) return ALDIC_OK;
+  RC_TRY(set_device(ctx));
+  SolveArgs a = icgn_args(im, n, d_coords, d_U0, d_F0, d_winsize, tol, d_U, d_F, d_H21, d_convIt,
+                          d_hole, d_zncc);
+  const int grid = (int)std::min<int64_t>(max_count, 2048);
+  StageTimer t(ctx, ALDIC_STAGE_FALLBACK);
+  HIP_TRY(aldic::launch_generic(0, a, d_list, d_count, 0, grid, ctx->stream));
+  return ALDIC_OK;
+}
+
 extern "C" int aldic_local_icgn_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
                                     const double* d_coords, const float* d_U0,
                                     const int32_t* d_winsize, double tol, float* d_U, float* d_F,
@@ -795,8 +897,6 @@ int64_t grid_count(int64_t a, int64_t b, int64_t ws, int64_t step) {
   if (step <= 0 || last < a) return 0;
   return (last - a) / step + 1;
 }
-
-constexpr int SLOT_GMAP = 20;
 
 int search_checks(int64_t M, int64_t N, const int64_t gridx[2], const int64_t gridy[2],
                   int32_t ws, int32_t step, const int64_t region[4]) {

@@ -1,0 +1,643 @@
+// cnn.hip -- the learned initial guess: a small CNN that maps a 48 x 48
+// reference/deformed patch pair to the six first-order warp parameters
+// [u_x v_x u_y v_y u v] that IC-GN starts from (no counterpart in the
+// reference; it replaces the integer search as the starting point).
+//
+// Every layer is a GEMM on the matrix cores: convolutions are im2col (NHWC
+// activations, K = 9*Cin padded to a multiple of 16) times the weight matrix
+// W[Cout][K]; the GEMM uses v_mfma_f32_32x32x16_bf16 (BF16 operands, FP32
+// accumulation) with both operands K-contiguous so every lane fetches its
+// 8-element fragment with one 16-byte load.  Backward passes reuse the same
+// GEMM on transposed operands (explicit transposes), split over K when the
+// reduction runs over the batch.
+#include <hip/hip_bf16.h>
+
+#include "aldic_common.h"
+#include "aldic_kernels.h"
+
+namespace {
+
+typedef __bf16 bf16;
+typedef bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ bf16 to_bf16(float v) { return (bf16)v; }
+__device__ __forceinline__ float from_bf16(bf16 v) { return (float)v; }
+
+// ---------------------------------------------------------------------------
+// GEMM  C[M x N] = A[M x K] * Bt[N x K]^T
+//   Bt row stride ldb (elements); K % 16 == 0; ldb a multiple of 8.  One
+//   wave computes 32 rows x 32*NB columns with NB MFMA 32x32x16
+//   accumulators; a 256-thread block stacks 4 waves along M (128 rows).
+//   blockIdx.z splits K into chunks of kchunk (multiple of 16).
+//   Fragment layout of v_mfma_f32_32x32x16_bf16 (wave64):
+//     A: lane l holds A[l % 32][8*(l/32) .. 8*(l/32)+7]
+//     B: lane l holds B[8*(l/32) .. +7][l % 32]  (= Bt[l % 32][same k])
+//     D: element i of lane l is D[8*(i/4) + 4*(l/32) + i%4][l % 32]
+//   A comes from one of three loaders (AMODE):
+//     0  a dense row-major matrix, row stride lda (multiple of 8);
+//     1  implicit im2col of a 3x3 / pad 1 convolution over NHWC bf16
+//        activations with Cin a multiple of 8: row (n, oy, ox), column
+//        (ky*3 + kx)*Cin + ci, so 8 consecutive columns are 8 channels of
+//        one tap -- one 16-byte load or zero padding;
+//     2  the same with Cin = 2: 8 columns are 4 taps x 2 channels.
+//   EPI 0: write FP32 partial sums to C + z*M*ldc;
+//   EPI 1: add bias[n], optional ReLU, write bf16 Cb and/or FP32 C.
+// ---------------------------------------------------------------------------
+struct ConvGeom {
+  int H, W, Cin, stride, Ho, Wo;
+};
+
+template <int AMODE>
+struct ALoader {
+  const bf16* base;
+  int n, iy0, ix0;  // conv: batch index, top-left input coordinate of the row
+  bool ok;
+  __device__ void init(const bf16* A, int lda, int row, bool rowok, const ConvGeom& g, int kh) {
+    ok = rowok;
+    if (AMODE == 0) {
+      base = A + (int64_t)row * lda + kh;
+    } else {
+      const int hw = g.Ho * g.Wo;
+      n = row / hw;
+      const int rem = row - n * hw;
+      const int oy = rem / g.Wo, ox = rem - oy * g.Wo;
+      iy0 = oy * g.stride - 1;
+      ix0 = ox * g.stride - 1;
+      base = A + (int64_t)n * g.H * g.W * g.Cin;
+    }
+  }
+  __device__ bf16x8 load(int k, int kh, const ConvGeom& g) const {
+    bf16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (!ok) return v;
+    if (AMODE == 0) {
+      v = *(const bf16x8*)(base + k);
+    } else if (AMODE == 1) {
+      const int kk = k + kh;
+      const int tap = kk / g.Cin, ci = kk - tap * g.Cin;
+      const int ky = tap / 3, kx = tap - 3 * ky;
+      const int iy = iy0 + ky, ix = ix0 + kx;
+      if (tap < 9 && iy >= 0 && iy < g.H && ix >= 0 && ix < g.W)
+        v = *(const bf16x8*)(base + ((int64_t)iy * g.W + ix) * g.Cin + ci);
+    } else {
+      const int tap0 = (k + kh) / 2;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int tap = tap0 + q;
+        const int ky = tap / 3, kx = tap - 3 * ky;
+        const int iy = iy0 + ky, ix = ix0 + kx;
+        if (tap < 9 && iy >= 0 && iy < g.H && ix >= 0 && ix < g.W) {
+          const bf16* src = base + ((int64_t)iy * g.W + ix) * 2;
+          v[2 * q] = src[0];
+          v[2 * q + 1] = src[1];
+        }
+      }
+    }
+    return v;
+  }
+};
+
+template <int NB, int EPI, int AMODE>
+__global__ __launch_bounds__(256) void gemm_kernel(const bf16* __restrict__ A, int lda,
+                                                   const bf16* __restrict__ Bt, int ldb, int M,
+                                                   int N, int K, int kchunk, float* __restrict__ C,
+                                                   int ldc, const float* __restrict__ bias,
+                                                   int relu, bf16* __restrict__ Cb, int ldcb,
+                                                   ConvGeom geom) {
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int m0 = blockIdx.x * 128 + wave * 32;
+  const int n0 = blockIdx.y * (32 * NB);
+  const int kbeg = blockIdx.z * kchunk;
+  const int kend = min(K, kbeg + kchunk);
+  const int r = lane % 32, kh = 8 * (lane / 32);
+  floatx16 acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++)
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[j][i] = 0.0f;
+  ALoader<AMODE> al;
+  al.init(A, lda, m0 + r, (m0 + r) < M, geom, kh);
+  const bf16* bp[NB];
+  bool bcol[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    bcol[j] = (n0 + 32 * j + r) < N;
+    bp[j] = Bt + (int64_t)(n0 + 32 * j + r) * ldb + kh;
+  }
+  const bf16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int k = kbeg; k < kend; k += 16) {
+    const bf16x8 a = al.load(k, kh, geom);
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const bf16x8 b = bcol[j] ? *(const bf16x8*)(bp[j] + k) : zero;
+      acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[j], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    const int col = n0 + 32 * j + r;
+    if (col >= N) continue;
+    const float bv = (EPI == 1 && bias) ? bias[col] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const int row = m0 + 8 * (i / 4) + 4 * (lane / 32) + (i % 4);
+      if (row >= M) continue;
+      float v = acc[j][i];
+      if (EPI == 0) {
+        C[(int64_t)blockIdx.z * M * ldc + (int64_t)row * ldc + col] = v;
+      } else {
+        v += bv;
+        if (relu) v = fmaxf(v, 0.0f);
+        if (C) C[(int64_t)row * ldc + col] = v;
+        if (Cb) Cb[(int64_t)row * ldcb + col] = to_bf16(v);
+      }
+    }
+  }
+}
+
+// sum of split-K partials (+ bias, ReLU) -> FP32 and/or bf16
+__global__ void splitk_reduce_kernel(const float* __restrict__ P, int splits, int M, int N, int ldp,
+                                     const float* __restrict__ bias, int relu,
+                                     float* __restrict__ C, int ldc, bf16* __restrict__ Cb,
+                                     int ldcb) {
+  const int64_t total = (int64_t)M * N;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int row = (int)(e / N), col = (int)(e % N);
+    float v = 0.0f;
+    for (int z = 0; z < splits; z++) v += P[(int64_t)z * M * ldp + (int64_t)row * ldp + col];
+    if (bias) v += bias[col];
+    if (relu) v = fmaxf(v, 0.0f);
+    if (C) C[(int64_t)row * ldc + col] = v;
+    if (Cb) Cb[(int64_t)row * ldcb + col] = to_bf16(v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// im2col / col2im for 3x3 convolutions, pad 1, NHWC bf16 activations.
+// col row (n, oy, ox), column (ky*3 + kx)*Cin + ci, zero padded to Kp.
+// ---------------------------------------------------------------------------
+__global__ void im2col_kernel(const bf16* __restrict__ x, int B, int H, int W, int Cin, int stride,
+                              int Ho, int Wo, int Kp, bf16* __restrict__ col) {
+  const int64_t total = (int64_t)B * Ho * Wo * Kp;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int kk = (int)(e % Kp);
+    const int64_t m = e / Kp;
+    const int ox = (int)(m % Wo), oy = (int)((m / Wo) % Ho), n = (int)(m / ((int64_t)Wo * Ho));
+    bf16 v = to_bf16(0.0f);
+    if (kk < 9 * Cin) {
+      const int ci = kk % Cin, t = kk / Cin, kx = t % 3, ky = t / 3;
+      const int iy = oy * stride + ky - 1, ix = ox * stride + kx - 1;
+      if (iy >= 0 && iy < H && ix >= 0 && ix < W)
+        v = x[(((int64_t)n * H + iy) * W + ix) * Cin + ci];
+    }
+    col[e] = v;
+  }
+}
+
+// dx(n, y, x, ci) = sum over the (ky, kx) taps that read it of dcol
+__global__ void col2im_kernel(const float* __restrict__ dcol, int B, int H, int W, int Cin,
+                              int stride, int Ho, int Wo, int Kp, float* __restrict__ dx) {
+  const int64_t total = (int64_t)B * H * W * Cin;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int ci = (int)(e % Cin);
+    const int64_t p = e / Cin;
+    const int x = (int)(p % W), y = (int)((p / W) % H), n = (int)(p / ((int64_t)W * H));
+    float s = 0.0f;
+    for (int ky = 0; ky < 3; ky++) {
+      const int ty = y + 1 - ky;
+      if (ty < 0 || ty % stride) continue;
+      const int oy = ty / stride;
+      if (oy >= Ho) continue;
+      for (int kx = 0; kx < 3; kx++) {
+        const int tx = x + 1 - kx;
+        if (tx < 0 || tx % stride) continue;
+        const int ox = tx / stride;
+        if (ox >= Wo) continue;
+        s += dcol[(((int64_t)n * Ho + oy) * Wo + ox) * Kp + (ky * 3 + kx) * Cin + ci];
+      }
+    }
+    dx[e] = s;
+  }
+}
+
+// out[c][r] = in[r][c] (bf16 -> bf16 or FP32 -> bf16), 32 x 32 LDS tiles
+template <typename TI>
+__global__ void transpose_kernel(const TI* __restrict__ in, int R, int Cc, int ldi,
+                                 bf16* __restrict__ out, int ldo) {
+  __shared__ float t[32][33];
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+  for (int i = ty; i < 32; i += 8) {
+    const int r = r0 + i, c = c0 + tx;
+    t[i][tx] = (r < R && c < Cc) ? (float)in[(int64_t)r * ldi + c] : 0.0f;
+  }
+  __syncthreads();
+  for (int i = ty; i < 32; i += 8) {
+    const int c = c0 + i, r = r0 + tx;
+    if (c < Cc && r < R) out[(int64_t)c * ldo + r] = to_bf16(t[tx][i]);
+  }
+}
+
+// g = g * (y > 0) with y the layer's post-ReLU output; FP32 in, bf16 out
+__global__ void relu_back_kernel(const float* __restrict__ g, const bf16* __restrict__ y,
+                                 int64_t n, bf16* __restrict__ gb, float* __restrict__ gf) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const float v = from_bf16(y[e]) > 0.0f ? g[e] : 0.0f;
+    gb[e] = to_bf16(v);
+    if (gf) gf[e] = v;
+  }
+}
+
+// column sums of an M x N FP32 matrix (bias gradients)
+__global__ void colsum_kernel(const float* __restrict__ g, int M, int ldg, float* __restrict__ out) {
+  const int col = blockIdx.x;
+  float s = 0.0f;
+  for (int r = threadIdx.x; r < M; r += blockDim.x) s += g[(int64_t)r * ldg + col];
+  s = wave_sum(s);
+  __shared__ float red[4];
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[col] = red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ void f32_to_bf16_kernel(const float* __restrict__ in, int64_t n, bf16* __restrict__ out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = to_bf16(in[e]);
+}
+
+// Adam on FP32 master weights; refreshes the bf16 copy.  mask (may be
+// null) zeroes the update of padding entries.
+__global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v, int64_t n, float lr,
+                            float b1, float b2, float eps, float c1, float c2, float wd,
+                            const float* __restrict__ mask, bf16* __restrict__ wb) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const float gg = g[e] + wd * w[e];
+    const float mm = b1 * m[e] + (1.0f - b1) * gg;
+    const float vv = b2 * v[e] + (1.0f - b2) * gg * gg;
+    m[e] = mm;
+    v[e] = vv;
+    float upd = lr * (mm * c1) / (sqrtf(vv * c2) + eps);
+    if (mask) upd *= mask[e];
+    const float nw = w[e] - upd;
+    w[e] = nw;
+    if (wb) wb[e] = to_bf16(nw);
+  }
+}
+
+// loss = mean over the batch of sum_k s_k (pred_k - label_k)^2; d = dloss/dpred
+__global__ void mse_kernel(const float* __restrict__ pred, int ldp, const float* __restrict__ lab,
+                           const float* __restrict__ scale, int B, float* __restrict__ d, int ldd,
+                           float* __restrict__ loss) {
+  float s = 0.0f;
+  for (int i = threadIdx.x; i < B * 6; i += blockDim.x) {
+    const int b = i / 6, k = i % 6;
+    const float e = pred[(int64_t)b * ldp + k] - lab[b * 6 + k];
+    s += scale[k] * e * e;
+    d[(int64_t)b * ldd + k] = 2.0f * scale[k] * e / (float)B;
+  }
+  s = wave_sum(s);
+  __shared__ float red[16];
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.0f;
+    for (int k = 0; k < (int)(blockDim.x / 64); k++) t += red[k];
+    *loss = t / (float)B;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Patch pairs: for each node, the 48 x 48 reference patch centred on the
+// node and the deformed patch centred on node + shift (integer prior),
+// each normalised to zero mean and unit variance, as NHWC bf16 [B,48,48,2].
+// Image pixel (i, j) 1-based; patch pixel (py, px) = image
+// (c - 24 + px, c' - 24 + py) along (dim 1, dim 2).  One wave per node.
+// ---------------------------------------------------------------------------
+constexpr int PATCH = 48;
+
+__global__ __launch_bounds__(256) void patch_kernel(const float* __restrict__ f,
+                                                    const float* __restrict__ g, int M, int N,
+                                                    const double* __restrict__ coords, int n,
+                                                    int first, int count,
+                                                    const float* __restrict__ prior,
+                                                    bf16* __restrict__ out) {
+  const int lane = threadIdx.x % 64;
+  const int local = blockIdx.x * 4 + threadIdx.x / 64;
+  if (local >= count) return;
+  const int node = first + local;
+  const int cx = (int)mround(coords[node]), cy = (int)mround(coords[node + n]);
+  int sx = 0, sy = 0;
+  if (prior) {
+    sx = (int)rintf(prior[2 * node]);
+    sy = (int)rintf(prior[2 * node + 1]);
+  }
+  constexpr int NP = PATCH * PATCH;
+  constexpr int PER = NP / 64;  // 36 pixels per lane
+  float vf[PER], vg[PER];
+  float s[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int p = lane + 64 * k;
+    const int px = p % PATCH, py = p / PATCH;
+    int i = cx - PATCH / 2 + px, j = cy - PATCH / 2 + py;  // 1-based
+    i = min(max(i, 1), M);
+    j = min(max(j, 1), N);
+    vf[k] = f[(int64_t)(i - 1) + (int64_t)(j - 1) * M];
+    int i2 = min(max(cx + sx - PATCH / 2 + px, 1), M), j2 = min(max(cy + sy - PATCH / 2 + py, 1), N);
+    vg[k] = g[(int64_t)(i2 - 1) + (int64_t)(j2 - 1) * M];
+    s[0] += vf[k];
+    s[1] += vf[k] * vf[k];
+    s[2] += vg[k];
+    s[3] += vg[k] * vg[k];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) s[q] = wave_sum(s[q]);
+  const float mf = s[0] / NP, mg = s[2] / NP;
+  const float sf = rsqrtf(fmaxf(s[1] / NP - mf * mf, 1e-12f));
+  const float sg = rsqrtf(fmaxf(s[3] / NP - mg * mg, 1e-12f));
+  bf16* o = out + (int64_t)local * NP * 2;
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int p = lane + 64 * k;
+    o[2 * p] = to_bf16((vf[k] - mf) * sf);
+    o[2 * p + 1] = to_bf16((vg[k] - mg) * sg);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Network output -> IC-GN starting point.  out[b][0..5] are the normalised
+// [u_x v_x u_y v_y du dv]; U0 = round(prior) + du*su, F0 = F*sf.
+// ---------------------------------------------------------------------------
+__global__ void predict_kernel(const float* __restrict__ out, int ldo, int first, int count,
+                               const float* __restrict__ prior, float sf, float su,
+                               float* __restrict__ U0, float* __restrict__ F0) {
+  const int local = blockIdx.x * blockDim.x + threadIdx.x;
+  if (local >= count) return;
+  const float* y = out + (int64_t)local * ldo;
+  const int b = first + local;
+  const float px = prior ? rintf(prior[2 * b]) : 0.0f;
+  const float py = prior ? rintf(prior[2 * b + 1]) : 0.0f;
+  U0[2 * b] = px + y[4] * su;
+  U0[2 * b + 1] = py + y[5] * su;
+#pragma unroll
+  for (int k = 0; k < 4; k++) F0[4 * b + k] = y[k] * sf;
+}
+
+// ---------------------------------------------------------------------------
+// Training data: a mosaic of square tiles of seeded speckle.  Spots are
+// generated per 8 x 8 px cell from a counter-based hash (Poisson count with
+// mean density*64, uniform positions), so any pixel can be rendered without
+// a spot list.  The reference image samples the spot field at X = x; in the
+// deformed image every tile has its own affine displacement
+// u(X) = t + A (X - c) about its centre node c, and pixel x samples the
+// field at X = c + (I + A)^-1 (x - c - t).  Grey levels: 20 + 215*min(I/2,1)
+// scaled by `scale`, plus Gaussian noise, rounded and clipped to [0, top].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+__device__ __forceinline__ float hash_unit(uint32_t a, uint32_t b, uint32_t c) {
+  const uint32_t h = mix32(a * 0x9E3779B1u ^ mix32(b + 0x632BE5ABu * c + 0x85EBCA77u));
+  return ((h >> 8) + 0.5f) * (1.0f / 16777216.0f);
+}
+
+constexpr int CELL = 8;
+
+__device__ float spot_field(float X, float Y, uint32_t seed, float lambda, float inv_r2,
+                            float reach) {
+  const int cx0 = (int)floorf((X - reach) / CELL), cx1 = (int)floorf((X + reach) / CELL);
+  const int cy0 = (int)floorf((Y - reach) / CELL), cy1 = (int)floorf((Y + reach) / CELL);
+  const float e0 = __expf(-lambda);
+  float acc = 0.0f;
+  for (int cy = cy0; cy <= cy1; cy++)
+    for (int cx = cx0; cx <= cx1; cx++) {
+      const uint32_t cell = (uint32_t)(cx + 4096) * 16411u + (uint32_t)(cy + 4096);
+      // Poisson count by inversion of the CDF
+      const float uc = hash_unit(seed, cell, 0);
+      int cnt = 0;
+      float pk = e0, cdf = e0;
+      while (uc > cdf && cnt < 12) {
+        cnt++;
+        pk *= lambda / cnt;
+        cdf += pk;
+      }
+      for (int k = 0; k < cnt; k++) {
+        const float sx = (cx + hash_unit(seed, cell, 2 * k + 1)) * CELL;
+        const float sy = (cy + hash_unit(seed, cell, 2 * k + 2)) * CELL;
+        const float dx = X - sx, dy = Y - sy;
+        const float d2 = dx * dx + dy * dy;
+        if (d2 < reach * reach) acc += __expf(-d2 * inv_r2);
+      }
+    }
+  return acc;
+}
+
+__global__ void mosaic_kernel(int M, int N, int tile, const float* __restrict__ warps,
+                              int ntx, uint32_t seed, uint32_t noise_seed, float density,
+                              float radius, float scale, float noise, float top,
+                              float* __restrict__ img) {
+  const int64_t total = (int64_t)M * N;
+  const float lambda = density * CELL * CELL;
+  const float inv_r2 = 1.0f / (radius * radius);
+  const float reach = 4.0f * radius;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % M) + 1, j = (int)(e / M) + 1;  // 1-based pixel
+    float X = (float)i, Y = (float)j;
+    if (warps) {
+      // pixels past the last whole tile belong to the last tile
+      const int tx = min((i - 1) / tile, ntx - 1), ty = min((j - 1) / tile, N / tile - 1);
+      const float* w = warps + 6 * ((int64_t)ty * ntx + tx);
+      const float cx = (float)(tx * tile + tile / 2), cy = (float)(ty * tile + tile / 2);
+      // w = [u_x v_x u_y v_y u v]; (I + A) = [1+u_x u_y; v_x 1+v_y]
+      const float a = 1.0f + w[0], b = w[2], c = w[1], d = 1.0f + w[3];
+      const float det = a * d - b * c;
+      const float rx = X - cx - w[4], ry = Y - cy - w[5];
+      X = cx + (d * rx - b * ry) / det;
+      Y = cy + (-c * rx + a * ry) / det;
+    }
+    const float I = spot_field(X, Y, seed, lambda, inv_r2, reach);
+    float gl = (20.0f + 215.0f * fminf(I * 0.5f, 1.0f)) * scale;
+    // Box-Muller noise
+    const float u1 = hash_unit(noise_seed, (uint32_t)e, 7), u2 = hash_unit(noise_seed, (uint32_t)e, 9);
+    gl += noise * scale * sqrtf(-2.0f * __logf(u1)) * __cosf(6.2831853f * u2);
+    img[e] = fminf(fmaxf(rintf(gl), 0.0f), top);
+  }
+}
+
+int grid_for(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace
+
+namespace aldic {
+
+template <int AMODE>
+static void launch_gemm(dim3 grid, int NB, bool fused, const bf16* a, int lda, const bf16* b,
+                        int ldb, int M, int N, int K, int kchunk, float* C, int ldc,
+                        const float* bias, int relu, bf16* cb, int ldcb, ConvGeom g,
+                        hipStream_t s) {
+#define LAUNCH(NBV, E)                                                                         \
+  gemm_kernel<NBV, E, AMODE><<<grid, 256, 0, s>>>(a, lda, b, ldb, M, N, K, kchunk, C, ldc, bias, \
+                                                  relu, cb, ldcb, g)
+  if (fused) {
+    if (NB == 1) LAUNCH(1, 1);
+    else if (NB == 2) LAUNCH(2, 1);
+    else LAUNCH(4, 1);
+  } else {
+    if (NB == 1) LAUNCH(1, 0);
+    else if (NB == 2) LAUNCH(2, 0);
+    else LAUNCH(4, 0);
+  }
+#undef LAUNCH
+}
+
+static hipError_t gemm_any(int amode, const void* A, int lda, const void* Bt, int ldb, int M,
+                           int N, int K, int splits, float* C, int ldc, const float* bias,
+                           int relu, void* Cb, int ldcb, ConvGeom g, hipStream_t s) {
+  if (K % 16 || ldb % 8 || splits < 1 || (amode == 0 && lda % 8)) return hipErrorInvalidValue;
+  if (amode == 1 && g.Cin % 8) return hipErrorInvalidValue;
+  if (amode == 2 && g.Cin != 2) return hipErrorInvalidValue;
+  const int ksteps = K / 16;
+  const int per = (ksteps + splits - 1) / splits;
+  const int kchunk = per * 16;
+  splits = (ksteps + per - 1) / per;
+  const int NB = N <= 32 ? 1 : (N <= 64 ? 2 : 4);
+  dim3 grid((M + 127) / 128, (N + 32 * NB - 1) / (32 * NB), splits);
+  const bool fused = splits == 1 && (Cb != nullptr || bias != nullptr || relu);
+  const bf16* a = (const bf16*)A;
+  const bf16* b = (const bf16*)Bt;
+  if (amode == 0)
+    launch_gemm<0>(grid, NB, fused, a, lda, b, ldb, M, N, K, kchunk, C, ldc, bias, relu,
+                   (bf16*)Cb, ldcb, g, s);
+  else if (amode == 1)
+    launch_gemm<1>(grid, NB, fused, a, lda, b, ldb, M, N, K, kchunk, C, ldc, bias, relu,
+                   (bf16*)Cb, ldcb, g, s);
+  else
+    launch_gemm<2>(grid, NB, fused, a, lda, b, ldb, M, N, K, kchunk, C, ldc, bias, relu,
+                   (bf16*)Cb, ldcb, g, s);
+  return hipGetLastError();
+}
+
+hipError_t cnn_gemm(const void* A, int lda, const void* Bt, int ldb, int M, int N, int K,
+                    int splits, float* C, int ldc, const float* bias, int relu, void* Cb, int ldcb,
+                    hipStream_t s) {
+  return gemm_any(0, A, lda, Bt, ldb, M, N, K, splits, C, ldc, bias, relu, Cb, ldcb,
+                  ConvGeom{0, 0, 0, 0, 0, 0}, s);
+}
+
+hipError_t cnn_conv(const void* x, int B, int H, int W, int Cin, int stride, int Ho, int Wo,
+                    const void* Wt, int Kp, int Cout, const float* bias, int relu, void* y,
+                    float* yf, hipStream_t s) {
+  const int M = B * Ho * Wo;
+  return gemm_any(Cin == 2 ? 2 : 1, x, 0, Wt, Kp, M, Cout, Kp, 1, yf, Cout, bias, relu, y, Cout,
+                  ConvGeom{H, W, Cin, stride, Ho, Wo}, s);
+}
+
+hipError_t cnn_splitk_reduce(const float* P, int splits, int M, int N, int ldp, const float* bias,
+                             int relu, float* C, int ldc, void* Cb, int ldcb, hipStream_t s) {
+  splitk_reduce_kernel<<<grid_for((int64_t)M * N), 256, 0, s>>>(P, splits, M, N, ldp, bias, relu, C,
+                                                                 ldc, (bf16*)Cb, ldcb);
+  return hipGetLastError();
+}
+
+hipError_t cnn_im2col(const void* x, int B, int H, int W, int Cin, int stride, int Ho, int Wo,
+                      int Kp, void* col, hipStream_t s) {
+  im2col_kernel<<<grid_for((int64_t)B * Ho * Wo * Kp), 256, 0, s>>>(
+      (const bf16*)x, B, H, W, Cin, stride, Ho, Wo, Kp, (bf16*)col);
+  return hipGetLastError();
+}
+
+hipError_t cnn_col2im(const float* dcol, int B, int H, int W, int Cin, int stride, int Ho, int Wo,
+                      int Kp, float* dx, hipStream_t s) {
+  col2im_kernel<<<grid_for((int64_t)B * H * W * Cin), 256, 0, s>>>(dcol, B, H, W, Cin, stride, Ho,
+                                                                    Wo, Kp, dx);
+  return hipGetLastError();
+}
+
+hipError_t cnn_transpose_bf16(const void* in, int R, int Cc, int ldi, void* out, int ldo,
+                              hipStream_t s) {
+  dim3 grid((Cc + 31) / 32, (R + 31) / 32);
+  transpose_kernel<bf16><<<grid, 256, 0, s>>>((const bf16*)in, R, Cc, ldi, (bf16*)out, ldo);
+  return hipGetLastError();
+}
+
+hipError_t cnn_transpose_f32(const float* in, int R, int Cc, int ldi, void* out, int ldo,
+                             hipStream_t s) {
+  dim3 grid((Cc + 31) / 32, (R + 31) / 32);
+  transpose_kernel<float><<<grid, 256, 0, s>>>(in, R, Cc, ldi, (bf16*)out, ldo);
+  return hipGetLastError();
+}
+
+hipError_t cnn_relu_back(const float* g, const void* y, int64_t n, void* gb, float* gf,
+                         hipStream_t s) {
+  relu_back_kernel<<<grid_for(n), 256, 0, s>>>(g, (const bf16*)y, n, (bf16*)gb, gf);
+  return hipGetLastError();
+}
+
+hipError_t cnn_colsum(const float* g, int M, int N, int ldg, float* out, hipStream_t s) {
+  colsum_kernel<<<N, 256, 0, s>>>(g, M, ldg, out);
+  return hipGetLastError();
+}
+
+hipError_t cnn_to_bf16(const float* in, int64_t n, void* out, hipStream_t s) {
+  f32_to_bf16_kernel<<<grid_for(n), 256, 0, s>>>(in, n, (bf16*)out);
+  return hipGetLastError();
+}
+
+hipError_t cnn_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                    float b2, float eps, float c1, float c2, float wd, const float* mask,
+                    void* wb, hipStream_t s) {
+  adam_kernel<<<grid_for(n), 256, 0, s>>>(w, g, m, v, n, lr, b1, b2, eps, c1, c2, wd, mask,
+                                           (bf16*)wb);
+  return hipGetLastError();
+}
+
+hipError_t cnn_mse(const float* pred, int ldp, const float* lab, const float* scale, int B,
+                   float* d, int ldd, float* loss, hipStream_t s) {
+  mse_kernel<<<1, 1024, 0, s>>>(pred, ldp, lab, scale, B, d, ldd, loss);
+  return hipGetLastError();
+}
+
+hipError_t cnn_patches(const float* f, const float* g, int M, int N, const double* coords, int n,
+                       int first, int count, const float* prior, void* out, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (first < 0 || first + count > n) return hipErrorInvalidValue;
+  patch_kernel<<<(count + 3) / 4, 256, 0, s>>>(f, g, M, N, coords, n, first, count, prior,
+                                               (bf16*)out);
+  return hipGetLastError();
+}
+
+hipError_t cnn_predict(const float* out, int ldo, int first, int count, const float* prior,
+                       float sf, float su, float* U0, float* F0, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  predict_kernel<<<(count + 255) / 256, 256, 0, s>>>(out, ldo, first, count, prior, sf, su, U0,
+                                                     F0);
+  return hipGetLastError();
+}
+
+hipError_t cnn_mosaic(int M, int N, int tile, const float* warps, int ntx, uint32_t seed,
+                      uint32_t noise_seed, float density, float radius, float scale, float noise,
+                      float top, float* img, hipStream_t s) {
+  mosaic_kernel<<<grid_for((int64_t)M * N), 256, 0, s>>>(M, N, tile, warps, ntx, seed, noise_seed,
+                                                          density, radius, scale, noise, top, img);
+  return hipGetLastError();
+}
+
+}  // namespace aldic

@@ -298,7 +298,10 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
 
   float P[6];
   if (MODE == MODE_ICGN) {
-    P[0] = P[1] = P[2] = P[3] = 0.0f;
+    // P0 = [0 0 0 0 U0] (funICGNQuadtree.m:41), or [F0 U0] when a first
+    // guess of the gradients is supplied (learned initial guess)
+#pragma unroll
+    for (int k = 0; k < 4; k++) P[k] = a.FOld ? uni(a.FOld[4 * node + k]) : 0.0f;
     P[4] = uni(a.U0[2 * node]);
     P[5] = uni(a.U0[2 * node + 1]);
   } else {
@@ -778,7 +781,7 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
     float P[6];
     float UOld0 = 0.0f, UOld1 = 0.0f, vd0 = 0.0f, vd1 = 0.0f;
     if (MODE == MODE_ICGN) {
-      P[0] = P[1] = P[2] = P[3] = 0.0f;
+      for (int k = 0; k < 4; k++) P[k] = a.FOld ? a.FOld[4 * node + k] : 0.0f;
       P[4] = a.U0[2 * node];
       P[5] = a.U0[2 * node + 1];
     } else {

@@ -35,6 +35,15 @@ struct SearchArgs {
   int64_t kmax;
   double *x, *y, *u, *v, *ccmax;
   int *ipr, *ipc;
+  // node-list mode (list != nullptr): templates centred on the listed
+  // nodes, count read on the device, results written as IC-GN starting
+  // points U0 (and F0 = 0) of those nodes
+  const int* list;
+  const int* lcount;
+  const double* ncoords;
+  int nn;
+  float* U0;
+  float* F0;
 };
 
 __device__ void search_point_params(const SearchArgs& A, int t, int* ii, int* jj, int* L, int* R,
@@ -54,21 +63,45 @@ __device__ void search_point_params(const SearchArgs& A, int t, int* ii, int* jj
   *D = d < 0 ? 0 : d;
 }
 
-__global__ __launch_bounds__(256) void search_kernel(SearchArgs A) {
-  extern __shared__ float smem[];
-  __shared__ float red[8 * 4];
-  __shared__ float bestv[256];
-  __shared__ int besti[256];
-  const int t = blockIdx.x;
+// Template origin and clamped region of a listed node: template centred on
+// the rounded node, clamps as funIntegerSearch.m:139-152 with the whole
+// image as the ROI.  False when the template leaves the image.
+__device__ bool search_node_params(const SearchArgs& A, int node, int* ii, int* jj, int* L, int* R,
+                                   int* U, int* D) {
+  const int h = A.ws / 2;
+  *ii = (int)mround(A.ncoords[node]) - h;
+  *jj = (int)mround(A.ncoords[node + A.nn]) - h;
+  if (*ii < 1 || *jj < 1 || *ii + A.ws > A.M || *jj + A.ws > A.N) return false;
+  int l = A.reg[0], r = A.reg[1], u = A.reg[2], d = A.reg[3];
+  if (*ii - l < 1) l = *ii - 1;
+  if (*jj - u < 1) u = *jj - 1;
+  if (*ii + A.ws + r > A.M) r = A.M - *ii - A.ws - 1;
+  if (*jj + A.ws + d > A.N) d = A.N - *jj - A.ws - 1;
+  *L = l < 0 ? 0 : l;
+  *R = r < 0 ? 0 : r;
+  *U = u < 0 ? 0 : u;
+  *D = d < 0 ? 0 : d;
+  return true;
+}
+
+__device__ void search_one(const SearchArgs& A, int t, int slot, float* smem, float* red,
+                           float* bestv, int* besti) {
   const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
   int ii, jj, L, R, U, D;
-  search_point_params(A, t, &ii, &jj, &L, &R, &U, &D);
+  int node = -1;
+  if (A.list) {
+    node = A.list[t];
+    if (!search_node_params(A, node, &ii, &jj, &L, &R, &U, &D)) return;  // block-uniform
+  } else {
+    search_point_params(A, t, &ii, &jj, &L, &R, &U, &D);
+  }
   const int s = A.ws + 1;
   const int n = s * s;
   const int W1 = s + L + R, W2 = s + U + D;
   const int K1 = W1 - s + 3, K2 = W2 - s + 3;
   const int M = A.M;
-  const int o = (t / A.nX) + (t % A.nX) * A.nY;  // u(cj, ci), nY x nX column-major
+  // u(cj, ci), nY x nX column-major (grid mode only)
+  const int o = A.list ? t : (t / A.nX) + (t % A.nX) * A.nY;
 
   // LDS: template (mean removed), window slice, column sums
   const int SW = KT1 + s - 1;        // slice rows (dim 1)
@@ -97,8 +130,9 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs A) {
   diff = red[1] + red[3] + red[5] + red[7];
   __syncthreads();
   if (diff == 0.0f) {
-    // normxcorr2 throws on a constant template -> NaN (funIntegerSearch.m:171-174)
-    if (tid == 0) {
+    // normxcorr2 throws on a constant template -> NaN (funIntegerSearch.m:171-174);
+    // a listed node keeps its starting point
+    if (tid == 0 && !A.list) {
       const double nan = __builtin_nan("");
       A.u[o] = nan;
       A.v[o] = nan;
@@ -122,7 +156,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs A) {
   __syncthreads();
   const float denT = sqrtf(red[0] + red[1] + red[2] + red[3]);
   const float inv_n = 1.0f / (float)n;
-  float* gmap = A.gmap + (int64_t)t * A.kmax;
+  float* gmap = A.gmap + (int64_t)slot * A.kmax;
 
   float my_best = -1e30f;
   int my_idx = 0x7fffffff;
@@ -264,13 +298,36 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs A) {
         maxf = Ac[0] + xo * Ac[1] + yo * Ac[2] + xo * yo * Ac[3] + xo * xo * Ac[4] + yo * yo * Ac[5];
       }
     }
-    A.u[o] = ypeak - 2 - (double)L;
-    A.v[o] = xpeak - 2 - (double)U;
-    A.ccmax[o] = maxf;
-    A.ipr[o] = yp;
-    A.ipc[o] = xp;
-    A.x[o] = (double)(ii + ii + A.ws) / 2.0;
-    A.y[o] = (double)(jj + jj + A.ws) / 2.0;
+    if (A.list) {
+      A.U0[2 * node] = (float)(ypeak - 2 - (double)L);
+      A.U0[2 * node + 1] = (float)(xpeak - 2 - (double)U);
+      if (A.F0)
+        for (int k = 0; k < 4; k++) A.F0[4 * node + k] = 0.0f;
+    } else {
+      A.u[o] = ypeak - 2 - (double)L;
+      A.v[o] = xpeak - 2 - (double)U;
+      A.ccmax[o] = maxf;
+      A.ipr[o] = yp;
+      A.ipc[o] = xp;
+      A.x[o] = (double)(ii + ii + A.ws) / 2.0;
+      A.y[o] = (double)(jj + jj + A.ws) / 2.0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void search_kernel(SearchArgs A) {
+  extern __shared__ float smem[];
+  __shared__ float red[8 * 4];
+  __shared__ float bestv[256];
+  __shared__ int besti[256];
+  if (A.list) {
+    const int total = *A.lcount;
+    for (int t = blockIdx.x; t < total; t += gridDim.x) {
+      search_one(A, t, blockIdx.x, smem, red, bestv, besti);
+      __syncthreads();
+    }
+  } else {
+    search_one(A, blockIdx.x, blockIdx.x, smem, red, bestv, besti);
   }
 }
 
@@ -284,12 +341,48 @@ size_t search_smem_bytes(int ws) {
   return sizeof(float) * ((size_t)s * s + (size_t)SW * SC + 2 * (size_t)KT2 * SW);
 }
 
+static hipError_t search_attr(int ws, size_t* smem) {
+  *smem = search_smem_bytes(ws);
+  if (*smem > 160 * 1024) return hipErrorInvalidValue;
+  if (*smem > 64 * 1024)
+    return hipFuncSetAttribute((const void*)search_kernel,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
+  return hipSuccess;
+}
+
+hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
+                               const double* coords, int n, const int* list, const int* count,
+                               int grid, int ws, const int reg[4], float* gmap, int64_t kmax,
+                               float* U0, float* F0, hipStream_t s) {
+  SearchArgs A{};
+  A.f = f;
+  A.g = g;
+  A.M = M;
+  A.N = N;
+  A.ws = ws;
+  for (int k = 0; k < 4; k++) A.reg[k] = reg[k];
+  A.gmap = gmap;
+  A.kmax = kmax;
+  A.list = list;
+  A.lcount = count;
+  A.ncoords = coords;
+  A.nn = n;
+  A.U0 = U0;
+  A.F0 = F0;
+  size_t smem = 0;
+  hipError_t e = search_attr(ws, &smem);
+  if (e != hipSuccess) return e;
+  if (grid <= 0) return hipSuccess;
+  search_kernel<<<grid, 256, smem, s>>>(A);
+  return hipGetLastError();
+}
+
 hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, int nY,
                          const int gridx[2], const int gridy[2], int ws, int step,
                          const int reg[4], float* gmap, int64_t kmax, double* x, double* y,
                          double* u, double* v, double* ccmax, int* ipr, int* ipc,
                          hipStream_t s) {
-  SearchArgs A;
+  SearchArgs A{};
   A.f = f;
   A.g = g;
   A.M = M;
@@ -312,13 +405,9 @@ hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, i
   A.ccmax = ccmax;
   A.ipr = ipr;
   A.ipc = ipc;
-  const size_t smem = search_smem_bytes(ws);
-  if (smem > 160 * 1024) return hipErrorInvalidValue;
-  if (smem > 64 * 1024) {
-    hipError_t e = hipFuncSetAttribute((const void*)search_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
-    if (e != hipSuccess) return e;
-  }
+  size_t smem = 0;
+  hipError_t e = search_attr(ws, &smem);
+  if (e != hipSuccess) return e;
   if (nX * nY == 0) return hipSuccess;
   search_kernel<<<nX * nY, 256, smem, s>>>(A);
   return hipGetLastError();

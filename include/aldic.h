@@ -17,6 +17,9 @@
  * Per-subset failures are not errors: they are reported through convIt
  * exactly as the reference reports them (-1 where MATLAB would throw,
  * 101 not converged, 102 >= 40% masked, 103 empty subset).
+ *
+ * Image, plan and network handles may be destroyed before or after the
+ * context that made them; every other call needs a live context.
  */
 #ifndef ALDIC_H
 #define ALDIC_H
@@ -61,6 +64,8 @@ int aldic_ctx_synchronize(aldic_ctx* ctx);
 #define ALDIC_STAGE_ICGN_C0 2      /* +0..3: fast kernel per size class, */
                                    /* +4: generic (> 65 px), +5: handed-on subsets */
 #define ALDIC_STAGE_SUBPB1_C0 8    /* same layout for Subpb1 */
+#define ALDIC_STAGE_CNN 14         /* learned starting point: patches + network */
+#define ALDIC_STAGE_FALLBACK 15    /* gate, node search and IC-GN rerun */
 #define ALDIC_NSTAGES 16
 int aldic_ctx_set_timing(aldic_ctx* ctx, int32_t enable);
 int aldic_ctx_stage_times(aldic_ctx* ctx, double* ms, int32_t* launches, int32_t reset);
@@ -158,6 +163,15 @@ int aldic_local_icgn_autotune_dev(aldic_ctx* ctx, const aldic_images* im, aldic_
 /* Fix the launch shape of one size class (solver 0: IC-GN, 1: Subpb1). */
 int aldic_plan_set_variant(aldic_plan* plan, int32_t solver, int32_t cls, int32_t variant);
 
+/* Same as aldic_local_icgn_dev with a starting guess of the gradients:
+ * P0 = [F0 U0] instead of [0 0 0 0 U0] (funICGNQuadtree.m:41).  d_F0 is 4n
+ * floats [u_x v_x u_y v_y] per node, or NULL for zeros.  The learned
+ * starting point feeds this; it has no counterpart in the reference. */
+int aldic_local_icgn_f0_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
+                            const double* d_coords, const float* d_U0, const float* d_F0,
+                            const int32_t* d_winsize, double tol, float* d_U, float* d_F,
+                            float* d_H21, int32_t* d_convIt, uint8_t* d_hole, float* d_zncc);
+
 /* Device-resident Subpb1. */
 int aldic_subpb1_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
                      const double* d_coords, const float* d_UOld, const float* d_FOld,
@@ -193,6 +207,72 @@ int aldic_integer_search_dev(aldic_ctx* ctx, const float* d_f, const float* d_g,
                              int32_t winsize, int32_t winstepsize, const int64_t region[4],
                              double* d_x, double* d_y, double* d_u, double* d_v,
                              double* d_ccmax, int32_t* d_ipeak_row, int32_t* d_ipeak_col);
+
+/* ---------------------------------------------------------------------------
+ * Learned starting point (no counterpart in the reference: it stands beside
+ * IntegerSearchQuadtree as the source of IC-GN's P0) and its fallback.
+ * ------------------------------------------------------------------------- */
+typedef struct aldic_cnn aldic_cnn;
+
+/* A network with room for batches of up to max_batch (multiple of 16)
+ * 48 x 48 patch pairs.  Parameters start at zero: load them with
+ * aldic_cnn_set_params. */
+int aldic_cnn_create(aldic_ctx* ctx, int32_t max_batch, aldic_cnn** out);
+int aldic_cnn_destroy(aldic_cnn* net);
+/* FP32 parameter vector: per layer W [cout][Kp] (K padded to 16) then bias.
+ * The layout is fixed by the architecture: net may be NULL. */
+int aldic_cnn_num_params(const aldic_cnn* net, int64_t* n);
+/* info = {w_off, b_off, cout, K, Kp, cin, stride, input side, is_conv};
+ * net may be NULL. */
+int aldic_cnn_layout(const aldic_cnn* net, int32_t layer, int64_t info[9]);
+/* Output units: outputs are [u_x v_x u_y v_y]/sf and [du dv]/su. */
+int aldic_cnn_scales(double* sf, double* su);
+int aldic_cnn_set_params(aldic_cnn* net, const float* host, int64_t n);  /* resets Adam */
+int aldic_cnn_get_params(aldic_cnn* net, float* host, int64_t n);
+int aldic_cnn_get_grads(aldic_cnn* net, float* host, int64_t n);
+int aldic_cnn_set_loss_weights(aldic_cnn* net, const float w[6]);
+/* d_x: B x 48 x 48 x 2 bf16 (NHWC); d_out: B x 6 FP32 (may be NULL). */
+int aldic_cnn_forward_dev(aldic_cnn* net, const void* d_x, int64_t B, float* d_out);
+/* One step of mean-squared error on normalised labels (B x 6 FP32):
+ * forward, backward into the gradient buffer and, when update != 0, Adam
+ * with rate lr.  d_loss (device float, may be NULL) receives the loss. */
+int aldic_cnn_train_step_dev(aldic_cnn* net, const void* d_x, const float* d_labels, int64_t B,
+                             float lr, int32_t update, float* d_loss);
+/* Patch pairs of the nodes: reference at the rounded node, deformed at the
+ * node + round(d_prior) (2n floats, NULL = 0), each normalised. */
+int aldic_cnn_patches_dev(aldic_ctx* ctx, const aldic_images* im, const double* d_coords,
+                          int64_t n, const float* d_prior, void* d_out);
+/* Patches + network + output scaling: d_U0 = round(prior) + du, d_F0 = F. */
+int aldic_cnn_predict_dev(aldic_cnn* net, const aldic_images* im, const double* d_coords,
+                          int64_t n, const float* d_prior, float* d_U0, float* d_F0);
+/* Training images: seeded speckle mosaic of tile x tile tiles, each tile of
+ * the deformed image under its own affine displacement about its centre
+ * node (d_warps: 6 floats [u_x v_x u_y v_y u v] per tile, tiles along dim 1
+ * first; NULL = reference image). */
+int aldic_speckle_mosaic_dev(aldic_ctx* ctx, int64_t M, int64_t N, int32_t tile,
+                             const float* d_warps, uint32_t seed, uint32_t noise_seed,
+                             double density, double radius, double scale, double noise,
+                             double top, float* d_img);
+/* Gate: nodes with convIt outside [1, max_iter] or ZNCC below min_zncc
+ * (d_zncc may be NULL) are listed in d_list; the count goes to d_count. */
+int aldic_gate_dev(aldic_ctx* ctx, int64_t n, const int32_t* d_convIt, const float* d_zncc,
+                   int32_t max_iter, double min_zncc, int32_t* d_list, int32_t* d_count);
+/* Integer search (funIntegerSearch.m:139-203 per node, the whole image as
+ * ROI) of the template (winsize+1)^2 centred on each listed node; writes
+ * U0 (and F0 = 0, d_F0 may be NULL) of those nodes.  At most max_count
+ * entries; the count is read on the device. */
+int aldic_search_nodes_dev(aldic_ctx* ctx, const aldic_images* im, const double* d_coords,
+                           int64_t n, const int32_t* d_list, const int32_t* d_count,
+                           int64_t max_count, int32_t winsize, const int64_t region[4],
+                           float* d_U0, float* d_F0);
+/* IC-GN (funICGNQuadtree.m) of the listed nodes only, from [F0 U0]; other
+ * nodes' outputs are left as they are. */
+int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im, const double* d_coords,
+                              int64_t n, const float* d_U0, const float* d_F0,
+                              const int32_t* d_winsize, const int32_t* d_list,
+                              const int32_t* d_count, int64_t max_count, double tol, float* d_U,
+                              float* d_F, float* d_H21, int32_t* d_convIt, uint8_t* d_hole,
+                              float* d_zncc);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,256 @@
+"""GPU checks of the learned initial guess through the C ABI.
+
+  * forward pass and gradients of the BF16 network against the FP64 numpy
+    oracle (oracle/cnn_oracle.py) evaluated with the same bfloat16 rounding
+    points: relative Frobenius error <= 1e-2 on the outputs and <= 2e-2 per
+    parameter tensor on the gradients (FP32 accumulation order and rare
+    bfloat16 rounding flips are the only differences);
+  * one Adam step against its closed form (first step: w - lr*g/(|g|+eps));
+  * patch extraction against numpy (bfloat16 resolution, 1e-2);
+  * the training-image renderer against IC-GN: labels of the tile warps
+    recovered to 0.02 px / 1e-3;
+  * gate, node search and list IC-GN against their host restatements;
+  * a short training run: the trained network predicts held-out warps far
+    better than a constant, and IC-GN started from it needs fewer
+    evaluations than IC-GN started from the prior alone.
+"""
+import numpy as np
+import pytest
+
+import cnn_oracle as co
+from _cases import c1_grid, c1_pair
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def L(pkg):
+    from aldic_amd import api, learned
+    return api, learned
+
+
+def _upload_bf16(api, ctx, x):
+    from aldic_amd import learned
+    return api.DeviceArray.from_host(ctx, learned.to_bf16_bits(x).reshape(-1), np.uint16)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _params(learned, seed):
+    rng = np.random.default_rng(seed)
+    p = learned.he_init(seed)
+    lay = learned.layout()
+    last = lay[-1]
+    p[last["w_off"]:last["w_off"] + last["cout"] * last["Kp"]] *= 10.0
+    for Lr in lay:
+        p[Lr["b_off"]:Lr["b_off"] + Lr["cout"]] = rng.normal(0, 0.05, Lr["cout"])
+    return p
+
+
+def test_forward_matches_oracle(L):
+    api, learned = L
+    ctx = api.default_context()
+    B = 48
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(B, 48, 48, 2)).astype(np.float32)
+    p = _params(learned, 1)
+    net = learned.Network(ctx, max_batch=64, params=p)
+    dx = _upload_bf16(api, ctx, x)
+    out = api.DeviceArray(ctx, B * 6, np.float32)
+    net.forward_dev(dx.ptr, B, out.ptr)
+    got = out.to_host().reshape(B, 6)
+    want, _ = co.forward(p.astype(np.float64), x.astype(np.float64), quant=True)
+    print("forward rel err", _rel(got, want))
+    assert _rel(got, want) < 1e-2
+    net.close()
+
+
+def test_gradients_match_oracle(L):
+    api, learned = L
+    ctx = api.default_context()
+    B = 32
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(B, 48, 48, 2)).astype(np.float32)
+    lab = rng.normal(size=(B, 6)).astype(np.float32)
+    p = _params(learned, 3)
+    net = learned.Network(ctx, max_batch=64, params=p)
+    dx = _upload_bf16(api, ctx, x)
+    dl = api.DeviceArray.from_host(ctx, lab.reshape(-1), np.float32)
+    dloss = api.DeviceArray(ctx, 1, np.float32)
+    net.train_step_dev(dx.ptr, dl.ptr, B, 0.0, update=False, d_loss=dloss.ptr)
+    g = net.get_grads().astype(np.float64)
+    loss_o, g_o, _ = co.loss_and_grads(p.astype(np.float64), x.astype(np.float64),
+                                       lab.astype(np.float64), quant=True)
+    assert float(dloss.to_host()[0]) == pytest.approx(loss_o, rel=1e-2)
+    for i, Lr in enumerate(learned.layout()):
+        w = slice(Lr["w_off"], Lr["w_off"] + Lr["cout"] * Lr["Kp"])
+        b = slice(Lr["b_off"], Lr["b_off"] + Lr["cout"])
+        ew, eb = _rel(g[w], g_o[w]), _rel(g[b], g_o[b])
+        print(f"layer {i}: dW rel {ew:.2e} db rel {eb:.2e}")
+        assert ew < 2e-2 and eb < 2e-2, i
+    # padding entries of W stay exactly zero
+    used = np.zeros_like(g, dtype=bool)
+    for Lr in learned.layout():
+        for r in range(Lr["cout"]):
+            used[Lr["w_off"] + r * Lr["Kp"]:Lr["w_off"] + r * Lr["Kp"] + Lr["K"]] = True
+        used[Lr["b_off"]:Lr["b_off"] + Lr["cout"]] = True
+    assert np.all(g[~used] == 0.0)
+    # one Adam step from zero moments: w - lr * g / (|g| + eps)
+    lr = 1e-3
+    net.train_step_dev(dx.ptr, dl.ptr, B, lr, update=True)
+    g1 = net.get_grads().astype(np.float64)
+    w1 = net.get_params().astype(np.float64)
+    want = p.astype(np.float64) - lr * g1 / (np.abs(g1) + 1e-8)
+    assert np.max(np.abs(w1 - want)) < 1e-6
+    net.close()
+
+
+def test_patches_match_numpy(L, pkg):
+    api, learned = L
+    ctx = api.default_context()
+    f, g, _, _ = c1_pair()
+    coords = c1_grid()[0][:40]
+    n = len(coords)
+    prior = np.random.default_rng(5).uniform(-6, 6, (n, 2)).astype(np.float32)
+    im = pkg.Images(f, g)
+    cF = np.asfortranarray(coords).reshape(-1, order="F")
+    dc = api.DeviceArray.from_host(ctx, cF, np.float64)
+    dp = api.DeviceArray.from_host(ctx, prior.reshape(-1), np.float32)
+    out = api.DeviceArray(ctx, n * 48 * 48, np.float32)  # n x 48 x 48 x 2 bf16
+    learned.patches_dev(im, dc.ptr, n, dp.ptr, out.ptr)
+    got = learned.from_bf16_bits(out.to_host().view(np.uint16)).reshape(n, 48, 48, 2)
+    M, N = f.shape
+    for k in range(n):
+        cx, cy = int(np.round(coords[k, 0])), int(np.round(coords[k, 1]))
+        sx, sy = int(np.rint(prior[k, 0])), int(np.rint(prior[k, 1]))
+        ii = np.clip(cx - 24 + np.arange(48), 1, M) - 1
+        jj = np.clip(cy - 24 + np.arange(48), 1, N) - 1
+        ii2 = np.clip(cx + sx - 24 + np.arange(48), 1, M) - 1
+        jj2 = np.clip(cy + sy - 24 + np.arange(48), 1, N) - 1
+        for c, (img, a, b) in enumerate(((f, ii, jj), (g, ii2, jj2))):
+            P = img[np.ix_(a, b)].T  # rows = py (dim 2), columns = px (dim 1)
+            P = (P - P.mean()) / P.std()
+            assert np.max(np.abs(got[k, :, :, c] - P)) < 2e-2, (k, c)
+    im.close()
+
+
+def test_mosaic_labels_recover_tile_warps(L):
+    api, learned = L
+    ctx = api.default_context()
+    src = learned.MosaicSource(ctx, tiles=12, seed=7)
+    st = src.make(0)
+    print({k: v for k, v in st.items() if np.isscalar(v)})
+    assert st["kept"] >= 0.95 * st["total"]
+    assert st["label_rms_u"] < 0.02
+    # the normalised labels are what the warps say, to IC-GN accuracy
+    sf, su = learned.scales()
+    assert np.max(np.abs(st["labels"][:, 4:] * su - st["truth_u"])) < 0.1
+    assert np.max(np.abs(st["labels"][:, :4] * sf - st["truth_f"])) < 5e-3
+    src.close()
+
+
+def test_gate_search_and_list_icgn(L, pkg, orc):
+    api, learned = L
+    ctx = api.default_context()
+    f, g, _, _ = c1_pair()
+    coords = c1_grid()[0]
+    n = len(coords)
+    s0 = orc.integer_search(f, g, [1, 512], [1, 512], 30, 16, [10, 10, 10, 10], nthreads=8)
+    U0 = np.zeros(2 * n)
+    U0[0::2] = s0["u"].T.reshape(-1)
+    U0[1::2] = s0["v"].T.reshape(-1)
+    # gate against its definition on synthetic statuses
+    rng = np.random.default_rng(3)
+    conv = rng.choice([-1, 1, 2, 5, 29, 30, 31, 60, 101, 102, 103], n).astype(np.int32)
+    zn = rng.uniform(0.8, 1.0, n).astype(np.float32)
+    zn[::17] = np.nan
+    dconv = api.DeviceArray.from_host(ctx, conv, np.int32)
+    dzn = api.DeviceArray.from_host(ctx, zn, np.float32)
+    dlist = api.DeviceArray(ctx, n, np.int32)
+    dcnt = api.DeviceArray(ctx, 1, np.int32)
+    api.gate_dev(ctx, n, dconv.ptr, dzn.ptr, 30, 0.9, dlist.ptr, dcnt.ptr)
+    cnt = int(dcnt.to_host()[0])
+    want = np.nonzero((conv < 1) | (conv > 30) | ~(zn >= 0.9))[0]
+    assert cnt == len(want)
+    assert np.array_equal(np.sort(dlist.to_host()[:cnt]), want)
+    # node search: the integer search of a template centred on each node
+    im = pkg.Images(f, g)
+    sel = np.arange(0, n, 7).astype(np.int32)
+    nodes = coords.copy()
+    # x reaches both image edges (the region clamps); y stays where no
+    # clamp applies, so a one-point oracle grid can be built below
+    nodes[:, 0] = np.clip(nodes[:, 0], 20, 490)
+    nodes[:, 1] = np.clip(nodes[:, 1], 26, 486)
+    cF = np.asfortranarray(nodes).reshape(-1, order="F")
+    dc = api.DeviceArray.from_host(ctx, cF, np.float64)
+    dsel = api.DeviceArray.from_host(ctx, sel, np.int32)
+    dscnt = api.DeviceArray.from_host(ctx, np.array([len(sel)], np.int32), np.int32)
+    dU0 = api.DeviceArray.from_host(ctx, np.full(2 * n, 77.0, np.float32), np.float32)
+    dF0 = api.DeviceArray.from_host(ctx, np.full(4 * n, 5.0, np.float32), np.float32)
+    api.search_nodes_dev(im, dc.ptr, n, dsel.ptr, dscnt.ptr, n, 30, [10, 10, 10, 10], dU0.ptr,
+                         dF0.ptr)
+    U0g = dU0.to_host().reshape(-1, 2)
+    F0g = dF0.to_host().reshape(-1, 4)
+    ws = 30
+    for k in sel:
+        ii = int(np.round(nodes[k, 0])) - ws // 2
+        jj = int(np.round(nodes[k, 1])) - ws // 2
+        # the same search through the oracle: a one-point grid at (ii, jj)
+        # whose x ROI ends at the image edge (XList = ii:step:512-ws-step)
+        step = 512 - ws - ii
+        s1 = orc.integer_search(f, g, [ii, 512], [jj, jj + ws + step], ws, step,
+                                [10, 10, 10, 10])
+        assert s1["u"].shape == (1, 1)
+        assert abs(U0g[k, 0] - s1["u"][0, 0]) <= 1e-3 + 1e-6
+        assert abs(U0g[k, 1] - s1["v"][0, 0]) <= 1e-3 + 1e-6
+        assert np.all(F0g[k] == 0.0)
+    untouched = np.setdiff1d(np.arange(n), sel)
+    assert np.all(U0g[untouched] == 77.0) and np.all(F0g[untouched] == 5.0)
+    # list IC-GN over every node equals the plan IC-GN (same semantics,
+    # workgroup kernel instead of the wave kernels)
+    dc2 = api.DeviceArray.from_host(ctx, np.asfortranarray(coords).reshape(-1, order="F"),
+                                    np.float64)
+    dU0b = api.DeviceArray.from_host(ctx, np.nan_to_num(U0).reshape(-1), np.float32)
+    dws = api.DeviceArray.from_host(ctx, np.full(n, 30, np.int32), np.int32)
+    outs = []
+    for mode in ("plan", "list"):
+        dU = api.DeviceArray(ctx, 2 * n, np.float32)
+        dF = api.DeviceArray(ctx, 4 * n, np.float32)
+        dcv = api.DeviceArray(ctx, n, np.int32)
+        dz = api.DeviceArray(ctx, n, np.float32)
+        if mode == "plan":
+            plan = api.Plan(30, n, ctx=ctx)
+            api.local_icgn_f0_dev(im, plan, dc2.ptr, dU0b.ptr, None, dws.ptr, 1e-3, dU.ptr,
+                                  dF.ptr, d_conv=dcv.ptr, d_zncc=dz.ptr)
+        else:
+            allv = api.DeviceArray.from_host(ctx, np.arange(n, dtype=np.int32), np.int32)
+            cntv = api.DeviceArray.from_host(ctx, np.array([n], np.int32), np.int32)
+            api.local_icgn_list_dev(im, dc2.ptr, n, dU0b.ptr, None, dws.ptr, allv.ptr, cntv.ptr,
+                                    n, 1e-3, dU.ptr, dF.ptr, d_conv=dcv.ptr, d_zncc=dz.ptr)
+        outs.append((dU.to_host(), dF.to_host(), dcv.to_host()))
+    (Ua, Fa, ca), (Ub, Fb, cb) = outs
+    ok = (ca >= 1) & (ca <= 100) & (cb >= 1) & (cb <= 100)
+    assert np.mean(ca != cb) <= 0.03
+    assert np.nanmax(np.abs(Ua - Ub).reshape(-1, 2)[ok]) <= 1e-3
+    im.close()
+
+
+def test_training_gains_over_prior_alone(L):
+    """A short training run; the network must beat a constant prediction
+    by a wide margin and cut IC-GN's evaluations on held-out subsets."""
+    api, learned = L
+    ctx = api.default_context()
+    net = learned.Network(ctx, max_batch=256, seed=0)
+    hist = learned.train(net, samples=150_000, batch=256, lr=2e-3, tiles=48, seed=11,
+                         log_every=100, log=print)
+    ev = learned.evaluate(net, seed=12345, tiles=24)
+    print("eval", ev, "train s", hist["seconds"])
+    # a constant prediction of a uniform +-12 px translation has RMS ~9.8 px
+    assert ev["rms_u_px"] < 1.5
+    # IC-GN from the network vs IC-GN from the prior alone (U0 = 0, F0 = 0)
+    res = learned.icgn_gain(net, seed=4321, tiles=24)
+    print(res)
+    assert res["learned"]["good"] > res["prior"]["good"] + 0.2
+    net.close()
