@@ -30,6 +30,7 @@ struct SolveArgs {
   const float* vdual;   // Subpb1: 2n
   float mu;
   float tol;
+  int maxit;            // evaluation cap k <= maxit (0: the reference's 100)
   float* U;             // 2n
   float* F;             // 4n (IC-GN), may be null
   float* H21;           // n x 21 column-major (IC-GN), may be null
@@ -61,18 +62,17 @@ hipError_t launch_gradient_f32(const float* ref, const float* mask, int64_t M, i
 hipError_t launch_gradient_out_f64(const double* ref, const double* mask, int64_t M, int64_t N,
                                    double* DfDx, double* DfDy, hipStream_t s);
 
-// integer search (search.hip)
+// integer search (search.hip); scratch of search_scratch_bytes(points, ...)
 size_t search_smem_bytes(int ws);
+size_t search_scratch_bytes(int maxpts, int ws, const int reg[4]);
 hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, int nY,
                          const int gridx[2], const int gridy[2], int ws, int step,
-                         const int reg[4], float* gmap, int64_t kmax, double* x, double* y,
-                         double* u, double* v, double* ccmax, int* ipr, int* ipc,
-                         hipStream_t s);
-
+                         const int reg[4], void* scratch, double* x, double* y, double* u,
+                         double* v, double* ccmax, int* ipr, int* ipc, hipStream_t s);
 hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
                                const double* coords, int n, const int* list, const int* count,
-                               int grid, int ws, const int reg[4], float* gmap, int64_t kmax,
-                               float* U0, float* F0, hipStream_t s);
+                               int max_count, int ws, const int reg[4], void* scratch, float* U0,
+                               float* F0, hipStream_t s);
 
 // search grid -> node initial guess (prep.hip)
 hipError_t launch_grid_to_nodes(const double* gu, const double* gv, int nX, int nY, double x0,
@@ -92,6 +92,8 @@ hipError_t cnn_gemm(const void* A, int lda, const void* Bt, int ldb, int M, int 
 hipError_t cnn_conv(const void* x, int B, int H, int W, int Cin, int stride, int Ho, int Wo,
                     const void* Wt, int Kp, int Cout, const float* bias, int relu, void* y,
                     float* yf, hipStream_t s);
+hipError_t cnn_dense(const void* x, int B, int K, const void* Wt, int Cout, const float* bias,
+                     int relu, void* y, hipStream_t s);
 hipError_t cnn_splitk_reduce(const float* P, int splits, int M, int N, int ldp, const float* bias,
                              int relu, float* C, int ldc, void* Cb, int ldcb, hipStream_t s);
 hipError_t cnn_im2col(const void* x, int B, int H, int W, int Cin, int stride, int Ho, int Wo,

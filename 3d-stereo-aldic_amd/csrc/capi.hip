@@ -68,8 +68,8 @@ int stage(aldic_ctx* ctx, int slot, size_t count, T** out) {
 
 
 // staging slots of the device-resident search entry points
-constexpr int SLOT_GMAP = 20;
-constexpr int SLOT_GMAP_NODES = 21;
+constexpr int SLOT_SEARCH = 20;
+constexpr int SLOT_SEARCH_NODES = 21;
 
 int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, int rect,
                aldic_plan* p) {
@@ -532,9 +532,11 @@ __global__ void gate_kernel(int n, const int32_t* conv, const float* zncc, int m
 extern "C" int aldic_local_icgn_f0_dev(aldic_ctx* ctx, const aldic_images* im,
                                        const aldic_plan* plan, const double* d_coords,
                                        const float* d_U0, const float* d_F0,
-                                       const int32_t* d_winsize, double tol, float* d_U,
-                                       float* d_F, float* d_H21, int32_t* d_convIt,
+                                       const int32_t* d_winsize, double tol, int32_t max_iter,
+                                       float* d_U, float* d_F, float* d_H21, int32_t* d_convIt,
                                        uint8_t* d_hole, float* d_zncc) {
+  if (max_iter < 0 || max_iter > 100)
+    return fail(ALDIC_EINVAL, "local_icgn_f0_dev: max_iter must be in [0, 100]");
   if (!ctx || !images_ok(im) || !plan || plan->rect)
     return fail(ALDIC_EINVAL, "local_icgn_dev: bad context/images/plan");
   if (plan->n == 0) return ALDIC_OK;
@@ -543,6 +545,7 @@ extern "C" int aldic_local_icgn_f0_dev(aldic_ctx* ctx, const aldic_images* im,
   RC_TRY(set_device(ctx));
   SolveArgs a = icgn_args(im, plan->n, d_coords, d_U0, d_F0, d_winsize, tol, d_U, d_F, d_H21,
                           d_convIt, d_hole, d_zncc);
+  a.maxit = max_iter;
   return run_solver(ctx, 0, a, plan);
 }
 
@@ -576,15 +579,15 @@ extern "C" int aldic_search_nodes_dev(aldic_ctx* ctx, const aldic_images* im,
     return fail(ALDIC_EINVAL, "search_nodes_dev: winsize too large for one workgroup's LDS");
   if (max_count <= 0 || n == 0) return ALDIC_OK;
   RC_TRY(set_device(ctx));
-  const int grid = (int)std::min<int64_t>(max_count, 1024);
-  const int64_t kmax = (region[0] + region[1] + 3) * (region[2] + region[3] + 3);
-  float* gmap;
-  RC_TRY(stage(ctx, SLOT_GMAP_NODES, (size_t)(kmax * grid), &gmap));
+  if (max_count > n) max_count = n;
   const int rg[4] = {(int)region[0], (int)region[1], (int)region[2], (int)region[3]};
+  void* scratch;
+  RC_TRY(staging(ctx, SLOT_SEARCH_NODES,
+                 aldic::search_scratch_bytes((int)max_count, winsize, rg), &scratch));
   StageTimer t(ctx, ALDIC_STAGE_FALLBACK);
   HIP_TRY(aldic::launch_search_nodes(im->fm, im->g, (int)im->M, (int)im->N, d_coords, (int)n,
-                                     d_list, d_count, grid, winsize, rg, gmap, kmax, d_U0, d_F0,
-                                     ctx->stream));
+                                     d_list, d_count, (int)max_count, winsize, rg, scratch, d_U0,
+                                     d_F0, ctx->stream));
   return ALDIC_OK;
 }
 
@@ -929,14 +932,14 @@ extern "C" int aldic_integer_search_dev(aldic_ctx* ctx, const float* d_f, const 
   if (!d_x || !d_y || !d_u || !d_v || !d_ccmax || !d_ipr || !d_ipc)
     return fail(ALDIC_EINVAL, "integer search: NULL output");
   RC_TRY(set_device(ctx));
-  const int64_t kmax = (region[0] + region[1] + 3) * (region[2] + region[3] + 3);
-  float* gmap;
-  RC_TRY(stage(ctx, SLOT_GMAP, (size_t)(kmax * nX * nY), &gmap));
   const int gx[2] = {(int)gridx[0], (int)gridx[1]}, gy[2] = {(int)gridy[0], (int)gridy[1]};
   const int rg[4] = {(int)region[0], (int)region[1], (int)region[2], (int)region[3]};
+  void* scratch;
+  RC_TRY(staging(ctx, SLOT_SEARCH, aldic::search_scratch_bytes((int)(nX * nY), winsize, rg),
+                 &scratch));
   StageTimer t(ctx, ALDIC_STAGE_SEARCH);
   HIP_TRY(aldic::launch_search(d_f, d_g, (int)M, (int)N, (int)nX, (int)nY, gx, gy, winsize,
-                               winstepsize, rg, gmap, kmax, d_x, d_y, d_u, d_v, d_ccmax, d_ipr,
+                               winstepsize, rg, scratch, d_x, d_y, d_u, d_v, d_ccmax, d_ipr,
                                d_ipc, ctx->stream));
   return ALDIC_OK;
 }

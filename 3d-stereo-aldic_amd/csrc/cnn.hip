@@ -12,6 +12,8 @@
 // reduction runs over the batch.
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
+
 #include "aldic_common.h"
 #include "aldic_kernels.h"
 
@@ -152,6 +154,223 @@ __global__ __launch_bounds__(256) void gemm_kernel(const bf16* __restrict__ A, i
         if (Cb) Cb[(int64_t)row * ldcb + col] = to_bf16(v);
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Inference-shaped GEMM: the whole weight matrix Bt[N][K] (N = 32*NB) is
+// staged once per workgroup in LDS (rows padded by 16 bytes), and the
+// workgroup loops over 256-row tiles of A.  Each wave computes 64 rows x N
+// (two MFMA row tiles share every B fragment); the next A fragments are
+// loaded while the current ones feed the matrix cores.  Epilogue: bias,
+// optional ReLU, bf16 store (row stride N).
+// ---------------------------------------------------------------------------
+template <int NB, int AMODE>
+__global__ __launch_bounds__(256) void gemm_lds_kernel(const bf16* __restrict__ A, int lda,
+                                                       const bf16* __restrict__ Bt, int M, int K,
+                                                       const float* __restrict__ bias, int relu,
+                                                       bf16* __restrict__ Cb, ConvGeom geom,
+                                                       int ntiles) {
+  extern __shared__ bf16x8 sB[];
+  constexpr int N = 32 * NB;
+  const int kq = K / 8, rs = kq + 1;
+  for (int e = threadIdx.x; e < N * kq; e += 256) {
+    const int row = e / kq, q = e - row * kq;
+    sB[row * rs + q] = *(const bf16x8*)(Bt + (int64_t)row * K + q * 8);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int r = lane % 32, kh = 8 * (lane / 32), hq = lane / 32;
+  float bv[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) bv[j] = bias ? bias[32 * j + r] : 0.0f;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int m0 = tile * 256 + wave * 64;
+    ALoader<AMODE> a0, a1;
+    a0.init(A, lda, m0 + r, (m0 + r) < M, geom, kh);
+    a1.init(A, lda, m0 + 32 + r, (m0 + 32 + r) < M, geom, kh);
+    floatx16 acc0[NB], acc1[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        acc0[j][i] = 0.0f;
+        acc1[j][i] = 0.0f;
+      }
+    bf16x8 x0 = a0.load(0, kh, geom), x1 = a1.load(0, kh, geom);
+    for (int k = 0; k < K; k += 16) {
+      bf16x8 n0 = x0, n1 = x1;
+      if (k + 16 < K) {
+        n0 = a0.load(k + 16, kh, geom);
+        n1 = a1.load(k + 16, kh, geom);
+      }
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        const bf16x8 b = sB[(32 * j + r) * rs + k / 8 + hq];
+        acc0[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, b, acc0[j], 0, 0, 0);
+        acc1[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, b, acc1[j], 0, 0, 0);
+      }
+      x0 = n0;
+      x1 = n1;
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        const int col = 32 * j + r;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          const int row = m0 + 32 * h + 8 * (i / 4) + 4 * hq + (i % 4);
+          if (row >= M) continue;
+          float v = (h ? acc1[j][i] : acc0[j][i]) + bv[j];
+          if (relu) v = fmaxf(v, 0.0f);
+          Cb[(int64_t)row * N + col] = to_bf16(v);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 / pad 1 convolution as an implicit GEMM with compile-time geometry
+// of the reduction: CIN input channels (2, 32 or 64), K = 9*CIN (32 for
+// CIN = 2, zero padded), N = 32*NB output channels.  Weights live in LDS
+// for the whole workgroup; per 256-row tile every lane computes the nine
+// tap offsets of its two rows once, and the fully unrolled k-loop has no
+// index arithmetic left, so the compiler can issue the A loads of many
+// k-steps ahead of the matrix-core work that consumes them.
+// ---------------------------------------------------------------------------
+template <int NB, int CIN>
+__global__ __launch_bounds__(256) void conv_lds_kernel(const bf16* __restrict__ x,
+                                                       const bf16* __restrict__ Wt, int M,
+                                                       const float* __restrict__ bias, int relu,
+                                                       bf16* __restrict__ out, ConvGeom g,
+                                                       int ntiles) {
+  constexpr int N = 32 * NB;
+  constexpr int K = CIN == 2 ? 32 : 9 * CIN;
+  constexpr int KQ = K / 8, RS = KQ + 1;
+  constexpr int STEPS = K / 16;
+  __shared__ bf16x8 sB[N * RS];
+  for (int e = threadIdx.x; e < N * KQ; e += 256) {
+    const int row = e / KQ, q = e - row * KQ;
+    sB[row * RS + q] = *(const bf16x8*)(Wt + (int64_t)row * K + q * 8);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const int r = lane % 32, hq = lane / 32, kh = 8 * hq;
+  float bv[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) bv[j] = bias ? bias[32 * j + r] : 0.0f;
+  const int hw = g.Ho * g.Wo;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int m0 = tile * 256 + wave * 64;
+    const bf16* base[2];
+    int off[2][9];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int row = m0 + 32 * h + r;
+      const bool ok = row < M;
+      const int n = ok ? row / hw : 0;
+      const int rem = row - n * hw;
+      const int oy = rem / g.Wo, ox = rem - (rem / g.Wo) * g.Wo;
+      base[h] = x + (int64_t)n * g.H * g.W * CIN;
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        const int iy = oy * g.stride - 1 + t / 3, ix = ox * g.stride - 1 + t % 3;
+        off[h][t] = (ok && iy >= 0 && iy < g.H && ix >= 0 && ix < g.W)
+                        ? (iy * g.W + ix) * CIN : -1;
+      }
+    }
+    floatx16 acc[2][NB];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc[h][j][i] = 0.0f;
+    if (CIN == 2) {
+#pragma unroll
+      for (int st = 0; st < STEPS; st++) {
+        bf16x8 a[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          bf16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+          // columns 16*st + kh .. +7 are taps 8*st + 4*hq .. +3, two channels each
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int t0 = 8 * st + q, t1 = 8 * st + 4 + q;
+            const int o = hq ? (t1 < 9 ? off[h][t1 < 9 ? t1 : 0] : -1)
+                             : (t0 < 9 ? off[h][t0 < 9 ? t0 : 0] : -1);
+            if (o >= 0) {
+              const uint32_t w2 = *(const uint32_t*)(base[h] + o);
+              v[2 * q] = __builtin_bit_cast(bf16, (uint16_t)(w2 & 0xffffu));
+              v[2 * q + 1] = __builtin_bit_cast(bf16, (uint16_t)(w2 >> 16));
+            }
+          }
+          a[h] = v;
+        }
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+          const bf16x8 b = sB[(32 * j + r) * RS + 2 * st + hq];
+          acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b, acc[0][j], 0, 0, 0);
+          acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b, acc[1][j], 0, 0, 0);
+        }
+      }
+    } else {
+      // one tap = SPT k-steps; the fragments of tap t+1 are in flight while
+      // tap t feeds the matrix cores
+      constexpr int SPT = CIN == 2 ? 1 : CIN / 16;
+      bf16x8 cur[2][SPT], nxt[2][SPT];
+      auto load_tap = [&](int t, bf16x8 (&dst)[2][SPT]) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int o = off[h][t];
+#pragma unroll
+          for (int c = 0; c < SPT; c++) {
+            bf16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (o >= 0) v = *(const bf16x8*)(base[h] + o + c * 16 + kh);
+            dst[h][c] = v;
+          }
+        }
+      };
+      load_tap(0, cur);
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        if (t + 1 < 9) load_tap(t + 1, nxt);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int c = 0; c < SPT; c++) {
+          const int st = t * SPT + c;
+#pragma unroll
+          for (int j = 0; j < NB; j++) {
+            const bf16x8 b = sB[(32 * j + r) * RS + 2 * st + hq];
+            acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[0][c], b, acc[0][j], 0, 0, 0);
+            acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[1][c], b, acc[1][j], 0, 0, 0);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (t + 1 < 9) {
+#pragma unroll
+          for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int c = 0; c < SPT; c++) cur[h][c] = nxt[h][c];
+        }
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+        const int col = 32 * j + r;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+          const int row = m0 + 32 * h + 8 * (i / 4) + 4 * hq + (i % 4);
+          if (row >= M) continue;
+          float v = acc[h][j][i] + bv[j];
+          if (relu) v = fmaxf(v, 0.0f);
+          out[(int64_t)row * N + col] = to_bf16(v);
+        }
+      }
   }
 }
 
@@ -362,12 +581,14 @@ __global__ __launch_bounds__(256) void patch_kernel(const float* __restrict__ f,
   const float mf = s[0] / NP, mg = s[2] / NP;
   const float sf = rsqrtf(fmaxf(s[1] / NP - mf * mf, 1e-12f));
   const float sg = rsqrtf(fmaxf(s[3] / NP - mg * mg, 1e-12f));
-  bf16* o = out + (int64_t)local * NP * 2;
+  // one 4-byte store per pixel: (reference, deformed) channel pair
+  uint32_t* o = reinterpret_cast<uint32_t*>(out + (int64_t)local * NP * 2);
 #pragma unroll
   for (int k = 0; k < PER; k++) {
     const int p = lane + 64 * k;
-    o[2 * p] = to_bf16((vf[k] - mf) * sf);
-    o[2 * p + 1] = to_bf16((vg[k] - mg) * sg);
+    const uint32_t lo = __builtin_bit_cast(uint16_t, to_bf16((vf[k] - mf) * sf));
+    const uint32_t hi = __builtin_bit_cast(uint16_t, to_bf16((vg[k] - mg) * sg));
+    o[p] = lo | (hi << 16);
   }
 }
 
@@ -542,12 +763,67 @@ hipError_t cnn_gemm(const void* A, int lda, const void* Bt, int ldb, int M, int 
                   ConvGeom{0, 0, 0, 0, 0, 0}, s);
 }
 
+template <int NB, int AMODE>
+static hipError_t launch_lds(const bf16* A, int lda, const bf16* Bt, int M, int K,
+                             const float* bias, int relu, bf16* Cb, ConvGeom g, hipStream_t s) {
+  const size_t smem = (size_t)32 * NB * (K / 8 + 1) * 16;
+  if (smem > 160 * 1024) return hipErrorInvalidValue;
+  static bool attr = false;  // one attribute call per instantiation
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)gemm_lds_kernel<NB, AMODE>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  const int ntiles = (M + 255) / 256;
+  const int per_cu = smem > 80 * 1024 ? 1 : (smem > 40 * 1024 ? 2 : 4);
+  const int grid = std::max(1, std::min(ntiles, 256 * per_cu * 2));
+  gemm_lds_kernel<NB, AMODE><<<grid, 256, smem, s>>>(A, lda, Bt, M, K, bias, relu, Cb, g, ntiles);
+  return hipGetLastError();
+}
+
+template <int NB, int CIN>
+static hipError_t launch_conv(const bf16* x, const bf16* Wt, int M, const float* bias, int relu,
+                              bf16* out, ConvGeom g, hipStream_t s) {
+  constexpr int K = CIN == 2 ? 32 : 9 * CIN;
+  constexpr size_t smem = (size_t)32 * NB * (K / 8 + 1) * 16;
+  const int ntiles = (M + 255) / 256;
+  const int per_cu = smem > 80 * 1024 ? 1 : (smem > 40 * 1024 ? 2 : 4);
+  const int grid = std::max(1, std::min(ntiles, 256 * per_cu * 2));
+  conv_lds_kernel<NB, CIN><<<grid, 256, 0, s>>>(x, Wt, M, bias, relu, out, g, ntiles);
+  return hipGetLastError();
+}
+
 hipError_t cnn_conv(const void* x, int B, int H, int W, int Cin, int stride, int Ho, int Wo,
                     const void* Wt, int Kp, int Cout, const float* bias, int relu, void* y,
                     float* yf, hipStream_t s) {
   const int M = B * Ho * Wo;
+  const ConvGeom g{H, W, Cin, stride, Ho, Wo};
+  const bool tab = (Cin == 2 && Kp == 32) || (Cin == 32 && Kp == 288) || (Cin == 64 && Kp == 576);
+  if (!yf && tab && (Cout == 32 || Cout == 64)) {
+    const bf16* a = (const bf16*)x;
+    const bf16* w = (const bf16*)Wt;
+    bf16* o = (bf16*)y;
+    if (Cin == 2)
+      return Cout == 32 ? launch_conv<1, 2>(a, w, M, bias, relu, o, g, s)
+                        : launch_conv<2, 2>(a, w, M, bias, relu, o, g, s);
+    if (Cin == 32)
+      return Cout == 32 ? launch_conv<1, 32>(a, w, M, bias, relu, o, g, s)
+                        : launch_conv<2, 32>(a, w, M, bias, relu, o, g, s);
+    return Cout == 32 ? launch_conv<1, 64>(a, w, M, bias, relu, o, g, s)
+                      : launch_conv<2, 64>(a, w, M, bias, relu, o, g, s);
+  }
   return gemm_any(Cin == 2 ? 2 : 1, x, 0, Wt, Kp, M, Cout, Kp, 1, yf, Cout, bias, relu, y, Cout,
-                  ConvGeom{H, W, Cin, stride, Ho, Wo}, s);
+                  g, s);
+}
+
+hipError_t cnn_dense(const void* x, int B, int K, const void* Wt, int Cout, const float* bias,
+                     int relu, void* y, hipStream_t s) {
+  if (Cout == 128 && K % 16 == 0 && (size_t)128 * (K / 8 + 1) * 16 <= 160 * 1024)
+    return launch_lds<4, 0>((const bf16*)x, K, (const bf16*)Wt, B, K, bias, relu, (bf16*)y,
+                            ConvGeom{0, 0, 0, 0, 0, 0}, s);
+  return gemm_any(0, x, K, Wt, K, B, Cout, K, 1, nullptr, Cout, bias, relu, y, Cout,
+                  ConvGeom{0, 0, 0, 0, 0, 0}, s);
 }
 
 hipError_t cnn_splitk_reduce(const float* P, int splits, int M, int N, int ldp, const float* bias,

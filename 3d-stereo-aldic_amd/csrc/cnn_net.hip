@@ -155,8 +155,8 @@ int forward(aldic_cnn* net, const void* x, int B) {
     in = net->act[l];
   }
   const Layer& f1 = net->L[NCONV];
-  HIP_TRY(aldic::cnn_gemm(in, f1.Kp, bfp(net->wb, f1.w_off), f1.Kp, B, f1.cout, f1.Kp, 1,
-                          nullptr, f1.cout, net->wf + f1.b_off, 1, net->act[NCONV], f1.cout, s));
+  HIP_TRY(aldic::cnn_dense(in, B, f1.Kp, bfp(net->wb, f1.w_off), f1.cout, net->wf + f1.b_off, 1,
+                           net->act[NCONV], s));
   const Layer& f2 = net->L[NCONV + 1];
   HIP_TRY(aldic::cnn_gemm(net->act[NCONV], f2.Kp, bfp(net->wb, f2.w_off), f2.Kp, B, f2.cout,
                           f2.Kp, 1, net->out, f2.cout, net->wf + f2.b_off, 0, nullptr, 0, s));

@@ -470,6 +470,7 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
 
   // ---- Gauss-Newton loop (:135-422) ----
   const float tol = a.tol;
+  const int maxit = a.maxit > 0 ? a.maxit : 100;
   float normNew = 1.0f, normAbs = 1.0f, normInit = 0.0f;
   int step = 0;
   int tx0 = -1000000, ty0 = -1000000;  // tile origin (0-based image index)
@@ -481,7 +482,7 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
   const float lo_u = 3.0f - (float)x0, hi_u = (float)(M - 2 - x0);
   const float lo_v = 3.0f - (float)y0, hi_v = (float)(N - 2 - y0);
 
-  while (step <= 100 && normNew > tol && normAbs > tol) {
+  while (step <= maxit && normNew > tol && normAbs > tol) {
     step++;
     const float A11 = 1.0f + P[0], A12 = P[2], A21 = P[1], A22 = 1.0f + P[3];
     // warp (:149-152) in displacement-relative form.  The warp is affine,
@@ -944,13 +945,14 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
       // its mask block replaces it with the g mask)
       const uint8_t* fsel = sbwf;
       const float tol = a.tol;
+      const int maxit = a.maxit > 0 ? a.maxit : 100;
       float normNew = 1.0f, normAbs = 1.0f, normInit = 0.0f;
       int step = 0;
       bool have_bwg = false, err = false, defer_oob = false;
       float sum_bwg = 0.0f;
       const int ws0 = wx0;
       int cur_w = wx;
-      while (step <= 100 && normNew > tol && normAbs > tol) {
+      while (step <= maxit && normNew > tol && normAbs > tol) {
         step++;
         if (MODE == MODE_ICGN && step > 1 && c0 > 0.0f) {
           // :139-145 -- a change of size throws at the next elementwise op

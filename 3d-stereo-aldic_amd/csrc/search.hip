@@ -1,19 +1,30 @@
-// search.hip -- whole-field integer search of ALDIC (func/funIntegerSearch.m:
-// 100-203): per grid point, the ZNCC of the (ws+1)^2 template against the
-// search window (normxcorr2, textbook zero-padded form), restricted to the
-// lag window the reference crops (:166), then findpeak (func/findpeak.m).
+// search.hip -- integer search of ALDIC (func/funIntegerSearch.m:100-203):
+// per point, the ZNCC of the (ws+1)^2 template against the search window
+// (normxcorr2, textbook zero-padded form), restricted to the lag window the
+// reference crops (:166), then findpeak (func/findpeak.m).
 //
-// One 256-thread workgroup per grid point.  The lag window is processed in
-// tiles of 64 (dim-1 lags, one per lane) x 32 (dim-2 lags, 8 per lane over
-// 4 waves).  Per tile the zero-padded window slice is staged in LDS; each
-// lane slides an 8-value register window along dim 2 so one LDS load feeds
-// 8 FMAs, and the template value is an LDS broadcast.  The local sums of
-// the window and its square (the normxcorr2 denominator) are exact box sums
-// computed in LDS from column sums, not a summed-area table (FP32 SATs lose
-// 1e-3 relative on the sum of squares).  ZNCC values go to a per-point map
-// in global memory; the first maximum in column-major order and the 3x3
-// quadratic fit (FP64, rounded to 1e-3 as findpeak.m:58-59) finish on the
+// Three launches, each spread over the whole chip whatever the number of
+// points:
+//   prep   one workgroup per point: window clamps, the template with its
+//          mean removed (stored transposed and zero padded so the tile
+//          kernel reads it with scalar loads), its norm;
+//   tiles  one workgroup per (point, lag tile): a tile is 64 dim-1 lags
+//          (lanes) x 32 dim-2 lags (4 waves x 8 registers).  The window
+//          slice is staged in LDS; each lane slides an 8-value register
+//          window along dim 2 so one LDS load feeds 8 FMAs, and the template
+//          value comes from a scalar register.  The normxcorr2 denominators
+//          are exact box sums from LDS column sums (an FP32 summed-area
+//          table loses 1e-3 relative on the sum of squares).  Each tile
+//          keeps its first maximum in column-major order;
+//   final  one wave per point: the first maximum over the tiles, then the
+//          3x3 quadratic fit of findpeak.m (FP64, rounded to 1e-3), its nine
+//          ZNCC values recomputed at the peak's neighbours.
+// Points are either the reference's grid (funIntegerSearch.m:106-107) or a
+// device list of nodes whose templates are centred on the nodes (the
+// fallback of the learned starting point); the list count is read on the
 // device.
+#include <algorithm>
+
 #include "aldic_common.h"
 #include "aldic_kernels.h"
 
@@ -23,60 +34,68 @@ constexpr int KT1 = 64;  // dim-1 lags per tile (lanes)
 constexpr int KT2 = 32;  // dim-2 lags per tile (4 waves x 8)
 constexpr int RB = 8;    // register block along dim 2
 
+struct PointPrm {
+  int ii, jj;      // template origin (1-based)
+  int L, R, U, D;  // clamped search region
+  int valid;       // 0: constant template (normxcorr2 throws) or template off the image
+  float denT;      // || T - mean(T) ||
+};
+
 struct SearchArgs {
   const float* f;  // M x N reference (normalised)
   const float* g;  // M x N deformed
   int M, N;
-  int nX, nY;
-  int gx0, gx1, gy0, gy1;  // ROI (1-based)
-  int ws, step;
-  int reg[4];  // L R U D
-  float* gmap;  // per-point lag maps, stride kmax
-  int64_t kmax;
-  double *x, *y, *u, *v, *ccmax;
-  int *ipr, *ipc;
-  // node-list mode (list != nullptr): templates centred on the listed
-  // nodes, count read on the device, results written as IC-GN starting
-  // points U0 (and F0 = 0) of those nodes
+  int ws, sp;      // template side s = ws + 1, padded row length sp
+  int reg[4];      // L R U D
+  int ntiles, nt1; // lag tiles per point (max over points), along dim 1
+  // grid mode
+  int nX, nY, gx0, gx1, gy0, gy1, step;
+  // list mode (list != nullptr)
   const int* list;
   const int* lcount;
   const double* ncoords;
   int nn;
   float* U0;
   float* F0;
+  // scratch
+  PointPrm* prm;
+  float* tmpl;  // per point s x sp, transposed: tmpl[a*sp + b] = T(a, b) - mean
+  float* tbv;   // per (point, tile) best value
+  int* tbi;     // and its lag index
+  // grid outputs
+  double *x, *y, *u, *v, *ccmax;
+  int *ipr, *ipc;
 };
 
-__device__ void search_point_params(const SearchArgs& A, int t, int* ii, int* jj, int* L, int* R,
-                                    int* U, int* D) {
-  const int ci = t % A.nX, cj = t / A.nX;
-  *ii = A.gx0 + ci * A.step;
-  *jj = A.gy0 + cj * A.step;
-  int l = A.reg[0], r = A.reg[1], u = A.reg[2], d = A.reg[3];
-  // per-point clamps, funIntegerSearch.m:139-152 (right/down use the ROI end)
-  if (*ii - l < 1) l = *ii - 1;
-  if (*jj - u < 1) u = *jj - 1;
-  if (*ii + A.ws + r > A.M) r = A.gx1 - *ii - A.ws - 1;
-  if (*jj + A.ws + d > A.N) d = A.gy1 - *jj - A.ws - 1;
-  *L = l < 0 ? 0 : l;
-  *R = r < 0 ? 0 : r;
-  *U = u < 0 ? 0 : u;
-  *D = d < 0 ? 0 : d;
+__device__ __forceinline__ int num_points(const SearchArgs& A) {
+  return A.list ? *A.lcount : A.nX * A.nY;
 }
 
-// Template origin and clamped region of a listed node: template centred on
-// the rounded node, clamps as funIntegerSearch.m:139-152 with the whole
-// image as the ROI.  False when the template leaves the image.
-__device__ bool search_node_params(const SearchArgs& A, int node, int* ii, int* jj, int* L, int* R,
-                                   int* U, int* D) {
-  const int h = A.ws / 2;
-  *ii = (int)mround(A.ncoords[node]) - h;
-  *jj = (int)mround(A.ncoords[node + A.nn]) - h;
-  if (*ii < 1 || *jj < 1 || *ii + A.ws > A.M || *jj + A.ws > A.N) return false;
+// Template origin and per-point clamps of funIntegerSearch.m:139-152 (the
+// right/down clamps use the ROI end, as the reference does).
+__device__ bool point_params(const SearchArgs& A, int t, int* ii, int* jj, int* L, int* R, int* U,
+                             int* D) {
+  int gx1, gy1;
+  if (A.list) {
+    const int node = A.list[t];
+    const int h = A.ws / 2;
+    *ii = (int)mround(A.ncoords[node]) - h;
+    *jj = (int)mround(A.ncoords[node + A.nn]) - h;
+    gx1 = A.M;
+    gy1 = A.N;
+    if (*ii < 1 || *jj < 1 || *ii + A.ws > A.M || *jj + A.ws > A.N) return false;
+  } else {
+    const int ci = t % A.nX, cj = t / A.nX;
+    *ii = A.gx0 + ci * A.step;
+    *jj = A.gy0 + cj * A.step;
+    gx1 = A.gx1;
+    gy1 = A.gy1;
+  }
   int l = A.reg[0], r = A.reg[1], u = A.reg[2], d = A.reg[3];
   if (*ii - l < 1) l = *ii - 1;
   if (*jj - u < 1) u = *jj - 1;
-  if (*ii + A.ws + r > A.M) r = A.M - *ii - A.ws - 1;
-  if (*jj + A.ws + d > A.N) d = A.N - *jj - A.ws - 1;
+  if (*ii + A.ws + r > A.M) r = gx1 - *ii - A.ws - 1;
+  if (*jj + A.ws + d > A.N) d = gy1 - *jj - A.ws - 1;
   *L = l < 0 ? 0 : l;
   *R = r < 0 ? 0 : r;
   *U = u < 0 ? 0 : u;
@@ -84,94 +103,97 @@ __device__ bool search_node_params(const SearchArgs& A, int node, int* ii, int* 
   return true;
 }
 
-__device__ void search_one(const SearchArgs& A, int t, int slot, float* smem, float* red,
-                           float* bestv, int* besti) {
+// ---------------------------------------------------------------------------
+// prep: one 256-thread workgroup per point
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void search_prep_kernel(SearchArgs A) {
+  __shared__ float red[8];
   const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
-  int ii, jj, L, R, U, D;
-  int node = -1;
-  if (A.list) {
-    node = A.list[t];
-    if (!search_node_params(A, node, &ii, &jj, &L, &R, &U, &D)) return;  // block-uniform
-  } else {
-    search_point_params(A, t, &ii, &jj, &L, &R, &U, &D);
-  }
-  const int s = A.ws + 1;
-  const int n = s * s;
-  const int W1 = s + L + R, W2 = s + U + D;
-  const int K1 = W1 - s + 3, K2 = W2 - s + 3;
-  const int M = A.M;
-  // u(cj, ci), nY x nX column-major (grid mode only)
-  const int o = A.list ? t : (t / A.nX) + (t % A.nX) * A.nY;
-
-  // LDS: template (mean removed), window slice, column sums
-  const int SW = KT1 + s - 1;        // slice rows (dim 1)
-  const int SC = KT2 + s - 1 + 16;   // slice columns (dim 2), 16 spare zero columns
-  float* Tp = smem;                  // s*s
-  float* sub = Tp + n;               // SW*SC
-  float* CB = sub + SW * SC;         // KT2*SW
-  float* CB2 = CB + KT2 * SW;        // KT2*SW
-
-  // ---- template C = f(ii:ii+ws, jj:jj+ws) ----
-  float st = 0.0f, diff = 0.0f;
-  const float t0 = A.f[(int64_t)(ii - 1) + (int64_t)(jj - 1) * M];
-  for (int p = tid; p < n; p += 256) {
-    const float v = A.f[(int64_t)(ii - 1 + p % s) + (int64_t)(jj - 1 + p / s) * M];
-    Tp[p] = v;
-    st += v;
-    diff += (v != t0) ? 1.0f : 0.0f;
-  }
-  float sums[2] = {wave_sum(st), wave_sum(diff)};
-  if (lane == 0) {
-    red[w * 2] = sums[0];
-    red[w * 2 + 1] = sums[1];
-  }
-  __syncthreads();
-  st = red[0] + red[2] + red[4] + red[6];
-  diff = red[1] + red[3] + red[5] + red[7];
-  __syncthreads();
-  if (diff == 0.0f) {
-    // normxcorr2 throws on a constant template -> NaN (funIntegerSearch.m:171-174);
-    // a listed node keeps its starting point
-    if (tid == 0 && !A.list) {
-      const double nan = __builtin_nan("");
-      A.u[o] = nan;
-      A.v[o] = nan;
-      A.ccmax[o] = nan;
-      A.ipr[o] = 0;
-      A.ipc[o] = 0;
-      A.x[o] = (double)(ii + ii + A.ws) / 2.0;
-      A.y[o] = (double)(jj + jj + A.ws) / 2.0;
+  const int total = num_points(A);
+  const int s = A.ws + 1, n = s * s;
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    PointPrm P{};
+    const bool inside = point_params(A, t, &P.ii, &P.jj, &P.L, &P.R, &P.U, &P.D);
+    float st = 0.0f, diff = 0.0f;
+    if (inside) {
+      const float t0 = A.f[(int64_t)(P.ii - 1) + (int64_t)(P.jj - 1) * A.M];
+      for (int p = tid; p < n; p += 256) {
+        const float v = A.f[(int64_t)(P.ii - 1 + p % s) + (int64_t)(P.jj - 1 + p / s) * A.M];
+        st += v;
+        diff += (v != t0) ? 1.0f : 0.0f;
+      }
     }
-    return;
+    st = wave_sum(st);
+    diff = wave_sum(diff);
+    if (lane == 0) {
+      red[w * 2] = st;
+      red[w * 2 + 1] = diff;
+    }
+    __syncthreads();
+    st = red[0] + red[2] + red[4] + red[6];
+    diff = red[1] + red[3] + red[5] + red[7];
+    __syncthreads();
+    const float tmean = st / (float)n;
+    float* T = A.tmpl + (int64_t)t * s * A.sp;
+    float ss = 0.0f;
+    // transposed and padded: T[a*sp + b], zeros for b >= s
+    for (int e = tid; e < s * A.sp; e += 256) {
+      const int a = e / A.sp, b = e - a * A.sp;
+      float d = 0.0f;
+      if (inside && b < s) {
+        d = A.f[(int64_t)(P.ii - 1 + a) + (int64_t)(P.jj - 1 + b) * A.M] - tmean;
+        ss += d * d;
+      }
+      T[e] = d;
+    }
+    ss = wave_sum(ss);
+    if (lane == 0) red[w] = ss;
+    __syncthreads();
+    if (tid == 0) {
+      P.denT = sqrtf(red[0] + red[1] + red[2] + red[3]);
+      P.valid = (inside && diff != 0.0f) ? 1 : 0;
+      A.prm[t] = P;
+    }
+    __syncthreads();
   }
-  const float tmean = st / (float)n;
-  float ss = 0.0f;
-  for (int p = tid; p < n; p += 256) {
-    const float d = Tp[p] - tmean;
-    Tp[p] = d;
-    ss += d * d;
-  }
-  ss = wave_sum(ss);
-  if (lane == 0) red[w] = ss;
-  __syncthreads();
-  const float denT = sqrtf(red[0] + red[1] + red[2] + red[3]);
-  const float inv_n = 1.0f / (float)n;
-  float* gmap = A.gmap + (int64_t)slot * A.kmax;
+}
 
-  float my_best = -1e30f;
-  int my_idx = 0x7fffffff;
-  const int g_r0 = ii - L - 1, g_c0 = jj - U - 1;  // 0-based image index of window (0,0)
-
-  for (int q0 = 0; q0 < K2; q0 += KT2) {
-    for (int p0 = 0; p0 < K1; p0 += KT1) {
-      __syncthreads();
-      // slice rows: window rows p0-1 .. p0-1+SW-1; columns q0-1 .. q0-1+SC-1
+// ---------------------------------------------------------------------------
+// tiles: one 256-thread workgroup per (point, lag tile)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void search_tile_kernel(SearchArgs A,
+                                                          const float* __restrict__ tmpl,
+                                                          const float* __restrict__ g) {
+  extern __shared__ float smem[];
+  __shared__ float bestv[256];
+  __shared__ int besti[256];
+  const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+  const int total = num_points(A) * A.ntiles;
+  const int s = A.ws + 1;
+  const int M = A.M;
+  const int SW = KT1 + s - 1;       // slice rows (dim 1)
+  const int SC = KT2 + s - 1 + 16;  // slice columns (dim 2), 16 spare zero columns
+  float* sub = smem;                // SW*SC
+  float* CB = sub + SW * SC;        // KT2*SW
+  float* CB2 = CB + KT2 * SW;       // KT2*SW
+  const float inv_n = 1.0f / (float)(s * s);
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int t = item / A.ntiles, tile = item - t * A.ntiles;
+    const PointPrm P = A.prm[t];
+    const int K1 = P.L + P.R + 3, K2 = P.U + P.D + 3;
+    const int W1 = s + P.L + P.R, W2 = s + P.U + P.D;
+    const int p0 = (tile % A.nt1) * KT1, q0 = (tile / A.nt1) * KT2;
+    float my_best = -1e30f;
+    int my_idx = 0x7fffffff;
+    if (P.valid && p0 < K1 && q0 < K2) {
+      const int g_r0 = P.ii - P.L - 1, g_c0 = P.jj - P.U - 1;  // 0-based window (0,0)
+      // slice: window rows p0-1 .. p0-1+SW-1, columns q0-1 .. q0-1+SC-1
       for (int e = tid; e < SW * SC; e += 256) {
         const int i = e % SW, j = e / SW;
         const int wr = p0 - 1 + i, wc = q0 - 1 + j;
         float v = 0.0f;
         if (wr >= 0 && wr < W1 && wc >= 0 && wc < W2 && j < KT2 + s - 1)
-          v = A.g[(int64_t)(g_r0 + wr) + (int64_t)(g_c0 + wc) * M];
+          v = g[(int64_t)(g_r0 + wr) + (int64_t)(g_c0 + wc) * M];
         sub[e] = v;
       }
       __syncthreads();
@@ -196,11 +218,13 @@ __device__ void search_one(const SearchArgs& A, int t, int slot, float* smem, fl
         }
       }
       // numerator: lane -> dim-1 lag p0+lane, wave w -> dim-2 lags q0+8w..+7
+      const float* T = tmpl + (int64_t)t * s * A.sp;
       float num[RB];
 #pragma unroll
       for (int r = 0; r < RB; r++) num[r] = 0.0f;
       for (int a = 0; a < s; a++) {
         const float* colp = sub + (w * RB) * SW + lane + a;
+        const float* Ta = T + a * A.sp;
         float win[2 * RB];
 #pragma unroll
         for (int k = 0; k < RB; k++) win[k] = colp[k * SW];
@@ -209,11 +233,9 @@ __device__ void search_one(const SearchArgs& A, int t, int slot, float* smem, fl
           for (int k = 0; k < RB; k++) win[RB + k] = colp[(b0 + RB + k) * SW];
 #pragma unroll
           for (int k = 0; k < RB; k++) {
-            if (b0 + k < s) {
-              const float tv = Tp[a + (b0 + k) * s];
+            const float tv = Ta[b0 + k];  // uniform: scalar load (zero past s)
 #pragma unroll
-              for (int r = 0; r < RB; r++) num[r] += tv * win[k + r];
-            }
+            for (int r = 0; r < RB; r++) num[r] = fmaf(tv, win[k + r], num[r]);
           }
 #pragma unroll
           for (int k = 0; k < RB; k++) win[k] = win[RB + k];
@@ -236,12 +258,11 @@ __device__ void search_one(const SearchArgs& A, int t, int slot, float* smem, fl
         const float varA = sa2 - sa * sa * inv_n;
         float gam = 0.0f;
         if (varA > 1e-6f * sa2 && varA > 0.0f) {
-          gam = num[r] / (denT * sqrtf(varA));
+          gam = num[r] / (P.denT * sqrtf(varA));
           if (fabsf(gam) - 1.0f > 1e-5f) gam = 0.0f;
         }
         if (p < K1 && q < K2) {
           const int idx = p + q * K1;
-          gmap[idx] = gam;
           if (gam > my_best || (gam == my_best && idx < my_idx)) {
             my_best = gam;
             my_idx = idx;
@@ -249,41 +270,130 @@ __device__ void search_one(const SearchArgs& A, int t, int slot, float* smem, fl
         }
       }
     }
+    bestv[tid] = my_best;
+    besti[tid] = my_idx;
+    __syncthreads();
+    if (tid < 64) {
+      float bv = bestv[tid];
+      int bi = besti[tid];
+      for (int k = tid + 64; k < 256; k += 64) {
+        if (bestv[k] > bv || (bestv[k] == bv && besti[k] < bi)) {
+          bv = bestv[k];
+          bi = besti[k];
+        }
+      }
+      // wave reduction with the same rule
+      for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_down(bv, off);
+        const int oi = __shfl_down(bi, off);
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
+      }
+      if (tid == 0) {
+        A.tbv[item] = bv;
+        A.tbi[item] = bi;
+      }
+    }
+    __syncthreads();
   }
-  bestv[tid] = my_best;
-  besti[tid] = my_idx;
-  __syncthreads();
-  if (tid == 0) {
-    float bv = bestv[0];
-    int bi = besti[0];
-    for (int k = 1; k < 256; k++) {
-      if (bestv[k] > bv || (bestv[k] == bv && besti[k] < bi)) {
-        bv = bestv[k];
-        bi = besti[k];
+}
+
+// ZNCC at one lag (p, q) of a point, the formula of the tile kernel:
+// window(wr, wc) = g(g_r0 + wr, g_c0 + wc) inside W1 x W2, zero outside.
+__device__ float zncc_at(const SearchArgs& A, const float* T, const PointPrm& P, int p, int q,
+                         int lane) {
+  const int s = A.ws + 1, n = s * s;
+  const int W1 = s + P.L + P.R, W2 = s + P.U + P.D;
+  const int g_r0 = P.ii - P.L - 1, g_c0 = P.jj - P.U - 1;
+  float num = 0.0f, sa = 0.0f, sa2 = 0.0f;
+  for (int e = lane; e < n; e += 64) {
+    const int a = e % s, b = e / s;
+    const int wr = p - 1 + a, wc = q - 1 + b;
+    float v = 0.0f;
+    if (wr >= 0 && wr < W1 && wc >= 0 && wc < W2)
+      v = A.g[(int64_t)(g_r0 + wr) + (int64_t)(g_c0 + wc) * A.M];
+    num += T[a * A.sp + b] * v;
+    sa += v;
+    sa2 += v * v;
+  }
+  num = wave_sum(num);
+  sa = wave_sum(sa);
+  sa2 = wave_sum(sa2);
+  const float varA = sa2 - sa * sa / (float)n;
+  float gam = 0.0f;
+  if (varA > 1e-6f * sa2 && varA > 0.0f) {
+    gam = num / (P.denT * sqrtf(varA));
+    if (fabsf(gam) - 1.0f > 1e-5f) gam = 0.0f;
+  }
+  return gam;
+}
+
+// ---------------------------------------------------------------------------
+// final: one wave per point
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void search_final_kernel(SearchArgs A) {
+  const int lane = threadIdx.x % 64;
+  const int total = num_points(A);
+  // uniform trip count per wave: every lane of a wave has the same t
+  for (int t = blockIdx.x * 4 + threadIdx.x / 64; t < total; t += gridDim.x * 4) {
+    const PointPrm P = A.prm[t];
+    const int K1 = P.L + P.R + 3, K2 = P.U + P.D + 3;
+    const int node = A.list ? A.list[t] : -1;
+    const int o = A.list ? t : (t / A.nX) + (t % A.nX) * A.nY;  // nY x nX column-major
+    if (!P.valid) {
+      // normxcorr2 throws on a constant template -> NaN (funIntegerSearch.m:171-174);
+      // a listed node keeps its starting point
+      if (!A.list && lane == 0) {
+        const double nan = __builtin_nan("");
+        A.u[o] = nan;
+        A.v[o] = nan;
+        A.ccmax[o] = nan;
+        A.ipr[o] = 0;
+        A.ipc[o] = 0;
+        A.x[o] = (double)(P.ii + P.ii + A.ws) / 2.0;
+        A.y[o] = (double)(P.jj + P.jj + A.ws) / 2.0;
+      }
+      continue;
+    }
+    // first maximum in column-major order over the tiles
+    float bv = -1e30f;
+    int bi = 0x7fffffff;
+    for (int k = 0; k < A.ntiles; k++) {
+      const float v = A.tbv[(int64_t)t * A.ntiles + k];
+      const int i = A.tbi[(int64_t)t * A.ntiles + k];
+      if (v > bv || (v == bv && i < bi)) {
+        bv = v;
+        bi = i;
       }
     }
     const int yp = bi % K1 + 1, xp = bi / K1 + 1;  // 1-based row, column
     double xpeak = xp, ypeak = yp, maxf = bv;
     if (!(xp == 1 || xp == K2 || yp == 1 || yp == K1)) {
+      const float* T = A.tmpl + (int64_t)t * (A.ws + 1) * A.sp;
+      double uu[3][3];
+      for (int dc = -1; dc <= 1; dc++)
+        for (int dr = -1; dr <= 1; dr++)
+          uu[dc + 1][dr + 1] = zncc_at(A, T, P, yp - 1 + dr, xp - 1 + dc, lane);
       // 3x3 least squares of u = A1 + A2 x + A3 y + A4 xy + A5 x^2 + A6 y^2
       // (x: column offset, y: row offset), findpeak.m:37-65
       double su = 0, sxu = 0, syu = 0, sxyu = 0, sx2u = 0, sy2u = 0;
       for (int dc = -1; dc <= 1; dc++)
         for (int dr = -1; dr <= 1; dr++) {
-          const double uu = gmap[(yp - 1 + dr) + (xp - 1 + dc) * K1];
-          su += uu;
-          sxu += dc * uu;
-          syu += dr * uu;
-          sxyu += dc * dr * uu;
-          sx2u += dc * dc * uu;
-          sy2u += dr * dr * uu;
+          const double v = uu[dc + 1][dr + 1];
+          su += v;
+          sxu += dc * v;
+          syu += dr * v;
+          sxyu += dc * dr * v;
+          sx2u += dc * dc * v;
+          sy2u += dr * dr * v;
         }
       double Ac[6];
       Ac[1] = sxu / 6.0;
       Ac[2] = syu / 6.0;
       Ac[3] = sxyu / 4.0;
-      // [9 6 6; 6 6 4; 6 4 6] [A1 A5 A6]' = [su sx2u sy2u]'
-      // closed form of that normal-equation block
+      // [9 6 6; 6 6 4; 6 4 6] [A1 A5 A6]' = [su sx2u sy2u]', closed form
       Ac[0] = (5.0 * su - 3.0 * sx2u - 3.0 * sy2u) / 9.0;
       Ac[4] = (-3.0 * su + 4.5 * sx2u) / 9.0;
       Ac[5] = (-3.0 * su + 4.5 * sy2u) / 9.0;
@@ -295,40 +405,66 @@ __device__ void search_one(const SearchArgs& A, int t, int slot, float* smem, fl
         yo = round(1000 * yo) / 1000;
         xpeak = xp + xo;
         ypeak = yp + yo;
-        maxf = Ac[0] + xo * Ac[1] + yo * Ac[2] + xo * yo * Ac[3] + xo * xo * Ac[4] + yo * yo * Ac[5];
+        maxf = Ac[0] + xo * Ac[1] + yo * Ac[2] + xo * yo * Ac[3] + xo * xo * Ac[4] +
+               yo * yo * Ac[5];
       }
     }
-    if (A.list) {
-      A.U0[2 * node] = (float)(ypeak - 2 - (double)L);
-      A.U0[2 * node + 1] = (float)(xpeak - 2 - (double)U);
-      if (A.F0)
-        for (int k = 0; k < 4; k++) A.F0[4 * node + k] = 0.0f;
-    } else {
-      A.u[o] = ypeak - 2 - (double)L;
-      A.v[o] = xpeak - 2 - (double)U;
-      A.ccmax[o] = maxf;
-      A.ipr[o] = yp;
-      A.ipc[o] = xp;
-      A.x[o] = (double)(ii + ii + A.ws) / 2.0;
-      A.y[o] = (double)(jj + jj + A.ws) / 2.0;
+    if (lane == 0) {
+      if (A.list) {
+        A.U0[2 * node] = (float)(ypeak - 2 - (double)P.L);
+        A.U0[2 * node + 1] = (float)(xpeak - 2 - (double)P.U);
+        if (A.F0)
+          for (int k = 0; k < 4; k++) A.F0[4 * node + k] = 0.0f;
+      } else {
+        A.u[o] = ypeak - 2 - (double)P.L;
+        A.v[o] = xpeak - 2 - (double)P.U;
+        A.ccmax[o] = maxf;
+        A.ipr[o] = yp;
+        A.ipc[o] = xp;
+        A.x[o] = (double)(P.ii + P.ii + A.ws) / 2.0;
+        A.y[o] = (double)(P.jj + P.jj + A.ws) / 2.0;
+      }
     }
   }
 }
 
-__global__ __launch_bounds__(256) void search_kernel(SearchArgs A) {
-  extern __shared__ float smem[];
-  __shared__ float red[8 * 4];
-  __shared__ float bestv[256];
-  __shared__ int besti[256];
-  if (A.list) {
-    const int total = *A.lcount;
-    for (int t = blockIdx.x; t < total; t += gridDim.x) {
-      search_one(A, t, blockIdx.x, smem, red, bestv, besti);
-      __syncthreads();
-    }
-  } else {
-    search_one(A, blockIdx.x, blockIdx.x, smem, red, bestv, besti);
+void tile_counts(const int reg[4], int* nt1, int* ntiles) {
+  const int K1 = reg[0] + reg[1] + 3, K2 = reg[2] + reg[3] + 3;
+  *nt1 = (K1 + KT1 - 1) / KT1;
+  *ntiles = *nt1 * ((K2 + KT2 - 1) / KT2);
+}
+
+int pad_len(int ws) { return (ws + 1 + RB - 1) / RB * RB; }
+
+size_t up256(size_t v) { return (v + 255) / 256 * 256; }
+
+void carve(SearchArgs& A, void* scratch, int maxpts) {
+  char* p = static_cast<char*>(scratch);
+  A.prm = reinterpret_cast<PointPrm*>(p);
+  p += up256((size_t)maxpts * sizeof(PointPrm));
+  A.tmpl = reinterpret_cast<float*>(p);
+  p += up256((size_t)maxpts * (A.ws + 1) * A.sp * 4);
+  A.tbv = reinterpret_cast<float*>(p);
+  p += up256((size_t)maxpts * A.ntiles * 4);
+  A.tbi = reinterpret_cast<int*>(p);
+}
+
+hipError_t run(const SearchArgs& A, int maxpts, hipStream_t s) {
+  const size_t smem = aldic::search_smem_bytes(A.ws);
+  if (smem > 160 * 1024) return hipErrorInvalidValue;
+  static bool attr = false;
+  if (smem > 64 * 1024 && !attr) {
+    hipError_t e = hipFuncSetAttribute((const void*)search_tile_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+    attr = true;
   }
+  if (maxpts <= 0) return hipSuccess;
+  search_prep_kernel<<<std::min(maxpts, 2048), 256, 0, s>>>(A);
+  const int64_t items = (int64_t)maxpts * A.ntiles;
+  search_tile_kernel<<<(int)std::min<int64_t>(items, 4096), 256, smem, s>>>(A, A.tmpl, A.g);
+  search_final_kernel<<<std::min((maxpts + 3) / 4, 1024), 256, 0, s>>>(A);
+  return hipGetLastError();
 }
 
 }  // namespace
@@ -338,50 +474,44 @@ namespace aldic {
 size_t search_smem_bytes(int ws) {
   const int s = ws + 1;
   const int SW = KT1 + s - 1, SC = KT2 + s - 1 + 16;
-  return sizeof(float) * ((size_t)s * s + (size_t)SW * SC + 2 * (size_t)KT2 * SW);
+  return sizeof(float) * ((size_t)SW * SC + 2 * (size_t)KT2 * SW);
 }
 
-static hipError_t search_attr(int ws, size_t* smem) {
-  *smem = search_smem_bytes(ws);
-  if (*smem > 160 * 1024) return hipErrorInvalidValue;
-  if (*smem > 64 * 1024)
-    return hipFuncSetAttribute((const void*)search_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)*smem);
-  return hipSuccess;
+size_t search_scratch_bytes(int maxpts, int ws, const int reg[4]) {
+  int nt1, nt;
+  tile_counts(reg, &nt1, &nt);
+  return up256((size_t)maxpts * sizeof(PointPrm)) +
+         up256((size_t)maxpts * (ws + 1) * pad_len(ws) * 4) +
+         2 * up256((size_t)maxpts * nt * 4);
 }
 
 hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
                                const double* coords, int n, const int* list, const int* count,
-                               int grid, int ws, const int reg[4], float* gmap, int64_t kmax,
-                               float* U0, float* F0, hipStream_t s) {
+                               int max_count, int ws, const int reg[4], void* scratch, float* U0,
+                               float* F0, hipStream_t s) {
   SearchArgs A{};
   A.f = f;
   A.g = g;
   A.M = M;
   A.N = N;
   A.ws = ws;
+  A.sp = pad_len(ws);
   for (int k = 0; k < 4; k++) A.reg[k] = reg[k];
-  A.gmap = gmap;
-  A.kmax = kmax;
+  tile_counts(reg, &A.nt1, &A.ntiles);
   A.list = list;
   A.lcount = count;
   A.ncoords = coords;
   A.nn = n;
   A.U0 = U0;
   A.F0 = F0;
-  size_t smem = 0;
-  hipError_t e = search_attr(ws, &smem);
-  if (e != hipSuccess) return e;
-  if (grid <= 0) return hipSuccess;
-  search_kernel<<<grid, 256, smem, s>>>(A);
-  return hipGetLastError();
+  carve(A, scratch, max_count);
+  return run(A, max_count, s);
 }
 
 hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, int nY,
                          const int gridx[2], const int gridy[2], int ws, int step,
-                         const int reg[4], float* gmap, int64_t kmax, double* x, double* y,
-                         double* u, double* v, double* ccmax, int* ipr, int* ipc,
-                         hipStream_t s) {
+                         const int reg[4], void* scratch, double* x, double* y, double* u,
+                         double* v, double* ccmax, int* ipr, int* ipc, hipStream_t s) {
   SearchArgs A{};
   A.f = f;
   A.g = g;
@@ -394,10 +524,10 @@ hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, i
   A.gy0 = gridy[0];
   A.gy1 = gridy[1];
   A.ws = ws;
+  A.sp = pad_len(ws);
   A.step = step;
   for (int k = 0; k < 4; k++) A.reg[k] = reg[k];
-  A.gmap = gmap;
-  A.kmax = kmax;
+  tile_counts(reg, &A.nt1, &A.ntiles);
   A.x = x;
   A.y = y;
   A.u = u;
@@ -405,12 +535,8 @@ hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, i
   A.ccmax = ccmax;
   A.ipr = ipr;
   A.ipc = ipc;
-  size_t smem = 0;
-  hipError_t e = search_attr(ws, &smem);
-  if (e != hipSuccess) return e;
-  if (nX * nY == 0) return hipSuccess;
-  search_kernel<<<nX * nY, 256, smem, s>>>(A);
-  return hipGetLastError();
+  carve(A, scratch, nX * nY);
+  return run(A, nX * nY, s);
 }
 
 }  // namespace aldic

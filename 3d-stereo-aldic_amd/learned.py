@@ -422,7 +422,8 @@ class LearnedInit:
                              self.d_F0.ptr)
         # 3. IC-GN
         local_icgn_f0_dev(im, self.plan, self.d_coords, self.d_U0.ptr, self.d_F0.ptr,
-                          self.d_winsize, self.tol, d_U, d_F, d_conv=d_conv, d_zncc=d_zncc)
+                          self.d_winsize, self.tol, d_U, d_F, d_conv=d_conv, d_zncc=d_zncc,
+                          max_iter=self.max_iter)
         # 4. gate, 5. search at failed nodes, 6. IC-GN rerun
         gate_dev(ctx, self.n, d_conv, d_zncc, self.max_iter, self.min_zncc, self.d_list.ptr,
                  self.d_count.ptr)

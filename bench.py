@@ -12,11 +12,23 @@ batch with inputs resident in HBM:
   -> initial guess of every node from the search grid
   -> first-order IC-GN of every node (tol 1e-3, <= 101 evaluations).
 
+The learned-init step replaces the dense search:
+
+  sparse integer search (same templates and radius, 128 px grid)
+  -> coarse prior at every node -> CNN on the 48 x 48 patch pair at the
+  node and at node + round(prior) -> [F0 U0] -> IC-GN
+  -> gate (more than 30 evaluations or ZNCC < 0.9) -> integer search at
+  the failed nodes -> IC-GN rerun of those nodes.
+
+The network is trained first (not timed) on the device, on 2M patch pairs
+labelled by IC-GN on seeded speckle mosaics.
+
 Converged = status 1..100, ZNCC >= 0.9.  The JSON line reports converged
-subsets/s, the RMS error of the converged subsets against the known field,
+subsets/s of the learned-init step (value) and of the search-init step
+(fft_init), the RMS error of the converged subsets against the known field,
 the roofline of the dominant kernel (its HIP-event time, algorithmic bytes)
-and, on rank 0, the FP64 CPU restatement (oracle, OpenMP) timed on a
-bounded sample on the host cores.
+and, on rank 0, the FP64 CPU restatement (oracle, OpenMP) timed on the same
+workload on the host cores.
 """
 from __future__ import annotations
 
@@ -80,7 +92,10 @@ def cpu_baseline(W, n_sample, threads, U_gpu=None):
     nY, nX = s["u"].shape
     rng = np.random.default_rng(0)
     n = len(W["ws"])
-    idx = np.sort(rng.choice(n, size=min(n_sample, n), replace=False))
+    if n_sample <= 0 or n_sample >= n:
+        idx = np.arange(n)
+    else:
+        idx = np.sort(rng.choice(n, size=n_sample, replace=False))
     coords = np.asfortranarray(W["coords"][idx])
     x0 = W["search_ws"] / 2 + 1
     fx = np.clip((coords[:, 0] - x0) / W["search_step"], 0, nX - 1)
@@ -109,10 +124,57 @@ def cpu_baseline(W, n_sample, threads, U_gpu=None):
         diag["gpu_vs_oracle_max_du_px"] = float(np.nanmax(du[conv_ok]))
     return diag, {"value": float(conv_ok.sum() / t_total), "unit": "converged subsets/s",
             "cores": threads, "kind": "port",
-            "sample": (f"FP64 oracle: integer search over the full {nX}x{nY} grid "
-                       f"({t_search:.1f} s, pro-rated to the sample) + IC-GN of {len(idx)} "
-                       f"random nodes of the {n}-node workload ({t_icgn:.1f} s), "
+            "sample": (f"FP64 oracle, search-init path: integer search over the full "
+                       f"{nX}x{nY} grid ({t_search:.1f} s"
+                       + (", pro-rated to the sample" if len(idx) < n else "")
+                       + f") + IC-GN of {len(idx)} of the {n} nodes ({t_icgn:.1f} s), "
                        f"{threads} OpenMP threads")}
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def subset_class(side):
+    return np.where(side <= 11, 0, np.where(side <= 17, 1, np.where(side <= 33, 2,
+                    np.where(side <= 65, 3, 4))))
+
+
+def accuracy(W, U, conv, z):
+    """Converged mask, overall RMS and per-size statistics against the field."""
+    cx, cy = W["coords"][:, 0], W["coords"][:, 1]
+    ok = (conv >= 1) & (conv <= 100) & (z >= 0.9)
+    eu = U[0::2] - W["uf"](cx, cy)
+    ev = U[1::2] - W["vf"](cx, cy)
+    e = np.hypot(eu, ev)
+    rms = float(np.sqrt(np.mean(e[ok] ** 2))) if ok.any() else float("nan")
+    by_size = {}
+    for w in np.unique(W["ws"]):
+        sel = W["ws"] == w
+        oks = ok & sel
+        by_size[int(w)] = {"nodes": int(sel.sum()), "converged": int(oks.sum()),
+                           "rms_px": round(float(np.sqrt(np.mean(e[oks] ** 2))), 5) if oks.any()
+                           else None,
+                           "median_px": round(float(np.median(e[oks])), 5) if oks.any() else None,
+                           "mean_iters": round(float(conv[oks].mean()), 2) if oks.any() else None}
+    return ok, rms, by_size
+
+
+def timed(ctx, api, step, steps, warmup):
+    for _ in range(warmup):
+        step()
+    ctx.synchronize()
+    api.set_timing(ctx, True)
+    api.stage_times(ctx, reset=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    stages = api.stage_times(ctx, reset=True)
+    api.set_timing(ctx, False)
+    return dt, stages
 
 
 def main():
@@ -120,9 +182,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=3000)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="IC-GN nodes of the CPU baseline (0 = all)")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--train-samples", type=int, default=2_000_000)
+    ap.add_argument("--init", choices=["both", "search", "learned"], default="both")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the local matching path is "
@@ -132,9 +197,11 @@ def main():
 
     pkg = ge.load_package()
     api = pkg.api
+    from aldic_amd import learned
     t0 = time.perf_counter()
     W = make_workload()
     t_gen = time.perf_counter() - t0
+    progress(f"workload ready ({t_gen:.1f} s, {len(W['ws'])} nodes)")
     M, N = W["M"], W["N"]
     n = len(W["ws"])
     ctx = pkg.default_context()
@@ -160,7 +227,7 @@ def main():
     d_z = DA(ctx, n, np.float32)
     x0 = sws / 2 + 1
 
-    def step():
+    def step_search():
         api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst,
                                [r, r, r, r], *[o.ptr for o in s_out], *[p.ptr for p in s_pk])
         api.grid_to_nodes_dev(ctx, s_out[2].ptr, s_out[3].ptr, nX, nY, x0, x0, sst, d_coords.ptr,
@@ -175,56 +242,78 @@ def main():
                           d_U0.ptr)
     chosen = api.local_icgn_autotune_dev(im, plan, d_coords.ptr, d_U0.ptr, d_ws.ptr, 1e-3, d_U.ptr,
                                          d_F.ptr, d_conv.ptr, d_z.ptr, reps=3)
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    api.set_timing(ctx, True)
-    api.stage_times(ctx, reset=True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    stages = api.stage_times(ctx, reset=True)
-    api.set_timing(ctx, False)
-
-    conv = d_conv.to_host()
-    U = d_U.to_host().astype(np.float64)
-    z = d_z.to_host()
-    cx, cy = W["coords"][:, 0], W["coords"][:, 1]
-    ok = (conv >= 1) & (conv <= 100) & (z >= 0.9)
-    eu = U[0::2] - W["uf"](cx, cy)
-    ev = U[1::2] - W["vf"](cx, cy)
-    rms = float(np.sqrt(np.mean(eu[ok] ** 2 + ev[ok] ** 2))) if ok.any() else float("nan")
-    n_conv = int(ok.sum())
-    value = n_conv / dt
-    e = np.hypot(eu, ev)
-    by_size = {}
-    for w in np.unique(W["ws"]):
-        sel = W["ws"] == w
-        oks = ok & sel
-        by_size[int(w)] = {"nodes": int(sel.sum()), "converged": int(oks.sum()),
-                           "rms_px": round(float(np.sqrt(np.mean(e[oks] ** 2))), 5) if oks.any()
-                           else None,
-                           "median_px": round(float(np.median(e[oks])), 5) if oks.any() else None,
-                           "mean_iters": round(float(conv[oks].mean()), 2) if oks.any() else None}
-
-    # ---- roofline of the dominant kernel (HIP events on the ctx stream) ----
+    res = {}
     side = W["ws"] + 1
-    cls = np.where(side <= 11, 0, np.where(side <= 17, 1, np.where(side <= 33, 2,
-                   np.where(side <= 65, 3, 4))))
-    kern = {}
-    for c in range(4):
-        name = f"icgn_c{c}"
-        if name in stages:
-            ms, cnt = stages[name]
-            byt = float(icgn_bytes(side[cls == c]).sum())
-            kern[name] = (ms / cnt, byt)
+    cls = subset_class(side)
+
+    def record(name, dt, stages, U, conv, z, extra=None):
+        ok, rms, by_size = accuracy(W, U, conv, z)
+        kern = {}
+        for c in range(4):
+            key = f"icgn_c{c}"
+            if key in stages:
+                ms, cnt = stages[key]
+                kern[key] = (ms / cnt, float(icgn_bytes(side[cls == c]).sum()))
+        res[name] = dict(dt=dt, stages=stages, ok=ok, rms=rms, by_size=by_size, kern=kern,
+                         U=U, **(extra or {}))
+
+    progress(f"autotuned launch shapes {list(chosen)}")
+    if args.init in ("both", "search"):
+        dt, stages = timed(ctx, api, step_search, args.steps, args.warmup)
+        progress(f"search-init step {dt * 1e3:.3f} ms")
+        record("search", dt, stages, d_U.to_host().astype(np.float64), d_conv.to_host(),
+               d_z.to_host())
+
+    if args.init in ("both", "learned"):
+        # ---- train the network on the device (not timed) ----
+        t0 = time.perf_counter()
+        net = learned.Network(ctx, max_batch=8192, seed=0)
+        hist = learned.train(net, samples=args.train_samples, batch=256, lr=2e-3, tiles=64,
+                             seed=4, log_every=1000, log=progress)
+        ev = learned.evaluate(net, seed=12345, tiles=32)
+        t_train = time.perf_counter() - t0
+        li = learned.LearnedInit(net, im, d_f.ptr, d_g.ptr, plan, d_coords.ptr, n, d_ws.ptr,
+                                 prior_grid=([1, M], [1, N], sws, 128), search_ws=sws,
+                                 region=(r, r, r, r), max_iter=30, min_zncc=0.9, tol=1e-3)
+
+        def step_learned():
+            li.run(d_U.ptr, d_F.ptr, d_conv.ptr, d_z.ptr)
+
+        dt, stages = timed(ctx, api, step_learned, args.steps, args.warmup)
+        progress(f"learned-init step {dt * 1e3:.3f} ms")
+        train = {"samples": hist["samples"], "steps": hist["steps"],
+                 "kept_frac": round(hist["kept"] / max(hist["total"], 1), 5),
+                 "seconds": round(t_train, 1), "final_loss": round(hist["loss"][-1], 5)
+                 if hist["loss"] else None,
+                 "heldout_rms_u_px": round(ev["rms_u_px"], 4),
+                 "heldout_rms_F": round(ev["rms_F"], 5)}
+        record("learned", dt, stages, d_U.to_host().astype(np.float64), d_conv.to_host(),
+               d_z.to_host(), dict(fallback=li.fallback_count(), train=train,
+                                   prior_points=li.nX * li.nY))
+
+    head = res.get("learned") or res["search"]
+    n_conv = int(head["ok"].sum())
+    value = n_conv / head["dt"]
+    # ---- roofline of the dominant IC-GN kernel (HIP events on the ctx stream) ----
+    kern = head["kern"]
     dom = max(kern, key=lambda k: kern[k][0])
     dom_ms, dom_bytes = kern[dom]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    per_step_ms = {k: round(v[0] / args.steps, 4) for k, v in stages.items()}
+
+    def summary(name):
+        d = res[name]
+        nc = int(d["ok"].sum())
+        out = {"value": round(nc / d["dt"], 1), "ms_per_step": round(d["dt"] * 1e3, 4),
+               "converged": nc, "converged_frac": round(nc / n, 5), "rms_px": round(d["rms"], 5),
+               "rms_ok": bool(d["rms"] <= 0.01), "by_subset_size": d["by_size"],
+               "stage_ms_per_step": {k: round(v[0] / args.steps, 4)
+                                     for k, v in d["stages"].items()}}
+        if name == "learned":
+            out["fallback"] = d["fallback"]
+            out["fallback_rate"] = round(d["fallback"] / n, 5)
+            out["prior_search_points"] = d["prior_points"]
+            out["train"] = d["train"]
+        return out
 
     line = {
         "metric": "converged IC-GN subsets/s at <=0.01 px RMS, learned vs FFT init, 1 MI355X",
@@ -233,22 +322,24 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt * 1e3, 4),
+        "ms_per_step": round(head["dt"] * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "stereo pair 2048x2048 12-bit speckle, hole r=300, "
-                               f"{n} nodes with 8-64 px subsets, ZNCC search r=50 on a "
-                               f"{nX}x{nY} grid, FFT-free direct ZNCC init, IC-GN tol 1e-3",
-                   "init": "search", "nodes": n, "search_points": ng},
+                               f"{n} nodes with 8-64 px subsets; init: "
+                               + ("learned (CNN on a 128 px sparse-search prior, gate + "
+                                  "search fallback)" if "learned" in res else
+                                  f"ZNCC search r=50 on a {nX}x{nY} grid")
+                               + "; IC-GN tol 1e-3",
+                   "init": "learned" if "learned" in res else "search", "nodes": n,
+                   "search_points": ng},
         "converged": n_conv,
         "converged_frac": round(n_conv / n, 5),
-        "rms_px": round(rms, 5),
-        "rms_ok": bool(rms <= 0.01),
-        "by_subset_size": by_size,
-        "stage_ms_per_step": per_step_ms,
+        "rms_px": round(head["rms"], 5),
+        "rms_ok": bool(head["rms"] <= 0.01),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -256,8 +347,18 @@ def main():
         "gen_s": round(t_gen, 1),
         "icgn_launch_shapes": [int(c) for c in chosen],
     }
+    for name in ("learned", "search"):
+        if name in res:
+            line["learned_init" if name == "learned" else "fft_init"] = summary(name)
+    if "learned" in res and "search" in res:
+        line["learned_vs_fft"] = round((res["learned"]["ok"].sum() / res["learned"]["dt"]) /
+                                       (res["search"]["ok"].sum() / res["search"]["dt"]), 3)
     if not args.no_cpu:
-        diag, line["cpu_baseline"] = cpu_baseline(W, args.cpu_sample, args.cpu_threads, U)
+        progress("CPU baseline (FP64 oracle)")
+        # the oracle runs the search-init path: compare it with the GPU's
+        ref = res.get("search")
+        diag, line["cpu_baseline"] = cpu_baseline(W, args.cpu_sample, args.cpu_threads,
+                                                  ref["U"] if ref else None)
         line["oracle_sample"] = diag
     else:
         line["cpu_baseline"] = None

@@ -165,12 +165,15 @@ int aldic_plan_set_variant(aldic_plan* plan, int32_t solver, int32_t cls, int32_
 
 /* Same as aldic_local_icgn_dev with a starting guess of the gradients:
  * P0 = [F0 U0] instead of [0 0 0 0 U0] (funICGNQuadtree.m:41).  d_F0 is 4n
- * floats [u_x v_x u_y v_y] per node, or NULL for zeros.  The learned
- * starting point feeds this; it has no counterpart in the reference. */
+ * floats [u_x v_x u_y v_y] per node, or NULL for zeros.  max_iter caps the
+ * loop at k <= max_iter (0: the reference's 100, funICGNQuadtree.m:135);
+ * a subset stopped by the cap reports 101.  The learned starting point
+ * feeds this; it has no counterpart in the reference. */
 int aldic_local_icgn_f0_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
                             const double* d_coords, const float* d_U0, const float* d_F0,
-                            const int32_t* d_winsize, double tol, float* d_U, float* d_F,
-                            float* d_H21, int32_t* d_convIt, uint8_t* d_hole, float* d_zncc);
+                            const int32_t* d_winsize, double tol, int32_t max_iter, float* d_U,
+                            float* d_F, float* d_H21, int32_t* d_convIt, uint8_t* d_hole,
+                            float* d_zncc);
 
 /* Device-resident Subpb1. */
 int aldic_subpb1_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,

@@ -243,8 +243,8 @@ def test_training_gains_over_prior_alone(L):
     api, learned = L
     ctx = api.default_context()
     net = learned.Network(ctx, max_batch=256, seed=0)
-    hist = learned.train(net, samples=150_000, batch=256, lr=2e-3, tiles=48, seed=11,
-                         log_every=100, log=print)
+    hist = learned.train(net, samples=1_000_000, batch=256, lr=2e-3, tiles=48, seed=11,
+                         log_every=1000, log=print)
     ev = learned.evaluate(net, seed=12345, tiles=24)
     print("eval", ev, "train s", hist["seconds"])
     # a constant prediction of a uniform +-12 px translation has RMS ~9.8 px
