@@ -35,6 +35,8 @@ SIGNATURES = {
     "aldic_memcpy_htod": (C.c_int, [_vp, _vp, _vp, _i64]),
     "aldic_memcpy_dtoh": (C.c_int, [_vp, _vp, _vp, _i64]),
     "aldic_ctx_set_timing": (C.c_int, [_vp, C.c_int32]),
+    "aldic_ctx_record": (C.c_int, [_vp, C.c_int32]),
+    "aldic_ctx_wait": (C.c_int, [_vp, _vp, C.c_int32]),
     "aldic_ctx_stage_times": (C.c_int, [_vp, _vp, _vp, C.c_int32]),
     "aldic_grid_to_nodes_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.c_double, C.c_double,
                                           C.c_double, _vp, _i64, _vp]),
@@ -85,7 +87,7 @@ SIGNATURES = {
     "aldic_gate_dev": (C.c_int, [_vp, _i64, _vp, _vp, C.c_int32, C.c_double, _vp, _vp]),
     "aldic_search_nodes_dev": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, C.c_int32,
                                          C.POINTER(_i64), _vp, _vp]),
-    "aldic_local_icgn_list_dev": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64,
+    "aldic_local_icgn_list_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                             C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
 }
 

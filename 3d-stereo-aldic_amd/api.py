@@ -57,6 +57,14 @@ class Context:
     def synchronize(self):
         check(lib().aldic_ctx_synchronize(self.ptr))
 
+    def record(self, mark: int):
+        """Mark the point reached by the work submitted so far."""
+        check(lib().aldic_ctx_record(self.ptr, int(mark)))
+
+    def wait(self, other: "Context", mark: int):
+        """Make later work on this context wait for `mark` of `other`."""
+        check(lib().aldic_ctx_wait(self.ptr, other.ptr, int(mark)))
+
     def close(self):
         if self.ptr:
             lib().aldic_ctx_destroy(self.ptr)
@@ -540,6 +548,10 @@ class Plan:
         self.ptr = p
         self.n = int(n)
 
+    def set_variant(self, solver: int, cls: int, variant: int):
+        """Fix the launch shape of one size class (solver 0 IC-GN, 1 Subpb1)."""
+        check(lib().aldic_plan_set_variant(self.ptr, int(solver), int(cls), int(variant)))
+
     def close(self):
         if getattr(self, "ptr", None):
             lib().aldic_plan_destroy(self.ptr)
@@ -553,10 +565,10 @@ class Plan:
 
 
 def local_icgn_dev(images: Images, plan: Plan, d_coords, d_U0, d_winsize, tol, d_U, d_F=None,
-                   d_H21=None, d_conv=None, d_hole=None, d_zncc=None):
-    check(lib().aldic_local_icgn_dev(images.ctx.ptr, images.ptr, plan.ptr, d_coords, d_U0,
-                                     d_winsize, float(tol), d_U, d_F, d_H21, d_conv, d_hole,
-                                     d_zncc))
+                   d_H21=None, d_conv=None, d_hole=None, d_zncc=None, ctx: Context | None = None):
+    c = ctx or images.ctx
+    check(lib().aldic_local_icgn_dev(c.ptr, images.ptr, plan.ptr, d_coords, d_U0, d_winsize,
+                                     float(tol), d_U, d_F, d_H21, d_conv, d_hole, d_zncc))
 
 
 def subpb1_dev(images: Images, plan: Plan, d_coords, d_UOld, d_FOld, d_vdual, d_winsize_xy, mu,
@@ -597,20 +609,24 @@ def gate_dev(ctx: Context, n, d_conv, d_zncc, max_iter, min_zncc, d_list, d_coun
 
 
 def search_nodes_dev(images: Images, d_coords, n, d_list, d_count, max_count, winsize, region,
-                     d_U0, d_F0=None):
-    """Integer search centred on each listed node -> U0 (and F0 = 0)."""
+                     d_U0, d_F0=None, ctx: Context | None = None):
+    """Integer search centred on each listed node -> U0 (and F0 = 0); runs
+    on ctx (default: the images' context)."""
     rg = (C.c_int64 * 4)(*[int(v) for v in region])
-    check(lib().aldic_search_nodes_dev(images.ctx.ptr, images.ptr, d_coords, int(n), d_list,
-                                       d_count, int(max_count), int(winsize), rg, d_U0, d_F0))
+    c = ctx or images.ctx
+    check(lib().aldic_search_nodes_dev(c.ptr, images.ptr, d_coords, int(n), d_list, d_count,
+                                       int(max_count), int(winsize), rg, d_U0, d_F0))
 
 
-def local_icgn_list_dev(images: Images, d_coords, n, d_U0, d_F0, d_winsize, d_list, d_count,
-                        max_count, tol, d_U, d_F=None, d_H21=None, d_conv=None, d_hole=None,
-                        d_zncc=None):
-    """IC-GN of the listed nodes only (outputs of the others untouched)."""
-    check(lib().aldic_local_icgn_list_dev(images.ctx.ptr, images.ptr, d_coords, int(n), d_U0,
-                                          d_F0, d_winsize, d_list, d_count, int(max_count),
-                                          float(tol), d_U, d_F, d_H21, d_conv, d_hole, d_zncc))
+def local_icgn_list_dev(images: Images, plan: Plan, d_coords, d_U0, d_F0, d_winsize, d_list,
+                        d_count, tol, d_U, d_F=None, d_H21=None, d_conv=None, d_hole=None,
+                        d_zncc=None, ctx: Context | None = None):
+    """IC-GN of the listed nodes of a plan only (outputs of the others
+    untouched); runs on ctx (default: the images' context)."""
+    c = ctx or images.ctx
+    check(lib().aldic_local_icgn_list_dev(c.ptr, images.ptr, plan.ptr, d_coords, d_U0, d_F0,
+                                          d_winsize, d_list, d_count, float(tol), d_U, d_F, d_H21,
+                                          d_conv, d_hole, d_zncc))
 
 
 STAGES = {0: "search", 1: "grid_to_nodes", 2: "icgn_c0", 3: "icgn_c1", 4: "icgn_c2",

@@ -40,6 +40,8 @@ struct aldic_ctx {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   double stage_ms[ALDIC_NSTAGES] = {0};
   int stage_cnt[ALDIC_NSTAGES] = {0};
+  // cross-context ordering points (aldic_ctx_record / aldic_ctx_wait)
+  hipEvent_t marks[ALDIC_NMARKS] = {nullptr};
 };
 
 // Handles record their device so they can be destroyed after their context.
@@ -63,6 +65,9 @@ struct aldic_plan {
   int cnt[5] = {0, 0, 0, 0, 0};
   int* defer_cnt = nullptr;
   int* defer_list = nullptr;
+  // per-class lists of a node subset (IC-GN of listed nodes): capacity cnt[c]
+  int* sub_lists[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int* sub_cnt = nullptr;  // 5 counts
   // launch shape per size class and solver (0 IC-GN, 1 Subpb1)
   int var[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 };

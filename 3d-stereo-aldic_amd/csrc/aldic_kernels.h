@@ -31,6 +31,7 @@ struct SolveArgs {
   float mu;
   float tol;
   int maxit;            // evaluation cap k <= maxit (0: the reference's 100)
+  const int* dcnt;      // fast kernels: list length read on the device (cnt = capacity)
   float* U;             // 2n
   float* F;             // 4n (IC-GN), may be null
   float* H21;           // n x 21 column-major (IC-GN), may be null
@@ -48,6 +49,15 @@ int num_variants();
 hipError_t launch_generic(int mode, const SolveArgs& a, const int* list, const int* count,
                           int fixed_count, int grid, hipStream_t s);
 int generic_max_pixels();
+// Sort the nodes list[0 .. *count) into per-size-class lists (appended at
+// counts[c]; counts must start at zero).
+struct ClassLists {
+  int* list[5];
+  int* counts;
+};
+hipError_t launch_bucket(const SolveArgs& a, const int* list, const int* count, int max_count,
+                         ClassLists out, hipStream_t s);
+__global__ void zero_ints_kernel(int* p, int n);
 
 // image preparation (prep.hip)
 hipError_t launch_prep_planes(const double* ref, const double* def, const double* mask,
@@ -115,6 +125,9 @@ hipError_t cnn_mse(const float* pred, int ldp, const float* lab, const float* sc
                    float* d, int ldd, float* loss, hipStream_t s);
 hipError_t cnn_patches(const float* f, const float* g, int M, int N, const double* coords, int n,
                        int first, int count, const float* prior, void* out, hipStream_t s);
+hipError_t cnn_front(const float* f, const float* g, int M, int N, const double* coords, int n,
+                     int first, int count, const float* prior, const void* W1, const float* b1,
+                     const void* W2, const float* b2, void* out, hipStream_t s);
 hipError_t cnn_predict(const float* out, int ldo, int first, int count, const float* prior,
                        float sf, float su, float* U0, float* F0, hipStream_t s);
 hipError_t cnn_mosaic(int M, int N, int tile, const float* warps, int ntx, uint32_t seed,

@@ -99,12 +99,18 @@ int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, 
   }
   HIP_TRY(hipMalloc(&p->defer_cnt, sizeof(int)));
   HIP_TRY(hipMalloc(&p->defer_list, sizeof(int) * std::max<int64_t>(n, 1)));
+  for (int c = 0; c < 5; c++)
+    if (p->cnt[c] > 0) HIP_TRY(hipMalloc(&p->sub_lists[c], sizeof(int) * p->cnt[c]));
+  HIP_TRY(hipMalloc(&p->sub_cnt, sizeof(int) * 5));
   return ALDIC_OK;
 }
 
 void free_plan(aldic_plan* p) {
-  for (int c = 0; c < 5; c++)
+  for (int c = 0; c < 5; c++) {
     if (p->lists[c]) (void)hipFree(p->lists[c]);
+    if (p->sub_lists[c]) (void)hipFree(p->sub_lists[c]);
+  }
+  if (p->sub_cnt) (void)hipFree(p->sub_cnt);
   if (p->defer_cnt) (void)hipFree(p->defer_cnt);
   if (p->defer_list) (void)hipFree(p->defer_list);
 }
@@ -204,10 +210,30 @@ extern "C" int aldic_ctx_destroy(aldic_ctx* ctx) {
   }
   for (hipEvent_t e : ctx->ev_pool)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ctx->marks)
+    if (e) (void)hipEventDestroy(e);
   for (void* p : ctx->buf)
     if (p) (void)hipFree(p);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_ctx_record(aldic_ctx* ctx, int32_t mark) {
+  if (!ctx || mark < 0 || mark >= ALDIC_NMARKS) return fail(ALDIC_EINVAL, "ctx_record: bad argument");
+  RC_TRY(set_device(ctx));
+  if (!ctx->marks[mark])
+    HIP_TRY(hipEventCreateWithFlags(&ctx->marks[mark], hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(ctx->marks[mark], ctx->stream));
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_ctx_wait(aldic_ctx* ctx, aldic_ctx* other, int32_t mark) {
+  if (!ctx || !other || mark < 0 || mark >= ALDIC_NMARKS)
+    return fail(ALDIC_EINVAL, "ctx_wait: bad argument");
+  if (!other->marks[mark]) return ALDIC_OK;  // never recorded: nothing to wait for
+  RC_TRY(set_device(ctx));
+  HIP_TRY(hipStreamWaitEvent(ctx->stream, other->marks[mark], 0));
   return ALDIC_OK;
 }
 
@@ -591,25 +617,48 @@ extern "C" int aldic_search_nodes_dev(aldic_ctx* ctx, const aldic_images* im,
   return ALDIC_OK;
 }
 
-// IC-GN over the flagged nodes d_list[0 .. *d_count) with the workgroup
-// kernel (any subset size); the count is read on the device
+// IC-GN over the listed nodes d_list[0 .. *d_count) of a plan: the nodes
+// are sorted into per-size-class lists on the device and run through the
+// same wave kernels as the whole plan (launched at each class's capacity,
+// the count read on the device), so no host round trip is needed
 extern "C" int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im,
-                                         const double* d_coords, int64_t n, const float* d_U0,
-                                         const float* d_F0, const int32_t* d_winsize,
-                                         const int32_t* d_list, const int32_t* d_count,
-                                         int64_t max_count, double tol, float* d_U, float* d_F,
-                                         float* d_H21, int32_t* d_convIt, uint8_t* d_hole,
-                                         float* d_zncc) {
-  if (!ctx || !images_ok(im)) return fail(ALDIC_EINVAL, "local_icgn_list_dev: bad context/images");
+                                         const aldic_plan* plan, const double* d_coords,
+                                         const float* d_U0, const float* d_F0,
+                                         const int32_t* d_winsize, const int32_t* d_list,
+                                         const int32_t* d_count, double tol, float* d_U,
+                                         float* d_F, float* d_H21, int32_t* d_convIt,
+                                         uint8_t* d_hole, float* d_zncc) {
+  if (!ctx || !images_ok(im) || !plan || plan->rect)
+    return fail(ALDIC_EINVAL, "local_icgn_list_dev: bad context/images/plan");
   if (!d_coords || !d_U0 || !d_winsize || !d_list || !d_count || !d_U || !d_convIt)
     return fail(ALDIC_EINVAL, "local_icgn_list_dev: NULL array");
-  if (max_count <= 0 || n == 0) return ALDIC_OK;
+  if (plan->n == 0) return ALDIC_OK;
   RC_TRY(set_device(ctx));
-  SolveArgs a = icgn_args(im, n, d_coords, d_U0, d_F0, d_winsize, tol, d_U, d_F, d_H21, d_convIt,
-                          d_hole, d_zncc);
-  const int grid = (int)std::min<int64_t>(max_count, 2048);
+  hipStream_t s = ctx->stream;
+  SolveArgs a = icgn_args(im, plan->n, d_coords, d_U0, d_F0, d_winsize, tol, d_U, d_F, d_H21,
+                          d_convIt, d_hole, d_zncc);
+  a.defer_cnt = plan->defer_cnt;
+  a.defer_list = plan->defer_list;
   StageTimer t(ctx, ALDIC_STAGE_FALLBACK);
-  HIP_TRY(aldic::launch_generic(0, a, d_list, d_count, 0, grid, ctx->stream));
+  aldic::ClassLists cl;
+  for (int c = 0; c < 5; c++) cl.list[c] = plan->sub_lists[c];
+  cl.counts = plan->sub_cnt;
+  HIP_TRY(aldic::launch_bucket(a, d_list, d_count, (int)plan->n, cl, s));
+  zero_ints<<<1, 64, 0, s>>>(plan->defer_cnt, 1);
+  HIP_TRY(hipGetLastError());
+  for (int c = 0; c < 4; c++) {
+    if (plan->cnt[c] == 0) continue;
+    a.list = plan->sub_lists[c];
+    a.cnt = plan->cnt[c];
+    a.dcnt = plan->sub_cnt + c;
+    HIP_TRY(aldic::launch_fast(0, c, plan->var[0][c], a, s));
+  }
+  a.dcnt = nullptr;
+  if (plan->cnt[4] > 0)
+    HIP_TRY(aldic::launch_generic(0, a, plan->sub_lists[4], plan->sub_cnt + 4, 0,
+                                  std::min(plan->cnt[4], 2048), s));
+  const int grid = (int)std::min<int64_t>(std::max<int64_t>(plan->n, 1), 1024);
+  HIP_TRY(aldic::launch_generic(0, a, plan->defer_list, plan->defer_cnt, 0, grid, s));
   return ALDIC_OK;
 }
 

@@ -537,9 +537,78 @@ __global__ void mse_kernel(const float* __restrict__ pred, int ldp, const float*
 // node and the deformed patch centred on node + shift (integer prior),
 // each normalised to zero mean and unit variance, as NHWC bf16 [B,48,48,2].
 // Image pixel (i, j) 1-based; patch pixel (py, px) = image
-// (c - 24 + px, c' - 24 + py) along (dim 1, dim 2).  One wave per node.
+// (c - 24 + px, c' - 24 + py) along (dim 1, dim 2).  One 256-thread
+// workgroup per node; the moments are summed in a fixed order (9 values
+// per thread, wave sums, then the 4 waves in order) that the fused front
+// end repeats, so both produce the same patches.
 // ---------------------------------------------------------------------------
 constexpr int PATCH = 48;
+
+struct PatchMoments {
+  float mf, sf, mg, sg;
+};
+
+// the 36 pixels of each image a thread owns: p = tid + 256 k
+__device__ __forceinline__ void patch_raw(const float* __restrict__ f, const float* __restrict__ g,
+                                          int M, int N, int cx, int cy, int sxs, int sys,
+                                          float* vf, float* vg) {
+  constexpr int NP = PATCH * PATCH;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < NP / 256; k++) {
+    const int p = tid + 256 * k;
+    const int px = p % PATCH, py = p / PATCH;
+    const int i = min(max(cx - PATCH / 2 + px, 1), M), j = min(max(cy - PATCH / 2 + py, 1), N);
+    vf[k] = f[(int64_t)(i - 1) + (int64_t)(j - 1) * M];
+    const int i2 = min(max(cx + sxs - PATCH / 2 + px, 1), M);
+    const int j2 = min(max(cy + sys - PATCH / 2 + py, 1), N);
+    vg[k] = g[(int64_t)(i2 - 1) + (int64_t)(j2 - 1) * M];
+  }
+}
+
+__device__ __forceinline__ PatchMoments patch_moments(const float* vf, const float* vg,
+                                                      float* red) {
+  constexpr int NP = PATCH * PATCH;
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  float st[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < NP / 256; k++) {
+    st[0] += vf[k];
+    st[1] += vf[k] * vf[k];
+    st[2] += vg[k];
+    st[3] += vg[k] * vg[k];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const float v = wave_sum(st[q]);
+    if (lane == 0) red[q * 4 + wave] = v;
+  }
+  __syncthreads();
+  float S[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) S[q] = red[q * 4] + red[q * 4 + 1] + red[q * 4 + 2] + red[q * 4 + 3];
+  __syncthreads();  // red is reused by the next node
+  PatchMoments m;
+  m.mf = S[0] / NP;
+  m.mg = S[2] / NP;
+  m.sf = rsqrtf(fmaxf(S[1] / NP - m.mf * m.mf, 1e-12f));
+  m.sg = rsqrtf(fmaxf(S[3] / NP - m.mg * m.mg, 1e-12f));
+  return m;
+}
+
+__device__ __forceinline__ uint32_t patch_pair(float vf, float vg, const PatchMoments& m) {
+  const uint32_t lo = __builtin_bit_cast(uint16_t, to_bf16((vf - m.mf) * m.sf));
+  const uint32_t hi = __builtin_bit_cast(uint16_t, to_bf16((vg - m.mg) * m.sg));
+  return lo | (hi << 16);
+}
+
+__device__ __forceinline__ void node_shift(const double* coords, int n, const float* prior,
+                                           int node, int* cx, int* cy, int* sxs, int* sys) {
+  *cx = (int)mround(coords[node]);
+  *cy = (int)mround(coords[node + n]);
+  *sxs = prior ? (int)rintf(prior[2 * node]) : 0;
+  *sys = prior ? (int)rintf(prior[2 * node + 1]) : 0;
+}
 
 __global__ __launch_bounds__(256) void patch_kernel(const float* __restrict__ f,
                                                     const float* __restrict__ g, int M, int N,
@@ -547,48 +616,18 @@ __global__ __launch_bounds__(256) void patch_kernel(const float* __restrict__ f,
                                                     int first, int count,
                                                     const float* __restrict__ prior,
                                                     bf16* __restrict__ out) {
-  const int lane = threadIdx.x % 64;
-  const int local = blockIdx.x * 4 + threadIdx.x / 64;
-  if (local >= count) return;
-  const int node = first + local;
-  const int cx = (int)mround(coords[node]), cy = (int)mround(coords[node + n]);
-  int sx = 0, sy = 0;
-  if (prior) {
-    sx = (int)rintf(prior[2 * node]);
-    sy = (int)rintf(prior[2 * node + 1]);
-  }
   constexpr int NP = PATCH * PATCH;
-  constexpr int PER = NP / 64;  // 36 pixels per lane
-  float vf[PER], vg[PER];
-  float s[4] = {0, 0, 0, 0};
+  __shared__ float red[16];
+  for (int local = blockIdx.x; local < count; local += gridDim.x) {
+    int cx, cy, sxs, sys;
+    node_shift(coords, n, prior, first + local, &cx, &cy, &sxs, &sys);
+    float vf[NP / 256], vg[NP / 256];
+    patch_raw(f, g, M, N, cx, cy, sxs, sys, vf, vg);
+    const PatchMoments m = patch_moments(vf, vg, red);
+    // one 4-byte store per pixel: (reference, deformed) channel pair
+    uint32_t* o = reinterpret_cast<uint32_t*>(out + (int64_t)local * NP * 2);
 #pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int p = lane + 64 * k;
-    const int px = p % PATCH, py = p / PATCH;
-    int i = cx - PATCH / 2 + px, j = cy - PATCH / 2 + py;  // 1-based
-    i = min(max(i, 1), M);
-    j = min(max(j, 1), N);
-    vf[k] = f[(int64_t)(i - 1) + (int64_t)(j - 1) * M];
-    int i2 = min(max(cx + sx - PATCH / 2 + px, 1), M), j2 = min(max(cy + sy - PATCH / 2 + py, 1), N);
-    vg[k] = g[(int64_t)(i2 - 1) + (int64_t)(j2 - 1) * M];
-    s[0] += vf[k];
-    s[1] += vf[k] * vf[k];
-    s[2] += vg[k];
-    s[3] += vg[k] * vg[k];
-  }
-#pragma unroll
-  for (int q = 0; q < 4; q++) s[q] = wave_sum(s[q]);
-  const float mf = s[0] / NP, mg = s[2] / NP;
-  const float sf = rsqrtf(fmaxf(s[1] / NP - mf * mf, 1e-12f));
-  const float sg = rsqrtf(fmaxf(s[3] / NP - mg * mg, 1e-12f));
-  // one 4-byte store per pixel: (reference, deformed) channel pair
-  uint32_t* o = reinterpret_cast<uint32_t*>(out + (int64_t)local * NP * 2);
-#pragma unroll
-  for (int k = 0; k < PER; k++) {
-    const int p = lane + 64 * k;
-    const uint32_t lo = __builtin_bit_cast(uint16_t, to_bf16((vf[k] - mf) * sf));
-    const uint32_t hi = __builtin_bit_cast(uint16_t, to_bf16((vg[k] - mg) * sg));
-    o[p] = lo | (hi << 16);
+    for (int k = 0; k < NP / 256; k++) o[threadIdx.x + 256 * k] = patch_pair(vf[k], vg[k], m);
   }
 }
 
@@ -695,6 +734,130 @@ __global__ void mosaic_kernel(int M, int N, int tile, const float* __restrict__ 
     const float u1 = hash_unit(noise_seed, (uint32_t)e, 7), u2 = hash_unit(noise_seed, (uint32_t)e, 9);
     gl += noise * scale * sqrtf(-2.0f * __logf(u1)) * __cosf(6.2831853f * u2);
     img[e] = fminf(fmaxf(rintf(gl), 0.0f), top);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Inference front end, fused: per node, the normalised 48 x 48 patch pair
+// (as patch_kernel), conv 2->32 stride 2 (48 -> 24) and conv 32->32
+// stride 2 (24 -> 12), with the patch and the 24 x 24 x 32 activation kept
+// in LDS.  The matrix-core arithmetic is the same as the layer-by-layer
+// path (same fragments, same k order, same bias/ReLU/bf16 rounding), so the
+// 12 x 12 x 32 output is identical to it.  One workgroup per node at a
+// time (persistent over nodes).
+// ---------------------------------------------------------------------------
+constexpr int FR_L1C = 40;  // padded channel stride of the L1 activation in LDS
+
+__global__ __launch_bounds__(256) void front_kernel(const float* __restrict__ f,
+                                                    const float* __restrict__ g, int M, int N,
+                                                    const double* __restrict__ coords, int n,
+                                                    int first, int count,
+                                                    const float* __restrict__ prior,
+                                                    const bf16* __restrict__ W1,
+                                                    const float* __restrict__ b1,
+                                                    const bf16* __restrict__ W2,
+                                                    const float* __restrict__ b2,
+                                                    bf16* __restrict__ out) {
+  constexpr int NP = PATCH * PATCH;
+  constexpr int K1 = 32, K2 = 288;
+  constexpr int RS1 = K1 / 8 + 1, RS2 = K2 / 8 + 1;
+  __shared__ uint32_t sx[NP];                  // patch: (ref, def) bf16 pairs
+  __shared__ bf16 sa1[24 * 24 * FR_L1C];       // L1 output, NHWC, padded channels
+  __shared__ bf16x8 sW1[32 * RS1];
+  __shared__ bf16x8 sW2[32 * RS2];
+  __shared__ float red[16];
+  const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+  const int r = lane % 32, hq = lane / 32, kh = 8 * hq;
+  for (int e = tid; e < 32 * (K1 / 8); e += 256) {
+    const int row = e / (K1 / 8), q = e % (K1 / 8);
+    sW1[row * RS1 + q] = *(const bf16x8*)(W1 + row * K1 + q * 8);
+  }
+  for (int e = tid; e < 32 * (K2 / 8); e += 256) {
+    const int row = e / (K2 / 8), q = e % (K2 / 8);
+    sW2[row * RS2 + q] = *(const bf16x8*)(W2 + row * K2 + q * 8);
+  }
+  const float bias1 = b1[r], bias2 = b2[r];
+  __syncthreads();
+  // the pixels of the next node are loaded while this node's convolutions run
+  float vf[NP / 256], vg[NP / 256];
+  if (blockIdx.x < count) {
+    int cx, cy, sxs, sys;
+    node_shift(coords, n, prior, first + blockIdx.x, &cx, &cy, &sxs, &sys);
+    patch_raw(f, g, M, N, cx, cy, sxs, sys, vf, vg);
+  }
+  for (int local = blockIdx.x; local < count; local += gridDim.x) {
+    // ---- patch pair, normalised (patch_kernel's arithmetic) ----
+    const PatchMoments m = patch_moments(vf, vg, red);
+#pragma unroll
+    for (int k = 0; k < NP / 256; k++) sx[tid + 256 * k] = patch_pair(vf[k], vg[k], m);
+    __syncthreads();
+    if (local + (int)gridDim.x < count) {
+      int cx, cy, sxs, sys;
+      node_shift(coords, n, prior, first + local + gridDim.x, &cx, &cy, &sxs, &sys);
+      patch_raw(f, g, M, N, cx, cy, sxs, sys, vf, vg);
+    }
+    // ---- L1: 576 rows (24 x 24), K 32 (taps 0..8 x 2 channels, padded), N 32 ----
+    for (int tile = wave; tile < 18; tile += 4) {
+      const int row = tile * 32 + r;
+      const int oy = row / 24, ox = row % 24;
+      floatx16 acc;
+#pragma unroll
+      for (int i = 0; i < 16; i++) acc[i] = 0.0f;
+#pragma unroll
+      for (int st2 = 0; st2 < 2; st2++) {
+        bf16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int t = 8 * st2 + 4 * hq + q;
+          const int ky = t / 3, kx = t % 3;
+          const int iy = oy * 2 - 1 + ky, ix = ox * 2 - 1 + kx;
+          if (t < 9 && iy >= 0 && iy < PATCH && ix >= 0 && ix < PATCH) {
+            const uint32_t w2 = sx[iy * PATCH + ix];
+            v[2 * q] = __builtin_bit_cast(bf16, (uint16_t)(w2 & 0xffffu));
+            v[2 * q + 1] = __builtin_bit_cast(bf16, (uint16_t)(w2 >> 16));
+          }
+        }
+        const bf16x8 b = sW1[r * RS1 + 2 * st2 + hq];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v, b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int orow = tile * 32 + 8 * (i / 4) + 4 * hq + (i % 4);
+        const float v = fmaxf(acc[i] + bias1, 0.0f);
+        sa1[orow * FR_L1C + r] = to_bf16(v);
+      }
+    }
+    __syncthreads();
+    // ---- L2: 144 rows (12 x 12), K 288 (tap-major, 32 channels), N 32 ----
+    for (int tile = wave; tile < 5; tile += 4) {
+      const int row = tile * 32 + r;
+      const bool ok = row < 144;
+      const int oy = ok ? row / 12 : 0, ox = ok ? row % 12 : 0;
+      floatx16 acc;
+#pragma unroll
+      for (int i = 0; i < 16; i++) acc[i] = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        const int iy = oy * 2 - 1 + t / 3, ix = ox * 2 - 1 + t % 3;
+        const bool in = ok && iy >= 0 && iy < 24 && ix >= 0 && ix < 24;
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+          bf16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+          if (in) v = *(const bf16x8*)(sa1 + (iy * 24 + ix) * FR_L1C + c * 16 + kh);
+          const bf16x8 b = sW2[r * RS2 + 2 * (2 * t + c) + hq];
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v, b, acc, 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int orow = tile * 32 + 8 * (i / 4) + 4 * hq + (i % 4);
+        if (orow < 144) {
+          const float v = fmaxf(acc[i] + bias2, 0.0f);
+          out[((int64_t)local * 144 + orow) * 32 + r] = to_bf16(v);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -895,8 +1058,19 @@ hipError_t cnn_patches(const float* f, const float* g, int M, int N, const doubl
                        int first, int count, const float* prior, void* out, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   if (first < 0 || first + count > n) return hipErrorInvalidValue;
-  patch_kernel<<<(count + 3) / 4, 256, 0, s>>>(f, g, M, N, coords, n, first, count, prior,
-                                               (bf16*)out);
+  patch_kernel<<<std::min(count, 4096), 256, 0, s>>>(f, g, M, N, coords, n, first, count, prior,
+                                                      (bf16*)out);
+  return hipGetLastError();
+}
+
+hipError_t cnn_front(const float* f, const float* g, int M, int N, const double* coords, int n,
+                     int first, int count, const float* prior, const void* W1, const float* b1,
+                     const void* W2, const float* b2, void* out, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (first < 0 || first + count > n) return hipErrorInvalidValue;
+  front_kernel<<<std::min(count, 2048), 256, 0, s>>>(f, g, M, N, coords, n, first, count, prior,
+                                                      (const bf16*)W1, b1, (const bf16*)W2, b2,
+                                                      (bf16*)out);
   return hipGetLastError();
 }
 

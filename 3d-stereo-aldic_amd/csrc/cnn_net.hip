@@ -40,6 +40,12 @@ constexpr LayerDef LAYERS[NLAYER] = {
     {64, 64, 2, 6}, {64, 64, 1, 3},  {576, 128, 0, 0}, {128, 6, 0, 0},
 };
 
+// the fused front end (cnn.hip: front_kernel) is written for this geometry
+static_assert(LAYERS[0].cin == 2 && LAYERS[0].cout == 32 && LAYERS[0].stride == 2 &&
+                  LAYERS[0].hin == 48 && LAYERS[1].cin == 32 && LAYERS[1].cout == 32 &&
+                  LAYERS[1].stride == 2 && LAYERS[1].hin == 24,
+              "front_kernel geometry");
+
 struct Layer {
   int cin, cout, stride, H, W, Ho, Wo;
   int K, Kp;  // reduction length and its padding to a multiple of 16
@@ -108,16 +114,19 @@ struct aldic_cnn {
   int64_t part_cap = 0;
   float* dout = nullptr;   // dloss/dout, FP32 [bmax x 16]
   void* doutb = nullptr;   // bf16 copy
-  void* patches = nullptr; // network input for predict_dev
   float* loss = nullptr;
   float* lscale = nullptr;
 };
 
 namespace {
 
+// Zeroed device buffer.  hipMemset runs on the null stream, which the
+// context's non-blocking stream does not wait for: finish it before any
+// kernel on that stream can touch the buffer.
 int cnn_alloc(void** p, size_t bytes) {
   HIP_TRY(hipMalloc(p, std::max<size_t>(bytes, 256)));
   HIP_TRY(hipMemset(*p, 0, std::max<size_t>(bytes, 256)));
+  HIP_TRY(hipDeviceSynchronize());
   return ALDIC_OK;
 }
 
@@ -132,7 +141,7 @@ int cnn_alloc_t(T** p, size_t count, size_t elem) {
 void cnn_free(aldic_cnn* net) {
   void* ptrs[] = {net->wf, net->wb, net->gw, net->am, net->av, net->out, net->col, net->colT,
                   net->dYT, net->dYb, net->dYf, net->dcol, net->dX, net->WT, net->WT2, net->part,
-                  net->dout, net->doutb, net->patches, net->loss, net->lscale};
+                  net->dout, net->doutb, net->loss, net->lscale};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (int l = 0; l < NLAYER; l++)
@@ -144,11 +153,12 @@ inline const uint16_t* bfp(const void* base, int64_t off) {
 }
 inline uint16_t* bfp(void* base, int64_t off) { return static_cast<uint16_t*>(base) + off; }
 
-// Forward pass over B inputs; activations stay in net->act for backward.
-int forward(aldic_cnn* net, const void* x, int B) {
+// Forward pass over B inputs from layer l0 on (l0 > 0: the input is
+// act[l0 - 1]); activations stay in net->act for backward.
+int forward(aldic_cnn* net, const void* x, int B, int l0 = 0) {
   hipStream_t s = net->ctx->stream;
-  const void* in = x;
-  for (int l = 0; l < NCONV; l++) {
+  const void* in = l0 == 0 ? x : net->act[l0 - 1];
+  for (int l = l0; l < NCONV; l++) {
     const Layer& y = net->L[l];
     HIP_TRY(aldic::cnn_conv(in, B, y.H, y.W, y.cin, y.stride, y.Ho, y.Wo, bfp(net->wb, y.w_off),
                             y.Kp, y.cout, net->wf + y.b_off, 1, net->act[l], nullptr, s));
@@ -237,6 +247,42 @@ int backward(aldic_cnn* net, const void* x, int B) {
   return ALDIC_OK;
 }
 
+// Buffers that only the layer-by-layer forward (layer-0 activation) and
+// training (im2col, transposes, gradients) use.
+int ensure_layer0(aldic_cnn* net) {
+  if (net->act[0]) return ALDIC_OK;
+  const Layer& y = net->L[0];
+  return cnn_alloc(&net->act[0], (size_t)net->bmax * y.Ho * y.Wo * y.cout * 2);
+}
+
+int ensure_training(aldic_cnn* net) {
+  if (net->col) return ALDIC_OK;
+  const int64_t mb = net->bmax;
+  int64_t max_act = 0, max_col = 0, max_wt = 0, max_dx = 0;
+  for (int l = 0; l < NLAYER; l++) {
+    const Layer& y = net->L[l];
+    const int64_t rows = mb * y.Ho * y.Wo;
+    max_act = std::max(max_act, rows * y.cout);
+    max_col = std::max(max_col, rows * y.Kp);
+    max_wt = std::max<int64_t>(max_wt, (int64_t)y.Kp * std::max(y.cout, 16));
+    max_dx = std::max(max_dx, mb * y.H * y.W * y.cin);
+  }
+  const size_t f4 = sizeof(float), b2 = 2;
+  RC_TRY(cnn_alloc_t(&net->colT, max_col, b2));
+  RC_TRY(cnn_alloc_t(&net->dYT, max_act, b2));
+  RC_TRY(cnn_alloc_t(&net->dYb, std::max(max_act, max_dx), b2));
+  RC_TRY(cnn_alloc_t(&net->dYf, std::max(max_act, max_dx), f4));
+  RC_TRY(cnn_alloc_t(&net->dcol, max_col, f4));
+  RC_TRY(cnn_alloc_t(&net->dX, max_dx, f4));
+  RC_TRY(cnn_alloc_t(&net->WT, max_wt, b2));
+  RC_TRY(cnn_alloc_t(&net->WT2, (size_t)net->L[NCONV + 1].Kp * 16, b2));
+  RC_TRY(cnn_alloc_t(&net->part, net->part_cap, f4));
+  RC_TRY(cnn_alloc_t(&net->dout, mb * 16, f4));
+  RC_TRY(cnn_alloc_t(&net->doutb, mb * 16, b2));
+  RC_TRY(cnn_alloc_t(&net->col, max_col, b2));  // last: marks the set complete
+  return ALDIC_OK;
+}
+
 int check_net(const aldic_cnn* net) {
   if (!net || !net->ctx) return aldic::set_error(ALDIC_EINVAL, "NULL network");
   HIP_TRY(hipSetDevice(net->ctx->device));
@@ -261,16 +307,6 @@ extern "C" int aldic_cnn_create(aldic_ctx* ctx, int32_t max_batch, aldic_cnn** o
   net->device = ctx->device;
   net->bmax = max_batch;
   const int64_t off = build_layers(net->L);
-  int64_t max_act = 0, max_col = 0, max_wt = 0, max_dx = 0, max_dyt = 0;
-  for (int l = 0; l < NLAYER; l++) {
-    const Layer& y = net->L[l];
-    const int64_t rows = (int64_t)max_batch * y.Ho * y.Wo;
-    max_act = std::max(max_act, rows * y.cout);
-    max_col = std::max(max_col, rows * y.Kp);
-    max_wt = std::max<int64_t>(max_wt, (int64_t)y.Kp * std::max(y.cout, 16));
-    max_dx = std::max(max_dx, (int64_t)max_batch * y.H * y.W * y.cin);
-    max_dyt = std::max(max_dyt, rows * y.cout);
-  }
   // the flattened 3 x 3 x 64 output of the last convolution feeds fc1
   const Layer& c6 = net->L[NCONV - 1];
   if (c6.Ho * c6.Wo * c6.cout != net->L[NCONV].K) {
@@ -289,22 +325,12 @@ extern "C" int aldic_cnn_create(aldic_ctx* ctx, int32_t max_batch, aldic_cnn** o
   A(&net->gw, off, f4);
   A(&net->am, off, f4);
   A(&net->av, off, f4);
-  for (int l = 0; l < NLAYER; l++)
+  // activations from layer 1 on (the fused front end writes layer 1); the
+  // layer-0 activation, the patch buffer and the training scratch are
+  // allocated on first use, so an inference-only network stays small
+  for (int l = 1; l < NLAYER; l++)
     A(&net->act[l], (size_t)max_batch * net->L[l].Ho * net->L[l].Wo * net->L[l].cout, b2);
   A(&net->out, (size_t)max_batch * 6, f4);
-  A(&net->col, max_col, b2);
-  A(&net->colT, max_col, b2);
-  A(&net->dYT, max_dyt, b2);
-  A(&net->dYb, std::max(max_act, max_dx), b2);
-  A(&net->dYf, std::max(max_act, max_dx), f4);
-  A(&net->dcol, max_col, f4);
-  A(&net->dX, max_dx, f4);
-  A(&net->WT, max_wt, b2);
-  A(&net->WT2, (size_t)net->L[NCONV + 1].Kp * 16, b2);
-  A(&net->part, net->part_cap, f4);
-  A(&net->dout, (size_t)max_batch * 16, f4);
-  A(&net->doutb, (size_t)max_batch * 16, b2);
-  A(&net->patches, (size_t)max_batch * PATCH * PATCH * 2, b2);
   A(&net->loss, 1, f4);
   A(&net->lscale, 6, f4);
   if (rc == ALDIC_OK) {
@@ -386,7 +412,9 @@ extern "C" int aldic_cnn_get_grads(aldic_cnn* net, float* host, int64_t n) {
 extern "C" int aldic_cnn_set_loss_weights(aldic_cnn* net, const float w[6]) {
   RC_TRY(check_net(net));
   if (!w) return aldic::set_error(ALDIC_EINVAL, "cnn_set_loss_weights: NULL");
-  HIP_TRY(hipMemcpy(net->lscale, w, 6 * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpyAsync(net->lscale, w, 6 * sizeof(float), hipMemcpyHostToDevice,
+                         net->ctx->stream));
+  HIP_TRY(hipStreamSynchronize(net->ctx->stream));
   return ALDIC_OK;
 }
 
@@ -394,6 +422,7 @@ extern "C" int aldic_cnn_forward_dev(aldic_cnn* net, const void* d_x, int64_t B,
   RC_TRY(check_net(net));
   RC_TRY(check_batch(net, B));
   if (!d_x) return aldic::set_error(ALDIC_EINVAL, "cnn_forward_dev: NULL input");
+  RC_TRY(ensure_layer0(net));
   RC_TRY(forward(net, d_x, (int)B));
   if (d_out)
     HIP_TRY(hipMemcpyAsync(d_out, net->out, B * 6 * sizeof(float), hipMemcpyDeviceToDevice,
@@ -408,6 +437,8 @@ extern "C" int aldic_cnn_train_step_dev(aldic_cnn* net, const void* d_x, const f
   if (B % 16) return aldic::set_error(ALDIC_EINVAL, "cnn_train_step_dev: batch must be a multiple of 16");
   if (!d_x || !d_labels) return aldic::set_error(ALDIC_EINVAL, "cnn_train_step_dev: NULL input");
   hipStream_t s = net->ctx->stream;
+  RC_TRY(ensure_layer0(net));
+  RC_TRY(ensure_training(net));
   RC_TRY(forward(net, d_x, (int)B));
   HIP_TRY(aldic::cnn_mse(net->out, 6, d_labels, net->lscale, (int)B, net->dout, 16, net->loss, s));
   RC_TRY(backward(net, d_x, (int)B));
@@ -445,11 +476,15 @@ extern "C" int aldic_cnn_predict_dev(aldic_cnn* net, const aldic_images* im,
     return aldic::set_error(ALDIC_EINVAL, "cnn_predict_dev: NULL array");
   hipStream_t s = net->ctx->stream;
   StageTimer t(net->ctx, ALDIC_STAGE_CNN);
+  const Layer& c1 = net->L[0];
+  const Layer& c2 = net->L[1];
   for (int64_t first = 0; first < n; first += net->bmax) {
     const int cnt = (int)std::min<int64_t>(net->bmax, n - first);
-    HIP_TRY(aldic::cnn_patches(im->fm, im->g, (int)im->M, (int)im->N, d_coords, (int)n,
-                               (int)first, cnt, d_prior, net->patches, s));
-    RC_TRY(forward(net, net->patches, cnt));
+    // patches + the first two convolutions fused, activations in LDS
+    HIP_TRY(aldic::cnn_front(im->fm, im->g, (int)im->M, (int)im->N, d_coords, (int)n, (int)first,
+                             cnt, d_prior, bfp(net->wb, c1.w_off), net->wf + c1.b_off,
+                             bfp(net->wb, c2.w_off), net->wf + c2.b_off, net->act[1], s));
+    RC_TRY(forward(net, nullptr, cnt, 2));
     HIP_TRY(aldic::cnn_predict(net->out, 6, (int)first, cnt, d_prior, SCALE_F, SCALE_U, d_U0,
                                d_F0, s));
   }

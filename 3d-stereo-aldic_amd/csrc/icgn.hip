@@ -19,6 +19,8 @@
 // Both restart nothing the other has written: a handed-on subset is solved
 // from scratch by the generic kernel, so the result is the reference's
 // whatever path produced it.
+#include <algorithm>
+
 #include "aldic_common.h"
 #include "aldic_kernels.h"
 
@@ -273,7 +275,8 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
   const int sub = threadIdx.x / NT;
   const int tid = threadIdx.x % NT;
   const int slot = blockIdx.x * SUBS + sub;
-  if (slot >= a.cnt) return;  // uniform per subset group (SUBS == 1 when NW > 1)
+  const int cnt = a.dcnt ? *a.dcnt : a.cnt;
+  if (slot >= cnt) return;  // uniform per subset group (SUBS == 1 when NW > 1)
   // every value of one subset is uniform across its waves: keep it scalar
   const int node = __builtin_amdgcn_readfirstlane(a.list[slot]);
   float* tile = tiles[sub];
@@ -1276,5 +1279,39 @@ hipError_t launch_generic(int mode, const SolveArgs& a, const int* list, const i
 }
 
 int generic_max_pixels() { return GEN_MAXPIX; }
+
+}  // namespace aldic
+
+namespace {
+__global__ void bucket_kernel(SolveArgs a, const int* list, const int* count,
+                              aldic::ClassLists out) {
+  const int total = *count;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int node = list[i];
+    int x0, y0, wx, wy;
+    node_window(a, node, &x0, &y0, &wx, &wy);
+    const int side = max(wx, wy) + 1;
+    int c = 4;
+    for (int k = 3; k >= 0; k--)
+      if (side <= aldic::CLS_SIDE[k]) c = k;
+    out.list[c][atomicAdd(&out.counts[c], 1)] = node;
+  }
+}
+}  // namespace
+
+__global__ void aldic::zero_ints_kernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+namespace aldic {
+
+hipError_t launch_bucket(const SolveArgs& a, const int* list, const int* count, int max_count,
+                         ClassLists out, hipStream_t s) {
+  zero_ints_kernel<<<1, 64, 0, s>>>(out.counts, 5);
+  const int grid = std::max(1, std::min((max_count + 255) / 256, 1024));
+  bucket_kernel<<<grid, 256, 0, s>>>(a, list, count, out);
+  return hipGetLastError();
+}
 
 }  // namespace aldic

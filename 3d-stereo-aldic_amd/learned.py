@@ -376,15 +376,26 @@ class LearnedInit:
     """Device buffers and launch sequence of the learned-init local step on
     a fixed node set.
 
-    run(): sparse integer search -> prior at every node -> network ->
-    IC-GN from [F0 U0] -> gate (convIt in [1, max_iter] and ZNCC >= min_zncc)
-    -> integer search at the failed nodes -> IC-GN rerun of those nodes."""
+    run(k): sparse integer search -> prior at every node -> network ->
+    IC-GN from [F0 U0] (at most max_iter evaluations) -> gate (convIt in
+    [1, max_iter] and ZNCC >= min_zncc) -> integer search at the failed
+    nodes -> IC-GN rerun of those nodes (the reference's 100-evaluation
+    cap).
+
+    With a second context (ctx_fb, with its own plan of the same nodes) the
+    steps of consecutive frames are pipelined: the fallback of step k runs
+    on ctx_fb while step k+1 starts on the main context.  Outputs and the
+    per-step buffers are double-buffered by k % 2; outputs(k) returns step
+    k's (U, F, convIt, ZNCC) device arrays."""
 
     def __init__(self, net: Network, images: Images, d_f, d_g, plan: Plan, d_coords, n,
                  d_winsize, prior_grid, search_ws=32, region=(50, 50, 50, 50), max_iter=30,
-                 min_zncc=0.9, tol=1e-3):
+                 min_zncc=0.9, tol=1e-3, ctx_fb: Context | None = None, plan_fb: Plan | None = None):
         ctx = images.ctx
+        self.ctx = ctx
+        self.ctx_fb = ctx_fb or ctx
         self.net, self.images, self.plan = net, images, plan
+        self.plan_fb = plan_fb or plan
         self.d_f, self.d_g = d_f, d_g  # FP32 planes the prior search runs on
         self.d_coords, self.n, self.d_winsize = d_coords, int(n), d_winsize
         self.region = tuple(int(r) for r in region)
@@ -400,38 +411,55 @@ class LearnedInit:
         self.g = {k: DeviceArray(ctx, npnt, np.float64) for k in ("x", "y", "u", "v", "cc")}
         self.g_ipr = DeviceArray(ctx, npnt, np.int32)
         self.g_ipc = DeviceArray(ctx, npnt, np.int32)
-        self.d_prior = DeviceArray(ctx, 2 * self.n, np.float32)
-        self.d_U0 = DeviceArray(ctx, 2 * self.n, np.float32)
-        self.d_F0 = DeviceArray(ctx, 4 * self.n, np.float32)
-        self.d_list = DeviceArray(ctx, max(self.n, 1), np.int32)
-        self.d_count = DeviceArray(ctx, 1, np.int32)
+        nn = self.n
+        self.bufs = []
+        for _ in range(2):
+            self.bufs.append(dict(
+                prior=DeviceArray(ctx, 2 * nn, np.float32), U0=DeviceArray(ctx, 2 * nn, np.float32),
+                F0=DeviceArray(ctx, 4 * nn, np.float32), list=DeviceArray(ctx, max(nn, 1), np.int32),
+                count=DeviceArray(ctx, 1, np.int32), U=DeviceArray(ctx, 2 * nn, np.float32),
+                F=DeviceArray(ctx, 4 * nn, np.float32), conv=DeviceArray(ctx, nn, np.int32),
+                z=DeviceArray(ctx, nn, np.float32)))
 
-    def run(self, d_U, d_F, d_conv, d_zncc):
-        ctx, im = self.images.ctx, self.images
+    def outputs(self, k):
+        b = self.bufs[k % 2]
+        return b["U"], b["F"], b["conv"], b["z"]
+
+    def run(self, k=0):
+        A, B, im = self.ctx, self.ctx_fb, self.images
+        sb = k % 2
+        b = self.bufs[sb]
+        two = B is not A
+        if two:
+            A.wait(B, sb)  # the fallback of step k-2 has released these buffers
         # 1. coarse prior from a sparse integer search
-        integer_search_dev(ctx, self.d_f, self.d_g, im.M, im.N, self.gx, self.gy, self.pws,
+        integer_search_dev(A, self.d_f, self.d_g, im.M, im.N, self.gx, self.gy, self.pws,
                            self.pstep, self.region, self.g["x"].ptr, self.g["y"].ptr,
                            self.g["u"].ptr, self.g["v"].ptr, self.g["cc"].ptr, self.g_ipr.ptr,
                            self.g_ipc.ptr)
         x0 = self.gx[0] + self.pws / 2.0
         y0 = self.gy[0] + self.pws / 2.0
-        grid_to_nodes_dev(ctx, self.g["u"].ptr, self.g["v"].ptr, self.nX, self.nY, x0, y0,
-                          self.pstep, self.d_coords, self.n, self.d_prior.ptr)
+        grid_to_nodes_dev(A, self.g["u"].ptr, self.g["v"].ptr, self.nX, self.nY, x0, y0,
+                          self.pstep, self.d_coords, self.n, b["prior"].ptr)
         # 2. network -> [F0 U0]
-        self.net.predict_dev(im, self.d_coords, self.n, self.d_prior.ptr, self.d_U0.ptr,
-                             self.d_F0.ptr)
-        # 3. IC-GN
-        local_icgn_f0_dev(im, self.plan, self.d_coords, self.d_U0.ptr, self.d_F0.ptr,
-                          self.d_winsize, self.tol, d_U, d_F, d_conv=d_conv, d_zncc=d_zncc,
-                          max_iter=self.max_iter)
-        # 4. gate, 5. search at failed nodes, 6. IC-GN rerun
-        gate_dev(ctx, self.n, d_conv, d_zncc, self.max_iter, self.min_zncc, self.d_list.ptr,
-                 self.d_count.ptr)
-        search_nodes_dev(im, self.d_coords, self.n, self.d_list.ptr, self.d_count.ptr, self.n,
-                         self.search_ws, self.region, self.d_U0.ptr, self.d_F0.ptr)
-        local_icgn_list_dev(im, self.d_coords, self.n, self.d_U0.ptr, self.d_F0.ptr,
-                            self.d_winsize, self.d_list.ptr, self.d_count.ptr, self.n, self.tol,
-                            d_U, d_F, d_conv=d_conv, d_zncc=d_zncc)
+        self.net.predict_dev(im, self.d_coords, self.n, b["prior"].ptr, b["U0"].ptr, b["F0"].ptr)
+        # 3. IC-GN, 4. gate
+        local_icgn_f0_dev(im, self.plan, self.d_coords, b["U0"].ptr, b["F0"].ptr, self.d_winsize,
+                          self.tol, b["U"].ptr, b["F"].ptr, d_conv=b["conv"].ptr,
+                          d_zncc=b["z"].ptr, max_iter=self.max_iter)
+        gate_dev(A, self.n, b["conv"].ptr, b["z"].ptr, self.max_iter, self.min_zncc,
+                 b["list"].ptr, b["count"].ptr)
+        if two:
+            A.record(2 + sb)
+            B.wait(A, 2 + sb)
+        # 5. search at the failed nodes, 6. IC-GN rerun of those nodes
+        search_nodes_dev(im, self.d_coords, self.n, b["list"].ptr, b["count"].ptr, self.n,
+                         self.search_ws, self.region, b["U0"].ptr, b["F0"].ptr, ctx=B)
+        local_icgn_list_dev(im, self.plan_fb, self.d_coords, b["U0"].ptr, b["F0"].ptr,
+                            self.d_winsize, b["list"].ptr, b["count"].ptr, self.tol, b["U"].ptr,
+                            b["F"].ptr, d_conv=b["conv"].ptr, d_zncc=b["z"].ptr, ctx=B)
+        if two:
+            B.record(sb)
 
-    def fallback_count(self):
-        return int(self.d_count.to_host()[0])
+    def fallback_count(self, k=0):
+        return int(self.bufs[k % 2]["count"].to_host()[0])

@@ -160,21 +160,30 @@ def accuracy(W, U, conv, z):
     return ok, rms, by_size
 
 
-def timed(ctx, api, step, steps, warmup):
-    for _ in range(warmup):
-        step()
-    ctx.synchronize()
-    api.set_timing(ctx, True)
-    api.stage_times(ctx, reset=True)
-    ctx.synchronize()
+def timed(ctxs, api, step, steps, warmup):
+    """Wall time per step of `steps` calls step(k) after `warmup` untimed
+    ones, every context drained on both sides; stage times summed over the
+    contexts."""
+    for k in range(warmup):
+        step(k)
+    for c in ctxs:
+        c.synchronize()
+    for c in ctxs:
+        api.set_timing(c, True)
+        api.stage_times(c, reset=True)
     t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    ctx.synchronize()
+    for k in range(warmup, warmup + steps):
+        step(k)
+    for c in ctxs:
+        c.synchronize()
     dt = (time.perf_counter() - t0) / steps
-    stages = api.stage_times(ctx, reset=True)
-    api.set_timing(ctx, False)
-    return dt, stages
+    stages = {}
+    for c in ctxs:
+        for name, (ms, cnt) in api.stage_times(c, reset=True).items():
+            m0, c0 = stages.get(name, (0.0, 0))
+            stages[name] = (m0 + ms, c0 + cnt)
+        api.set_timing(c, False)
+    return dt, stages, warmup + steps - 1
 
 
 def main():
@@ -226,14 +235,27 @@ def main():
     d_conv = DA(ctx, n, np.int32)
     d_z = DA(ctx, n, np.float32)
     x0 = sws / 2 + 1
+    # a second context (stream) pipelines consecutive frames: the IC-GN of
+    # step k runs on ctx2 while the search of step k+1 runs on ctx; outputs
+    # are double-buffered by k % 2
+    ctx2 = api.Context(0)
+    plan2 = api.Plan(W["ws"], n, ctx=ctx2)
+    sets = [dict(U0=DA(ctx, 2 * n, np.float32), U=DA(ctx, 2 * n, np.float32),
+                 F=DA(ctx, 4 * n, np.float32), conv=DA(ctx, n, np.int32),
+                 z=DA(ctx, n, np.float32)) for _ in range(2)]
 
-    def step_search():
+    def step_search(k):
+        b = sets[k % 2]
+        ctx.wait(ctx2, k % 2)  # the IC-GN of step k-2 has released b
         api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst,
                                [r, r, r, r], *[o.ptr for o in s_out], *[p.ptr for p in s_pk])
         api.grid_to_nodes_dev(ctx, s_out[2].ptr, s_out[3].ptr, nX, nY, x0, x0, sst, d_coords.ptr,
-                              n, d_U0.ptr)
-        api.local_icgn_dev(im, plan, d_coords.ptr, d_U0.ptr, d_ws.ptr, 1e-3, d_U.ptr, d_F.ptr,
-                           None, d_conv.ptr, None, d_z.ptr)
+                              n, b["U0"].ptr)
+        ctx.record(2 + k % 2)
+        ctx2.wait(ctx, 2 + k % 2)
+        api.local_icgn_dev(im, plan2, d_coords.ptr, b["U0"].ptr, d_ws.ptr, 1e-3, b["U"].ptr,
+                           b["F"].ptr, None, b["conv"].ptr, None, b["z"].ptr, ctx=ctx2)
+        ctx2.record(k % 2)
 
     # plan-time autotune of the IC-GN launch shapes on this node set (not timed)
     api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst, [r, r, r, r],
@@ -258,38 +280,43 @@ def main():
                          U=U, **(extra or {}))
 
     progress(f"autotuned launch shapes {list(chosen)}")
+    for c in range(4):
+        plan2.set_variant(0, c, chosen[c])
+        plan2.set_variant(1, c, chosen[c])
     if args.init in ("both", "search"):
-        dt, stages = timed(ctx, api, step_search, args.steps, args.warmup)
+        dt, stages, last = timed([ctx, ctx2], api, step_search, args.steps, args.warmup)
         progress(f"search-init step {dt * 1e3:.3f} ms")
-        record("search", dt, stages, d_U.to_host().astype(np.float64), d_conv.to_host(),
-               d_z.to_host())
+        b = sets[last % 2]
+        record("search", dt, stages, b["U"].to_host().astype(np.float64), b["conv"].to_host(),
+               b["z"].to_host())
 
     if args.init in ("both", "learned"):
         # ---- train the network on the device (not timed) ----
         t0 = time.perf_counter()
-        net = learned.Network(ctx, max_batch=8192, seed=0)
+        net = learned.Network(ctx, max_batch=256, seed=0)
         hist = learned.train(net, samples=args.train_samples, batch=256, lr=2e-3, tiles=64,
                              seed=4, log_every=1000, log=progress)
         ev = learned.evaluate(net, seed=12345, tiles=32)
         t_train = time.perf_counter() - t0
-        li = learned.LearnedInit(net, im, d_f.ptr, d_g.ptr, plan, d_coords.ptr, n, d_ws.ptr,
+        # inference copy of the trained weights, one batch for all nodes
+        inet = learned.Network(ctx, max_batch=(n + 15) // 16 * 16, params=net.get_params())
+        net.close()
+        li = learned.LearnedInit(inet, im, d_f.ptr, d_g.ptr, plan, d_coords.ptr, n, d_ws.ptr,
                                  prior_grid=([1, M], [1, N], sws, 128), search_ws=sws,
-                                 region=(r, r, r, r), max_iter=30, min_zncc=0.9, tol=1e-3)
-
-        def step_learned():
-            li.run(d_U.ptr, d_F.ptr, d_conv.ptr, d_z.ptr)
-
-        dt, stages = timed(ctx, api, step_learned, args.steps, args.warmup)
+                                 region=(r, r, r, r), max_iter=30, min_zncc=0.9, tol=1e-3,
+                                 ctx_fb=ctx2, plan_fb=plan2)
+        dt, stages, last = timed([ctx, ctx2], api, li.run, args.steps, args.warmup)
         progress(f"learned-init step {dt * 1e3:.3f} ms")
+        lU, lF, lconv, lz = li.outputs(last)
         train = {"samples": hist["samples"], "steps": hist["steps"],
                  "kept_frac": round(hist["kept"] / max(hist["total"], 1), 5),
                  "seconds": round(t_train, 1), "final_loss": round(hist["loss"][-1], 5)
                  if hist["loss"] else None,
                  "heldout_rms_u_px": round(ev["rms_u_px"], 4),
                  "heldout_rms_F": round(ev["rms_F"], 5)}
-        record("learned", dt, stages, d_U.to_host().astype(np.float64), d_conv.to_host(),
-               d_z.to_host(), dict(fallback=li.fallback_count(), train=train,
-                                   prior_points=li.nX * li.nY))
+        record("learned", dt, stages, lU.to_host().astype(np.float64), lconv.to_host(),
+               lz.to_host(), dict(fallback=li.fallback_count(last), train=train,
+                                  prior_points=li.nX * li.nY))
 
     head = res.get("learned") or res["search"]
     n_conv = int(head["ok"].sum())

@@ -67,6 +67,16 @@ int aldic_ctx_synchronize(aldic_ctx* ctx);
 #define ALDIC_STAGE_CNN 14         /* learned starting point: patches + network */
 #define ALDIC_STAGE_FALLBACK 15    /* gate, node search and IC-GN rerun */
 #define ALDIC_NSTAGES 16
+/* Ordering between contexts (one stream each) on one device, for callers
+ * that pipeline independent frames over two contexts: aldic_ctx_record
+ * marks the point reached by the work submitted so far to ctx (mark
+ * 0 .. ALDIC_NMARKS-1, re-recording replaces it); aldic_ctx_wait makes
+ * work submitted later to ctx wait for the last recording of `mark` on
+ * other.  No counterpart in the reference. */
+#define ALDIC_NMARKS 8
+int aldic_ctx_record(aldic_ctx* ctx, int32_t mark);
+int aldic_ctx_wait(aldic_ctx* ctx, aldic_ctx* other, int32_t mark);
+
 int aldic_ctx_set_timing(aldic_ctx* ctx, int32_t enable);
 int aldic_ctx_stage_times(aldic_ctx* ctx, double* ms, int32_t* launches, int32_t reset);
 
@@ -268,14 +278,14 @@ int aldic_search_nodes_dev(aldic_ctx* ctx, const aldic_images* im, const double*
                            int64_t n, const int32_t* d_list, const int32_t* d_count,
                            int64_t max_count, int32_t winsize, const int64_t region[4],
                            float* d_U0, float* d_F0);
-/* IC-GN (funICGNQuadtree.m) of the listed nodes only, from [F0 U0]; other
- * nodes' outputs are left as they are. */
-int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im, const double* d_coords,
-                              int64_t n, const float* d_U0, const float* d_F0,
+/* IC-GN (funICGNQuadtree.m) of the listed nodes only, from [F0 U0]: list
+ * entries are distinct node indices of the plan; other nodes' outputs are
+ * left as they are. */
+int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
+                              const double* d_coords, const float* d_U0, const float* d_F0,
                               const int32_t* d_winsize, const int32_t* d_list,
-                              const int32_t* d_count, int64_t max_count, double tol, float* d_U,
-                              float* d_F, float* d_H21, int32_t* d_convIt, uint8_t* d_hole,
-                              float* d_zncc);
+                              const int32_t* d_count, double tol, float* d_U, float* d_F,
+                              float* d_H21, int32_t* d_convIt, uint8_t* d_hole, float* d_zncc);
 
 #ifdef __cplusplus
 }

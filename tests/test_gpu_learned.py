@@ -136,6 +136,42 @@ def test_patches_match_numpy(L, pkg):
     im.close()
 
 
+def test_fused_predict_equals_layer_path(L, pkg):
+    """predict_dev (patches and the first two convolutions fused in LDS)
+    against patches_dev + forward_dev: the same network outputs."""
+    api, learned = L
+    ctx = api.default_context()
+    f, g, _, _ = c1_pair()
+    coords = c1_grid()[0]
+    n = len(coords)
+    rng = np.random.default_rng(8)
+    prior = rng.uniform(-6, 6, (n, 2)).astype(np.float32)
+    p = _params(learned, 9)
+    net = learned.Network(ctx, max_batch=256, params=p)
+    im = pkg.Images(f, g)
+    dc = api.DeviceArray.from_host(ctx, np.asfortranarray(coords).reshape(-1, order="F"),
+                                   np.float64)
+    dp = api.DeviceArray.from_host(ctx, prior.reshape(-1), np.float32)
+    dU0 = api.DeviceArray(ctx, 2 * n, np.float32)
+    dF0 = api.DeviceArray(ctx, 4 * n, np.float32)
+    net.predict_dev(im, dc.ptr, n, dp.ptr, dU0.ptr, dF0.ptr)  # chunks of 256
+    U0 = dU0.to_host().reshape(-1, 2)
+    F0 = dF0.to_host().reshape(-1, 4)
+    x = api.DeviceArray(ctx, n * 48 * 48, np.float32)
+    learned.patches_dev(im, dc.ptr, n, dp.ptr, x.ptr)
+    out = api.DeviceArray(ctx, 6 * n, np.float32)
+    for s0 in range(0, n, 256):
+        B = min(256, n - s0)
+        net.forward_dev(x.ptr + s0 * 48 * 48 * 4, B, out.ptr + s0 * 24)
+    y = out.to_host().reshape(n, 6)
+    sf, su = learned.scales()
+    np.testing.assert_allclose(F0, y[:, :4] * np.float32(sf), rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(U0, np.rint(prior) + y[:, 4:] * np.float32(su), rtol=1e-6,
+                               atol=1e-5)
+    im.close()
+    net.close()
+
+
 def test_mosaic_labels_recover_tile_warps(L):
     api, learned = L
     ctx = api.default_context()
@@ -227,8 +263,9 @@ def test_gate_search_and_list_icgn(L, pkg, orc):
         else:
             allv = api.DeviceArray.from_host(ctx, np.arange(n, dtype=np.int32), np.int32)
             cntv = api.DeviceArray.from_host(ctx, np.array([n], np.int32), np.int32)
-            api.local_icgn_list_dev(im, dc2.ptr, n, dU0b.ptr, None, dws.ptr, allv.ptr, cntv.ptr,
-                                    n, 1e-3, dU.ptr, dF.ptr, d_conv=dcv.ptr, d_zncc=dz.ptr)
+            plan = api.Plan(30, n, ctx=ctx)
+            api.local_icgn_list_dev(im, plan, dc2.ptr, dU0b.ptr, None, dws.ptr, allv.ptr,
+                                    cntv.ptr, 1e-3, dU.ptr, dF.ptr, d_conv=dcv.ptr, d_zncc=dz.ptr)
         outs.append((dU.to_host(), dF.to_host(), dcv.to_host()))
     (Ua, Fa, ca), (Ub, Fb, cb) = outs
     ok = (ca >= 1) & (ca <= 100) & (cb >= 1) & (cb <= 100)
