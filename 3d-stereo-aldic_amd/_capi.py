@@ -87,6 +87,9 @@ SIGNATURES = {
     "aldic_gate_dev": (C.c_int, [_vp, _i64, _vp, _vp, C.c_int32, C.c_double, _vp, _vp]),
     "aldic_search_nodes_dev": (C.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i64, C.c_int32,
                                          C.POINTER(_i64), _vp, _vp]),
+    "aldic_search_grid_nodes_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.POINTER(_i64),
+                                              C.POINTER(_i64), C.c_int32, C.c_int32,
+                                              C.POINTER(_i64), _vp, _i64, _vp, _vp, _vp, _vp]),
     "aldic_local_icgn_list_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                             C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
 }

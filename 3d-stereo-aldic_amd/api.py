@@ -618,6 +618,18 @@ def search_nodes_dev(images: Images, d_coords, n, d_list, d_count, max_count, wi
                                        int(max_count), int(winsize), rg, d_U0, d_F0))
 
 
+def search_grid_nodes_dev(ctx: Context, d_f, d_g, M, N, gridx, gridy, winsize, winstepsize,
+                          region, d_coords, n, d_list, d_count, d_U0, d_F0=None):
+    """Search-init U0 of the listed nodes: only the grid points their
+    bilinear transfer reads are searched."""
+    gx = (C.c_int64 * 2)(int(gridx[0]), int(gridx[-1]))
+    gy = (C.c_int64 * 2)(int(gridy[0]), int(gridy[-1]))
+    rg = (C.c_int64 * 4)(*[int(v) for v in region])
+    check(lib().aldic_search_grid_nodes_dev(ctx.ptr, d_f, d_g, int(M), int(N), gx, gy,
+                                            int(winsize), int(winstepsize), rg, d_coords, int(n),
+                                            d_list, d_count, d_U0, d_F0))
+
+
 def local_icgn_list_dev(images: Images, plan: Plan, d_coords, d_U0, d_F0, d_winsize, d_list,
                         d_count, tol, d_U, d_F=None, d_H21=None, d_conv=None, d_hole=None,
                         d_zncc=None, ctx: Context | None = None):

@@ -84,6 +84,20 @@ hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
                                int max_count, int ws, const int reg[4], void* scratch, float* U0,
                                float* F0, hipStream_t s);
 
+// grid search over a device list of grid points (indices ci + cj*nX)
+hipError_t launch_search_points(const float* f, const float* g, int M, int N, int nX, int nY,
+                                const int gridx[2], const int gridy[2], int ws, int step,
+                                const int reg[4], const int* points, const int* count,
+                                void* scratch, double* x, double* y, double* u, double* v,
+                                double* ccmax, int* ipr, int* ipc, hipStream_t s);
+hipError_t launch_grid_to_listed_nodes(const double* gu, const double* gv, int nX, int nY,
+                                       double x0, double y0, double step, const double* coords,
+                                       int n, const int* list, const int* count, float* U0,
+                                       float* F0, hipStream_t s);
+hipError_t launch_mark_cells(int nX, int nY, double x0, double y0, double step,
+                             const double* coords, int n, const int* list, const int* count,
+                             int* flags, int* points, int* npoints, hipStream_t s);
+
 // search grid -> node initial guess (prep.hip)
 hipError_t launch_grid_to_nodes(const double* gu, const double* gv, int nX, int nY, double x0,
                                 double y0, double step, const double* coords, int n, float* U0,

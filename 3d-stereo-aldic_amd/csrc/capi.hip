@@ -70,6 +70,7 @@ int stage(aldic_ctx* ctx, int slot, size_t count, T** out) {
 // staging slots of the device-resident search entry points
 constexpr int SLOT_SEARCH = 20;
 constexpr int SLOT_SEARCH_NODES = 21;
+constexpr int SLOT_SEARCH_GRID_NODES = 22;
 
 int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, int rect,
                aldic_plan* p) {
@@ -990,6 +991,58 @@ extern "C" int aldic_integer_search_dev(aldic_ctx* ctx, const float* d_f, const 
   HIP_TRY(aldic::launch_search(d_f, d_g, (int)M, (int)N, (int)nX, (int)nY, gx, gy, winsize,
                                winstepsize, rg, scratch, d_x, d_y, d_u, d_v, d_ccmax, d_ipr,
                                d_ipc, ctx->stream));
+  return ALDIC_OK;
+}
+
+// The search-init start of the listed nodes: the grid points their
+// bilinear transfer reads are found on the device, searched (grid-list
+// mode) and interpolated onto the listed nodes only.
+extern "C" int aldic_search_grid_nodes_dev(aldic_ctx* ctx, const float* d_f, const float* d_g,
+                                           int64_t M, int64_t N, const int64_t gridx[2],
+                                           const int64_t gridy[2], int32_t winsize,
+                                           int32_t winstepsize, const int64_t region[4],
+                                           const double* d_coords, int64_t n,
+                                           const int32_t* d_list, const int32_t* d_count,
+                                           float* d_U0, float* d_F0) {
+  if (!ctx || !d_f || !d_g || !d_coords || !d_list || !d_count || !d_U0)
+    return fail(ALDIC_EINVAL, "search_grid_nodes_dev: NULL argument");
+  RC_TRY(search_checks(M, N, gridx, gridy, winsize, winstepsize, region));
+  const int64_t nX = grid_count(gridx[0], gridx[1], winsize, winstepsize);
+  const int64_t nY = grid_count(gridy[0], gridy[1], winsize, winstepsize);
+  if (nX * nY == 0 || n == 0) return ALDIC_OK;
+  if (n > 0x7fffffff) return fail(ALDIC_EINVAL, "search_grid_nodes_dev: too many nodes");
+  RC_TRY(set_device(ctx));
+  const int m = (int)(nX * nY);
+  const int gx[2] = {(int)gridx[0], (int)gridx[1]}, gy[2] = {(int)gridy[0], (int)gridy[1]};
+  const int rg[4] = {(int)region[0], (int)region[1], (int)region[2], (int)region[3]};
+  // grid-sized work arrays: flags, point list, count, x y u v cc (double), peaks
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t ib = up(sizeof(int) * m), db = up(sizeof(double) * m);
+  const size_t work = 2 * ib + 256 + 5 * db + 2 * ib;
+  const size_t sb = aldic::search_scratch_bytes(m, winsize, rg);
+  void* base;
+  RC_TRY(staging(ctx, SLOT_SEARCH_GRID_NODES, work + sb, &base));
+  char* p = static_cast<char*>(base);
+  int* flags = reinterpret_cast<int*>(p);
+  int* points = reinterpret_cast<int*>(p + ib);
+  int* npoints = reinterpret_cast<int*>(p + 2 * ib);
+  double* gd = reinterpret_cast<double*>(p + 2 * ib + 256);
+  double *x = gd, *y = gd + db / 8, *u = gd + 2 * (db / 8), *v = gd + 3 * (db / 8),
+         *cc = gd + 4 * (db / 8);
+  int* ipr = reinterpret_cast<int*>(p + 2 * ib + 256 + 5 * db);
+  int* ipc = reinterpret_cast<int*>(p + 3 * ib + 256 + 5 * db);
+  void* scratch = p + work;
+  const double x0 = gridx[0] + winsize / 2.0, y0 = gridy[0] + winsize / 2.0;
+  hipStream_t s = ctx->stream;
+  StageTimer t(ctx, ALDIC_STAGE_FALLBACK);
+  HIP_TRY(aldic::launch_mark_cells((int)nX, (int)nY, x0, y0, (double)winstepsize, d_coords,
+                                   (int)n, d_list, d_count, flags, points, npoints, s));
+  HIP_TRY(aldic::launch_search_points(d_f, d_g, (int)M, (int)N, (int)nX, (int)nY, gx, gy, winsize,
+                                      winstepsize, rg, points, npoints, scratch, x, y, u, v, cc,
+                                      ipr, ipc, s));
+  HIP_TRY(aldic::launch_grid_to_listed_nodes(u, v, (int)nX, (int)nY, x0, y0,
+                                             (double)winstepsize, d_coords, (int)n, d_list,
+                                             d_count, d_U0, d_F0, s));
   return ALDIC_OK;
 }
 

@@ -527,9 +527,25 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
       tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
       ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
       if constexpr (NW > 1) __syncthreads();
-      for (int e = tid; e < TS * TS; e += NT) {
-        const int rr = e % TS, cc = e / TS;
-        tile[e] = a.g[(int64_t)(tx0 + rr) + (int64_t)(ty0 + cc) * M];
+      // the tile's loads go out in groups of TG before their LDS stores, so
+      // a group costs one memory latency (compile-time trip counts)
+      constexpr int TR = (TS * TS + NT - 1) / NT;
+      constexpr int TG = 8;
+      const float* gt = a.g + (int64_t)tx0 + (int64_t)ty0 * M;
+#pragma unroll 1
+      for (int i0 = 0; i0 < TR; i0 += TG) {
+        float tv[TG];
+#pragma unroll
+        for (int i = 0; i < TG; i++) {
+          const int e = tid + (i0 + i) * NT;
+          const int rr = e % TS, cc = e / TS;
+          tv[i] = (i0 + i < TR && e < TS * TS) ? gt[rr + (int64_t)cc * M] : 0.0f;
+        }
+#pragma unroll
+        for (int i = 0; i < TG; i++) {
+          const int e = tid + (i0 + i) * NT;
+          if (i0 + i < TR && e < TS * TS) tile[e] = tv[i];
+        }
       }
       if constexpr (NW > 1)
         __syncthreads();

@@ -133,21 +133,38 @@ hipError_t launch_place_df(const double* DfDx, const double* DfDy, int64_t M, in
 // read.
 // ---------------------------------------------------------------------------
 namespace {
+// Grid cell (ix, iy) and weights (tx, ty) of node k: bilinear between the
+// four grid nodes around it, constant beyond the outer ones.
+__device__ __forceinline__ void grid_cell(const double* coords, int n, int k, int nX, int nY,
+                                          double x0, double y0, double step, int* ix, int* iy,
+                                          double* tx, double* ty) {
+  double fx = (coords[k] - x0) / step, fy = (coords[k + n] - y0) / step;
+  fx = fmin(fmax(fx, 0.0), (double)(nX - 1));
+  fy = fmin(fmax(fy, 0.0), (double)(nY - 1));
+  *ix = min((int)floor(fx), max(nX - 2, 0));
+  *iy = min((int)floor(fy), max(nY - 2, 0));
+  *tx = fx - *ix;
+  *ty = fy - *iy;
+}
+
+// list (may be null): the nodes to fill, count read on the device; F0
+// (may be null) of those nodes is set to zero
 __global__ __launch_bounds__(256) void grid_to_nodes_kernel(const double* __restrict__ gu,
                                                             const double* __restrict__ gv,
                                                             int nX, int nY, double x0, double y0,
                                                             double step,
                                                             const double* __restrict__ coords,
-                                                            int n, float* __restrict__ U0) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  double fx = (coords[k] - x0) / step, fy = (coords[k + n] - y0) / step;
-  fx = fmin(fmax(fx, 0.0), (double)(nX - 1));
-  fy = fmin(fmax(fy, 0.0), (double)(nY - 1));
-  int ix = (int)floor(fx), iy = (int)floor(fy);
-  ix = min(ix, max(nX - 2, 0));
-  iy = min(iy, max(nY - 2, 0));
-  const double tx = fx - ix, ty = fy - iy;
+                                                            int n, const int* __restrict__ list,
+                                                            const int* __restrict__ count,
+                                                            float* __restrict__ U0,
+                                                            float* __restrict__ F0) {
+  const int total = list ? *count : n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int k = list ? list[i] : i;
+  int ix, iy;
+  double tx, ty;
+  grid_cell(coords, n, k, nX, nY, x0, y0, step, &ix, &iy, &tx, &ty);
   const int ix1 = min(ix + 1, nX - 1), iy1 = min(iy + 1, nY - 1);
   // grids are nY x nX column-major: (cj, ci) -> cj + ci*nY
   auto at = [&](const double* a, int ci, int cj) { return a[cj + (int64_t)ci * nY]; };
@@ -157,6 +174,37 @@ __global__ __launch_bounds__(256) void grid_to_nodes_kernel(const double* __rest
                    (1 - tx) * ty * at(gv, ix, iy1) + tx * ty * at(gv, ix1, iy1);
   U0[2 * k] = (float)u;
   U0[2 * k + 1] = (float)v;
+  if (F0)
+    for (int q = 0; q < 4; q++) F0[4 * k + q] = 0.0f;
+}
+
+// flags of the grid nodes the listed nodes interpolate from
+__global__ void mark_cells_kernel(int nX, int nY, double x0, double y0, double step,
+                                  const double* __restrict__ coords, int n,
+                                  const int* __restrict__ list, const int* __restrict__ count,
+                                  int* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= *count) return;
+  int ix, iy;
+  double tx, ty;
+  grid_cell(coords, n, list[i], nX, nY, x0, y0, step, &ix, &iy, &tx, &ty);
+  const int ix1 = min(ix + 1, nX - 1), iy1 = min(iy + 1, nY - 1);
+  // grid point index t = ci + cj*nX (the search's point order)
+  flags[ix + iy * nX] = 1;
+  flags[ix1 + iy * nX] = 1;
+  flags[ix + iy1 * nX] = 1;
+  flags[ix1 + iy1 * nX] = 1;
+}
+
+__global__ void compact_flags_kernel(const int* __restrict__ flags, int m, int* __restrict__ out,
+                                     int* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m && flags[i]) out[atomicAdd(count, 1)] = i;
+}
+
+__global__ void fill_int_kernel(int* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 }  // namespace
 
@@ -166,7 +214,30 @@ hipError_t launch_grid_to_nodes(const double* gu, const double* gv, int nX, int 
                                 hipStream_t s) {
   if (n <= 0) return hipSuccess;
   grid_to_nodes_kernel<<<(n + 255) / 256, 256, 0, s>>>(gu, gv, nX, nY, x0, y0, step, coords, n,
-                                                       U0);
+                                                       nullptr, nullptr, U0, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_to_listed_nodes(const double* gu, const double* gv, int nX, int nY,
+                                       double x0, double y0, double step, const double* coords,
+                                       int n, const int* list, const int* count, float* U0,
+                                       float* F0, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  grid_to_nodes_kernel<<<(n + 255) / 256, 256, 0, s>>>(gu, gv, nX, nY, x0, y0, step, coords, n,
+                                                       list, count, U0, F0);
+  return hipGetLastError();
+}
+
+hipError_t launch_mark_cells(int nX, int nY, double x0, double y0, double step,
+                             const double* coords, int n, const int* list, const int* count,
+                             int* flags, int* points, int* npoints, hipStream_t s) {
+  const int m = nX * nY;
+  fill_int_kernel<<<(m + 255) / 256, 256, 0, s>>>(flags, m, 0);
+  fill_int_kernel<<<1, 64, 0, s>>>(npoints, 1, 0);
+  if (n > 0)
+    mark_cells_kernel<<<(n + 255) / 256, 256, 0, s>>>(nX, nY, x0, y0, step, coords, n, list,
+                                                      count, flags);
+  compact_flags_kernel<<<(m + 255) / 256, 256, 0, s>>>(flags, m, points, npoints);
   return hipGetLastError();
 }
 }  // namespace aldic

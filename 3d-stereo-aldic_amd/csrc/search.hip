@@ -19,10 +19,9 @@
 //   final  one wave per point: the first maximum over the tiles, then the
 //          3x3 quadratic fit of findpeak.m (FP64, rounded to 1e-3), its nine
 //          ZNCC values recomputed at the peak's neighbours.
-// Points are either the reference's grid (funIntegerSearch.m:106-107) or a
-// device list of nodes whose templates are centred on the nodes (the
-// fallback of the learned starting point); the list count is read on the
-// device.
+// Points are the reference's grid (funIntegerSearch.m:106-107), a device
+// list of some of its points, or a device list of nodes whose templates are
+// centred on the nodes; list counts are read on the device.
 #include <algorithm>
 
 #include "aldic_common.h"
@@ -50,8 +49,10 @@ struct SearchArgs {
   int ntiles, nt1; // lag tiles per point (max over points), along dim 1
   // grid mode
   int nX, nY, gx0, gx1, gy0, gy1, step;
-  // list mode (list != nullptr)
+  // list mode (list != nullptr): nodes; grid-list mode (glist != nullptr):
+  // grid point indices ci + cj*nX.  Either count is lcount.
   const int* list;
+  const int* glist;
   const int* lcount;
   const double* ncoords;
   int nn;
@@ -68,7 +69,12 @@ struct SearchArgs {
 };
 
 __device__ __forceinline__ int num_points(const SearchArgs& A) {
-  return A.list ? *A.lcount : A.nX * A.nY;
+  return (A.list || A.glist) ? *A.lcount : A.nX * A.nY;
+}
+
+// grid index of point t (grid modes)
+__device__ __forceinline__ int grid_index(const SearchArgs& A, int t) {
+  return A.glist ? A.glist[t] : t;
 }
 
 // Template origin and per-point clamps of funIntegerSearch.m:139-152 (the
@@ -85,7 +91,8 @@ __device__ bool point_params(const SearchArgs& A, int t, int* ii, int* jj, int* 
     gy1 = A.N;
     if (*ii < 1 || *jj < 1 || *ii + A.ws > A.M || *jj + A.ws > A.N) return false;
   } else {
-    const int ci = t % A.nX, cj = t / A.nX;
+    const int gi = grid_index(A, t);
+    const int ci = gi % A.nX, cj = gi / A.nX;
     *ii = A.gx0 + ci * A.step;
     *jj = A.gy0 + cj * A.step;
     gx1 = A.gx1;
@@ -341,7 +348,8 @@ __global__ __launch_bounds__(256) void search_final_kernel(SearchArgs A) {
     const PointPrm P = A.prm[t];
     const int K1 = P.L + P.R + 3, K2 = P.U + P.D + 3;
     const int node = A.list ? A.list[t] : -1;
-    const int o = A.list ? t : (t / A.nX) + (t % A.nX) * A.nY;  // nY x nX column-major
+    const int gi = A.list ? 0 : grid_index(A, t);
+    const int o = A.list ? t : (gi / A.nX) + (gi % A.nX) * A.nY;  // nY x nX column-major
     if (!P.valid) {
       // normxcorr2 throws on a constant template -> NaN (funIntegerSearch.m:171-174);
       // a listed node keeps its starting point
@@ -506,6 +514,40 @@ hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
   A.F0 = F0;
   carve(A, scratch, max_count);
   return run(A, max_count, s);
+}
+
+hipError_t launch_search_points(const float* f, const float* g, int M, int N, int nX, int nY,
+                                const int gridx[2], const int gridy[2], int ws, int step,
+                                const int reg[4], const int* points, const int* count,
+                                void* scratch, double* x, double* y, double* u, double* v,
+                                double* ccmax, int* ipr, int* ipc, hipStream_t s) {
+  SearchArgs A{};
+  A.f = f;
+  A.g = g;
+  A.M = M;
+  A.N = N;
+  A.nX = nX;
+  A.nY = nY;
+  A.gx0 = gridx[0];
+  A.gx1 = gridx[1];
+  A.gy0 = gridy[0];
+  A.gy1 = gridy[1];
+  A.ws = ws;
+  A.sp = pad_len(ws);
+  A.step = step;
+  for (int k = 0; k < 4; k++) A.reg[k] = reg[k];
+  tile_counts(reg, &A.nt1, &A.ntiles);
+  A.glist = points;
+  A.lcount = count;
+  A.x = x;
+  A.y = y;
+  A.u = u;
+  A.v = v;
+  A.ccmax = ccmax;
+  A.ipr = ipr;
+  A.ipc = ipc;
+  carve(A, scratch, nX * nY);
+  return run(A, nX * nY, s);
 }
 
 hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, int nY,

@@ -28,7 +28,7 @@ import numpy as np
 from ._capi import check, lib
 from .api import (Context, DeviceArray, Images, Plan, default_context, gate_dev,
                   grid_to_nodes_dev, integer_search_dev, local_icgn_f0_dev,
-                  local_icgn_list_dev, search_nodes_dev)
+                  local_icgn_list_dev, search_grid_nodes_dev)
 
 PATCH = 48
 
@@ -378,9 +378,9 @@ class LearnedInit:
 
     run(k): sparse integer search -> prior at every node -> network ->
     IC-GN from [F0 U0] (at most max_iter evaluations) -> gate (convIt in
-    [1, max_iter] and ZNCC >= min_zncc) -> integer search at the failed
-    nodes -> IC-GN rerun of those nodes (the reference's 100-evaluation
-    cap).
+    [1, max_iter] and ZNCC >= min_zncc) -> the search-init start of the
+    failed nodes (the points of the search grid their transfer reads) ->
+    IC-GN rerun of those nodes (the reference's 100-evaluation cap).
 
     With a second context (ctx_fb, with its own plan of the same nodes) the
     steps of consecutive frames are pipelined: the fallback of step k runs
@@ -389,7 +389,7 @@ class LearnedInit:
     k's (U, F, convIt, ZNCC) device arrays."""
 
     def __init__(self, net: Network, images: Images, d_f, d_g, plan: Plan, d_coords, n,
-                 d_winsize, prior_grid, search_ws=32, region=(50, 50, 50, 50), max_iter=30,
+                 d_winsize, prior_grid, fallback_grid, region=(50, 50, 50, 50), max_iter=30,
                  min_zncc=0.9, tol=1e-3, ctx_fb: Context | None = None, plan_fb: Plan | None = None):
         ctx = images.ctx
         self.ctx = ctx
@@ -399,7 +399,9 @@ class LearnedInit:
         self.d_f, self.d_g = d_f, d_g  # FP32 planes the prior search runs on
         self.d_coords, self.n, self.d_winsize = d_coords, int(n), d_winsize
         self.region = tuple(int(r) for r in region)
-        self.search_ws = int(search_ws)
+        # fallback_grid = (gridx, gridy, winsize, step): the search-init grid;
+        # a failed node restarts from exactly the U0 that grid gives it
+        self.fb_grid = fallback_grid
         self.max_iter, self.min_zncc, self.tol = int(max_iter), float(min_zncc), float(tol)
         # prior_grid = (gridx, gridy, winsize, step): a sparse search grid
         self.gx, self.gy, self.pws, self.pstep = prior_grid
@@ -452,9 +454,12 @@ class LearnedInit:
         if two:
             A.record(2 + sb)
             B.wait(A, 2 + sb)
-        # 5. search at the failed nodes, 6. IC-GN rerun of those nodes
-        search_nodes_dev(im, self.d_coords, self.n, b["list"].ptr, b["count"].ptr, self.n,
-                         self.search_ws, self.region, b["U0"].ptr, b["F0"].ptr, ctx=B)
+        # 5. the search-init start of the failed nodes (only the grid points
+        # they need), 6. IC-GN rerun of those nodes
+        fgx, fgy, fws, fstep = self.fb_grid
+        search_grid_nodes_dev(B, self.d_f, self.d_g, im.M, im.N, fgx, fgy, fws, fstep,
+                              self.region, self.d_coords, self.n, b["list"].ptr,
+                              b["count"].ptr, b["U0"].ptr, b["F0"].ptr)
         local_icgn_list_dev(im, self.plan_fb, self.d_coords, b["U0"].ptr, b["F0"].ptr,
                             self.d_winsize, b["list"].ptr, b["count"].ptr, self.tol, b["U"].ptr,
                             b["F"].ptr, d_conv=b["conv"].ptr, d_zncc=b["z"].ptr, ctx=B)

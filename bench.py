@@ -17,8 +17,9 @@ The learned-init step replaces the dense search:
   sparse integer search (same templates and radius, 128 px grid)
   -> coarse prior at every node -> CNN on the 48 x 48 patch pair at the
   node and at node + round(prior) -> [F0 U0] -> IC-GN
-  -> gate (more than 30 evaluations or ZNCC < 0.9) -> integer search at
-  the failed nodes -> IC-GN rerun of those nodes.
+  -> gate (more than 30 evaluations or ZNCC < 0.9) -> the search-init
+  start of the failed nodes (only the 32 px grid points they read) ->
+  IC-GN rerun of those nodes.
 
 The network is trained first (not timed) on the device, on 2M patch pairs
 labelled by IC-GN on seeded speckle mosaics.
@@ -48,10 +49,25 @@ import __graft_entry__ as ge  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+ROOFLINE_REPS = 5
+# HBM bytes per launch of the dominant IC-GN kernel (65 px class, this
+# workload, search start), rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in KiB per
+# dispatch, each scaled by the same counter's reading on a 16 MiB
+# device-to-device copy (FETCH 8207.2 KiB -> x1.996, WRITE 16384 KiB -> x1);
+# measured with this bench (profiles/icgn_class3_hbm_traffic.md)
+ICGN_C3_TRAFFIC = (246617.7 * 16384.0 / 8207.2 + 6690.0 * 1.0) * 1024.0
 
 
 def load(name, rel):
     return ge._load_file(name, os.path.join(ROOT, rel))
+
+
+def icgn_flops(side, iters):
+    """FP32 flops of one subset: 120 per pixel per evaluation (warp,
+    weights, 4x4 separable combine, moments, residual, b) + 46 per pixel
+    once (steepest-descent images, Hessian)."""
+    s = np.asarray(side, dtype=np.float64)
+    return (120.0 * np.asarray(iters, dtype=np.float64) + 46.0) * s * s
 
 
 def icgn_bytes(side):
@@ -302,7 +318,8 @@ def main():
         inet = learned.Network(ctx, max_batch=(n + 15) // 16 * 16, params=net.get_params())
         net.close()
         li = learned.LearnedInit(inet, im, d_f.ptr, d_g.ptr, plan, d_coords.ptr, n, d_ws.ptr,
-                                 prior_grid=([1, M], [1, N], sws, 128), search_ws=sws,
+                                 prior_grid=([1, M], [1, N], sws, 128),
+                                 fallback_grid=([1, M], [1, N], sws, sst),
                                  region=(r, r, r, r), max_iter=30, min_zncc=0.9, tol=1e-3,
                                  ctx_fb=ctx2, plan_fb=plan2)
         dt, stages, last = timed([ctx, ctx2], api, li.run, args.steps, args.warmup)
@@ -321,11 +338,36 @@ def main():
     head = res.get("learned") or res["search"]
     n_conv = int(head["ok"].sum())
     value = n_conv / head["dt"]
-    # ---- roofline of the dominant IC-GN kernel (HIP events on the ctx stream) ----
-    kern = head["kern"]
+    # ---- roofline of the dominant IC-GN kernel: ROOFLINE_REPS launches
+    # alone on one stream (the timed steps overlap it with the other
+    # stream's work), HIP events on that stream ----
+    progress("roofline pass")
+    api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst, [r, r, r, r],
+                           *[o.ptr for o in s_out], *[p.ptr for p in s_pk])
+    api.grid_to_nodes_dev(ctx, s_out[2].ptr, s_out[3].ptr, nX, nY, x0, x0, sst, d_coords.ptr, n,
+                          d_U0.ptr)
+    ctx.synchronize()
+    api.set_timing(ctx, True)
+    api.stage_times(ctx, reset=True)
+    for _ in range(ROOFLINE_REPS):
+        api.local_icgn_dev(im, plan, d_coords.ptr, d_U0.ptr, d_ws.ptr, 1e-3, d_U.ptr, d_F.ptr,
+                           None, d_conv.ptr, None, d_z.ptr)
+    rstages = api.stage_times(ctx, reset=True)
+    api.set_timing(ctx, False)
+    rconv = d_conv.to_host()
+    kern = {}
+    for c in range(4):
+        key = f"icgn_c{c}"
+        if key in rstages:
+            ms, cnt = rstages[key]
+            sel = cls == c
+            it = np.clip(rconv[sel], 1, 101)
+            kern[key] = (ms / cnt, float(icgn_bytes(side[sel]).sum()),
+                         float(icgn_flops(side[sel], it).sum()))
     dom = max(kern, key=lambda k: kern[k][0])
-    dom_ms, dom_bytes = kern[dom]
+    dom_ms, dom_bytes, dom_flops = kern[dom]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    tflops = dom_flops / (dom_ms * 1e-3) / 1e12
 
     def summary(name):
         d = res[name]
@@ -369,8 +411,12 @@ def main():
         "rms_ok": bool(head["rms"] <= 0.01),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel_ms": round(dom_ms, 4), "algorithmic_bytes": dom_bytes},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(ICGN_C3_TRAFFIC) if dom == "icgn_c3" else None,
+                     "kernel_ms": round(dom_ms, 4), "launches": ROOFLINE_REPS,
+                     "algorithmic_bytes": dom_bytes,
+                     "fp32": {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4)}},
         "gen_s": round(t_gen, 1),
         "icgn_launch_shapes": [int(c) for c in chosen],
     }

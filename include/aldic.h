@@ -278,6 +278,17 @@ int aldic_search_nodes_dev(aldic_ctx* ctx, const aldic_images* im, const double*
                            int64_t n, const int32_t* d_list, const int32_t* d_count,
                            int64_t max_count, int32_t winsize, const int64_t region[4],
                            float* d_U0, float* d_F0);
+/* The search-init starting point of the listed nodes only: the points of
+ * the funIntegerSearch grid (gridx, gridy, winsize, winstepsize, region as
+ * aldic_integer_search_dev) that aldic_grid_to_nodes_dev would read for
+ * them are searched, and U0 (F0 = 0, d_F0 may be NULL) of the listed
+ * nodes is interpolated from them -- the same U0 the whole-grid path gives
+ * those nodes.  d_f, d_g are the FP32 planes the search runs on. */
+int aldic_search_grid_nodes_dev(aldic_ctx* ctx, const float* d_f, const float* d_g, int64_t M,
+                                int64_t N, const int64_t gridx[2], const int64_t gridy[2],
+                                int32_t winsize, int32_t winstepsize, const int64_t region[4],
+                                const double* d_coords, int64_t n, const int32_t* d_list,
+                                const int32_t* d_count, float* d_U0, float* d_F0);
 /* IC-GN (funICGNQuadtree.m) of the listed nodes only, from [F0 U0]: list
  * entries are distinct node indices of the plan; other nodes' outputs are
  * left as they are. */

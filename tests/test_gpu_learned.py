@@ -274,6 +274,44 @@ def test_gate_search_and_list_icgn(L, pkg, orc):
     im.close()
 
 
+def test_search_grid_nodes_matches_whole_grid(L, pkg):
+    """The fallback start of listed nodes equals the whole-grid search +
+    transfer for those nodes, bit for bit; other nodes untouched."""
+    api, learned = L
+    ctx = api.default_context()
+    f, g, _, _ = c1_pair()
+    coords = c1_grid()[0]
+    n = len(coords)
+    M, N = f.shape
+    df = api.DeviceArray.from_host(ctx, f, np.float32)
+    dg = api.DeviceArray.from_host(ctx, g, np.float32)
+    dc = api.DeviceArray.from_host(ctx, np.asfortranarray(coords).reshape(-1, order="F"),
+                                   np.float64)
+    ws, step, reg = 30, 16, [10, 10, 10, 10]
+    nX = nY = len(range(1, M - ws - step + 1, step))
+    outs = [api.DeviceArray(ctx, nX * nY, np.float64) for _ in range(5)]
+    pks = [api.DeviceArray(ctx, nX * nY, np.int32) for _ in range(2)]
+    api.integer_search_dev(ctx, df.ptr, dg.ptr, M, N, [1, M], [1, N], ws, step, reg,
+                           *[o.ptr for o in outs], *[p.ptr for p in pks])
+    full = api.DeviceArray(ctx, 2 * n, np.float32)
+    api.grid_to_nodes_dev(ctx, outs[2].ptr, outs[3].ptr, nX, nY, 1 + ws / 2, 1 + ws / 2, step,
+                          dc.ptr, n, full.ptr)
+    sel = np.sort(np.random.default_rng(1).choice(n, 60, replace=False)).astype(np.int32)
+    dl = api.DeviceArray.from_host(ctx, sel, np.int32)
+    dn = api.DeviceArray.from_host(ctx, np.array([len(sel)], np.int32), np.int32)
+    U0 = api.DeviceArray.from_host(ctx, np.full(2 * n, 5.0, np.float32), np.float32)
+    F0 = api.DeviceArray.from_host(ctx, np.full(4 * n, 7.0, np.float32), np.float32)
+    api.search_grid_nodes_dev(ctx, df.ptr, dg.ptr, M, N, [1, M], [1, N], ws, step, reg, dc.ptr,
+                              n, dl.ptr, dn.ptr, U0.ptr, F0.ptr)
+    got = U0.to_host().reshape(-1, 2)
+    want = full.to_host().reshape(-1, 2)
+    np.testing.assert_array_equal(got[sel], want[sel])
+    rest = np.setdiff1d(np.arange(n), sel)
+    assert np.all(got[rest] == 5.0)
+    Fh = F0.to_host().reshape(-1, 4)
+    assert np.all(Fh[sel] == 0.0) and np.all(Fh[rest] == 7.0)
+
+
 def test_training_gains_over_prior_alone(L):
     """A short training run; the network must beat a constant prediction
     by a wide margin and cut IC-GN's evaluations on held-out subsets."""
