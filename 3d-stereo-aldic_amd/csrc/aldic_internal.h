@@ -63,6 +63,8 @@ struct aldic_plan {
   int rect = 0;
   int* lists[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int cnt[5] = {0, 0, 0, 0, 0};
+  // class c's subsets are all class_side(c) x class_side(c)
+  int full[5] = {0, 0, 0, 0, 0};
   int* defer_cnt = nullptr;
   int* defer_list = nullptr;
   // per-class lists of a node subset (IC-GN of listed nodes): capacity cnt[c]

@@ -32,6 +32,7 @@ struct SolveArgs {
   float tol;
   int maxit;            // evaluation cap k <= maxit (0: the reference's 100)
   const int* dcnt;      // fast kernels: list length read on the device (cnt = capacity)
+  int fixed_side;       // fast kernels: every listed subset has the class's full side
   float* U;             // 2n
   float* F;             // 4n (IC-GN), may be null
   float* H21;           // n x 21 column-major (IC-GN), may be null
@@ -44,6 +45,7 @@ struct SolveArgs {
 
 namespace aldic {
 int size_class(int S1, int S2);
+int class_side(int cls);  // the largest subset side of a fast-kernel class
 hipError_t launch_fast(int mode, int cls, int var, const SolveArgs& a, hipStream_t s);
 int num_variants();
 hipError_t launch_generic(int mode, const SolveArgs& a, const int* list, const int* count,

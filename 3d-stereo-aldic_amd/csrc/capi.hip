@@ -78,6 +78,7 @@ int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, 
   p->n = n;
   p->rect = rect;
   std::vector<int> lists[5];
+  bool full[5] = {true, true, true, true, false};
   for (int64_t i = 0; i < n; i++) {
     int wx, wy;
     if (rect) {
@@ -88,8 +89,11 @@ int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, 
     }
     if (wx < 0 || wy < 0 || (wx % 2) || (wy % 2))
       return fail(ALDIC_EINVAL, "winsize must be even and non-negative (validateDICpara.m:22-28)");
-    lists[aldic::size_class(wx + 1, wy + 1)].push_back((int)i);
+    const int c = aldic::size_class(wx + 1, wy + 1);
+    lists[c].push_back((int)i);
+    if (c < 4 && (wx + 1 != aldic::class_side(c) || wy + 1 != aldic::class_side(c))) full[c] = false;
   }
+  for (int c = 0; c < 5; c++) p->full[c] = full[c] && !lists[c].empty();
   for (int c = 0; c < 5; c++) {
     p->cnt[c] = (int)lists[c].size();
     if (p->cnt[c] > 0) {
@@ -129,6 +133,7 @@ int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
     if (p->cnt[c] == 0) continue;
     a.list = p->lists[c];
     a.cnt = p->cnt[c];
+    a.fixed_side = p->full[c];
     StageTimer t(ctx, base + c);
     HIP_TRY(aldic::launch_fast(mode, c, p->var[mode][c], a, s));
   }
@@ -652,6 +657,7 @@ extern "C" int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im,
     a.list = plan->sub_lists[c];
     a.cnt = plan->cnt[c];
     a.dcnt = plan->sub_cnt + c;
+    a.fixed_side = plan->full[c];
     HIP_TRY(aldic::launch_fast(0, c, plan->var[0][c], a, s));
   }
   a.dcnt = nullptr;
@@ -741,6 +747,7 @@ extern "C" int aldic_local_icgn_autotune_dev(aldic_ctx* ctx, const aldic_images*
     if (plan->cnt[c] == 0) continue;
     a.list = plan->lists[c];
     a.cnt = plan->cnt[c];
+    a.fixed_side = plan->full[c];
     float best = 1e30f;
     int bestv = 0;
     for (int v = 0; v < aldic::num_variants(); v++) {

@@ -265,7 +265,9 @@ constexpr int fast_block() {
   return NW == 1 ? 256 : ALDIC_WAVE * NW;
 }
 
-template <int MODE, int NW, int PPL, int TS>
+// FS > 0: every subset of the launch is FS x FS (the plan checked it), so
+// the pixel layout, the partial slot and the divisions are compile-time.
+template <int MODE, int NW, int PPL, int TS, int FS>
 __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
   constexpr int NT = ALDIC_WAVE * NW;
   constexpr int SUBS = fast_block<NW>() / NT;
@@ -295,8 +297,15 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
   y0 = __builtin_amdgcn_readfirstlane(y0);
   wx = __builtin_amdgcn_readfirstlane(wx);
   wy = __builtin_amdgcn_readfirstlane(wy);
-  const int hx = wx / 2, hy = wy / 2;
-  const int S1 = wx + 1, S2 = wy + 1;
+  if (FS > 0 && (wx != FS - 1 || wy != FS - 1)) {  // not the size the launch is built for
+    if (tid == 0) {
+      const int q = atomicAdd(a.defer_cnt, 1);
+      a.defer_list[q] = node;
+    }
+    return;
+  }
+  const int hx = FS > 0 ? (FS - 1) / 2 : wx / 2, hy = FS > 0 ? (FS - 1) / 2 : wy / 2;
+  const int S1 = FS > 0 ? FS : wx + 1, S2 = FS > 0 ? FS : wy + 1;
   const int NS = S1 * S2;
 
   float P[6];
@@ -1235,12 +1244,18 @@ int size_class(int S1, int S2) {
 
 int num_variants() { return NVAR; }
 
+int class_side(int cls) { return cls >= 0 && cls < 4 ? CLS_SIDE[cls] : 0; }
+
 template <int MODE, int C, int V>
 static void launch_shape(const SolveArgs& a, hipStream_t s) {
   constexpr int NW = VAR_NW[C][V];
   constexpr int block = fast_block<NW>();
   constexpr int subs = block / (ALDIC_WAVE * NW);
-  fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C]><<<(a.cnt + subs - 1) / subs, block, 0, s>>>(a);
+  const int grid = (a.cnt + subs - 1) / subs;
+  if (a.fixed_side)
+    fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], CLS_SIDE[C]><<<grid, block, 0, s>>>(a);
+  else
+    fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], 0><<<grid, block, 0, s>>>(a);
 }
 
 template <int MODE, int C>

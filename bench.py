@@ -14,7 +14,7 @@ batch with inputs resident in HBM:
 
 The learned-init step replaces the dense search:
 
-  sparse integer search (same templates and radius, 128 px grid)
+  sparse integer search (same templates and radius, 256 px grid)
   -> coarse prior at every node -> CNN on the 48 x 48 patch pair at the
   node and at node + round(prior) -> [F0 U0] -> IC-GN
   -> gate (more than 30 evaluations or ZNCC < 0.9) -> the search-init
@@ -213,6 +213,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-samples", type=int, default=2_000_000)
     ap.add_argument("--init", choices=["both", "search", "learned"], default="both")
+    ap.add_argument("--cap-sweep", default="",
+                    help="diagnostic: comma-separated IC-GN evaluation caps; times the "
+                         "search-start IC-GN at each cap and exits (no metric line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the local matching path is "
@@ -296,6 +299,22 @@ def main():
                          U=U, **(extra or {}))
 
     progress(f"autotuned launch shapes {list(chosen)}")
+    if args.cap_sweep:
+        for cap in [int(c) for c in args.cap_sweep.split(",")]:
+            api.local_icgn_f0_dev(im, plan, d_coords.ptr, d_U0.ptr, None, d_ws.ptr, 1e-3, d_U.ptr,
+                                  d_F.ptr, d_conv=d_conv.ptr, d_zncc=d_z.ptr, max_iter=cap)
+            ctx.synchronize()
+            api.set_timing(ctx, True)
+            api.stage_times(ctx, reset=True)
+            for _ in range(ROOFLINE_REPS):
+                api.local_icgn_f0_dev(im, plan, d_coords.ptr, d_U0.ptr, None, d_ws.ptr, 1e-3,
+                                      d_U.ptr, d_F.ptr, d_conv=d_conv.ptr, d_zncc=d_z.ptr,
+                                      max_iter=cap)
+            st = api.stage_times(ctx, reset=True)
+            api.set_timing(ctx, False)
+            print(json.dumps({"cap": cap, "stage_ms": {k: round(v[0] / v[1], 4)
+                                                       for k, v in st.items()}}), flush=True)
+        return
     for c in range(4):
         plan2.set_variant(0, c, chosen[c])
         plan2.set_variant(1, c, chosen[c])
@@ -318,7 +337,7 @@ def main():
         inet = learned.Network(ctx, max_batch=(n + 15) // 16 * 16, params=net.get_params())
         net.close()
         li = learned.LearnedInit(inet, im, d_f.ptr, d_g.ptr, plan, d_coords.ptr, n, d_ws.ptr,
-                                 prior_grid=([1, M], [1, N], sws, 128),
+                                 prior_grid=([1, M], [1, N], sws, 256),
                                  fallback_grid=([1, M], [1, N], sws, sst),
                                  region=(r, r, r, r), max_iter=30, min_zncc=0.9, tol=1e-3,
                                  ctx_fb=ctx2, plan_fb=plan2)
@@ -399,7 +418,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "stereo pair 2048x2048 12-bit speckle, hole r=300, "
                                f"{n} nodes with 8-64 px subsets; init: "
-                               + ("learned (CNN on a 128 px sparse-search prior, gate + "
+                               + ("learned (CNN on a 256 px sparse-search prior, gate + "
                                   "search fallback)" if "learned" in res else
                                   f"ZNCC search r=50 on a {nX}x{nY} grid")
                                + "; IC-GN tol 1e-3",

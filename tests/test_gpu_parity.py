@@ -205,6 +205,25 @@ def test_local_icgn_mixed_sizes(pkg, orc):
     _cmp_icgn(got, want, conv_tol_frac=0.05, ws=ws)
 
 
+@pytest.mark.parametrize("ws", [10, 16, 32, 64])
+def test_local_icgn_full_class(pkg, orc, ws):
+    """Every subset of a class at the class's full side (11, 17, 33, 65 px):
+    the compile-time-size wave kernels against the oracle."""
+    f, g, uf, vf = small_pair(M=256, N=240, seed=10, shift=(0.6, -0.2), strain=0.002)
+    rng = np.random.default_rng(ws)
+    n = 80
+    cx = rng.integers(40, 217, n).astype(float)
+    cy = rng.integers(40, 201, n).astype(float)
+    coords = np.asfortranarray(np.stack([cx, cy], 1))
+    U0 = np.zeros(2 * n)
+    U0[0::2] = np.round(uf(cx, cy))
+    U0[1::2] = np.round(vf(cx, cy))
+    im = pkg.Images(f, g)
+    got = pkg.local_icgn_raw(im, coords, U0, ws, 1e-3)
+    want = orc.local_icgn(orc.Images(f, g), coords, U0, np.full(n, ws, np.int32), 1e-3)
+    _cmp_icgn(got, want, conv_tol_frac=0.05)
+
+
 def test_subpb1(pkg, orc, c1_setup):
     f, g, uf, vf, coords, U0, _ = c1_setup
     want0 = orc.local_icgn(orc.Images(f, g), coords, U0, 30, 1e-3, nthreads=8)
