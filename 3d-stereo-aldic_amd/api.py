@@ -593,10 +593,13 @@ def grid_to_nodes_dev(ctx: Context, d_u, d_v, nX, nY, x0, y0, step, d_coords, n,
 
 
 def local_icgn_f0_dev(images: Images, plan: Plan, d_coords, d_U0, d_F0, d_winsize, tol, d_U,
-                      d_F=None, d_H21=None, d_conv=None, d_hole=None, d_zncc=None, max_iter=0):
+                      d_F=None, d_H21=None, d_conv=None, d_hole=None, d_zncc=None, max_iter=0,
+                      ctx: Context | None = None):
     """IC-GN from P0 = [F0 U0] (d_F0 None: zeros, as local_icgn_dev);
-    max_iter caps k (0: the reference's 100)."""
-    check(lib().aldic_local_icgn_f0_dev(images.ctx.ptr, images.ptr, plan.ptr, d_coords, d_U0,
+    max_iter caps k (0: the reference's 100); runs on ctx (default: the
+    images' context)."""
+    c = ctx or images.ctx
+    check(lib().aldic_local_icgn_f0_dev(c.ptr, images.ptr, plan.ptr, d_coords, d_U0,
                                         d_F0, d_winsize, float(tol), int(max_iter), d_U, d_F,
                                         d_H21, d_conv, d_hole, d_zncc))
 

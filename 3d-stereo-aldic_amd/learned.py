@@ -382,17 +382,22 @@ class LearnedInit:
     failed nodes (the points of the search grid their transfer reads) ->
     IC-GN rerun of those nodes (the reference's 100-evaluation cap).
 
-    With a second context (ctx_fb, with its own plan of the same nodes) the
-    steps of consecutive frames are pipelined: the fallback of step k runs
-    on ctx_fb while step k+1 starts on the main context.  Outputs and the
-    per-step buffers are double-buffered by k % 2; outputs(k) returns step
-    k's (U, F, convIt, ZNCC) device arrays."""
+    Steps of consecutive frames are pipelined over up to three contexts
+    (streams): the prior search and network of frame k run on the images'
+    context while IC-GN and the gate of frame k-1 run on ctx_icgn and the
+    fallback of frame k-2 on ctx_fb (each solver context with its own plan
+    of the same nodes).  The per-frame buffers are rotated over three sets;
+    outputs(k) returns frame k's (U, F, convIt, ZNCC) device arrays."""
+
+    NSETS = 3
 
     def __init__(self, net: Network, images: Images, d_f, d_g, plan: Plan, d_coords, n,
                  d_winsize, prior_grid, fallback_grid, region=(50, 50, 50, 50), max_iter=30,
-                 min_zncc=0.9, tol=1e-3, ctx_fb: Context | None = None, plan_fb: Plan | None = None):
+                 min_zncc=0.9, tol=1e-3, ctx_fb: Context | None = None, plan_fb: Plan | None = None,
+                 ctx_icgn: Context | None = None):
         ctx = images.ctx
         self.ctx = ctx
+        self.ctx_icgn = ctx_icgn or ctx
         self.ctx_fb = ctx_fb or ctx
         self.net, self.images, self.plan = net, images, plan
         self.plan_fb = plan_fb or plan
@@ -415,7 +420,7 @@ class LearnedInit:
         self.g_ipc = DeviceArray(ctx, npnt, np.int32)
         nn = self.n
         self.bufs = []
-        for _ in range(2):
+        for _ in range(self.NSETS):
             self.bufs.append(dict(
                 prior=DeviceArray(ctx, 2 * nn, np.float32), U0=DeviceArray(ctx, 2 * nn, np.float32),
                 F0=DeviceArray(ctx, 4 * nn, np.float32), list=DeviceArray(ctx, max(nn, 1), np.int32),
@@ -424,16 +429,17 @@ class LearnedInit:
                 z=DeviceArray(ctx, nn, np.float32)))
 
     def outputs(self, k):
-        b = self.bufs[k % 2]
+        b = self.bufs[k % self.NSETS]
         return b["U"], b["F"], b["conv"], b["z"]
 
     def run(self, k=0):
-        A, B, im = self.ctx, self.ctx_fb, self.images
-        sb = k % 2
+        A, B, C, im = self.ctx, self.ctx_icgn, self.ctx_fb, self.images
+        sb = k % self.NSETS
         b = self.bufs[sb]
-        two = B is not A
-        if two:
-            A.wait(B, sb)  # the fallback of step k-2 has released these buffers
+        # marks: A(sb) network done, B(3+sb) gate done, C(6 if sb == 0 else sb) fallback done
+        mA, mB, mC = sb, 3 + sb, 6 if sb == 0 else sb
+        if C is not A:
+            A.wait(C, mC)  # the fallback of frame k-3 has released set sb
         # 1. coarse prior from a sparse integer search
         integer_search_dev(A, self.d_f, self.d_g, im.M, im.N, self.gx, self.gy, self.pws,
                            self.pstep, self.region, self.g["x"].ptr, self.g["y"].ptr,
@@ -445,26 +451,29 @@ class LearnedInit:
                           self.pstep, self.d_coords, self.n, b["prior"].ptr)
         # 2. network -> [F0 U0]
         self.net.predict_dev(im, self.d_coords, self.n, b["prior"].ptr, b["U0"].ptr, b["F0"].ptr)
+        if B is not A:
+            A.record(mA)
+            B.wait(A, mA)
         # 3. IC-GN, 4. gate
         local_icgn_f0_dev(im, self.plan, self.d_coords, b["U0"].ptr, b["F0"].ptr, self.d_winsize,
                           self.tol, b["U"].ptr, b["F"].ptr, d_conv=b["conv"].ptr,
-                          d_zncc=b["z"].ptr, max_iter=self.max_iter)
-        gate_dev(A, self.n, b["conv"].ptr, b["z"].ptr, self.max_iter, self.min_zncc,
+                          d_zncc=b["z"].ptr, max_iter=self.max_iter, ctx=B)
+        gate_dev(B, self.n, b["conv"].ptr, b["z"].ptr, self.max_iter, self.min_zncc,
                  b["list"].ptr, b["count"].ptr)
-        if two:
-            A.record(2 + sb)
-            B.wait(A, 2 + sb)
+        if C is not B:
+            B.record(mB)
+            C.wait(B, mB)
         # 5. the search-init start of the failed nodes (only the grid points
         # they need), 6. IC-GN rerun of those nodes
         fgx, fgy, fws, fstep = self.fb_grid
-        search_grid_nodes_dev(B, self.d_f, self.d_g, im.M, im.N, fgx, fgy, fws, fstep,
+        search_grid_nodes_dev(C, self.d_f, self.d_g, im.M, im.N, fgx, fgy, fws, fstep,
                               self.region, self.d_coords, self.n, b["list"].ptr,
                               b["count"].ptr, b["U0"].ptr, b["F0"].ptr)
         local_icgn_list_dev(im, self.plan_fb, self.d_coords, b["U0"].ptr, b["F0"].ptr,
                             self.d_winsize, b["list"].ptr, b["count"].ptr, self.tol, b["U"].ptr,
-                            b["F"].ptr, d_conv=b["conv"].ptr, d_zncc=b["z"].ptr, ctx=B)
-        if two:
-            B.record(sb)
+                            b["F"].ptr, d_conv=b["conv"].ptr, d_zncc=b["z"].ptr, ctx=C)
+        if C is not A:
+            C.record(mC)
 
     def fallback_count(self, k=0):
-        return int(self.bufs[k % 2]["count"].to_host()[0])
+        return int(self.bufs[k % self.NSETS]["count"].to_host()[0])

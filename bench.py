@@ -259,6 +259,10 @@ def main():
     # are double-buffered by k % 2
     ctx2 = api.Context(0)
     plan2 = api.Plan(W["ws"], n, ctx=ctx2)
+    # the learned-init step pipelines over three: network (ctx), IC-GN and
+    # gate (ctx3, plan3), fallback (ctx2, plan2)
+    ctx3 = api.Context(0)
+    plan3 = api.Plan(W["ws"], n, ctx=ctx3)
     sets = [dict(U0=DA(ctx, 2 * n, np.float32), U=DA(ctx, 2 * n, np.float32),
                  F=DA(ctx, 4 * n, np.float32), conv=DA(ctx, n, np.int32),
                  z=DA(ctx, n, np.float32)) for _ in range(2)]
@@ -316,8 +320,9 @@ def main():
                                                        for k, v in st.items()}}), flush=True)
         return
     for c in range(4):
-        plan2.set_variant(0, c, chosen[c])
-        plan2.set_variant(1, c, chosen[c])
+        for pl in (plan2, plan3):
+            pl.set_variant(0, c, chosen[c])
+            pl.set_variant(1, c, chosen[c])
     if args.init in ("both", "search"):
         dt, stages, last = timed([ctx, ctx2], api, step_search, args.steps, args.warmup)
         progress(f"search-init step {dt * 1e3:.3f} ms")
@@ -336,12 +341,12 @@ def main():
         # inference copy of the trained weights, one batch for all nodes
         inet = learned.Network(ctx, max_batch=(n + 15) // 16 * 16, params=net.get_params())
         net.close()
-        li = learned.LearnedInit(inet, im, d_f.ptr, d_g.ptr, plan, d_coords.ptr, n, d_ws.ptr,
+        li = learned.LearnedInit(inet, im, d_f.ptr, d_g.ptr, plan3, d_coords.ptr, n, d_ws.ptr,
                                  prior_grid=([1, M], [1, N], sws, 256),
                                  fallback_grid=([1, M], [1, N], sws, sst),
                                  region=(r, r, r, r), max_iter=30, min_zncc=0.9, tol=1e-3,
-                                 ctx_fb=ctx2, plan_fb=plan2)
-        dt, stages, last = timed([ctx, ctx2], api, li.run, args.steps, args.warmup)
+                                 ctx_fb=ctx2, plan_fb=plan2, ctx_icgn=ctx3)
+        dt, stages, last = timed([ctx, ctx2, ctx3], api, li.run, args.steps, args.warmup)
         progress(f"learned-init step {dt * 1e3:.3f} ms")
         lU, lF, lconv, lz = li.outputs(last)
         train = {"samples": hist["samples"], "steps": hist["steps"],
