@@ -36,6 +36,10 @@ SIGNATURES = {
     "aldic_memcpy_dtoh": (C.c_int, [_vp, _vp, _vp, _i64]),
     "aldic_ctx_set_timing": (C.c_int, [_vp, C.c_int32]),
     "aldic_ctx_record": (C.c_int, [_vp, C.c_int32]),
+    "aldic_ctx_capture_begin": (C.c_int, [_vp]),
+    "aldic_ctx_capture_end": (C.c_int, [_vp, _pp]),
+    "aldic_graph_launch": (C.c_int, [_vp, _vp]),
+    "aldic_graph_destroy": (C.c_int, [_vp]),
     "aldic_ctx_wait": (C.c_int, [_vp, _vp, C.c_int32]),
     "aldic_ctx_stage_times": (C.c_int, [_vp, _vp, _vp, C.c_int32]),
     "aldic_grid_to_nodes_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.c_double, C.c_double,

@@ -65,9 +65,45 @@ class Context:
         """Make later work on this context wait for `mark` of `other`."""
         check(lib().aldic_ctx_wait(self.ptr, other.ptr, int(mark)))
 
+    def capture(self, fn):
+        """Record everything fn() submits to this context as a Graph (not
+        run)."""
+        check(lib().aldic_ctx_capture_begin(self.ptr))
+        try:
+            fn()
+        except BaseException:
+            p = C.c_void_p()
+            if lib().aldic_ctx_capture_end(self.ptr, C.byref(p)) == 0:
+                lib().aldic_graph_destroy(p)
+            raise
+        p = C.c_void_p()
+        check(lib().aldic_ctx_capture_end(self.ptr, C.byref(p)))
+        return Graph(p)
+
     def close(self):
         if self.ptr:
             lib().aldic_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured sequence of device work (aldic_graph)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+    def launch(self, ctx: Context):
+        check(lib().aldic_graph_launch(self.ptr, ctx.ptr))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().aldic_graph_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):

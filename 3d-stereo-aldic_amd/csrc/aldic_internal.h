@@ -6,6 +6,8 @@
 
 #include <string>
 #include <utility>
+#include <atomic>
+#include <memory>
 #include <vector>
 
 #include "../../include/aldic.h"
@@ -42,6 +44,17 @@ struct aldic_ctx {
   int stage_cnt[ALDIC_NSTAGES] = {0};
   // cross-context ordering points (aldic_ctx_record / aldic_ctx_wait)
   hipEvent_t marks[ALDIC_NMARKS] = {nullptr};
+  // graphs captured on this context that are still alive, plus one while a
+  // capture is open: staging buffers must not move while it is non-zero
+  // (shared with the graphs so either may be destroyed first)
+  std::shared_ptr<std::atomic<int>> pinned = std::make_shared<std::atomic<int>>(0);
+};
+
+struct aldic_graph {
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::shared_ptr<std::atomic<int>> pinned;  // of the capturing context
 };
 
 // Handles record their device so they can be destroyed after their context.

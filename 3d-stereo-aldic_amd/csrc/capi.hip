@@ -47,6 +47,10 @@ int staging(aldic_ctx* ctx, int slot, size_t bytes, void** out) {
     ctx->cap.resize(slot + 1, 0);
   }
   if (ctx->cap[slot] < bytes) {
+    if (ctx->pinned->load() > 0)
+      return fail(ALDIC_EINVAL, "staging buffer " + std::to_string(slot) +
+                                    " would grow while graphs captured on this context are alive "
+                                    "(run the sequence once before capturing)");
     if (ctx->buf[slot]) HIP_TRY(hipFree(ctx->buf[slot]));
     ctx->buf[slot] = nullptr;
     ctx->cap[slot] = 0;
@@ -240,6 +244,61 @@ extern "C" int aldic_ctx_wait(aldic_ctx* ctx, aldic_ctx* other, int32_t mark) {
   if (!other->marks[mark]) return ALDIC_OK;  // never recorded: nothing to wait for
   RC_TRY(set_device(ctx));
   HIP_TRY(hipStreamWaitEvent(ctx->stream, other->marks[mark], 0));
+  return ALDIC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// HIP graphs of a context's work: everything submitted to ctx between
+// capture_begin and capture_end (kernels only: no entry point on the
+// device-resident path issues a memset or an allocation once its staging
+// buffers exist) becomes one graph, replayed with its arguments as captured.
+// ---------------------------------------------------------------------------
+extern "C" int aldic_ctx_capture_begin(aldic_ctx* ctx) {
+  if (!ctx) return fail(ALDIC_EINVAL, "ctx_capture_begin: NULL context");
+  if (ctx->timing) return fail(ALDIC_EINVAL, "ctx_capture_begin: stage timing is on");
+  RC_TRY(set_device(ctx));
+  HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+  ctx->pinned->fetch_add(1);
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_ctx_capture_end(aldic_ctx* ctx, aldic_graph** out) {
+  if (!ctx || !out) return fail(ALDIC_EINVAL, "ctx_capture_end: NULL argument");
+  RC_TRY(set_device(ctx));
+  *out = nullptr;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(ctx->stream, &st));
+  if (st == hipStreamCaptureStatusNone)
+    return fail(ALDIC_EINVAL, "ctx_capture_end: no capture is open on this context");
+  auto* g = new aldic_graph();
+  g->device = ctx->device;
+  g->pinned = ctx->pinned;  // the open capture's count passes to the graph
+  hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    ctx->pinned->fetch_sub(1);
+    delete g;
+    return fail(ALDIC_EDEVICE, std::string("ctx_capture_end: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_graph_launch(aldic_graph* g, aldic_ctx* ctx) {
+  if (!g || !ctx) return fail(ALDIC_EINVAL, "graph_launch: NULL argument");
+  RC_TRY(set_device(ctx));
+  HIP_TRY(hipGraphLaunch(g->exec, ctx->stream));
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_graph_destroy(aldic_graph* g) {
+  if (!g) return ALDIC_OK;
+  (void)hipSetDevice(g->device);
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->pinned) g->pinned->fetch_sub(1);
+  delete g;
   return ALDIC_OK;
 }
 

@@ -432,14 +432,9 @@ class LearnedInit:
         b = self.bufs[k % self.NSETS]
         return b["U"], b["F"], b["conv"], b["z"]
 
-    def run(self, k=0):
-        A, B, C, im = self.ctx, self.ctx_icgn, self.ctx_fb, self.images
-        sb = k % self.NSETS
-        b = self.bufs[sb]
-        # marks: A(sb) network done, B(3+sb) gate done, C(6 if sb == 0 else sb) fallback done
-        mA, mB, mC = sb, 3 + sb, 6 if sb == 0 else sb
-        if C is not A:
-            A.wait(C, mC)  # the fallback of frame k-3 has released set sb
+    # ---- the three stages of one frame, each on its own context ----
+    def _stage_net(self, sb):
+        A, im, b = self.ctx, self.images, self.bufs[sb]
         # 1. coarse prior from a sparse integer search
         integer_search_dev(A, self.d_f, self.d_g, im.M, im.N, self.gx, self.gy, self.pws,
                            self.pstep, self.region, self.g["x"].ptr, self.g["y"].ptr,
@@ -451,18 +446,18 @@ class LearnedInit:
                           self.pstep, self.d_coords, self.n, b["prior"].ptr)
         # 2. network -> [F0 U0]
         self.net.predict_dev(im, self.d_coords, self.n, b["prior"].ptr, b["U0"].ptr, b["F0"].ptr)
-        if B is not A:
-            A.record(mA)
-            B.wait(A, mA)
+
+    def _stage_icgn(self, sb):
+        B, im, b = self.ctx_icgn, self.images, self.bufs[sb]
         # 3. IC-GN, 4. gate
         local_icgn_f0_dev(im, self.plan, self.d_coords, b["U0"].ptr, b["F0"].ptr, self.d_winsize,
                           self.tol, b["U"].ptr, b["F"].ptr, d_conv=b["conv"].ptr,
                           d_zncc=b["z"].ptr, max_iter=self.max_iter, ctx=B)
         gate_dev(B, self.n, b["conv"].ptr, b["z"].ptr, self.max_iter, self.min_zncc,
                  b["list"].ptr, b["count"].ptr)
-        if C is not B:
-            B.record(mB)
-            C.wait(B, mB)
+
+    def _stage_fallback(self, sb):
+        C, im, b = self.ctx_fb, self.images, self.bufs[sb]
         # 5. the search-init start of the failed nodes (only the grid points
         # they need), 6. IC-GN rerun of those nodes
         fgx, fgy, fws, fstep = self.fb_grid
@@ -472,8 +467,46 @@ class LearnedInit:
         local_icgn_list_dev(im, self.plan_fb, self.d_coords, b["U0"].ptr, b["F0"].ptr,
                             self.d_winsize, b["list"].ptr, b["count"].ptr, self.tol, b["U"].ptr,
                             b["F"].ptr, d_conv=b["conv"].ptr, d_zncc=b["z"].ptr, ctx=C)
+
+    def run(self, k=0, graphs=None):
+        """Frame k; graphs (from capture()) replays the stages instead of
+        issuing their launches."""
+        A, B, C = self.ctx, self.ctx_icgn, self.ctx_fb
+        sb = k % self.NSETS
+        # marks: A(sb) network done, B(3+sb) gate done, C(6 if sb == 0 else sb) fallback done
+        mA, mB, mC = sb, 3 + sb, 6 if sb == 0 else sb
+        if C is not A:
+            A.wait(C, mC)  # the fallback of frame k-3 has released set sb
+        if graphs:
+            graphs[0][sb].launch(A)
+        else:
+            self._stage_net(sb)
+        if B is not A:
+            A.record(mA)
+            B.wait(A, mA)
+        if graphs:
+            graphs[1][sb].launch(B)
+        else:
+            self._stage_icgn(sb)
+        if C is not B:
+            B.record(mB)
+            C.wait(B, mB)
+        if graphs:
+            graphs[2][sb].launch(C)
+        else:
+            self._stage_fallback(sb)
         if C is not A:
             C.record(mC)
+
+    def capture(self):
+        """One HIP graph per stage and buffer set (run every stage once
+        first, so staging buffers and kernel attributes exist)."""
+        out = [[], [], []]
+        for sb in range(self.NSETS):
+            out[0].append(self.ctx.capture(lambda: self._stage_net(sb)))
+            out[1].append(self.ctx_icgn.capture(lambda: self._stage_icgn(sb)))
+            out[2].append(self.ctx_fb.capture(lambda: self._stage_fallback(sb)))
+        return out
 
     def fallback_count(self, k=0):
         return int(self.bufs[k % self.NSETS]["count"].to_host()[0])

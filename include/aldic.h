@@ -77,6 +77,23 @@ int aldic_ctx_synchronize(aldic_ctx* ctx);
 int aldic_ctx_record(aldic_ctx* ctx, int32_t mark);
 int aldic_ctx_wait(aldic_ctx* ctx, aldic_ctx* other, int32_t mark);
 
+/* HIP graph of a context's work (the stereo/temporal local step repeated
+ * per frame): everything submitted to ctx between capture_begin and
+ * capture_end is recorded, not run; aldic_graph_launch replays it on a
+ * context's stream with the arguments as captured (device-resident counts
+ * are read when the graph runs).  Stage timing must be off while
+ * capturing.  Run the sequence once before capturing it: the context's
+ * staging buffers are pinned from capture_begin until every graph captured
+ * on it is destroyed, and a call that would have to grow one meanwhile
+ * fails with ALDIC_EINVAL.  Waits on marks recorded outside the capture
+ * cannot be captured; order graphs across contexts with record/wait
+ * between launches.  No counterpart in the reference. */
+typedef struct aldic_graph aldic_graph;
+int aldic_ctx_capture_begin(aldic_ctx* ctx);
+int aldic_ctx_capture_end(aldic_ctx* ctx, aldic_graph** out);
+int aldic_graph_launch(aldic_graph* graph, aldic_ctx* ctx);
+int aldic_graph_destroy(aldic_graph* graph);
+
 int aldic_ctx_set_timing(aldic_ctx* ctx, int32_t enable);
 int aldic_ctx_stage_times(aldic_ctx* ctx, double* ms, int32_t* launches, int32_t reset);
 

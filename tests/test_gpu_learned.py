@@ -329,3 +329,79 @@ def test_training_gains_over_prior_alone(L):
     print(res)
     assert res["learned"]["good"] > res["prior"]["good"] + 0.2
     net.close()
+
+
+def test_learned_pipeline_graphs_match_direct(L, pkg):
+    """The learned-init step gives the same frame whether it runs on one
+    context, pipelined over three, or replayed from per-stage HIP graphs
+    (each replay checked, with host synchronisations and other device work
+    between replays)."""
+    api, learned = L
+    ctx = api.default_context()
+    f, g, _, _ = c1_pair()
+    coords = c1_grid()[0]
+    n = len(coords)
+    M, N = f.shape
+    DA = api.DeviceArray
+    d_f = DA.from_host(ctx, f, np.float32)
+    d_g = DA.from_host(ctx, g, np.float32)
+    im = api.Images.from_device(d_f.ptr, d_g.ptr, M, N, ctx=ctx)
+    dc = DA.from_host(ctx, np.asfortranarray(coords).reshape(-1, order="F"), np.float64)
+    ws = np.full(n, 30, np.int32)
+    dws = DA.from_host(ctx, ws, np.int32)
+    # a network whose last layer is zero predicts the prior itself (U0 =
+    # prior, F0 = 0): most nodes pass the gate, the rest take the fallback
+    p = learned.he_init(0)
+    last = learned.layout()[-1]
+    p[last["w_off"]:last["w_off"] + last["cout"] * last["Kp"]] = 0.0
+    p[last["b_off"]:last["b_off"] + last["cout"]] = 0.0
+    net = learned.Network(ctx, max_batch=(n + 15) // 16 * 16, params=p)
+    kw = dict(prior_grid=([1, M], [1, N], 30, 128), fallback_grid=([1, M], [1, N], 30, 16),
+              region=(10, 10, 10, 10), max_iter=30, min_zncc=0.9, tol=1e-3)
+
+    def host(li, k):
+        return [a.to_host() for a in li.outputs(k)] + [li.fallback_count(k)]
+
+    def same(a, b):
+        for x, y in zip(a[:4], b[:4]):
+            np.testing.assert_array_equal(x, y)
+        assert a[4] == b[4]
+
+    one = learned.LearnedInit(net, im, d_f.ptr, d_g.ptr, api.Plan(ws, n, ctx=ctx), dc.ptr, n,
+                              dws.ptr, **kw)
+    one.run(0)
+    ref = host(one, 0)
+    nfb = ref[4]
+    assert 0 < nfb < n // 2, nfb
+    # the first row of nodes sits on the image edge (status 101/102 there)
+    ok = (ref[2] >= 1) & (ref[2] <= 100)
+    assert ok.mean() > 0.9
+    ctx2, ctx3 = api.Context(0), api.Context(0)
+    li = learned.LearnedInit(net, im, d_f.ptr, d_g.ptr, api.Plan(ws, n, ctx=ctx3), dc.ptr, n,
+                             dws.ptr, ctx_fb=ctx2, plan_fb=api.Plan(ws, n, ctx=ctx2),
+                             ctx_icgn=ctx3, **kw)
+    # pipelined, direct launches
+    for k in range(6):
+        li.run(k)
+    for c in (ctx, ctx2, ctx3):
+        c.synchronize()
+    for k in range(3, 6):
+        same(host(li, k), ref)
+    graphs = li.capture()
+    scratch = DA(ctx, 2 * n, np.float32)
+    for k in range(6, 15):
+        li.run(k, graphs)
+        if k % 3 == 2:
+            for c in (ctx, ctx2, ctx3):
+                c.synchronize()
+            for j in range(k - 2, k + 1):
+                same(host(li, j), ref)
+            # eager work on the same buffers between replays
+            one.run(1)
+            same(host(one, 1), ref)
+            scratch.to_host()
+    for gs in graphs:
+        for gr in gs:
+            gr.close()
+    net.close()
+    im.close()
