@@ -23,7 +23,9 @@ from .api import (  # noqa: E402
     funNormalizeImg,
     icgn_bad_points,
     local_icgn_raw,
+    nan_fill,
     remove_outliers,
+    scattered_natural,
     subpb1_bad_points,
     subpb1_raw,
 )
@@ -32,5 +34,5 @@ __all__ = [
     "AldicError", "Context", "Images", "IntegerSearchQuadtree", "LocalICGNQuadtree",
     "Subpb1Quadtree", "ba_interp2_spline", "default_context", "funImgGradient",
     "funIntegerSearch", "funNormalizeImg", "icgn_bad_points", "local_icgn_raw",
-    "remove_outliers", "subpb1_bad_points", "subpb1_raw", "lib",
+    "nan_fill", "remove_outliers", "scattered_natural", "subpb1_bad_points", "subpb1_raw", "lib",
 ]

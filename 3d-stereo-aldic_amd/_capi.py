@@ -47,6 +47,8 @@ SIGNATURES = {
     "aldic_interp_spline_f64": (C.c_int, [_vp, _dp, _i64, _i64, _i64, _dp, _dp, _i64, _dp]),
     "aldic_interp_spline_f32": (C.c_int, [_vp, _fp, _i64, _i64, _i64, _fp, _fp, _i64, _fp]),
     "aldic_img_gradient": (C.c_int, [_vp, _dp, _vp, _i64, _i64, _dp, _dp]),
+    "aldic_natural_interp": (C.c_int, [_dp, _dp, _dp, _i64, C.c_int32, _dp, _dp, _i64, _dp]),
+    "aldic_nan_fill": (C.c_int, [_dp, _i64, _dp, _vp, _vp, _i64, _i64, _vp]),
     "aldic_images_create": (C.c_int, [_vp, _dp, _dp, _vp, _vp, _vp, _i64, _i64, _pp]),
     "aldic_images_create_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i64, _pp]),
     "aldic_images_destroy": (C.c_int, [_vp]),

@@ -19,8 +19,10 @@ image x, node coordinates are 1-based, vectors are flat.  ``Df`` and
 
 Differences that a caller sees, all stated where they occur:
 * the natural-neighbour NaN fill after the bad-point rules
-  (LocalICGNQuadtree.m:208-325, Subpb1Quadtree.m:190-254) is not reproduced:
-  bad points come back as NaN (``fill_nans`` is reserved for it);
+  (LocalICGNQuadtree.m:200-325, Subpb1Quadtree.m:189-254) runs on the host in
+  libaldic (:func:`nan_fill`); where sites tie for the nearest one outside
+  the hull it takes the first, and degenerate (collinear) sites in the
+  global pass raise an error;
 * ``cc["qfactors"]`` (PCE/PPE) is not computed: it is unused downstream of
   IntegerSearchQuadtree with qDICOrNot = 0 (IntegerSearchQuadtree.m:159);
 * funRemoveOutliers' median test and inpaint_nans are applied on the host
@@ -332,23 +334,70 @@ def subpb1_bad_points(ConvItPerEle, maxIterNum=100):
 
 
 # ---------------------------------------------------------------------------
+# NaN fill of the drivers (host, natural-neighbour interpolation)
+# ---------------------------------------------------------------------------
+def scattered_natural(x, y, v, xq, yq):
+    """scatteredInterpolant(x, y, v, 'natural', 'nearest') evaluated at
+    (xq, yq).  v may hold several value columns (n x nv); the result has the
+    shape of xq (times nv columns when v is 2-D)."""
+    x = _F(x).reshape(-1)
+    y = _F(y).reshape(-1)
+    v = np.asarray(v, dtype=np.float64)
+    cols = v.reshape(x.size, -1)
+    xq = np.asarray(xq, dtype=np.float64)
+    shape = xq.shape
+    xqf = _F(xq).reshape(-1)
+    yqf = _F(yq).reshape(-1)
+    if xqf.size != yqf.size or cols.shape[0] != x.size or y.size != x.size:
+        raise ValueError("scattered_natural: sizes of x, y, v or xq, yq disagree")
+    out = np.zeros((xqf.size, cols.shape[1]), order="F")
+    check(lib().aldic_natural_interp(x, y, _F(cols), x.size, cols.shape[1], xqf, yqf, xqf.size,
+                                     out))
+    return out.reshape(shape) if v.ndim == 1 else out
+
+
+def nan_fill(coordinatesFEM, U, F=None, mask=None):
+    """The NaN fill after the bad-point rules: per 8-connected mask region,
+    then over every node (LocalICGNQuadtree.m:200-247, :290-325 with F and
+    mask = Df.ImgRefMask; Subpb1Quadtree.m:189-254 with F = None and mask =
+    DICpara.ImgRefMask).  Returns a filled copy of U."""
+    coords = _F(np.asarray(coordinatesFEM, dtype=np.float64)[:, :2])
+    n = coords.shape[0]
+    Uo = _F(U).reshape(-1).copy()
+    if Uo.size != 2 * n:
+        raise ValueError("nan_fill: U must hold 2 values per node")
+    Fp = None
+    if F is not None:
+        Ff = _F(F).reshape(-1)
+        if Ff.size != 4 * n:
+            raise ValueError("nan_fill: F must hold 4 values per node")
+        Fp = Ff.ctypes.data
+    mp, M, N = None, 0, 0
+    if mask is not None:
+        m8 = np.asfortranarray(np.asarray(mask) != 0, dtype=np.uint8)
+        M, N = m8.shape
+        mp = m8.ctypes.data
+    check(lib().aldic_nan_fill(coords, n, Uo, Fp, mp, M, N, None))
+    return Uo
+
+
+# ---------------------------------------------------------------------------
 # MATLAB-level drivers
 # ---------------------------------------------------------------------------
 def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="GaussNewton",
                       tol=1e-3, shapeFunctionOrder=1, ctx: Context | None = None,
-                      fill_nans=False):
+                      fill_nans=True):
     """[U,F,HtempPar,LocalTime,ConvItPerEle,LocalICGNBadPtNum,markCoordHoleStrain] =
     LocalICGNQuadtree(U0,coordinatesFEM,Df,ImgRef,ImgDef,DICpara,ICGNmethod,tol,
     shapeFunctionOrder)
 
     ``DICpara["winsize"]`` may also be an array with one winsize per node.
-    markCoordHoleStrain is 0-based (numpy indexing)."""
+    markCoordHoleStrain is 0-based (numpy indexing).  fill_nans=False
+    returns U with the bad nodes still NaN (before :200)."""
     if shapeFunctionOrder != 1:
         raise NotImplementedError("only first-order IC-GN (shapeFunctionOrder = 1) is built")
     if ICGNmethod != "GaussNewton":
         raise NotImplementedError("only ICGNmethod = 'GaussNewton' is used by the reference")
-    if fill_nans:
-        raise NotImplementedError("the natural-neighbour NaN fill is not reproduced")
     images = _images_for(Df, ImgRef, ImgDef, ctx)
     t0 = time.perf_counter()
     U, F, H, conv, hole, _z = local_icgn_raw(images, coordinatesFEM, U0, DICpara["winsize"], tol)
@@ -357,23 +406,28 @@ def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmetho
     U = U.copy()
     U[2 * bad] = np.nan
     U[2 * bad + 1] = np.nan
+    if fill_nans:
+        mask = None if Df is None else Df.get("ImgRefMask")
+        if mask is None:
+            mask = np.ones((images.M, images.N))
+        U = nan_fill(coordinatesFEM, U, F, mask)
     return U, F, H, LocalTime, conv.astype(np.float64), badnum, np.flatnonzero(hole == 1)
 
 
 def Subpb1Quadtree(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef, mu, beta, HPar,
                    ALSolveStep, DICpara, ICGNmethod="GaussNewton", tol=1e-2,
-                   ctx: Context | None = None, fill_nans=False):
+                   ctx: Context | None = None, fill_nans=True):
     """[U,HPar,ALSub1Time,ConvItPerEle,LocalICGNBadPtNum] = Subpb1Quadtree(UOld,FOld,udual,
     vdual,coordinatesFEM,Df,ImgRef,ImgDef,mu,beta,HPar,ALSolveStep,DICpara,ICGNmethod,tol)
 
     ``beta`` and ``HPar`` are accepted and unused, as in the reference's
     quadtree variant (the local H is rebuilt inside each subset).  The
     ``ALSolveStep == 1`` branch of the reference reads an undefined variable
-    (Subpb1Quadtree.m:101-107) and is rejected."""
+    (Subpb1Quadtree.m:101-107) and is rejected.  The region pass of the NaN
+    fill reads ``DICpara["ImgRefMask"]`` (all ones when absent);
+    fill_nans=False returns U with the bad nodes still NaN."""
     if ALSolveStep == 1:
         raise ValueError("ALSolveStep == 1 is not reachable in the reference (undefined Htemp)")
-    if fill_nans:
-        raise NotImplementedError("the natural-neighbour NaN fill is not reproduced")
     images = _images_for(Df, ImgRef, ImgDef, ctx)
     n = np.asarray(coordinatesFEM).shape[0]
     wl = DICpara.get("winsize_List")
@@ -386,6 +440,11 @@ def Subpb1Quadtree(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef,
     U = U.copy()
     U[2 * bad] = np.nan
     U[2 * bad + 1] = np.nan
+    if fill_nans:
+        mask = DICpara.get("ImgRefMask")
+        if mask is None:
+            mask = np.ones((images.M, images.N))
+        U = nan_fill(coordinatesFEM, U, None, mask)
     return U, HPar, ALSub1Time, conv.astype(np.float64), badnum
 
 

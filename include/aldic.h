@@ -315,6 +315,35 @@ int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im, const aldi
                               const int32_t* d_count, double tol, float* d_U, float* d_F,
                               float* d_H21, int32_t* d_convIt, uint8_t* d_hole, float* d_zncc);
 
+/* ---- host post-processing of the drivers (no device work) ---- */
+
+/* scatteredInterpolant(xs, ys, vals, 'natural', 'nearest') evaluated at
+ * (xq, yq): Sibson natural-neighbour interpolation inside the convex hull of
+ * the sites, linear along its edges, the nearest site's value outside (the
+ * first site in input order on a tie).  vals is n x nv column-major (nv
+ * value columns over one set of sites, e.g. u and v), out is nq x nv.
+ * Duplicate sites are merged with their values averaged.  Fewer than three
+ * distinct sites, or collinear sites, is ALDIC_EINVAL (the reference's
+ * interpolant construction fails there).  Replaces the scatteredInterpolant
+ * calls of LocalICGNQuadtree.m:235-241, :302-305 and
+ * Subpb1Quadtree.m:221-227, :239-242. */
+int aldic_natural_interp(const double* xs, const double* ys, const double* vals, int64_t n,
+                         int32_t nv, const double* xq, const double* yq, int64_t nq, double* out);
+/* The NaN fill that follows the bad-point rules of the drivers, in place on
+ * U (2n, interleaved; NaN marks a bad node).  Pass 1, when a node is NaN
+ * and mask (M x N, column-major, nonzero = inside) is given: per
+ * 8-connected mask region, the NaN nodes whose rounded position lies in it
+ * are interpolated from the region's valid nodes when there are at least
+ * three (a region whose sites are degenerate is skipped, the reference's
+ * catch).  Pass 2, when a node is still NaN or (F given, 4n) some F(4k+1)
+ * is NaN: U at every node becomes the interpolant of all valid nodes.
+ * LocalICGNQuadtree.m:200-247 + :290-325 (F given, mask = Df.ImgRefMask);
+ * Subpb1Quadtree.m:189-254 (F = NULL, mask = DICpara.ImgRefMask).
+ * regions_filled (may be NULL) receives the number of regions pass 1
+ * interpolated in. */
+int aldic_nan_fill(const double* coords, int64_t n, double* U, const double* F,
+                   const uint8_t* mask, int64_t M, int64_t N, int32_t* regions_filled);
+
 #ifdef __cplusplus
 }
 #endif

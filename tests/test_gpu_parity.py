@@ -12,10 +12,13 @@ Tolerances (floating point, the device computes in FP32 except where noted):
     FP32 round-off of b near convergence); deformation gradients
     |dF| <= 1e-5 where both stop after the same number of evaluations and
     <= 1e-3 (one converged Gauss-Newton step) where they stop one apart;
+  * the drivers' NaN fill (host code): 1e-9 against the oracle's from the
+    same pre-fill U;
   * integer search: integer ZNCC peaks bit-exact; sub-pixel u, v within
     1e-3 + 1e-9 (the reference rounds the fit to 1e-3, so an FP32 ZNCC can
     land on the other side of a rounding boundary).
 """
+import natural_oracle
 import numpy as np
 import pytest
 
@@ -140,10 +143,23 @@ def test_local_icgn_driver_outputs(pkg, c1_setup):
         1e-3, 1)
     assert U.shape == (1800,) and F.shape == (3600,) and H.shape == (900, 21)
     bad, num = pkg.icgn_bad_points(conv)
-    assert badnum == num and np.all(np.isnan(U[2 * bad]))
+    assert badnum == num and len(bad) > 0
     good = np.setdiff1d(np.arange(900), bad)
     eu = U[2 * good] - uf(coords[good, 0], coords[good, 1])
     assert np.sqrt(np.mean(eu ** 2)) < 0.03
+    # the NaN fill (LocalICGNQuadtree.m:200-325) against the oracle's, from
+    # the same pre-fill U
+    U1, F1, *_ = pkg.LocalICGNQuadtree(
+        U0, coords, Df, f, g, {"winsize": 30, "winstepsize": 16, "ClusterNo": 1}, "GaussNewton",
+        1e-3, 1, fill_nans=False)
+    assert np.all(np.isnan(U1[2 * bad])) and np.array_equal(U1[2 * good], U[2 * good])
+    assert np.all(np.isfinite(U))
+    want = natural_oracle.nan_fill(coords, U1, F1, Df["ImgRefMask"])
+    assert np.max(np.abs(U - want)) < 1e-9
+    # filled values follow the field as well as their neighbours do
+    ef = U[2 * bad] - uf(coords[bad, 0], coords[bad, 1])
+    inner = (coords[bad, 0] > 16) & (coords[bad, 1] > 16)
+    assert np.max(np.abs(ef[inner])) < 0.5
 
 
 def test_local_icgn_status_codes(pkg, orc):
@@ -241,6 +257,23 @@ def test_subpb1(pkg, orc, c1_setup):
     assert np.array_equal(convo == 0, conv == 0)
     assert np.mean(conv != convo) <= 0.03
     assert np.abs(U - Uo).reshape(-1, 2)[ok].max() <= 1e-3
+    # the driver: bad-point rules and NaN fill (Subpb1Quadtree.m:167-254)
+    mask = np.ones(f.shape)
+    mask[200:230, :] = 0  # two regions
+    Df = pkg.funImgGradient(f, g)
+    DICpara = {"winsize": 30, "winsize_List": wl, "ImgRefMask": mask}
+    args = (UOld, FOld, udual, vdual, coords, Df, f, g, 1e-3, 0.0, None, 2, DICpara, "GaussNewton",
+            1e-2)
+    Ud, _, _, convd, badnum = pkg.Subpb1Quadtree(*args)
+    Upre, _, _, convp, _ = pkg.Subpb1Quadtree(*args, fill_nans=False)
+    assert np.array_equal(convd, convp)
+    bad, num = pkg.subpb1_bad_points(convd)
+    assert badnum == num
+    # NaN: the bad nodes, and the nodes whose solve threw (convIt 0, U NaN)
+    nan_nodes = np.flatnonzero(np.isnan(Upre[0::2]))
+    assert np.all(np.isin(bad, nan_nodes)) and nan_nodes.size > 0
+    want = natural_oracle.nan_fill(coords, Upre, None, mask)
+    assert np.all(np.isfinite(Ud)) and np.max(np.abs(Ud - want)) < 1e-9
 
 
 def test_integer_search_c1(pkg, orc, c1_setup):
