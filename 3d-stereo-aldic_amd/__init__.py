@@ -12,10 +12,12 @@ lib()  # no library, no package: there is no CPU fallback
 
 from .api import (  # noqa: E402
     Context,
+    FEMesh,
     Images,
     IntegerSearchQuadtree,
     LocalICGNQuadtree,
     Subpb1Quadtree,
+    Subpb2Quadtree,
     ba_interp2_spline,
     default_context,
     funImgGradient,
@@ -31,8 +33,8 @@ from .api import (  # noqa: E402
 )
 
 __all__ = [
-    "AldicError", "Context", "Images", "IntegerSearchQuadtree", "LocalICGNQuadtree",
-    "Subpb1Quadtree", "ba_interp2_spline", "default_context", "funImgGradient",
+    "AldicError", "Context", "FEMesh", "Images", "IntegerSearchQuadtree",
+    "LocalICGNQuadtree", "Subpb1Quadtree", "Subpb2Quadtree", "ba_interp2_spline", "default_context", "funImgGradient",
     "funIntegerSearch", "funNormalizeImg", "icgn_bad_points", "local_icgn_raw",
     "nan_fill", "remove_outliers", "scattered_natural", "subpb1_bad_points", "subpb1_raw", "lib",
 ]

@@ -47,6 +47,14 @@ SIGNATURES = {
     "aldic_interp_spline_f64": (C.c_int, [_vp, _dp, _i64, _i64, _i64, _dp, _dp, _i64, _dp]),
     "aldic_interp_spline_f32": (C.c_int, [_vp, _fp, _i64, _i64, _i64, _fp, _fp, _i64, _fp]),
     "aldic_img_gradient": (C.c_int, [_vp, _dp, _vp, _i64, _i64, _dp, _dp]),
+    "aldic_fem_create": (C.c_int, [_vp, _dp, _i64, _vp, _i64, _vp, _i64, _vp, _i64, C.c_int32,
+                                   _pp]),
+    "aldic_fem_destroy": (C.c_int, [_vp]),
+    "aldic_fem_info": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "aldic_subpb2": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, _dp, _dp,
+                               _dp, _vp]),
+    "aldic_subpb2_dev": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, _vp,
+                                   _vp, _vp, C.c_int32, _vp]),
     "aldic_natural_interp": (C.c_int, [_dp, _dp, _dp, _i64, C.c_int32, _dp, _dp, _i64, _dp]),
     "aldic_nan_fill": (C.c_int, [_dp, _i64, _dp, _vp, _vp, _i64, _i64, _vp]),
     "aldic_images_create": (C.c_int, [_vp, _dp, _dp, _vp, _vp, _vp, _i64, _i64, _pp]),

@@ -382,6 +382,93 @@ def nan_fill(coordinatesFEM, U, F=None, mask=None):
 
 
 # ---------------------------------------------------------------------------
+# Subpb2Quadtree: the global FE step
+# ---------------------------------------------------------------------------
+class FEMesh:
+    """The FE Q4 quadtree mesh of the global step (aldic_fem): DICmesh's
+    coordinatesFEM (n x 2), elementsFEM (ne x 4 or ne x 8, 1-based, 0 = no
+    hanging node), dirichlet (1-based nodes) and neumann (n_edges x 4)."""
+
+    def __init__(self, coordinatesFEM, elementsFEM, dirichlet=None, neumann=None,
+                 GaussPtOrder=2, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        coords = _F(np.asarray(coordinatesFEM, dtype=np.float64)[:, :2])
+        el = np.asarray(elementsFEM, dtype=np.int64)
+        if el.ndim != 2 or el.shape[1] not in (4, 8):
+            raise ValueError("elementsFEM must be ne x 4 or ne x 8")
+        if el.shape[1] == 4:
+            el = np.concatenate([el, np.zeros((el.shape[0], 4), np.int64)], axis=1)
+        el = _F(el, np.int32)
+        d = np.zeros(0, np.int32) if dirichlet is None else np.asarray(dirichlet).reshape(-1)
+        d = _F(d, np.int32)
+        nm = None if neumann is None or np.size(neumann) == 0 else _F(np.asarray(neumann)
+                                                                        .reshape(-1, 4))
+        self.n, self.ne = coords.shape[0], el.shape[0]
+        p = C.c_void_p()
+        check(lib().aldic_fem_create(self.ctx.ptr, coords, self.n, el.ctypes.data, self.ne,
+                                     d.ctypes.data, d.size, None if nm is None else nm.ctypes.data,
+                                     0 if nm is None else nm.shape[0], int(GaussPtOrder),
+                                     C.byref(p)))
+        self.ptr = p
+
+    def info(self):
+        nnz, nf, nd = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().aldic_fem_info(self.ptr, C.byref(nnz), C.byref(nf), C.byref(nd)))
+        return {"nnz": nnz.value, "free": nf.value, "dirichlet": nd.value}
+
+    def solve(self, beta, mu, U, F, alpha=0.0, winstepsize=1.0, return_iterations=False):
+        U = _F(U).reshape(-1)
+        F = _F(F).reshape(-1)
+        if U.size != 2 * self.n or F.size != 4 * self.n:
+            raise ValueError("U must hold 2 and F 4 values per node")
+        out = np.zeros(2 * self.n, order="F")
+        it = C.c_int32()
+        check(lib().aldic_subpb2(self.ctx.ptr, self.ptr, float(beta), float(mu), float(alpha),
+                                 float(winstepsize), U, F, out, C.byref(it)))
+        return (out, it.value) if return_iterations else out
+
+    def solve_dev(self, ctx: Context, beta, mu, d_U, d_F, d_Uhat, alpha=0.0, winstepsize=1.0,
+                  max_iter=0):
+        it = C.c_int32()
+        check(lib().aldic_subpb2_dev(ctx.ptr, self.ptr, float(beta), float(mu), float(alpha),
+                                     float(winstepsize), d_U, d_F, d_Uhat, int(max_iter),
+                                     C.byref(it)))
+        return it.value
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().aldic_fem_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def Subpb2Quadtree(DICmesh, GaussPtOrder, beta, mu, U, F, udual, vdual, alpha, winstepsize,
+                   waitBarDisplayOrNot=0, ctx: Context | None = None):
+    """Uhat = Subpb2Quadtree(DICmesh, GaussPtOrder, beta, mu, U, F, udual, vdual,
+    alpha, winstepsize, waitBarDisplayOrNot) (func_quadtree/Subpb2Quadtree.m:
+    1-397).  ``DICmesh`` is a dict with coordinatesFEM, elementsFEM,
+    dirichlet and neumann, or an :class:`FEMesh` built once for the mesh.
+    udual and vdual are accepted and unused: the reference multiplies both
+    by zero (:42-44)."""
+    mesh = DICmesh
+    own = False
+    if not isinstance(mesh, FEMesh):
+        mesh = FEMesh(DICmesh["coordinatesFEM"], DICmesh["elementsFEM"],
+                      DICmesh.get("dirichlet"), DICmesh.get("neumann"), GaussPtOrder, ctx=ctx)
+        own = True
+    try:
+        return mesh.solve(beta, mu, U, F, alpha, winstepsize)
+    finally:
+        if own:
+            mesh.close()
+
+
+# ---------------------------------------------------------------------------
 # MATLAB-level drivers
 # ---------------------------------------------------------------------------
 def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmethod="GaussNewton",
@@ -742,8 +829,8 @@ def local_icgn_list_dev(images: Images, plan: Plan, d_coords, d_U0, d_F0, d_wins
 STAGES = {0: "search", 1: "grid_to_nodes", 2: "icgn_c0", 3: "icgn_c1", 4: "icgn_c2",
           5: "icgn_c3", 6: "icgn_generic", 7: "icgn_handed_on", 8: "subpb1_c0", 9: "subpb1_c1",
           10: "subpb1_c2", 11: "subpb1_c3", 12: "subpb1_generic", 13: "subpb1_handed_on",
-          14: "cnn", 15: "fallback"}
-NSTAGES = 16
+          14: "cnn", 15: "fallback", 16: "subpb2"}
+NSTAGES = 17
 
 
 def set_timing(ctx: Context, enable: bool):

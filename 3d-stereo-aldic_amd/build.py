@@ -25,8 +25,8 @@ FLAGS = [
     "-Wall", "-Wno-unused-parameter", "-Wno-unused-function",
 ]
 
-SOURCES = ["capi.hip", "cnn.hip", "cnn_net.hip", "icgn.hip", "interp.hip", "natural.cpp", "prep.hip",
-           "search.hip"]
+SOURCES = ["capi.hip", "cnn.hip", "cnn_net.hip", "fem.hip", "icgn.hip", "interp.hip", "natural.cpp",
+           "prep.hip", "search.hip"]
 
 
 def _compile(src: str) -> str:

@@ -66,7 +66,8 @@ int aldic_ctx_synchronize(aldic_ctx* ctx);
 #define ALDIC_STAGE_SUBPB1_C0 8    /* same layout for Subpb1 */
 #define ALDIC_STAGE_CNN 14         /* learned starting point: patches + network */
 #define ALDIC_STAGE_FALLBACK 15    /* gate, node search and IC-GN rerun */
-#define ALDIC_NSTAGES 16
+#define ALDIC_STAGE_SUBPB2 16      /* global FE step: assembly and solve */
+#define ALDIC_NSTAGES 17
 /* Ordering between contexts (one stream each) on one device, for callers
  * that pipeline independent frames over two contexts: aldic_ctx_record
  * marks the point reached by the work submitted so far to ctx (mark
@@ -314,6 +315,37 @@ int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im, const aldi
                               const int32_t* d_winsize, const int32_t* d_list,
                               const int32_t* d_count, double tol, float* d_U, float* d_F,
                               float* d_H21, int32_t* d_convIt, uint8_t* d_hole, float* d_zncc);
+
+/* ---- the global step of the ALM loop ---- */
+
+/* The FE Q4 mesh of Subpb2Quadtree (DICmesh): coords n x 2 (coordinatesFEM,
+ * column-major), elements ne x 8 (elementsFEM, column-major, 1-based; 0 in
+ * columns 5..8 = no hanging node on edge 2-3, 3-4, 4-1, 1-2), dirichlet nd
+ * 1-based nodes (DICmesh.dirichlet), neumann nn x 4 (node1, node2, n_x, n_y;
+ * DICmesh.neumann, NULL when empty), GaussPtOrder 2..5.  The sparsity of the
+ * global matrix and the order of its assembly sums are fixed here. */
+typedef struct aldic_fem aldic_fem;
+int aldic_fem_create(aldic_ctx* ctx, const double* coords, int64_t n, const int32_t* elements,
+                     int64_t ne, const int32_t* dirichlet, int64_t nd, const double* neumann,
+                     int64_t nn, int32_t gauss_order, aldic_fem** out);
+int aldic_fem_destroy(aldic_fem* fem);
+/* Nonzeros of the scalar matrix, free nodes, Dirichlet nodes. */
+int aldic_fem_info(const aldic_fem* fem, int64_t* nnz, int64_t* nfree, int64_t* ndir);
+/* Uhat = Subpb2Quadtree(DICmesh, GaussPtOrder, beta, mu, U, F, udual, vdual,
+ * alpha, winstepsize, 0) (func_quadtree/Subpb2Quadtree.m:1-397): U (2n), F
+ * (4n) and Uhat (2n) in FP64.  udual and vdual are not arguments: the
+ * reference multiplies both by zero (:42-44).  Nodes in no element (and
+ * the first node of unique(elementsFEM), the reference's :314-318) get 0,
+ * Dirichlet nodes get U, the rest the solution of the free system, here by
+ * Jacobi-preconditioned conjugate gradients to a relative residual of 1e-13
+ * per component (iterations, may be NULL, receives the count; max_iter <= 0
+ * picks 2n).  One call at a time per mesh. */
+int aldic_subpb2(aldic_ctx* ctx, aldic_fem* fem, double beta, double mu, double alpha,
+                 double winstepsize, const double* U, const double* F, double* Uhat,
+                 int32_t* iterations);
+int aldic_subpb2_dev(aldic_ctx* ctx, aldic_fem* fem, double beta, double mu, double alpha,
+                     double winstepsize, const double* d_U, const double* d_F, double* d_Uhat,
+                     int32_t max_iter, int32_t* iterations);
 
 /* ---- host post-processing of the drivers (no device work) ---- */
 
