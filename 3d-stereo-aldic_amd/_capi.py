@@ -55,6 +55,10 @@ SIGNATURES = {
                                _dp, _vp]),
     "aldic_subpb2_dev": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, _vp,
                                    _vp, _vp, C.c_int32, _vp]),
+    "aldic_images_planes": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "aldic_images_create_raw": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp, _i64, _i64, _vp, _vp,
+                                          _pp, _vp]),
+    "aldic_normalize_raw_dev": (C.c_int, [_vp, _vp, C.c_int32, _i64, _i64, _vp, _vp, _vp, _vp]),
     "aldic_natural_interp": (C.c_int, [_dp, _dp, _dp, _i64, C.c_int32, _dp, _dp, _i64, _dp]),
     "aldic_nan_fill": (C.c_int, [_dp, _i64, _dp, _vp, _vp, _i64, _i64, _vp]),
     "aldic_images_create": (C.c_int, [_vp, _dp, _dp, _vp, _vp, _vp, _i64, _i64, _pp]),

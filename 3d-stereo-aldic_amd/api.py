@@ -236,6 +236,56 @@ class Images:
         self.M, self.N = M, N
         return self
 
+    @classmethod
+    def from_raw(cls, ImgRef, ImgDef, ImgRefMask=None, gridxyROIRange=None,
+                 ctx: Context | None = None):
+        """Images from raw 8- or 16-bit pixel arrays (H x W, row-major as
+        decoded from the file): funNormalizeImg over the ROI and the
+        gradients run on the device (aldic_images_create_raw).  The MATLAB
+        image is the transpose, M = W by N = H.  Sets ``stats`` to
+        (mean_f, std_f, mean_g, std_g)."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        a = np.ascontiguousarray(ImgRef)
+        b = np.ascontiguousarray(ImgDef)
+        if a.dtype == np.uint8 and b.dtype == np.uint8:
+            bits = 8
+        elif a.dtype == np.uint16 and b.dtype == np.uint16:
+            bits = 16
+        else:
+            raise TypeError("raw images must both be uint8 or both uint16")
+        if a.shape != b.shape or a.ndim != 2:
+            raise ValueError("raw images must be two H x W arrays of one size")
+        H, W = a.shape
+        M, N = W, H
+        roi = gridxyROIRange or {"gridx": [1, M], "gridy": [1, N]}
+        gx = np.array(roi["gridx"], dtype=np.int64)
+        gy = np.array(roi["gridy"], dtype=np.int64)
+        mask = None if ImgRefMask is None else _F(ImgRefMask)
+        if mask is not None and mask.shape != (M, N):
+            raise ValueError("the mask is M x N = W x H")
+        stats = np.zeros(4)
+        p = C.c_void_p()
+        check(lib().aldic_images_create_raw(self.ctx.ptr, a.ctypes.data, b.ctypes.data, bits,
+                                            None if mask is None else mask.ctypes.data, M, N,
+                                            gx.ctypes.data, gy.ctypes.data, C.byref(p),
+                                            stats.ctypes.data))
+        self.ptr = p
+        self.M, self.N = M, N
+        self.stats = tuple(stats)
+        return self
+
+    def planes(self):
+        """Host copies of the resident planes: {fm, g, mask, fx, fy}, M x N FP32."""
+        ptrs = [C.c_void_p() for _ in range(5)]
+        check(lib().aldic_images_planes(self.ptr, *[C.byref(q) for q in ptrs]))
+        out = {}
+        for name, q in zip(("fm", "g", "mask", "fx", "fy"), ptrs):
+            a = np.empty((self.N, self.M), dtype=np.float32)
+            check(lib().aldic_memcpy_dtoh(self.ctx.ptr, a.ctypes.data, q, a.nbytes))
+            out[name] = a.T
+        return out
+
     def close(self):
         if getattr(self, "ptr", None):
             lib().aldic_images_destroy(self.ptr)

@@ -62,6 +62,12 @@ hipError_t launch_bucket(const SolveArgs& a, const int* list, const int* count, 
 __global__ void zero_ints_kernel(int* p, int n);
 
 // image preparation (prep.hip)
+// raw 8/16-bit images: exact sums over a 0-based inclusive ROI (sums[0..1]
+// = sum, sum of squares), then (raw - mean) / sd over all n pixels in FP64
+hipError_t launch_raw_sums(const void* raw, int bits, int64_t M, int64_t x0, int64_t x1,
+                           int64_t y0, int64_t y1, unsigned long long* sums, hipStream_t s);
+hipError_t launch_raw_normalize(const void* raw, int bits, int64_t n, double mean, double sd,
+                                double* out, hipStream_t s);
 hipError_t launch_prep_planes(const double* ref, const double* def, const double* mask,
                               int64_t M, int64_t N, float* fm, float* g, float* maskf,
                               hipStream_t s);

@@ -490,6 +490,135 @@ extern "C" int aldic_images_create(aldic_ctx* ctx, const double* ImgRef, const d
   return ALDIC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// raw 8/16-bit images: funNormalizeImg on the device
+// ---------------------------------------------------------------------------
+namespace {
+
+// ROI of funNormalizeImg.m:22-28, clamped to the image; 0-based inclusive
+int clamp_roi(const int64_t gridx[2], const int64_t gridy[2], int64_t M, int64_t N, int64_t r[4]) {
+  if (!gridx || !gridy) return fail(ALDIC_EINVAL, "normalize: NULL ROI");
+  int64_t a = gridx[0], b = gridx[1], c = gridy[0], d = gridy[1];
+  if (a < 1) a = 1;
+  if (b > M) b = M;
+  if (c < 1) c = 1;
+  if (d > N) d = N;
+  if (a > b || c > d) return fail(ALDIC_EINVAL, "normalize: empty ROI");
+  r[0] = a - 1;
+  r[1] = b - 1;
+  r[2] = c - 1;
+  r[3] = d - 1;
+  return ALDIC_OK;
+}
+
+// mean and std (n - 1 normalisation) from exact integer sums: the mean is
+// the correctly rounded S / n, the variance (n Q - S^2) / (n (n - 1)) is
+// formed exactly before one division
+void mean_std(unsigned long long S, unsigned long long Q, int64_t n, double* mean, double* sd) {
+  *mean = (double)S / (double)n;
+  if (n < 2) {
+    *sd = 0.0;
+    return;
+  }
+  const __int128 num = (__int128)n * (__int128)Q - (__int128)S * (__int128)S;
+  const long double var = (long double)num / ((long double)n * (long double)(n - 1));
+  *sd = (double)sqrtl(var);
+}
+
+int normalize_raw(aldic_ctx* ctx, const void* d_raw, int bits, int64_t M, int64_t N,
+                  const int64_t r[4], double* d_out, double* ms) {
+  hipStream_t s = ctx->stream;
+  unsigned long long* sums = nullptr;
+  RC_TRY(stage(ctx, 23, 2, &sums));
+  HIP_TRY(aldic::launch_raw_sums(d_raw, bits, M, r[0], r[1], r[2], r[3], sums, s));
+  unsigned long long h[2];
+  HIP_TRY(hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  double mean, sd;
+  mean_std(h[0], h[1], (r[1] - r[0] + 1) * (r[3] - r[2] + 1), &mean, &sd);
+  HIP_TRY(aldic::launch_raw_normalize(d_raw, bits, M * N, mean, sd, d_out, s));
+  if (ms) {
+    ms[0] = mean;
+    ms[1] = sd;
+  }
+  return ALDIC_OK;
+}
+
+}  // namespace
+
+extern "C" int aldic_images_planes(const aldic_images* im, const float** fm, const float** g,
+                                   const float** mask, const float** fx, const float** fy) {
+  if (!im) return fail(ALDIC_EINVAL, "images_planes: NULL images");
+  if (fm) *fm = im->fm;
+  if (g) *g = im->g;
+  if (mask) *mask = im->mask;
+  if (fx) *fx = im->fx;
+  if (fy) *fy = im->fy;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_normalize_raw_dev(aldic_ctx* ctx, const void* d_raw, int32_t bits, int64_t M,
+                                       int64_t N, const int64_t gridx[2], const int64_t gridy[2],
+                                       double* d_out, double* mean_std_out) {
+  if (!ctx || !d_raw || !d_out || M < 1 || N < 1)
+    return fail(ALDIC_EINVAL, "normalize_raw_dev: bad argument");
+  if (bits != 8 && bits != 16) return fail(ALDIC_EINVAL, "normalize_raw_dev: bits must be 8 or 16");
+  RC_TRY(set_device(ctx));
+  int64_t r[4];
+  RC_TRY(clamp_roi(gridx, gridy, M, N, r));
+  return normalize_raw(ctx, d_raw, bits, M, N, r, d_out, mean_std_out);
+}
+
+extern "C" int aldic_images_create_raw(aldic_ctx* ctx, const void* ImgRef, const void* ImgDef,
+                                       int32_t bits, const double* mask, int64_t M, int64_t N,
+                                       const int64_t gridx[2], const int64_t gridy[2],
+                                       aldic_images** out, double* stats) {
+  if (!ctx || !ImgRef || !ImgDef || !out) return fail(ALDIC_EINVAL, "images_raw: NULL argument");
+  if (bits != 8 && bits != 16) return fail(ALDIC_EINVAL, "images_raw: bits must be 8 or 16");
+  if (M < 7 || N < 7) return fail(ALDIC_EINVAL, "images_raw: smaller than 7 x 7");
+  RC_TRY(set_device(ctx));
+  int64_t r[4];
+  RC_TRY(clamp_roi(gridx, gridy, M, N, r));
+  auto* im = new aldic_images();
+  im->ctx = ctx;
+  im->device = ctx->device;
+  im->M = M;
+  im->N = N;
+  int rc = alloc_planes(im);
+  if (rc != ALDIC_OK) {
+    aldic_images_destroy(im);
+    return rc;
+  }
+  const int64_t MN = M * N;
+  const size_t rb = (size_t)MN * (bits / 8);
+  hipStream_t s = ctx->stream;
+  auto body = [&]() -> int {
+    unsigned char *rr, *rd;
+    double *dr, *dd, *dm = nullptr;
+    RC_TRY(stage(ctx, 5, rb, &rr));
+    RC_TRY(stage(ctx, 6, rb, &rd));
+    RC_TRY(stage(ctx, 0, (size_t)MN, &dr));
+    RC_TRY(stage(ctx, 1, (size_t)MN, &dd));
+    if (mask) RC_TRY(stage(ctx, 2, (size_t)MN, &dm));
+    HIP_TRY(hipMemcpyAsync(rr, ImgRef, rb, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(rd, ImgDef, rb, hipMemcpyHostToDevice, s));
+    if (mask) HIP_TRY(hipMemcpyAsync(dm, mask, sizeof(double) * MN, hipMemcpyHostToDevice, s));
+    RC_TRY(normalize_raw(ctx, rr, bits, M, N, r, dr, stats));
+    RC_TRY(normalize_raw(ctx, rd, bits, M, N, r, dd, stats ? stats + 2 : nullptr));
+    HIP_TRY(aldic::launch_prep_planes(dr, dd, dm, M, N, im->fm, im->g, im->mask, s));
+    HIP_TRY(aldic::launch_gradient(dr, dm, M, N, im->fx, im->fy, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ALDIC_OK;
+  };
+  rc = body();
+  if (rc != ALDIC_OK) {
+    aldic_images_destroy(im);
+    return rc;
+  }
+  *out = im;
+  return ALDIC_OK;
+}
+
 namespace {
 __global__ void fill_ones(float* p, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;

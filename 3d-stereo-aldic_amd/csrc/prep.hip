@@ -80,6 +80,46 @@ __global__ __launch_bounds__(256) void place_df_kernel(const double* __restrict_
   }
 }
 
+// Sum and sum of squares of a raw 8- or 16-bit image over the ROI
+// [x0, x1] x [y0, y1] (0-based, inclusive), exact in 64-bit integers (the
+// order of the atomic additions does not matter).
+template <class T>
+__global__ __launch_bounds__(256) void raw_sums_kernel(const T* __restrict__ raw, int64_t M,
+                                                       int64_t x0, int64_t x1, int64_t y0,
+                                                       int64_t y1,
+                                                       unsigned long long* __restrict__ sums) {
+  const int64_t w = x1 - x0 + 1, cnt = w * (y1 - y0 + 1);
+  unsigned long long s = 0, q = 0;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < cnt;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long v = raw[(x0 + k % w) + (y0 + k / w) * M];
+    s += v;
+    q += v * v;
+  }
+  for (int o = ALDIC_WAVE / 2; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o);
+    q += __shfl_xor(q, o);
+  }
+  if (threadIdx.x % ALDIC_WAVE == 0) {
+    atomicAdd(&sums[0], s);
+    atomicAdd(&sums[1], q);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void raw_normalize_kernel(const T* __restrict__ raw, int64_t n,
+                                                            double mean, double sd,
+                                                            double* __restrict__ out) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    out[k] = ((double)raw[k] - mean) / sd;
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 16384) g = 16384;
@@ -90,6 +130,30 @@ int grid_for(int64_t n) {
 }  // namespace
 
 namespace aldic {
+
+hipError_t launch_raw_sums(const void* raw, int bits, int64_t M, int64_t x0, int64_t x1,
+                           int64_t y0, int64_t y1, unsigned long long* sums, hipStream_t s) {
+  zero_u64_kernel<<<1, 64, 0, s>>>(sums, 2);
+  const int g = grid_for((x1 - x0 + 1) * (y1 - y0 + 1));
+  if (bits == 8)
+    raw_sums_kernel<unsigned char><<<g, 256, 0, s>>>(static_cast<const unsigned char*>(raw), M,
+                                                     x0, x1, y0, y1, sums);
+  else
+    raw_sums_kernel<unsigned short><<<g, 256, 0, s>>>(static_cast<const unsigned short*>(raw), M,
+                                                      x0, x1, y0, y1, sums);
+  return hipGetLastError();
+}
+
+hipError_t launch_raw_normalize(const void* raw, int bits, int64_t n, double mean, double sd,
+                                double* out, hipStream_t s) {
+  if (bits == 8)
+    raw_normalize_kernel<unsigned char><<<grid_for(n), 256, 0, s>>>(
+        static_cast<const unsigned char*>(raw), n, mean, sd, out);
+  else
+    raw_normalize_kernel<unsigned short><<<grid_for(n), 256, 0, s>>>(
+        static_cast<const unsigned short*>(raw), n, mean, sd, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_prep_planes(const double* ref, const double* def, const double* mask, int64_t M,
                               int64_t N, float* fm, float* g, float* maskf, hipStream_t s) {

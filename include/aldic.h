@@ -135,6 +135,29 @@ int aldic_images_create(aldic_ctx* ctx, const double* ImgRef, const double* ImgD
 int aldic_images_create_dev(aldic_ctx* ctx, const float* fm, const float* g, const float* mask,
                             const float* fx, const float* fy, int64_t M, int64_t N,
                             aldic_images** out);
+/* Images from raw 8- or 16-bit pixels as loadStereoImages decodes them
+ * (gui/loadStereoImages.m:79, double(raw)': a row-major H x W pixel buffer
+ * is the reference's M = W by N = H column-major image): each is
+ * normalised on the device by funNormalizeImg (func/funNormalizeImg.m:
+ * 22-40: (Img - mean) / std over the ROI gridx x gridy, clamped to the
+ * image; mean and std from exact integer sums) in FP64 and then prepared as
+ * aldic_images_create with Df computed on the device.  Uploads 1 or 2 bytes
+ * per pixel instead of 8.  stats (may be NULL) receives mean and std of
+ * ImgRef then ImgDef. */
+int aldic_images_create_raw(aldic_ctx* ctx, const void* ImgRef, const void* ImgDef, int32_t bits,
+                            const double* mask, int64_t M, int64_t N, const int64_t gridx[2],
+                            const int64_t gridy[2], aldic_images** out, double* stats);
+/* funNormalizeImg of one raw device image into an FP64 device image (M x N,
+ * same memory order); mean_std (host, may be NULL) receives mean and std. */
+int aldic_normalize_raw_dev(aldic_ctx* ctx, const void* d_raw, int32_t bits, int64_t M, int64_t N,
+                            const int64_t gridx[2], const int64_t gridy[2], double* d_out,
+                            double* mean_std);
+/* Device pointers of the resident FP32 planes (M x N each): fm = ImgRef
+ * .* mask, g = ImgDef, mask, and Df.DfDx / Df.DfDy on the full grid (zero
+ * outside rows/columns 4..M-3, 4..N-3).  Any may be NULL.  The planes
+ * belong to the handle. */
+int aldic_images_planes(const aldic_images* im, const float** fm, const float** g,
+                        const float** mask, const float** fx, const float** fy);
 int aldic_images_destroy(aldic_images* im);
 
 /* Per-node part of LocalICGNQuadtree (func_quadtree/LocalICGNQuadtree.m:
