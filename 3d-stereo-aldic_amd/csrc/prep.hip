@@ -7,6 +7,8 @@
 //     reference's cropped (M-6) x (N-6) FP64 arrays.
 // Threads run along dim 1 (image x), the contiguous axis, so every load of
 // the stencil is coalesced.
+#include <algorithm>
+
 #include "aldic_common.h"
 #include "aldic_kernels.h"
 
@@ -100,7 +102,19 @@ __global__ __launch_bounds__(256) void raw_sums_kernel(const T* __restrict__ raw
     s += __shfl_xor(s, o);
     q += __shfl_xor(q, o);
   }
+  // one atomic pair per block
+  __shared__ unsigned long long red[2][256 / ALDIC_WAVE];
+  const int wv = threadIdx.x / ALDIC_WAVE;
   if (threadIdx.x % ALDIC_WAVE == 0) {
+    red[0][wv] = s;
+    red[1][wv] = q;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x / ALDIC_WAVE); k++) {
+      s += red[0][k];
+      q += red[1][k];
+    }
     atomicAdd(&sums[0], s);
     atomicAdd(&sums[1], q);
   }
@@ -134,7 +148,7 @@ namespace aldic {
 hipError_t launch_raw_sums(const void* raw, int bits, int64_t M, int64_t x0, int64_t x1,
                            int64_t y0, int64_t y1, unsigned long long* sums, hipStream_t s) {
   zero_u64_kernel<<<1, 64, 0, s>>>(sums, 2);
-  const int g = grid_for((x1 - x0 + 1) * (y1 - y0 + 1));
+  const int g = std::min(grid_for((x1 - x0 + 1) * (y1 - y0 + 1)), 1024);
   if (bits == 8)
     raw_sums_kernel<unsigned char><<<g, 256, 0, s>>>(static_cast<const unsigned char*>(raw), M,
                                                      x0, x1, y0, y1, sums);

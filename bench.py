@@ -202,6 +202,115 @@ def timed(ctxs, api, step, steps, warmup):
     return dt, stages, warmup + steps - 1
 
 
+def global_step_mode(pkg, args):
+    """The rows around the local step on the stereo workload's geometry:
+    the quadtree mesh of the 2048 x 2048 hole mask (host), Subpb2 on it
+    (device, FP64), the NaN fill (host) and the raw 16-bit decode (device)
+    against the FP64 upload.  Prints one JSON line of timings."""
+    api = pkg.api
+    from aldic_amd import mesh as mesh_mod, quadtree
+    synth = load("aldic_synth_b", "3d-stereo-aldic_amd/synth.py")
+    M = N = 2048
+    uf, vf = synth.field_stereo(M, N)
+    mask = mesh_mod.hole_mask(M, N, (1024.5, 1024.5), 300.0)
+    step, ws = 16, 32
+    coords0, nX, nY = mesh_mod.uniform_grid([1, M], [1, N], ws, step)
+    x = coords0[:, 0].reshape(nY, nX)
+    y = coords0[:, 1].reshape(nY, nX)
+    out = {"mode": "global-step", "image": [M, N], "winstepsize": step, "winsize": ws,
+           "elementMinSize": 4}
+    t0 = time.perf_counter()
+    DICmesh = quadtree.MeshSetUp(x, y, {"ImgSize": (M, N), "winstepsize": step})
+    DICmesh["elementMinSize"] = 4
+    U0 = np.empty(2 * len(coords0))
+    U0[0::2] = uf(coords0[:, 0], coords0[:, 1])
+    U0[1::2] = vf(coords0[:, 0], coords0[:, 1])
+    qm, _, U0q = quadtree.GenerateQuadtreeMesh(U0, None, mask, DICmesh,
+                                               {"winstepsize": step, "winsize": ws})
+    out["mesh_s"] = round(time.perf_counter() - t0, 3)
+    c, el = qm["coordinatesFEM"], qm["elementsFEM"]
+    n = len(c)
+    out.update(nodes=n, elements=len(el), hanging=int((el[:, 4:] > 0).sum()),
+               dirichlet=len(qm["dirichlet"]))
+    progress(f"quadtree mesh: {n} nodes, {len(el)} elements ({out['mesh_s']} s)")
+    ctx = pkg.default_context()
+    t0 = time.perf_counter()
+    fem = pkg.FEMesh(c, el, qm["dirichlet"], None, 2, ctx=ctx)
+    out["fem_setup_s"] = round(time.perf_counter() - t0, 3)
+    info = fem.info()
+    out["nnz"] = info["nnz"]
+    h = 1e-3
+    U = np.empty(2 * n)
+    U[0::2] = uf(c[:, 0], c[:, 1])
+    U[1::2] = vf(c[:, 0], c[:, 1])
+    F = np.zeros(4 * n)
+    F[0::4] = (uf(c[:, 0] + h, c[:, 1]) - uf(c[:, 0] - h, c[:, 1])) / (2 * h)
+    F[1::4] = (vf(c[:, 0] + h, c[:, 1]) - vf(c[:, 0] - h, c[:, 1])) / (2 * h)
+    F[2::4] = (uf(c[:, 0], c[:, 1] + h) - uf(c[:, 0], c[:, 1] - h)) / (2 * h)
+    F[3::4] = (vf(c[:, 0], c[:, 1] + h) - vf(c[:, 0], c[:, 1] - h)) / (2 * h)
+    rng = np.random.default_rng(1)
+    Un = U + rng.normal(scale=0.02, size=2 * n)  # a local step's noise
+    DA = api.DeviceArray
+    dU = DA.from_host(ctx, Un, np.float64)
+    dF = DA.from_host(ctx, F, np.float64)
+    dO = DA(ctx, 2 * n, np.float64)
+    mu = 1e-3
+    beta = 1e-3 * step * step * mu
+    iters = fem.solve_dev(ctx, beta, mu, dU.ptr, dF.ptr, dO.ptr, winstepsize=step)
+    ctx.synchronize()
+    reps = max(args.steps, 5)
+    api.set_timing(ctx, True)
+    api.stage_times(ctx, reset=True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fem.solve_dev(ctx, beta, mu, dU.ptr, dF.ptr, dO.ptr, winstepsize=step)
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    st = api.stage_times(ctx, reset=True)
+    api.set_timing(ctx, False)
+    Uh = dO.to_host()
+    # nodes in no element (inside the hole) get 0, as in the reference
+    used = np.zeros(n, bool)
+    used[el[el > 0] - 1] = True
+    uu = np.repeat(used, 2)
+    out["subpb2_ms"] = round(dt * 1e3, 4)
+    out["subpb2_event_ms"] = round(st["subpb2"][0] / st["subpb2"][1], 4)
+    out["cg_iterations"] = iters
+    # bytes the CG iterations must touch: CSR values + columns, five 2-vectors
+    per_it = 12 * info["nnz"] + n * (5 * 16 + 8 + 1)
+    out["cg_bytes_per_iteration"] = per_it
+    out["cg_gbs"] = round(per_it * iters / dt / 1e9, 2) if dt > 0 else None
+    out["solved_nodes"] = int(used.sum())
+    out["subpb2_rms_vs_field_px"] = round(float(np.sqrt(np.mean((Uh - U)[uu] ** 2))), 5)
+    out["subpb2_rms_input_px"] = round(float(np.sqrt(np.mean((Un - U)[uu] ** 2))), 5)
+    progress(f"subpb2 {out['subpb2_ms']} ms, {iters} CG iterations")
+    fem.close()
+    # NaN fill of 2% bad nodes (host)
+    Ub = Un.copy()
+    bad = rng.choice(n, n // 50, replace=False)
+    Ub[2 * bad] = np.nan
+    Ub[2 * bad + 1] = np.nan
+    t0 = time.perf_counter()
+    Uf = pkg.nan_fill(c, Ub, None, mask)
+    out["nan_fill_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    out["nan_fill_nodes"] = int(len(bad))
+    # raw 16-bit decode vs the FP64 upload
+    f12, g12, _ = synth.make_pair(M, N, uf, vf, density=0.015, seed=2, bits=12)
+    rawL = np.ascontiguousarray(f12.T).astype(np.uint16)
+    rawR = np.ascontiguousarray(g12.T).astype(np.uint16)
+    roi = {"gridx": [1, M], "gridy": [1, N]}
+    (fn, gn), _ = pkg.funNormalizeImg([f12, g12], roi)
+    for name, make in (("raw16", lambda: api.Images.from_raw(rawL, rawR, mask, roi, ctx=ctx)),
+                       ("fp64", lambda: api.Images(fn, gn, mask, ctx=ctx))):
+        make().close()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            make().close()
+        out[f"images_{name}_ms"] = round((time.perf_counter() - t0) / 3 * 1e3, 2)
+    progress(f"images: raw16 {out['images_raw16_ms']} ms, fp64 {out['images_fp64_ms']} ms")
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +322,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-samples", type=int, default=2_000_000)
     ap.add_argument("--init", choices=["both", "search", "learned"], default="both")
+    ap.add_argument("--global-step", action="store_true",
+                    help="diagnostic: time the rows around the local step on the stereo "
+                         "workload (quadtree mesh, Subpb2, NaN fill, raw decode) and exit "
+                         "(no metric line)")
     ap.add_argument("--cap-sweep", default="",
                     help="diagnostic: comma-separated IC-GN evaluation caps; times the "
                          "search-start IC-GN at each cap and exits (no metric line)")
@@ -225,6 +338,9 @@ def main():
 
     pkg = ge.load_package()
     api = pkg.api
+    if args.global_step:
+        global_step_mode(pkg, args)
+        return
     from aldic_amd import learned
     t0 = time.perf_counter()
     W = make_workload()
