@@ -272,7 +272,9 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
   constexpr int NT = ALDIC_WAVE * NW;
   constexpr int SUBS = fast_block<NW>() / NT;
   static_assert(NW == 1 || SUBS == 1, "multi-wave subsets own their block");
-  __shared__ float tiles[SUBS][TS * TS];
+  // the tile starts TS + 1 floats into its slot, so that the gather's
+  // (-1, -1) tap offset is a non-negative base address
+  __shared__ float tiles[SUBS][TS * TS + TS + 1];
   __shared__ float red[NW * 32];
   const int sub = threadIdx.x / NT;
   const int tid = threadIdx.x % NT;
@@ -281,7 +283,8 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
   if (slot >= cnt) return;  // uniform per subset group (SUBS == 1 when NW > 1)
   // every value of one subset is uniform across its waves: keep it scalar
   const int node = __builtin_amdgcn_readfirstlane(a.list[slot]);
-  float* tile = tiles[sub];
+  float* tile = tiles[sub] + TS + 1;
+  const float* tileb = tiles[sub];  // tile - (TS + 1)
   const int M = a.M, N = a.N;
   if (M < TS || N < TS) {  // the tile would not fit in the image
     if (tid == 0) {
@@ -574,14 +577,16 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
     for (int k = 0; k < PPL; k++) {
       const float tx = fmaf(A11, pdx[k], fmaf(A12, pdy[k], cx));
       const float ty = fmaf(A21, pdx[k], fmaf(A22, pdy[k], cy));
-      const float fxl = floorf(tx), fyl = floorf(ty);
-      const float dxf = tx - fxl, dyf = ty - fyl;
-      // inside the tile by the bounding-box test above (one pixel to spare)
-      const int ix = (int)fxl - 1, iy = (int)fyl - 1;
+      // tx, ty >= 2 inside the tile (the bounding-box test above leaves one
+      // pixel to spare), where fract(t) = t - floor(t) exactly and the
+      // truncating conversion is the floor; the 4 x 4 cell starts one pixel
+      // up and left of it
+      const float dxf = __builtin_amdgcn_fractf(tx), dyf = __builtin_amdgcn_fractf(ty);
+      const int ix = (int)tx, iy = (int)ty;
       float wr[4], wc[4];
       spline_weights56(dxf, wr);
       spline_weights56(dyf, wc);
-      const float* base = tile + __umul24(iy, TS) + ix;
+      const float* base = tileb + __umul24(iy, TS) + ix;
       float acc = 0.0f;
 #pragma unroll
       for (int c = 0; c < 4; c++) {
@@ -620,10 +625,12 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
       }
       return;
     }
-    const float meang = s1[0] / (float)NS;
+    // compile-time size: the mean by a constant reciprocal; the norm and
+    // its reciprocal by the hardware instructions (1 ulp)
+    const float meang = FS > 0 ? s1[0] * (1.0f / (float)(FS * FS)) : s1[0] / (float)NS;
     const float varsum = fmaxf(fmaf(-(float)NS * meang, meang, s1[1]), 0.0f);
-    const float bottomg = (NS > 1) ? sqrtf(varsum) : 0.0f;
-    const float ig = 1.0f / bottomg;
+    const float bottomg = (NS > 1) ? __builtin_amdgcn_sqrtf(varsum) : 0.0f;
+    const float ig = __builtin_amdgcn_rcpf(bottomg);
     const float c2 = meang * ig;
     // residual r = (f - mean f)/bottom f - (g - mean g)/bottom g and b
     // (:365-374); empty slots carry zero gradients and contribute to b
@@ -654,7 +661,7 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
         bvec[k] = bottomf * v[k];
         nb += bvec[k] * bvec[k];
       }
-      bnorm = sqrtf(nb);
+      bnorm = __builtin_amdgcn_sqrtf(nb);
       last_zncc = 1.0f - 0.5f * v[6];
     } else {
       float v[3] = {0, 0, 0};
