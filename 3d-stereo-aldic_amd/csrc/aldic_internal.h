@@ -44,6 +44,10 @@ struct aldic_ctx {
   int stage_cnt[ALDIC_NSTAGES] = {0};
   // cross-context ordering points (aldic_ctx_record / aldic_ctx_wait)
   hipEvent_t marks[ALDIC_NMARKS] = {nullptr};
+  // a second stream (default priority) for the small size classes of a
+  // solve, forked from and joined back to `stream` with two events
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   // graphs captured on this context that are still alive, plus one while a
   // capture is open: staging buffers must not move while it is non-zero
   // (shared with the graphs so either may be destroyed first)
@@ -104,15 +108,17 @@ struct StageTimer {
   aldic_ctx* ctx;
   int id;
   hipEvent_t a = nullptr, b = nullptr;
-  StageTimer(aldic_ctx* c, int stage) : ctx(c), id(stage) {
+  hipStream_t st;
+  StageTimer(aldic_ctx* c, int stage, hipStream_t on = nullptr)
+      : ctx(c), id(stage), st(on ? on : c->stream) {
     if (!ctx->timing) return;
     a = take_event(ctx);
     b = take_event(ctx);
-    if (a) (void)hipEventRecord(a, ctx->stream);
+    if (a) (void)hipEventRecord(a, st);
   }
   ~StageTimer() {
     if (!ctx->timing || !a || !b) return;
-    (void)hipEventRecord(b, ctx->stream);
+    (void)hipEventRecord(b, st);
     ctx->pending.push_back({id, {a, b}});
   }
 };

@@ -125,27 +125,52 @@ void free_plan(aldic_plan* p) {
 }
 
 // Launch sequence shared by IC-GN (mode 0) and Subpb1 (mode 1).
+// The size classes of one solve are independent until the hand-on pass:
+// the largest class runs on the context stream, the others on the side
+// stream at the same time, joined before the handed-on subsets.
+int side_stream(aldic_ctx* ctx) {
+  if (ctx->side) return ALDIC_OK;
+  HIP_TRY(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&ctx->fork_ev, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&ctx->join_ev, hipEventDisableTiming));
+  return ALDIC_OK;
+}
+
+int largest_class(const int cnt[5]) {
+  int big = 0;
+  for (int c = 1; c < 5; c++)
+    if (cnt[c] > cnt[big]) big = c;
+  return big;
+}
+
 int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
   hipStream_t s = ctx->stream;
+  RC_TRY(side_stream(ctx));
   a.n = (int)p->n;
   a.defer_cnt = p->defer_cnt;
   a.defer_list = p->defer_list;
   const int base = mode == 0 ? ALDIC_STAGE_ICGN_C0 : ALDIC_STAGE_SUBPB1_C0;
   zero_ints<<<1, 64, 0, s>>>(p->defer_cnt, 1);
   HIP_TRY(hipGetLastError());
-  for (int c = 0; c < 4; c++) {
+  HIP_TRY(hipEventRecord(ctx->fork_ev, s));
+  HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
+  const int big = largest_class(p->cnt);
+  for (int c = 0; c < 5; c++) {
     if (p->cnt[c] == 0) continue;
-    a.list = p->lists[c];
-    a.cnt = p->cnt[c];
-    a.fixed_side = p->full[c];
-    StageTimer t(ctx, base + c);
-    HIP_TRY(aldic::launch_fast(mode, c, p->var[mode][c], a, s));
+    hipStream_t on = c == big ? s : ctx->side;
+    StageTimer t(ctx, base + c, on);
+    if (c < 4) {
+      a.list = p->lists[c];
+      a.cnt = p->cnt[c];
+      a.fixed_side = p->full[c];
+      HIP_TRY(aldic::launch_fast(mode, c, p->var[mode][c], a, on));
+    } else {
+      const int grid = std::min(p->cnt[4], 2048);
+      HIP_TRY(aldic::launch_generic(mode, a, p->lists[4], nullptr, p->cnt[4], grid, on));
+    }
   }
-  if (p->cnt[4] > 0) {
-    const int grid = std::min(p->cnt[4], 2048);
-    StageTimer t(ctx, base + 4);
-    HIP_TRY(aldic::launch_generic(mode, a, p->lists[4], nullptr, p->cnt[4], grid, s));
-  }
+  HIP_TRY(hipEventRecord(ctx->join_ev, ctx->side));
+  HIP_TRY(hipStreamWaitEvent(s, ctx->join_ev, 0));
   // subsets handed on by the fast kernels; the grid-stride loop reads the
   // count on the device, so no host round trip is needed
   const int grid = (int)std::min<int64_t>(std::max<int64_t>(p->n, 1), 1024);
@@ -224,6 +249,12 @@ extern "C" int aldic_ctx_destroy(aldic_ctx* ctx) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : ctx->buf)
     if (p) (void)hipFree(p);
+  if (ctx->side) {
+    (void)hipStreamSynchronize(ctx->side);
+    (void)hipStreamDestroy(ctx->side);
+  }
+  if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
+  if (ctx->join_ev) (void)hipEventDestroy(ctx->join_ev);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return ALDIC_OK;
@@ -840,6 +871,8 @@ extern "C" int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im,
   HIP_TRY(aldic::launch_bucket(a, d_list, d_count, (int)plan->n, cl, s));
   zero_ints<<<1, 64, 0, s>>>(plan->defer_cnt, 1);
   HIP_TRY(hipGetLastError());
+  // one stream: the lists are short, and a caller pipelining frames over
+  // several contexts already fills the hardware queues
   for (int c = 0; c < 4; c++) {
     if (plan->cnt[c] == 0) continue;
     a.list = plan->sub_lists[c];
