@@ -566,6 +566,35 @@ __device__ __forceinline__ void patch_raw(const float* __restrict__ f, const flo
   }
 }
 
+// The same pixels when the whole 48 x 48 window lies inside the image (no
+// clamp): one load per pixel from a per-node base and a per-thread byte
+// offset fixed for the launch.
+__device__ __forceinline__ bool patch_inside(int M, int N, int cx, int cy) {
+  return cx - PATCH / 2 >= 1 && cx + PATCH / 2 - 1 <= M && cy - PATCH / 2 >= 1 &&
+         cy + PATCH / 2 - 1 <= N;
+}
+
+__device__ __forceinline__ void patch_load(const float* __restrict__ f, const float* __restrict__ g,
+                                           int M, int N, int cx, int cy, int sxs, int sys,
+                                           float* vf, float* vg) {
+  constexpr int NP = PATCH * PATCH;
+  if (patch_inside(M, N, cx, cy) && patch_inside(M, N, cx + sxs, cy + sys)) {
+    const char* bf = reinterpret_cast<const char*>(
+        f + ((int64_t)(cy - PATCH / 2 - 1) * M + (cx - PATCH / 2 - 1)));
+    const char* bg = reinterpret_cast<const char*>(
+        g + ((int64_t)(cy + sys - PATCH / 2 - 1) * M + (cx + sxs - PATCH / 2 - 1)));
+#pragma unroll
+    for (int k = 0; k < NP / 256; k++) {
+      const uint32_t p = threadIdx.x + 256 * k, py = p / PATCH, px = p - PATCH * py;
+      const uint32_t off = (py * (uint32_t)M + px) * 4u;
+      vf[k] = *reinterpret_cast<const float*>(bf + off);
+      vg[k] = *reinterpret_cast<const float*>(bg + off);
+    }
+  } else {
+    patch_raw(f, g, M, N, cx, cy, sxs, sys, vf, vg);
+  }
+}
+
 __device__ __forceinline__ PatchMoments patch_moments(const float* vf, const float* vg,
                                                       float* red) {
   constexpr int NP = PATCH * PATCH;
@@ -748,7 +777,7 @@ __global__ void mosaic_kernel(int M, int N, int tile, const float* __restrict__ 
 // ---------------------------------------------------------------------------
 constexpr int FR_L1C = 40;  // padded channel stride of the L1 activation in LDS
 
-__global__ __launch_bounds__(256) void front_kernel(const float* __restrict__ f,
+__global__ __launch_bounds__(256, 2) void front_kernel(const float* __restrict__ f,
                                                     const float* __restrict__ g, int M, int N,
                                                     const double* __restrict__ coords, int n,
                                                     int first, int count,
@@ -759,9 +788,10 @@ __global__ __launch_bounds__(256) void front_kernel(const float* __restrict__ f,
                                                     const float* __restrict__ b2,
                                                     bf16* __restrict__ out) {
   constexpr int NP = PATCH * PATCH;
+  constexpr int PP = PATCH + 2;  // the patch with a zero border, so taps need no bounds test
   constexpr int K1 = 32, K2 = 288;
   constexpr int RS1 = K1 / 8 + 1, RS2 = K2 / 8 + 1;
-  __shared__ uint32_t sx[NP];                  // patch: (ref, def) bf16 pairs
+  __shared__ uint32_t sx[PP * PP];             // patch: (ref, def) bf16 pairs
   __shared__ bf16 sa1[24 * 24 * FR_L1C];       // L1 output, NHWC, padded channels
   __shared__ bf16x8 sW1[32 * RS1];
   __shared__ bf16x8 sW2[32 * RS2];
@@ -777,47 +807,64 @@ __global__ __launch_bounds__(256) void front_kernel(const float* __restrict__ f,
     sW2[row * RS2 + q] = *(const bf16x8*)(W2 + row * K2 + q * 8);
   }
   const float bias1 = b1[r], bias2 = b2[r];
+  for (int e = tid; e < PP * PP; e += 256) sx[e] = 0u;  // the border stays zero
+  // L1 taps of this lane: t = 8 st2 + 4 hq + q, (ky, kx) = (t / 3, t % 3), as
+  // an offset into the padded patch; t >= 9 is the zero padding of K
+  int toff0[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int t = 4 * hq + q;
+    toff0[q] = (t / 3) * PP + t % 3;
+  }
+  const int toff8 = 2 * PP + 2;  // t = 8, used by the hq = 0 half only
   __syncthreads();
   // the pixels of the next node are loaded while this node's convolutions run
   float vf[NP / 256], vg[NP / 256];
   if (blockIdx.x < count) {
     int cx, cy, sxs, sys;
     node_shift(coords, n, prior, first + blockIdx.x, &cx, &cy, &sxs, &sys);
-    patch_raw(f, g, M, N, cx, cy, sxs, sys, vf, vg);
+    patch_load(f, g, M, N, cx, cy, sxs, sys, vf, vg);
   }
   for (int local = blockIdx.x; local < count; local += gridDim.x) {
     // ---- patch pair, normalised (patch_kernel's arithmetic) ----
     const PatchMoments m = patch_moments(vf, vg, red);
 #pragma unroll
-    for (int k = 0; k < NP / 256; k++) sx[tid + 256 * k] = patch_pair(vf[k], vg[k], m);
+    for (int k = 0; k < NP / 256; k++) {
+      const uint32_t p = tid + 256 * k, py = p / PATCH, px = p - PATCH * py;
+      sx[(py + 1) * PP + px + 1] = patch_pair(vf[k], vg[k], m);
+    }
     __syncthreads();
     if (local + (int)gridDim.x < count) {
       int cx, cy, sxs, sys;
       node_shift(coords, n, prior, first + local + gridDim.x, &cx, &cy, &sxs, &sys);
-      patch_raw(f, g, M, N, cx, cy, sxs, sys, vf, vg);
+      patch_load(f, g, M, N, cx, cy, sxs, sys, vf, vg);
     }
     // ---- L1: 576 rows (24 x 24), K 32 (taps 0..8 x 2 channels, padded), N 32 ----
     for (int tile = wave; tile < 18; tile += 4) {
       const int row = tile * 32 + r;
       const int oy = row / 24, ox = row % 24;
+      // tap (ky, kx) of output (oy, ox) reads patch pixel (2 oy - 1 + ky, 2 ox - 1 + kx),
+      // padded index (2 oy + ky) * PP + 2 ox + kx
+      const int rb = 2 * oy * PP + 2 * ox;
       floatx16 acc;
 #pragma unroll
       for (int i = 0; i < 16; i++) acc[i] = 0.0f;
+      {
+        // taps 0..7: every element is a (ref, def) pair already in bf16
+        uint32_t w[4];
 #pragma unroll
-      for (int st2 = 0; st2 < 2; st2++) {
-        bf16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int t = 8 * st2 + 4 * hq + q;
-          const int ky = t / 3, kx = t % 3;
-          const int iy = oy * 2 - 1 + ky, ix = ox * 2 - 1 + kx;
-          if (t < 9 && iy >= 0 && iy < PATCH && ix >= 0 && ix < PATCH) {
-            const uint32_t w2 = sx[iy * PATCH + ix];
-            v[2 * q] = __builtin_bit_cast(bf16, (uint16_t)(w2 & 0xffffu));
-            v[2 * q + 1] = __builtin_bit_cast(bf16, (uint16_t)(w2 >> 16));
-          }
-        }
-        const bf16x8 b = sW1[r * RS1 + 2 * st2 + hq];
+        for (int q = 0; q < 4; q++) w[q] = sx[rb + toff0[q]];
+        const bf16x8 v = __builtin_bit_cast(bf16x8, w);
+        const bf16x8 b = sW1[r * RS1 + hq];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v, b, acc, 0, 0, 0);
+      }
+      {
+        // tap 8 (first pair of the hq = 0 half), zero padding elsewhere
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        const uint32_t t8 = sx[rb + toff8];
+        w[0] = hq == 0 ? t8 : 0u;
+        const bf16x8 v = __builtin_bit_cast(bf16x8, w);
+        const bf16x8 b = sW1[r * RS1 + 2 + hq];
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v, b, acc, 0, 0, 0);
       }
 #pragma unroll
