@@ -169,8 +169,8 @@ template <int NB, int AMODE>
 __global__ __launch_bounds__(256) void gemm_lds_kernel(const bf16* __restrict__ A, int lda,
                                                        const bf16* __restrict__ Bt, int M, int K,
                                                        const float* __restrict__ bias, int relu,
-                                                       bf16* __restrict__ Cb, ConvGeom geom,
-                                                       int ntiles) {
+                                                       bf16* __restrict__ Cb, int ldc,
+                                                       ConvGeom geom, int ntiles) {
   extern __shared__ bf16x8 sB[];
   constexpr int N = 32 * NB;
   const int kq = K / 8, rs = kq + 1;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256) void gemm_lds_kernel(const bf16* __restrict__ 
           if (row >= M) continue;
           float v = (h ? acc1[j][i] : acc0[j][i]) + bv[j];
           if (relu) v = fmaxf(v, 0.0f);
-          Cb[(int64_t)row * N + col] = to_bf16(v);
+          Cb[(int64_t)row * ldc + col] = to_bf16(v);
         }
       }
     }
@@ -975,7 +975,8 @@ hipError_t cnn_gemm(const void* A, int lda, const void* Bt, int ldb, int M, int 
 
 template <int NB, int AMODE>
 static hipError_t launch_lds(const bf16* A, int lda, const bf16* Bt, int M, int K,
-                             const float* bias, int relu, bf16* Cb, ConvGeom g, hipStream_t s) {
+                             const float* bias, int relu, bf16* Cb, int ldc, ConvGeom g,
+                             hipStream_t s) {
   const size_t smem = (size_t)32 * NB * (K / 8 + 1) * 16;
   if (smem > 160 * 1024) return hipErrorInvalidValue;
   static bool attr = false;  // one attribute call per instantiation
@@ -988,7 +989,8 @@ static hipError_t launch_lds(const bf16* A, int lda, const bf16* Bt, int M, int 
   const int ntiles = (M + 255) / 256;
   const int per_cu = smem > 80 * 1024 ? 1 : (smem > 40 * 1024 ? 2 : 4);
   const int grid = std::max(1, std::min(ntiles, 256 * per_cu * 2));
-  gemm_lds_kernel<NB, AMODE><<<grid, 256, smem, s>>>(A, lda, Bt, M, K, bias, relu, Cb, g, ntiles);
+  gemm_lds_kernel<NB, AMODE><<<grid, 256, smem, s>>>(A, lda, Bt, M, K, bias, relu, Cb, ldc, g,
+                                                      ntiles);
   return hipGetLastError();
 }
 
@@ -1029,9 +1031,18 @@ hipError_t cnn_conv(const void* x, int B, int H, int W, int Cin, int stride, int
 
 hipError_t cnn_dense(const void* x, int B, int K, const void* Wt, int Cout, const float* bias,
                      int relu, void* y, hipStream_t s) {
-  if (Cout == 128 && K % 16 == 0 && (size_t)128 * (K / 8 + 1) * 16 <= 160 * 1024)
-    return launch_lds<4, 0>((const bf16*)x, K, (const bf16*)Wt, B, K, bias, relu, (bf16*)y,
-                            ConvGeom{0, 0, 0, 0, 0, 0}, s);
+  // two 64-column halves: a workgroup then stages half the weights (75 KB
+  // at K = 576), so it can share a CU with the IC-GN kernels of another
+  // stream instead of waiting for one whose LDS is empty
+  if (Cout == 128 && K % 16 == 0 && (size_t)64 * (K / 8 + 1) * 16 <= 80 * 1024) {
+    for (int h = 0; h < 2; h++) {
+      const hipError_t e = launch_lds<2, 0>((const bf16*)x, K, (const bf16*)Wt + (int64_t)64 * h * K,
+                                            B, K, bias ? bias + 64 * h : nullptr, relu,
+                                            (bf16*)y + 64 * h, 128, ConvGeom{0, 0, 0, 0, 0, 0}, s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
   return gemm_any(0, x, K, Wt, K, B, Cout, K, 1, nullptr, Cout, bias, relu, y, Cout,
                   ConvGeom{0, 0, 0, 0, 0, 0}, s);
 }

@@ -478,10 +478,24 @@ def main():
                  if hist["loss"] else None,
                  "heldout_rms_u_px": round(ev["rms_u_px"], 4),
                  "heldout_rms_F": round(ev["rms_F"], 5)}
+        # the first IC-GN pass alone (network start, before the gate and the
+        # fallback) on the 65 px class: evaluations per subset
+        for c in (ctx, ctx2, ctx3):
+            c.synchronize()
+        li._stage_net(0)
+        ctx.synchronize()
+        li._stage_icgn(0)
+        ctx3.synchronize()
+        conv1 = li.bufs[0]["conv"].to_host()
+        big = W["ws"] == 64
+        c1 = conv1[big]
+        first = {"mean_evals_converged": round(float(c1[(c1 >= 1) & (c1 <= 30)].mean()), 3),
+                 "over_30": int((c1 > 30).sum()), "status_101": int((c1 == 101).sum()),
+                 "subsets": int(big.sum())}
         record("learned", dt, stages, lU.to_host().astype(np.float64), lconv.to_host(),
                lz.to_host(), dict(fallback=li.fallback_count(last), train=train,
                                   prior_points=li.nX * li.nY,
-                                  graph_ms=round(dtg * 1e3, 4)))
+                                  graph_ms=round(dtg * 1e3, 4), first_pass=first))
 
     head = res.get("learned") or res["search"]
     n_conv = int(head["ok"].sum())
@@ -530,6 +544,7 @@ def main():
             out["fallback_rate"] = round(d["fallback"] / n, 5)
             out["prior_search_points"] = d["prior_points"]
             out["graph_ms_per_step"] = d["graph_ms"]
+            out["first_pass_65px"] = d["first_pass"]
             out["train"] = d["train"]
         return out
 
