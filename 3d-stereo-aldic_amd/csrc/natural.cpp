@@ -9,10 +9,11 @@
 // the value is sum_i w_i v_i with Sibson's weights: w_i is the area that
 // q's Voronoi cell (in the diagram of the sites plus q) takes from the cell
 // of site i, over the area of q's cell.  q's cell is built by clipping a box
-// with the bisector half-planes of the sites around q; the search radius
-// doubles until it is at least twice the cell's circumradius, beyond which
-// no site can cut the cell, or until it holds every site (near the hull the
-// cell reaches far out; then only the box grows).  The area taken from site i is q's cell clipped
+// with the bisector half-planes of the sites around q, visited bucket ring
+// by bucket ring outwards until the sites left are at least twice the
+// cell's circumradius away, beyond which no site can cut the cell (near the
+// hull the cell reaches far out; if the box still bounds it after every
+// site, the box grows).  The area taken from site i is q's cell clipped
 // to the points nearer to i than to every other natural neighbour (a point
 // of q's cell lies in the old cell of one of them).  On the hull boundary
 // the weights reduce to linear interpolation between the two ends of the
@@ -26,7 +27,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstring>
+#include <functional>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/aldic.h"
@@ -75,6 +79,12 @@ double area(const std::vector<Vtx>& P) {
   }
   return 0.5 * s;
 }
+
+// Working storage of one query's Sibson evaluation; one per thread.
+struct Scratch {
+  std::vector<int> nn;
+  std::vector<Vtx> poly, tmp, tmp2;
+};
 
 class Interp {
  public:
@@ -156,7 +166,38 @@ class Interp {
   }
 
   // out: nv values
-  void eval(double qx, double qy, double* out) {
+  void eval(double qx, double qy, double* out) { eval(qx, qy, out, own_); }
+
+  // Queries k = 0..nq-1 at (qx[k], qy[k]); value c of query k goes to
+  // out[k * os + c * cs].  Queries are independent, so they are split over
+  // threads when there are enough of them; the values do not depend on the
+  // split.
+  void eval_many(const double* qx, const double* qy, int64_t nq, double* out, int64_t os,
+                 int64_t cs) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int64_t nt = std::min<int64_t>(std::min<int64_t>(hw, 16), nq / 128);
+    auto work = [&](int64_t k0, int64_t k1, Scratch& S) {
+      double o[16];
+      std::vector<double> ob(nv_ > 16 ? nv_ : 0);
+      double* op = nv_ > 16 ? ob.data() : o;
+      for (int64_t k = k0; k < k1; k++) {
+        eval(qx[k], qy[k], op, S);
+        for (int c = 0; c < nv_; c++) out[k * os + c * cs] = op[c];
+      }
+    };
+    if (nt <= 1) {
+      work(0, nq, own_);
+      return;
+    }
+    std::vector<Scratch> S(nt);
+    std::vector<std::thread> th;
+    th.reserve(nt);
+    for (int64_t t = 0; t < nt; t++)
+      th.emplace_back(work, nq * t / nt, nq * (t + 1) / nt, std::ref(S[t]));
+    for (auto& x : th) x.join();
+  }
+
+  void eval(double qx, double qy, double* out, Scratch& S) const {
     if (!std::isfinite(qx) || !std::isfinite(qy)) {
       for (int c = 0; c < nv_; c++) out[c] = std::nan("");
       return;
@@ -174,7 +215,7 @@ class Interp {
       for (int c = 0; c < nv_; c++) out[c] = v_[(size_t)s * nv_ + c];
       return;
     }
-    if (where == 1 && sibson(qx, qy, out)) return;
+    if (where == 1 && sibson(qx, qy, out, S)) return;
     if (where == 0) {  // on the hull boundary
       for (int c = 0; c < nv_; c++)
         out[c] = (1.0 - et) * v_[(size_t)ea * nv_ + c] + et * v_[(size_t)eb * nv_ + c];
@@ -261,51 +302,72 @@ class Interp {
     *et = t;
     return 0;
   }
-  bool sibson(double qx, double qy, double* out) {
-    double r = 2.5 * std::max(cellx_, celly_);
-    // beyond rall every site is a candidate and q's cell is exact whatever
-    // its size; only the box it is clipped from may still have to grow
+  bool sibson(double qx, double qy, double* out, Scratch& S) const {
+    // q's cell, clipped from a box around q: the buckets are visited in
+    // square rings around q's bucket, and the walk stops once every site
+    // left is at least twice the cell's circumradius from q (such a site's
+    // bisector cannot reach the cell).  A site whose bisector misses the
+    // current cell is skipped without touching the polygon.  If the box
+    // still bounds the cell when every site has been seen, q is on the hull
+    // in effect and the box grows.
     const double rall = std::hypot(std::max(qx - xmin_, xmax_ - qx), std::max(qy - ymin_, ymax_ - qy));
-    bool all_in = r >= rall;
-    double B = 2.0 * r;
-    std::vector<Vtx>& V = poly_;
-    std::vector<Vtx>& T = tmp_;
+    double B = 2.0 * (scale_ + rall);
+    std::vector<Vtx>& V = S.poly;
+    std::vector<Vtx>& T = S.tmp;
+    std::vector<int>& nn_ = S.nn;
+    std::vector<Vtx>& tmp2_ = S.tmp2;
+    const int cx = std::min(std::max((int)std::floor((qx - xmin_) / cellx_), 0), gnx_ - 1);
+    const int cy = std::min(std::max((int)std::floor((qy - ymin_) / celly_), 0), gny_ - 1);
+    const int tmax = std::max(std::max(cx, gnx_ - 1 - cx), std::max(cy, gny_ - 1 - cy));
     for (;;) {
-      cand_.clear();
-      for_cells(qx, qy, all_in ? rall + r : r, [&](int i) {
-        const double dx = x_[i] - qx, dy = y_[i] - qy;
-        if (all_in || dx * dx + dy * dy <= r * r) cand_.push_back(i);
-      });
-      std::sort(cand_.begin(), cand_.end(), [&](int a, int b) {
-        const double da = (x_[a] - qx) * (x_[a] - qx) + (y_[a] - qy) * (y_[a] - qy);
-        const double db = (x_[b] - qx) * (x_[b] - qx) + (y_[b] - qy) * (y_[b] - qy);
-        return da < db;
-      });
       V = {{-B, -B, -1}, {B, -B, -1}, {B, B, -1}, {-B, B, -1}};
-      for (int s : cand_) {
-        // nearer to q (the origin) than to s: s . p <= |s|^2 / 2
-        const double sx = x_[s] - qx, sy = y_[s] - qy;
-        clip(V, sx, sy, 0.5 * (sx * sx + sy * sy), s, T);
-        V.swap(T);
+      double R2 = 2.0 * B * B;
+      auto visit = [&](int c) {
+        for (int k = cstart_[c]; k < cstart_[c + 1]; k++) {
+          const int s = cidx_[k];
+          // nearer to q (the origin) than to s: s . p <= |s|^2 / 2
+          const double sx = x_[s] - qx, sy = y_[s] - qy;
+          const double h = 0.5 * (sx * sx + sy * sy);
+          if (h * h > R2 * (sx * sx + sy * sy)) continue;  // |s| > 2R
+          bool cuts = false;
+          for (const Vtx& v : V) cuts |= sx * v.x + sy * v.y - h > 0.0;
+          if (!cuts) continue;
+          clip(V, sx, sy, h, s, T);
+          V.swap(T);
+          R2 = 0.0;
+          for (const Vtx& v : V) R2 = std::max(R2, v.x * v.x + v.y * v.y);
+        }
+      };
+      for (int t = 0; t <= tmax; t++) {
+        if (t > 0) {
+          // sites outside rings 0..t-1 are at least this far from q
+          const double gx = std::min(qx - (xmin_ + (cx - t + 1) * cellx_),
+                                     (xmin_ + (cx + t) * cellx_) - qx);
+          const double gy = std::min(qy - (ymin_ + (cy - t + 1) * celly_),
+                                     (ymin_ + (cy + t) * celly_) - qy);
+          const double g = std::min(gx, gy) - 1e-9 * (cellx_ + celly_);
+          if (g > 0.0 && g * g > 4.0 * R2) break;
+        }
+        const int x0 = cx - t, x1 = cx + t, y0 = cy - t, y1 = cy + t;
+        for (int yy = std::max(y0, 0); yy <= std::min(y1, gny_ - 1); yy++) {
+          if (yy == y0 || yy == y1) {
+            for (int xx = std::max(x0, 0); xx <= std::min(x1, gnx_ - 1); xx++) visit(xx + yy * gnx_);
+          } else {
+            if (x0 >= 0) visit(x0 + yy * gnx_);
+            if (x1 < gnx_ && x1 != x0) visit(x1 + yy * gnx_);
+          }
+        }
       }
-      double R2 = 0.0;
       bool boxed = false;
       const double etol = 1e-12 * B;
       for (size_t k = 0; k < V.size(); k++) {
-        R2 = std::max(R2, V[k].x * V[k].x + V[k].y * V[k].y);
         const Vtx& p = V[k];
         const Vtx& q = V[(k + 1) % V.size()];
         if (p.lab < 0 && std::hypot(q.x - p.x, q.y - p.y) > etol) boxed = true;
       }
-      if (!boxed && (all_in || 4.0 * R2 <= r * r)) break;
-      if (all_in) {
-        if (B > 1e8 * (scale_ + rall)) return false;  // unbounded: q is on the hull in effect
-        B *= 16.0;
-        continue;
-      }
-      r *= 2.0;
-      B = 2.0 * r;
-      all_in = r >= rall;
+      if (!boxed) break;
+      if (B > 1e8 * (scale_ + rall)) return false;  // unbounded: q is on the hull in effect
+      B *= 16.0;
     }
     // natural neighbours: sites carrying an edge of positive length
     nn_.clear();
@@ -346,37 +408,84 @@ class Interp {
   double xmin_ = 0, xmax_ = 0, ymin_ = 0, ymax_ = 0, cell_ = 1, cellx_ = 1, celly_ = 1, scale_ = 1;
   int gnx_ = 1, gny_ = 1;
   std::vector<int> cstart_, cidx_;
-  std::vector<int> cand_, nn_;
-  std::vector<Vtx> poly_, tmp_, tmp2_;
+  Scratch own_;
 };
 
 // MATLAB round(): half away from zero
 inline int64_t mround(double v) { return (int64_t)(v < 0 ? std::ceil(v - 0.5) : std::floor(v + 0.5)); }
 
-// 8-connected components of mask != 0 (M x N column-major); labels 1..K, 0 off-mask
-int64_t label8(const uint8_t* mask, int64_t M, int64_t N, std::vector<int32_t>& lab) {
-  lab.assign((size_t)(M * N), 0);
-  std::vector<int64_t> stack;
-  int32_t K = 0;
-  for (int64_t s = 0; s < M * N; s++) {
-    if (!mask[s] || lab[s]) continue;
-    lab[s] = ++K;
-    stack.push_back(s);
-    while (!stack.empty()) {
-      const int64_t p = stack.back();
-      stack.pop_back();
-      const int64_t i = p % M, j = p / M;
-      for (int dj = -1; dj <= 1; dj++)
-        for (int di = -1; di <= 1; di++) {
-          const int64_t ii = i + di, jj = j + dj;
-          if (ii < 0 || jj < 0 || ii >= M || jj >= N) continue;
-          const int64_t q = ii + jj * M;
-          if (mask[q] && !lab[q]) {
-            lab[q] = K;
-            stack.push_back(q);
-          }
-        }
+// True if some byte of w is zero.
+inline bool has_zero_byte(uint64_t w) {
+  return ((w - 0x0101010101010101ULL) & ~w & 0x8080808080808080ULL) != 0;
+}
+
+// 8-connected components of mask != 0 (M x N, column-major), reported as a
+// component id per node pixel: lab[k] for pixel (pi[k], pj[k]) (0-based),
+// 0 where the pixel is off the mask, ids 1..K in order of first appearance
+// among the nodes.  Returns K.  Only the runs of each column are stored: a
+// run is joined (union-find) to every run of the next column that overlaps
+// it or touches it diagonally, which is 8-connectivity.
+int64_t label_nodes(const uint8_t* mask, int64_t M, int64_t N, const int64_t* pi,
+                    const int64_t* pj, int64_t n, int64_t* lab) {
+  std::vector<int64_t> cs((size_t)N + 1), ra, rb;
+  for (int64_t j = 0; j < N; j++) {
+    cs[j] = (int64_t)ra.size();
+    const uint8_t* col = mask + j * M;
+    int64_t i = 0;
+    while (i < M) {
+      uint64_t w;
+      while (i + 8 <= M && (std::memcpy(&w, col + i, 8), w == 0)) i += 8;
+      while (i < M && !col[i]) i++;
+      if (i >= M) break;
+      const int64_t a = i;
+      while (i + 8 <= M && (std::memcpy(&w, col + i, 8), !has_zero_byte(w))) i += 8;
+      while (i < M && col[i]) i++;
+      ra.push_back(a);
+      rb.push_back(i - 1);
     }
+  }
+  cs[N] = (int64_t)ra.size();
+  const int64_t R = cs[N];
+  std::vector<int64_t> par((size_t)R);
+  for (int64_t r = 0; r < R; r++) par[r] = r;
+  auto find = [&](int64_t x) {
+    while (par[x] != x) {
+      par[x] = par[par[x]];
+      x = par[x];
+    }
+    return x;
+  };
+  for (int64_t j = 0; j + 1 < N; j++) {
+    int64_t p = cs[j], q = cs[j + 1];
+    const int64_t pe = cs[j + 1], qe = cs[j + 2];
+    while (p < pe && q < qe) {
+      if (rb[p] + 1 < ra[q]) {
+        p++;
+      } else if (rb[q] + 1 < ra[p]) {
+        q++;
+      } else {
+        const int64_t x = find(p), y = find(q);
+        if (x != y) par[std::max(x, y)] = std::min(x, y);
+        if (rb[p] < rb[q])
+          p++;
+        else
+          q++;
+      }
+    }
+  }
+  std::vector<int64_t> id((size_t)R, 0);
+  int64_t K = 0;
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t i = pi[k], j = pj[k];
+    const auto b0 = ra.begin() + cs[j], b1 = ra.begin() + cs[j + 1];
+    const auto it = std::upper_bound(b0, b1, i);
+    lab[k] = 0;
+    if (it == b0) continue;
+    const int64_t r = (int64_t)(it - ra.begin()) - 1;
+    if (rb[r] < i) continue;
+    const int64_t root = find(r);
+    if (!id[root]) id[root] = ++K;
+    lab[k] = id[root];
   }
   return K;
 }
@@ -392,11 +501,7 @@ extern "C" int aldic_natural_interp(const double* xs, const double* ys, const do
   Interp I;
   const int rc = I.build(xs, ys, vals, n, nv);
   if (rc != ALDIC_OK) return rc;
-  std::vector<double> o(nv);
-  for (int64_t k = 0; k < nq; k++) {
-    I.eval(xq[k], yq[k], o.data());
-    for (int c = 0; c < nv; c++) out[k + (int64_t)c * nq] = o[c];
-  }
+  I.eval_many(xq, yq, nq, out, 1, nq);
   return ALDIC_OK;
 }
 
@@ -412,15 +517,15 @@ extern "C" int aldic_nan_fill(const double* coords, int64_t n, double* U, const 
   for (int64_t i = 0; i < n; i++) (isnan_u(i) ? nanidx : good).push_back(i);
   // ---- per connected mask region (LocalICGNQuadtree.m:203-247) ----
   if (!nanidx.empty() && mask) {
-    std::vector<int32_t> lab;
-    const int64_t K = label8(mask, M, N, lab);
-    std::vector<int32_t> node_lab(n, 0);
+    std::vector<int64_t> pi(n), pj(n), node_lab(n);
     for (int64_t i = 0; i < n; i++) {
       const int64_t rx = mround(cx[i]), ry = mround(cy[i]);
       if (rx < 1 || rx > M || ry < 1 || ry > N)
         return aldic::set_error(ALDIC_EINVAL, "nan_fill: a node lies outside the image");
-      node_lab[i] = lab[(size_t)((rx - 1) + (ry - 1) * M)];
+      pi[i] = rx - 1;
+      pj[i] = ry - 1;
     }
+    const int64_t K = label_nodes(mask, M, N, pi.data(), pj.data(), n, node_lab.data());
     std::vector<std::vector<int64_t>> src(K + 1), qry(K + 1);
     for (int64_t i : good) src[node_lab[i]].push_back(i);
     for (int64_t i : nanidx) qry[node_lab[i]].push_back(i);
@@ -447,11 +552,10 @@ extern "C" int aldic_nan_fill(const double* coords, int64_t n, double* U, const 
       }
       Interp I;
       if (I.build(sx.data(), sy.data(), sv.data(), ns, 2) != ALDIC_OK) continue;  // the region's catch
-      double o[2];
+      I.eval_many(qx.data(), qy.data(), nq, qo.data(), 2, 1);
       for (int64_t k = 0; k < nq; k++) {
-        I.eval(qx[k], qy[k], o);
-        U[2 * qry[r][k]] = o[0];
-        U[2 * qry[r][k] + 1] = o[1];
+        U[2 * qry[r][k]] = qo[2 * k];
+        U[2 * qry[r][k] + 1] = qo[2 * k + 1];
       }
       if (regions_filled) (*regions_filled)++;
     }
@@ -477,11 +581,6 @@ extern "C" int aldic_nan_fill(const double* coords, int64_t n, double* U, const 
   Interp I;
   const int rc = I.build(sx.data(), sy.data(), sv.data(), ns, 2);
   if (rc != ALDIC_OK) return rc;
-  double o[2];
-  for (int64_t i = 0; i < n; i++) {
-    I.eval(cx[i], cy[i], o);
-    U[2 * i] = o[0];
-    U[2 * i + 1] = o[1];
-  }
+  I.eval_many(cx, cy, n, U, 2, 1);
   return ALDIC_OK;
 }

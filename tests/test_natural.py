@@ -191,3 +191,54 @@ def test_nan_fill_region_rules(pkg):
     assert np.max(np.abs(got - want)) < 1e-9
     with pytest.raises(pkg.AldicError):  # every node NaN
         pkg.nan_fill(coords, np.full(2 * n, np.nan), None, mask)
+
+
+def test_nan_fill_diagonal_touch_is_one_region(pkg):
+    """Two mask blocks that touch only at a corner are one 8-connected
+    region: its NaN nodes are filled from both blocks' valid nodes and not
+    from the separate third block (whose values are far off)."""
+    M, N = 120, 120
+    mask = np.zeros((M, N))
+    mask[0:50, 0:50] = 1
+    mask[50:100, 50:100] = 1  # touches the first block at one corner
+    mask[105:120, 0:40] = 1   # separate
+    pts = [(x, y) for x in np.arange(5.0, 50, 9) for y in np.arange(5.0, 50, 9)]
+    pts += [(x, y) for x in np.arange(54.0, 100, 9) for y in np.arange(54.0, 100, 9)]
+    pts += [(x, y) for x in (108.0, 116.0) for y in np.arange(5.0, 40, 8)]
+    coords = np.asfortranarray(np.array(pts, dtype=float))
+    n = len(coords)
+    U = np.empty(2 * n)
+    U[0::2] = 0.02 * coords[:, 0] + 0.01 * coords[:, 1]
+    U[1::2] = -0.01 * coords[:, 0]
+    U[0::2][coords[:, 0] > 104] += 50.0
+    second = np.flatnonzero((coords[:, 0] > 50) & (coords[:, 0] < 100))
+    bad = second[::3]
+    U[2 * bad] = np.nan
+    U[2 * bad + 1] = np.nan
+    got = pkg.nan_fill(coords, U, None, mask)
+    want = no.nan_fill(coords, U, None, mask)
+    assert np.max(np.abs(got - want)) < 1e-9
+    assert np.max(np.abs(got[2 * bad])) < 10.0  # not pulled toward the third block
+
+
+def test_natural_interp_large_void_many_queries(pkg):
+    """A grid with a large empty disc: queries at its rim and inside it have
+    Voronoi cells reaching across the void; enough queries (300) that the
+    library splits them over threads.  Eight are checked against the oracle,
+    all 300 by linear precision."""
+    g = np.arange(0.0, 321.0, 16.0)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    x, y = X.ravel(), Y.ravel()
+    keep = np.hypot(x - 160, y - 160) > 90
+    x, y = x[keep], y[keep]
+    v = np.stack([np.sin(x / 31.0) + 0.01 * y, np.cos(y / 17.0)], axis=1)
+    rng = np.random.default_rng(5)
+    t = rng.uniform(0, 2 * np.pi, 300)
+    rr = np.concatenate([rng.uniform(80, 100, 150), rng.uniform(0, 80, 150)])
+    q = np.stack([160 + rr * np.cos(t), 160 + rr * np.sin(t)], 1)
+    got = pkg.scattered_natural(x, y, v, q[:, 0], q[:, 1])
+    pick = np.r_[0:4, 150:154]
+    want = no.Natural(x, y, v)(q[pick, 0], q[pick, 1])
+    assert np.max(np.abs(got[pick] - want)) <= 1e-9 * np.abs(v).max()
+    got = pkg.scattered_natural(x, y, _lin(x, y), q[:, 0], q[:, 1])
+    assert np.max(np.abs(got - _lin(q[:, 0], q[:, 1]))) < 1e-9
