@@ -68,6 +68,10 @@ struct aldic_fem {
   double* sc = nullptr;        // scalars (see cg kernels)
   int* ctr = nullptr;          // [0] finishing-block counter, [1] done flag, [2] iterations
   int nblk = 0;
+  // a chunk of CG iterations recorded once as a graph and replayed; rebuilt
+  // when the iteration limit it was recorded with changes
+  hipGraphExec_t cg_exec = nullptr;
+  int32_t cg_exec_maxit = -1;
 };
 
 namespace {
@@ -318,9 +322,11 @@ __global__ void setup_kernel(int64_t n, const int* __restrict__ rowptr, const in
 }
 
 // Block sums of NV values, then the last block to finish adds the block
-// sums in order into out (deterministic).
+// sums into out: thread t takes blocks t, t + FEM_T, ... in order, then the
+// same shuffle/LDS tree as the block sums.  The order is fixed, whatever
+// the order the blocks finish in, so the result is deterministic.
 template <int NV>
-__device__ void reduce_store(double (&v)[NV], double* part, int nblk, int* counter, double* out) {
+__device__ bool reduce_store(double (&v)[NV], double* part, int nblk, int* counter, double* out) {
   __shared__ double sh[NV][FEM_T / ALDIC_WAVE];
   __shared__ bool last;
   const int lane = threadIdx.x % ALDIC_WAVE, wv = threadIdx.x / ALDIC_WAVE;
@@ -342,16 +348,38 @@ __device__ void reduce_store(double (&v)[NV], double* part, int nblk, int* count
     last = atomicAdd(counter, 1) == nblk - 1;
   }
   __syncthreads();
-  if (last && threadIdx.x < NV) {
+  if (last) {
     __threadfence();
-    double s = 0.0;
-    for (int b = 0; b < nblk; b++) s += part[(int64_t)threadIdx.x * nblk + b];
-    out[threadIdx.x] = s;
+    double t[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      t[k] = 0.0;
+      for (int b = threadIdx.x; b < nblk; b += FEM_T) t[k] += part[(int64_t)k * nblk + b];
+      for (int o = ALDIC_WAVE / 2; o > 0; o >>= 1) t[k] += __shfl_xor(t[k], o);
+    }
+    __syncthreads();  // every wave is done reading sh from the block sums
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < NV; k++) sh[k][wv] = t[k];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+      double s = 0.0;
+      for (int w = 0; w < FEM_T / ALDIC_WAVE; w++) s += sh[threadIdx.x][w];
+      out[threadIdx.x] = s;
+    }
+    if (threadIdx.x == 0) *counter = 0;
   }
-  if (last && threadIdx.x == 0) *counter = 0;
+  return last;
 }
 
-// sc layout: [0,1] rz (x, y), [2,3] pq, [4,5] rr, [6,7] bb (||rhs||^2), [8,9] rz new
+// sc layout: (r . z) of even iterations at [0,1] with (r . r) after it at
+// [2,3], of odd iterations at [10,11] with (r . r) at [12,13]; (p . q) at
+// [4,5]; ||rhs||^2 at [6,7].  Iteration k reads its (r . z) from slot k & 1
+// and leaves the next one in the other slot, so no kernel overwrites a
+// scalar that another block of the same launch may still read.
+__device__ __forceinline__ double* rz_slot(double* sc, int s) { return sc + (s ? 10 : 0); }
+
 __global__ void cg_init_kernel(int64_t n, const unsigned char* __restrict__ kind,
                                const double* __restrict__ r, const double* __restrict__ dinv,
                                double* __restrict__ z, double* __restrict__ p,
@@ -407,33 +435,60 @@ __global__ void cg_spmv_kernel(int64_t n, const int* __restrict__ rowptr,
   double v[2] = {0, 0};
   if (i < n && kind[i] == 1) {
     double qx = 0.0, qy = 0.0;
-    for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-      const int j = col[k];
+    const double2* p2 = reinterpret_cast<const double2*>(p);
+    int k = rowptr[i];
+    const int ke = rowptr[i + 1];
+    // four entries at a time so their loads are in flight together; the
+    // sums keep the entry order
+    for (; k + 4 <= ke; k += 4) {
+      int j[4];
+      double a[4];
+      double2 pj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        j[u] = col[k + u];
+        a[u] = val[k + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) pj[u] = p2[j[u]];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        qx += a[u] * pj[u].x;  // p is zero off the free rows
+        qy += a[u] * pj[u].y;
+      }
+    }
+    for (; k < ke; k++) {
+      const double2 pj = p2[col[k]];
       const double a = val[k];
-      qx += a * p[2 * j];  // p is zero off the free rows
-      qy += a * p[2 * j + 1];
+      qx += a * pj.x;
+      qy += a * pj.y;
     }
     q[2 * i] = qx;
     q[2 * i + 1] = qy;
     v[0] = p[2 * i] * qx;
     v[1] = p[2 * i + 1] * qy;
   }
-  reduce_store<2>(v, part, nblk, ctr, sc + 2);
+  reduce_store<2>(v, part, nblk, ctr, sc + 4);
 }
 
-// x += a p, r -= a q, z = D^-1 r, then (r . z) and (r . r)
+// x += a p, r -= a q, z = D^-1 r, then (r . z) and (r . r) into the other
+// slot; the block that finishes the sums also counts the iteration and sets
+// the done flag (converged, iteration limit, or NaN)
 __global__ void cg_update_kernel(int64_t n, const unsigned char* __restrict__ kind,
                                  double* __restrict__ x, double* __restrict__ r,
                                  double* __restrict__ z, const double* __restrict__ p,
                                  const double* __restrict__ q, const double* __restrict__ dinv,
-                                 double* part, int nblk, int* ctr, double* sc) {
+                                 double* part, int nblk, int* ctr, double* sc, int par,
+                                 double tol2, int max_iter) {
   if (ctr[1]) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double* rz = rz_slot(sc, par);
+  double* rzn = rz_slot(sc, par ^ 1);
   double v[4] = {0, 0, 0, 0};
   if (i < n && kind[i] == 1) {
 #pragma unroll
     for (int c = 0; c < 2; c++) {
-      const double a = sc[2 + c] != 0.0 ? sc[c] / sc[2 + c] : 0.0;
+      const double a = sc[4 + c] != 0.0 ? rz[c] / sc[4 + c] : 0.0;
       x[2 * i + c] += a * p[2 * i + c];
       const double rr = r[2 * i + c] - a * q[2 * i + c];
       r[2 * i + c] = rr;
@@ -443,33 +498,32 @@ __global__ void cg_update_kernel(int64_t n, const unsigned char* __restrict__ ki
       v[2 + c] = rr * rr;
     }
   }
-  reduce_store<4>(v, part, nblk, ctr, sc + 8);
+  if (reduce_store<4>(v, part, nblk, ctr, rzn)) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int it = ctr[2] + 1;
+      ctr[2] = it;
+      const double rx = rzn[2], ry = rzn[3];
+      const bool conv = rx <= tol2 * sc[6] && ry <= tol2 * sc[7];
+      if (conv || it >= max_iter || !(rx == rx) || !(ry == ry)) ctr[1] = 1;
+    }
+  }
 }
 
-// p = z + (rz_new / rz) p; the finishing block of the previous kernel left
-// rz_new in sc[8..9] and rr in sc[10..11]
+// p = z + (rz_new / rz) p, rz_new from the slot the update filled
 __global__ void cg_dir_kernel(int64_t n, const unsigned char* __restrict__ kind,
-                              const double* __restrict__ z, double* __restrict__ p, double* sc,
-                              int* ctr, double tol2, int max_iter) {
+                              const double* __restrict__ z, double* __restrict__ p,
+                              const double* sc, const int* ctr, int par) {
   if (ctr[1]) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const double bx = sc[8] != 0.0 && sc[0] != 0.0 ? sc[8] / sc[0] : 0.0;
-  const double by = sc[9] != 0.0 && sc[1] != 0.0 ? sc[9] / sc[1] : 0.0;
+  const double* rz = sc + (par ? 10 : 0);
+  const double* rzn = sc + (par ? 0 : 10);
+  const double bx = rzn[0] != 0.0 && rz[0] != 0.0 ? rzn[0] / rz[0] : 0.0;
+  const double by = rzn[1] != 0.0 && rz[1] != 0.0 ? rzn[1] / rz[1] : 0.0;
   if (i < n && kind[i] == 1) {
     p[2 * i] = z[2 * i] + bx * p[2 * i];
     p[2 * i + 1] = z[2 * i + 1] + by * p[2 * i + 1];
   }
-}
-
-// Advance the scalars after every block of cg_dir_kernel has read them:
-// a one-thread launch ordered after it on the stream.
-__global__ void cg_step_kernel(double* sc, int* ctr, double tol2, int max_iter) {
-  if (ctr[1]) return;
-  sc[0] = sc[8];
-  sc[1] = sc[9];
-  ctr[2] += 1;
-  const bool conv = sc[10] <= tol2 * sc[6] && sc[11] <= tol2 * sc[7];
-  if (conv || ctr[2] >= max_iter || !(sc[10] == sc[10]) || !(sc[11] == sc[11])) ctr[1] = 1;
 }
 
 __global__ void fill_ints(int* p, int n, int v) {
@@ -511,6 +565,54 @@ void fem_free(aldic_fem* f) {
                   f->r, f->z, f->p, f->q, f->dinv, f->part, f->sc, f->ctr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (f->cg_exec) (void)hipGraphExecDestroy(f->cg_exec);
+  f->cg_exec = nullptr;
+}
+
+constexpr int CG_CHUNK = 16;  // even: a chunk starts on an even iteration
+
+void launch_cg_chunk(aldic_fem* f, hipStream_t s, double tol2, int max_iter) {
+  const int64_t n = f->n;
+  for (int k = 0; k < CG_CHUNK; k++) {
+    const int par = k & 1;
+    cg_spmv_kernel<<<f->nblk, FEM_T, 0, s>>>(n, f->rowptr, f->col, f->val, f->kind, f->p, f->q,
+                                             f->part, f->nblk, f->ctr, f->sc);
+    cg_update_kernel<<<f->nblk, FEM_T, 0, s>>>(n, f->kind, f->x, f->r, f->z, f->p, f->q, f->dinv,
+                                               f->part, f->nblk, f->ctr, f->sc, par, tol2,
+                                               max_iter);
+    cg_dir_kernel<<<f->nblk, FEM_T, 0, s>>>(n, f->kind, f->z, f->p, f->sc, f->ctr, par);
+  }
+}
+
+// The chunk as a graph: recorded on the caller's stream the first time (or
+// when the iteration limit changes), replayed after.  Returns false when it
+// cannot be recorded (the stream is already being captured, say); the
+// caller then launches the chunk directly.
+bool cg_chunk_graph(aldic_fem* f, hipStream_t s, double tol2, int max_iter) {
+  if (f->cg_exec && f->cg_exec_maxit == max_iter) return true;
+  if (f->cg_exec) {
+    (void)hipGraphExecDestroy(f->cg_exec);
+    f->cg_exec = nullptr;
+  }
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return false;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+  launch_cg_chunk(f, s, tol2, max_iter);
+  hipGraph_t g = nullptr;
+  if (hipStreamEndCapture(s, &g) != hipSuccess || !g) {
+    (void)hipGetLastError();
+    return false;
+  }
+  hipGraphExec_t ex = nullptr;
+  const bool ok = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0) == hipSuccess;
+  (void)hipGraphDestroy(g);
+  if (!ok) {
+    (void)hipGetLastError();
+    return false;
+  }
+  f->cg_exec = ex;
+  f->cg_exec_maxit = max_iter;
+  return true;
 }
 
 inline int blocks(int64_t n) { return (int)std::max<int64_t>(1, (n + FEM_T - 1) / FEM_T); }
@@ -727,18 +829,13 @@ extern "C" int aldic_subpb2_dev(aldic_ctx* ctx, aldic_fem* f, double beta, doubl
   cg_init_kernel<<<f->nblk, FEM_T, 0, s>>>(n, f->kind, f->r, f->dinv, f->z, f->p, f->b, d_U,
                                            f->part, f->nblk, f->ctr, f->sc);
   const double tol2 = 1e-26;  // (1e-13)^2
+  const bool graph = cg_chunk_graph(f, s, tol2, max_iter);
   int done = 0, iters = 0;
-  constexpr int CHUNK = 16;
   while (!done) {
-    for (int k = 0; k < CHUNK; k++) {
-      cg_spmv_kernel<<<f->nblk, FEM_T, 0, s>>>(n, f->rowptr, f->col, f->val, f->kind, f->p, f->q,
-                                               f->part, f->nblk, f->ctr, f->sc);
-      cg_update_kernel<<<f->nblk, FEM_T, 0, s>>>(n, f->kind, f->x, f->r, f->z, f->p, f->q,
-                                                 f->dinv, f->part, f->nblk, f->ctr, f->sc);
-      cg_dir_kernel<<<f->nblk, FEM_T, 0, s>>>(n, f->kind, f->z, f->p, f->sc, f->ctr, tol2,
-                                              max_iter);
-      cg_step_kernel<<<1, 1, 0, s>>>(f->sc, f->ctr, tol2, max_iter);
-    }
+    if (graph)
+      HIP_TRY(hipGraphLaunch(f->cg_exec, s));
+    else
+      launch_cg_chunk(f, s, tol2, max_iter);
     int h[4];
     HIP_TRY(hipMemcpyAsync(h, f->ctr, sizeof(h), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));

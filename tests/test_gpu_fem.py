@@ -86,3 +86,34 @@ def test_subpb2_larger_mesh(pkg):
     print(f"{len(coords)} nodes, {len(el)} elements, {it} CG iterations")
     assert np.max(np.abs(got - want)) / np.max(np.abs(U)) < 1e-9
     mesh.close()
+
+
+def test_subpb2_iteration_limit_and_replays(pkg):
+    """The CG chunks are replayed from a recorded graph that is rebuilt when
+    the iteration limit changes: a limit of 3 stops after exactly 3
+    iterations, twice with the same result, the default limit then gives
+    the converged result of the host entry point, and a limit of 5 after
+    it stops at 5."""
+    from aldic_amd import api
+    coords, el, bnd = quadtree_mesh(9, 7, 4.0, refine=[(2, 2), (5, 3), (6, 5)])
+    U, F = _fields(coords, 11)
+    n = len(coords)
+    mesh = pkg.FEMesh(coords, el, bnd, None, 2)
+    ctx = pkg.default_context()
+    dU = api.DeviceArray.from_host(ctx, U, np.float64)
+    dF = api.DeviceArray.from_host(ctx, F, np.float64)
+    dO = api.DeviceArray(ctx, 2 * n, np.float64)
+    it3 = mesh.solve_dev(ctx, 1e-3, 1e-3, dU.ptr, dF.ptr, dO.ptr, 0.0, 8.0, max_iter=3)
+    a3 = dO.to_host()
+    assert it3 == 3
+    assert mesh.solve_dev(ctx, 1e-3, 1e-3, dU.ptr, dF.ptr, dO.ptr, 0.0, 8.0, max_iter=3) == 3
+    assert np.array_equal(dO.to_host(), a3)
+    itc = mesh.solve_dev(ctx, 1e-3, 1e-3, dU.ptr, dF.ptr, dO.ptr, 0.0, 8.0)
+    conv = dO.to_host()
+    want, itw = mesh.solve(1e-3, 1e-3, U, F, 0.0, 8.0, return_iterations=True)
+    assert itc == itw > 3 and np.array_equal(conv, want)
+    assert not np.allclose(a3, conv, rtol=0, atol=1e-9 * np.abs(U).max())
+    assert mesh.solve_dev(ctx, 1e-3, 1e-3, dU.ptr, dF.ptr, dO.ptr, 0.0, 8.0, max_iter=5) == 5
+    ref = fo.subpb2(coords, el, bnd, None, 2, 1e-3, 1e-3, U, F, 0.0, 8.0)
+    assert np.max(np.abs(conv - ref)) / np.max(np.abs(U)) < 1e-9
+    mesh.close()
