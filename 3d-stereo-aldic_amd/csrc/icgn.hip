@@ -265,26 +265,24 @@ constexpr int fast_block() {
   return NW == 1 ? 256 : ALDIC_WAVE * NW;
 }
 
+// One subset (list entry slot) of fast_kernel; its wave group returns from
+// it together.  tileb is the group's LDS tile slot (TS * TS + TS + 1
+// floats): the tile starts TS + 1 floats into it, so that the gather's
+// (-1, -1) tap offset is a non-negative base address.
 // FS > 0: every subset of the launch is FS x FS (the plan checked it), so
 // the pixel layout, the partial slot and the divisions are compile-time.
-template <int MODE, int NW, int PPL, int TS, int FS>
-__global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
+template <int MODE, int NW, int PPL, int TS, int FS, bool WALK>
+__device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float* tileb,
+                                            float* red) {
   constexpr int NT = ALDIC_WAVE * NW;
-  constexpr int SUBS = fast_block<NW>() / NT;
-  static_assert(NW == 1 || SUBS == 1, "multi-wave subsets own their block");
-  // the tile starts TS + 1 floats into its slot, so that the gather's
-  // (-1, -1) tap offset is a non-negative base address
-  __shared__ float tiles[SUBS][TS * TS + TS + 1];
-  __shared__ float red[NW * 32];
-  const int sub = threadIdx.x / NT;
-  const int tid = threadIdx.x % NT;
-  const int slot = blockIdx.x * SUBS + sub;
-  const int cnt = a.dcnt ? *a.dcnt : a.cnt;
-  if (slot >= cnt) return;  // uniform per subset group (SUBS == 1 when NW > 1)
+  int tid = threadIdx.x % NT;
+  // in a list walk the lane index is made opaque to the optimiser, so that
+  // nothing derived from it is hoisted out of the walk and kept live across
+  // the whole solve (the one-entry kernel keeps its known range)
+  if constexpr (WALK) __asm__ volatile("" : "+v"(tid));
   // every value of one subset is uniform across its waves: keep it scalar
   const int node = __builtin_amdgcn_readfirstlane(a.list[slot]);
-  float* tile = tiles[sub] + TS + 1;
-  const float* tileb = tiles[sub];  // tile - (TS + 1)
+  float* tile = tileb + TS + 1;
   const int M = a.M, N = a.N;
   if (M < TS || N < TS) {  // the tile would not fit in the image
     if (tid == 0) {
@@ -721,6 +719,33 @@ __global__ __launch_bounds__(fast_block<NW>()) void fast_kernel(SolveArgs a) {
       write_icgn(a, node, st, P, h21, true, last_zncc);
     else
       write_subpb1(a, node, st, P);
+  }
+}
+
+// One subset per wave group (SUBS groups per block).  WALK: a list counted
+// on the device (a.dcnt) and launched at a bounded grid, which each group
+// walks with a grid stride; otherwise the grid covers the list and every
+// group takes one entry (a separate instance: the loop costs the one-entry
+// kernel registers; a walking multi-wave one is held to two waves per
+// SIMD).
+template <int MODE, int NW, int PPL, int TS, int FS, bool WALK>
+__global__ __launch_bounds__(fast_block<NW>(), WALK && NW > 1 ? 2 : 1) void fast_kernel(
+    SolveArgs a) {
+  constexpr int NT = ALDIC_WAVE * NW;
+  constexpr int SUBS = fast_block<NW>() / NT;
+  static_assert(NW == 1 || SUBS == 1, "multi-wave subsets own their block");
+  __shared__ float tiles[SUBS][TS * TS + TS + 1];
+  __shared__ float red[NW * 32];
+  const int sub = threadIdx.x / NT;
+  const int cnt = a.dcnt ? *a.dcnt : a.cnt;
+  // uniform per subset group (SUBS == 1 when NW > 1); the group's barriers
+  // and the reduction's barrier keep one entry's LDS use apart from the next
+  if constexpr (WALK) {
+    for (int slot = blockIdx.x * SUBS + sub; slot < cnt; slot += gridDim.x * SUBS)
+      fast_subset<MODE, NW, PPL, TS, FS, true>(a, slot, tiles[sub], red);
+  } else {
+    const int slot = blockIdx.x * SUBS + sub;
+    if (slot < cnt) fast_subset<MODE, NW, PPL, TS, FS, false>(a, slot, tiles[sub], red);
   }
 }
 
@@ -1229,6 +1254,7 @@ namespace aldic {
 constexpr int CLS_SIDE[4] = {11, 17, 33, 65};
 constexpr int CLS_TS[4] = {24, 32, 48, 84};
 constexpr int NVAR = 3;
+constexpr int LIST_GRID = 1024;  // blocks of a launch over a device-counted list
 constexpr int VAR_NW[4][NVAR] = {{1, 1, 1}, {1, 2, 2}, {1, 2, 4}, {4, 8, 5}};
 constexpr int VAR_PPL[4][NVAR] = {{2, 2, 2}, {5, 3, 3}, {18, 9, 5}, {17, 9, 14}};
 
@@ -1258,11 +1284,21 @@ static void launch_shape(const SolveArgs& a, hipStream_t s) {
   constexpr int NW = VAR_NW[C][V];
   constexpr int block = fast_block<NW>();
   constexpr int subs = block / (ALDIC_WAVE * NW);
+  // a list counted on the device is walked by a bounded grid (a few
+  // resident rounds), not launched at its full capacity: the idle blocks
+  // of a capacity launch would take slots from the kernels beside it
   const int grid = (a.cnt + subs - 1) / subs;
-  if (a.fixed_side)
-    fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], CLS_SIDE[C]><<<grid, block, 0, s>>>(a);
-  else
-    fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], 0><<<grid, block, 0, s>>>(a);
+  if (a.dcnt) {
+    const int g = std::min(grid, LIST_GRID);
+    if (a.fixed_side)
+      fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], CLS_SIDE[C], true><<<g, block, 0, s>>>(a);
+    else
+      fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], 0, true><<<g, block, 0, s>>>(a);
+  } else if (a.fixed_side) {
+    fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], CLS_SIDE[C], false><<<grid, block, 0, s>>>(a);
+  } else {
+    fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], 0, false><<<grid, block, 0, s>>>(a);
+  }
 }
 
 template <int MODE, int C>
