@@ -844,8 +844,8 @@ extern "C" int aldic_search_nodes_dev(aldic_ctx* ctx, const aldic_images* im,
 
 // IC-GN over the listed nodes d_list[0 .. *d_count) of a plan: the nodes
 // are sorted into per-size-class lists on the device and run through the
-// same wave kernels as the whole plan (launched at each class's capacity,
-// the count read on the device), so no host round trip is needed
+// same wave kernels as the whole plan (a bounded grid walks each class's
+// list, the count read on the device), so no host round trip is needed
 extern "C" int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im,
                                          const aldic_plan* plan, const double* d_coords,
                                          const float* d_U0, const float* d_F0,

@@ -26,6 +26,21 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ bf16 to_bf16(float v) { return (bf16)v; }
 __device__ __forceinline__ float from_bf16(bf16 v) { return (float)v; }
 
+// Elements 4 q0 .. 4 q0 + 3 of an MFMA 32x32 accumulator plus their biases,
+// optional ReLU, as four bf16 in one 8-byte word (the rounding of to_bf16).
+__device__ __forceinline__ uint2 bias_relu_pack4(const floatx16& acc, int q0, float4 b, bool relu) {
+  float v[4] = {acc[4 * q0] + b.x, acc[4 * q0 + 1] + b.y, acc[4 * q0 + 2] + b.z,
+                acc[4 * q0 + 3] + b.w};
+  if (relu) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = fmaxf(v[k], 0.0f);
+  }
+  uint32_t h[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) h[k] = __builtin_bit_cast(uint16_t, to_bf16(v[k]));
+  return make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+}
+
 // ---------------------------------------------------------------------------
 // GEMM  C[M x N] = A[M x K] * Bt[N x K]^T
 //   Bt row stride ldb (elements); K % 16 == 0; ldb a multiple of 8.  One
@@ -181,9 +196,6 @@ __global__ __launch_bounds__(256) void gemm_lds_kernel(const bf16* __restrict__ 
   __syncthreads();
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int r = lane % 32, kh = 8 * (lane / 32), hq = lane / 32;
-  float bv[NB];
-#pragma unroll
-  for (int j = 0; j < NB; j++) bv[j] = bias ? bias[32 * j + r] : 0.0f;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int m0 = tile * 256 + wave * 64;
     ALoader<AMODE> a0, a1;
@@ -207,26 +219,28 @@ __global__ __launch_bounds__(256) void gemm_lds_kernel(const bf16* __restrict__ 
 #pragma unroll
       for (int j = 0; j < NB; j++) {
         const bf16x8 b = sB[(32 * j + r) * rs + k / 8 + hq];
-        acc0[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x0, b, acc0[j], 0, 0, 0);
-        acc1[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(x1, b, acc1[j], 0, 0, 0);
+        acc0[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, x0, acc0[j], 0, 0, 0);
+        acc1[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, x1, acc1[j], 0, 0, 0);
       }
       x0 = n0;
       x1 = n1;
     }
+    // computed transposed (weights as the A operand): element i of lane
+    // (r, hq) is column 32 j + 8 (i / 4) + 4 hq + i % 4 of row m0 + 32 h + r,
+    // so runs of 4 columns go out as 8-byte words (ldc % 4 == 0)
 #pragma unroll
     for (int h = 0; h < 2; h++) {
+      const int row = m0 + 32 * h + r;
+      if (row >= M) continue;
 #pragma unroll
-      for (int j = 0; j < NB; j++) {
-        const int col = 32 * j + r;
+      for (int j = 0; j < NB; j++)
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-          const int row = m0 + 32 * h + 8 * (i / 4) + 4 * hq + (i % 4);
-          if (row >= M) continue;
-          float v = (h ? acc1[j][i] : acc0[j][i]) + bv[j];
-          if (relu) v = fmaxf(v, 0.0f);
-          Cb[(int64_t)row * ldc + col] = to_bf16(v);
+        for (int q0 = 0; q0 < 4; q0++) {
+          const int col = 32 * j + 8 * q0 + 4 * hq;
+          const float4 b = bias ? *(const float4*)(bias + col) : make_float4(0, 0, 0, 0);
+          *(uint2*)(Cb + (int64_t)row * ldc + col) =
+              bias_relu_pack4(h ? acc1[j] : acc0[j], q0, b, relu != 0);
         }
-      }
     }
   }
 }
@@ -258,9 +272,6 @@ __global__ __launch_bounds__(256) void conv_lds_kernel(const bf16* __restrict__ 
   __syncthreads();
   const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
   const int r = lane % 32, hq = lane / 32, kh = 8 * hq;
-  float bv[NB];
-#pragma unroll
-  for (int j = 0; j < NB; j++) bv[j] = bias ? bias[32 * j + r] : 0.0f;
   const int hw = g.Ho * g.Wo;
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int m0 = tile * 256 + wave * 64;
@@ -312,8 +323,8 @@ __global__ __launch_bounds__(256) void conv_lds_kernel(const bf16* __restrict__ 
 #pragma unroll
         for (int j = 0; j < NB; j++) {
           const bf16x8 b = sB[(32 * j + r) * RS + 2 * st + hq];
-          acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b, acc[0][j], 0, 0, 0);
-          acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b, acc[1][j], 0, 0, 0);
+          acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a[0], acc[0][j], 0, 0, 0);
+          acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a[1], acc[1][j], 0, 0, 0);
         }
       }
     } else {
@@ -344,8 +355,8 @@ __global__ __launch_bounds__(256) void conv_lds_kernel(const bf16* __restrict__ 
 #pragma unroll
           for (int j = 0; j < NB; j++) {
             const bf16x8 b = sB[(32 * j + r) * RS + 2 * st + hq];
-            acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[0][c], b, acc[0][j], 0, 0, 0);
-            acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cur[1][c], b, acc[1][j], 0, 0, 0);
+            acc[0][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, cur[0][c], acc[0][j], 0, 0, 0);
+            acc[1][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, cur[1][c], acc[1][j], 0, 0, 0);
           }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -357,20 +368,22 @@ __global__ __launch_bounds__(256) void conv_lds_kernel(const bf16* __restrict__ 
         }
       }
     }
+    // computed transposed (weights as the A operand): element i of lane
+    // (r, hq) is channel 32 j + 8 (i / 4) + 4 hq + i % 4 of output row
+    // m0 + 32 h + r, so runs of 4 channels go out as 8-byte words
 #pragma unroll
-    for (int h = 0; h < 2; h++)
+    for (int h = 0; h < 2; h++) {
+      const int row = m0 + 32 * h + r;
+      if (row >= M) continue;
 #pragma unroll
-      for (int j = 0; j < NB; j++) {
-        const int col = 32 * j + r;
+      for (int j = 0; j < NB; j++)
 #pragma unroll
-        for (int i = 0; i < 16; i++) {
-          const int row = m0 + 32 * h + 8 * (i / 4) + 4 * hq + (i % 4);
-          if (row >= M) continue;
-          float v = acc[h][j][i] + bv[j];
-          if (relu) v = fmaxf(v, 0.0f);
-          out[(int64_t)row * N + col] = to_bf16(v);
+        for (int q0 = 0; q0 < 4; q0++) {
+          const int ch = 32 * j + 8 * q0 + 4 * hq;
+          const float4 b = bias ? *(const float4*)(bias + ch) : make_float4(0, 0, 0, 0);
+          *(uint2*)(out + (int64_t)row * N + ch) = bias_relu_pack4(acc[h][j], q0, b, relu != 0);
         }
-      }
+    }
   }
 }
 
@@ -795,6 +808,7 @@ __global__ __launch_bounds__(256, 2) void front_kernel(const float* __restrict__
   __shared__ bf16 sa1[24 * 24 * FR_L1C];       // L1 output, NHWC, padded channels
   __shared__ bf16x8 sW1[32 * RS1];
   __shared__ bf16x8 sW2[32 * RS2];
+  __shared__ float4 sbias[16];  // b1 then b2, 4 channels per entry
   __shared__ float red[16];
   const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
   const int r = lane % 32, hq = lane / 32, kh = 8 * hq;
@@ -806,7 +820,7 @@ __global__ __launch_bounds__(256, 2) void front_kernel(const float* __restrict__
     const int row = e / (K2 / 8), q = e % (K2 / 8);
     sW2[row * RS2 + q] = *(const bf16x8*)(W2 + row * K2 + q * 8);
   }
-  const float bias1 = b1[r], bias2 = b2[r];
+  if (tid < 16) sbias[tid] = tid < 8 ? ((const float4*)b1)[tid] : ((const float4*)b2)[tid - 8];
   for (int e = tid; e < PP * PP; e += 256) sx[e] = 0u;  // the border stays zero
   // L1 taps of this lane: t = 8 st2 + 4 hq + q, (ky, kx) = (t / 3, t % 3), as
   // an offset into the padded patch; t >= 9 is the zero padding of K
@@ -840,6 +854,10 @@ __global__ __launch_bounds__(256, 2) void front_kernel(const float* __restrict__
       patch_load(f, g, M, N, cx, cy, sxs, sys, vf, vg);
     }
     // ---- L1: 576 rows (24 x 24), K 32 (taps 0..8 x 2 channels, padded), N 32 ----
+    // Computed transposed (weights as the A operand, the patch as B), so a
+    // lane ends with 16 channels of one output pixel, in runs of 4 that go
+    // to LDS as 8-byte words; the products and their k order are those of
+    // the untransposed layer.
     for (int tile = wave; tile < 18; tile += 4) {
       const int row = tile * 32 + r;
       const int oy = row / 24, ox = row % 24;
@@ -856,7 +874,7 @@ __global__ __launch_bounds__(256, 2) void front_kernel(const float* __restrict__
         for (int q = 0; q < 4; q++) w[q] = sx[rb + toff0[q]];
         const bf16x8 v = __builtin_bit_cast(bf16x8, w);
         const bf16x8 b = sW1[r * RS1 + hq];
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v, b, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, v, acc, 0, 0, 0);
       }
       {
         // tap 8 (first pair of the hq = 0 half), zero padding elsewhere
@@ -865,17 +883,18 @@ __global__ __launch_bounds__(256, 2) void front_kernel(const float* __restrict__
         w[0] = hq == 0 ? t8 : 0u;
         const bf16x8 v = __builtin_bit_cast(bf16x8, w);
         const bf16x8 b = sW1[r * RS1 + 2 + hq];
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v, b, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, v, acc, 0, 0, 0);
       }
+      // element i: channel 8 (i / 4) + 4 hq + i % 4 of pixel `row`
 #pragma unroll
-      for (int i = 0; i < 16; i++) {
-        const int orow = tile * 32 + 8 * (i / 4) + 4 * hq + (i % 4);
-        const float v = fmaxf(acc[i] + bias1, 0.0f);
-        sa1[orow * FR_L1C + r] = to_bf16(v);
+      for (int q0 = 0; q0 < 4; q0++) {
+        const int ch = 8 * q0 + 4 * hq;
+        *(uint2*)(sa1 + row * FR_L1C + ch) = bias_relu_pack4(acc, q0, sbias[ch / 4], true);
       }
     }
     __syncthreads();
     // ---- L2: 144 rows (12 x 12), K 288 (tap-major, 32 channels), N 32 ----
+    // transposed as L1: a lane ends with 16 channels of one output pixel
     for (int tile = wave; tile < 5; tile += 4) {
       const int row = tile * 32 + r;
       const bool ok = row < 144;
@@ -892,15 +911,15 @@ __global__ __launch_bounds__(256, 2) void front_kernel(const float* __restrict__
           bf16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
           if (in) v = *(const bf16x8*)(sa1 + (iy * 24 + ix) * FR_L1C + c * 16 + kh);
           const bf16x8 b = sW2[r * RS2 + 2 * (2 * t + c) + hq];
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(v, b, acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, v, acc, 0, 0, 0);
         }
       }
+      if (ok) {
+        bf16* o = out + ((int64_t)local * 144 + row) * 32;
 #pragma unroll
-      for (int i = 0; i < 16; i++) {
-        const int orow = tile * 32 + 8 * (i / 4) + 4 * hq + (i % 4);
-        if (orow < 144) {
-          const float v = fmaxf(acc[i] + bias2, 0.0f);
-          out[((int64_t)local * 144 + orow) * 32 + r] = to_bf16(v);
+        for (int q0 = 0; q0 < 4; q0++) {
+          const int ch = 8 * q0 + 4 * hq;
+          *(uint2*)(o + ch) = bias_relu_pack4(acc, q0, sbias[8 + ch / 4], true);
         }
       }
     }
@@ -979,6 +998,11 @@ static hipError_t launch_lds(const bf16* A, int lda, const bf16* Bt, int M, int 
                              hipStream_t s) {
   const size_t smem = (size_t)32 * NB * (K / 8 + 1) * 16;
   if (smem > 160 * 1024) return hipErrorInvalidValue;
+  // 8-byte output words: the row stride, the column block and the bias
+  // block must keep 4-element alignment
+  if (ldc % 4 != 0 || (reinterpret_cast<uintptr_t>(Cb) & 7) != 0 ||
+      (bias && (reinterpret_cast<uintptr_t>(bias) & 15) != 0))
+    return hipErrorInvalidValue;
   static bool attr = false;  // one attribute call per instantiation
   if (!attr) {
     hipError_t e = hipFuncSetAttribute((const void*)gemm_lds_kernel<NB, AMODE>,
@@ -999,6 +1023,10 @@ static hipError_t launch_conv(const bf16* x, const bf16* Wt, int M, const float*
                               bf16* out, ConvGeom g, hipStream_t s) {
   constexpr int K = CIN == 2 ? 32 : 9 * CIN;
   constexpr size_t smem = (size_t)32 * NB * (K / 8 + 1) * 16;
+  // 8-byte output words: the output and bias blocks keep 4-element alignment
+  if ((reinterpret_cast<uintptr_t>(out) & 7) != 0 ||
+      (bias && (reinterpret_cast<uintptr_t>(bias) & 15) != 0))
+    return hipErrorInvalidValue;
   const int ntiles = (M + 255) / 256;
   const int per_cu = smem > 80 * 1024 ? 1 : (smem > 40 * 1024 ? 2 : 4);
   const int grid = std::max(1, std::min(ntiles, 256 * per_cu * 2));
@@ -1126,6 +1154,10 @@ hipError_t cnn_front(const float* f, const float* g, int M, int N, const double*
                      const void* W2, const float* b2, void* out, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   if (first < 0 || first + count > n) return hipErrorInvalidValue;
+  // biases read as float4, outputs written as 8-byte words
+  if ((reinterpret_cast<uintptr_t>(b1) & 15) != 0 || (reinterpret_cast<uintptr_t>(b2) & 15) != 0 ||
+      (reinterpret_cast<uintptr_t>(out) & 7) != 0)
+    return hipErrorInvalidValue;
   front_kernel<<<std::min(count, 2048), 256, 0, s>>>(f, g, M, N, coords, n, first, count, prior,
                                                       (const bf16*)W1, b1, (const bf16*)W2, b2,
                                                       (bf16*)out);
