@@ -55,6 +55,8 @@ SIGNATURES = {
                                _dp, _vp]),
     "aldic_subpb2_dev": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double, _vp,
                                    _vp, _vp, C.c_int32, _vp]),
+    "aldic_subpb2_dev_async": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         _vp, _vp, _vp, C.c_int32, _vp]),
     "aldic_images_planes": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "aldic_images_create_raw": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp, _i64, _i64, _vp, _vp,
                                           _pp, _vp]),

@@ -485,6 +485,15 @@ class FEMesh:
                                      C.byref(it)))
         return it.value
 
+    def solve_dev_async(self, ctx: Context, beta, mu, d_U, d_F, d_Uhat, alpha=0.0,
+                        winstepsize=1.0, max_iter=64, d_iterations=None):
+        """solve_dev without a host synchronisation (recordable in a graph):
+        every chunk of 16 CG iterations up to max_iter is launched, and the
+        count lands in the device int d_iterations."""
+        check(lib().aldic_subpb2_dev_async(ctx.ptr, self.ptr, float(beta), float(mu), float(alpha),
+                                           float(winstepsize), d_U, d_F, d_Uhat, int(max_iter),
+                                           d_iterations))
+
     def close(self):
         if getattr(self, "ptr", None):
             lib().aldic_fem_destroy(self.ptr)

@@ -526,6 +526,9 @@ __global__ void cg_dir_kernel(int64_t n, const unsigned char* __restrict__ kind,
   }
 }
 
+// the iteration count of the last solve, for the caller's device buffer
+__global__ void iters_out_kernel(const int* ctr, int* out) { *out = ctr[2]; }
+
 __global__ void fill_ints(int* p, int n, int v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -802,17 +805,11 @@ extern "C" int aldic_fem_info(const aldic_fem* f, int64_t* nnz, int64_t* nfree, 
   return ALDIC_OK;
 }
 
-extern "C" int aldic_subpb2_dev(aldic_ctx* ctx, aldic_fem* f, double beta, double mu, double alpha,
-                                double winstepsize, const double* d_U, const double* d_F,
-                                double* d_Uhat, int32_t max_iter, int32_t* iterations) {
-  if (!ctx || !f || !d_U || !d_F || !d_Uhat)
-    return fail(ALDIC_EINVAL, "subpb2_dev: NULL argument");
-  if (f->nn > 0 && !(winstepsize != 0.0))
-    return fail(ALDIC_EINVAL, "subpb2_dev: winstepsize must be nonzero with Neumann edges");
-  if (max_iter <= 0) max_iter = (int32_t)std::min<int64_t>(std::max<int64_t>(2 * f->n, 100), 1 << 20);
-  HIP_TRY(hipSetDevice(ctx->device));
-  hipStream_t s = ctx->stream;
-  StageTimer t(ctx, ALDIC_STAGE_SUBPB2);
+namespace {
+
+// Assembly, right-hand side, Jacobi inverse and the CG start on stream s.
+void subpb2_setup(aldic_fem* f, double beta, double mu, double alpha, double winstepsize,
+                  const double* d_U, const double* d_F, hipStream_t s) {
   const int64_t n = f->n;
   element_kernel<<<blocks(f->ne), FEM_T, 0, s>>>(f->ne, n, f->elem, f->xy, f->gauss, beta, mu,
                                                  alpha, d_U, d_F, f->Ae, f->be);
@@ -828,23 +825,76 @@ extern "C" int aldic_subpb2_dev(aldic_ctx* ctx, aldic_fem* f, double beta, doubl
                                            f->part, f->nblk, f->ctr, f->sc);
   cg_init_kernel<<<f->nblk, FEM_T, 0, s>>>(n, f->kind, f->r, f->dinv, f->z, f->p, f->b, d_U,
                                            f->part, f->nblk, f->ctr, f->sc);
-  const double tol2 = 1e-26;  // (1e-13)^2
-  const bool graph = cg_chunk_graph(f, s, tol2, max_iter);
+}
+
+int subpb2_check(aldic_ctx* ctx, aldic_fem* f, double winstepsize, const double* d_U,
+                 const double* d_F, double* d_Uhat, const char* who) {
+  if (!ctx || !f || !d_U || !d_F || !d_Uhat)
+    return fail(ALDIC_EINVAL, std::string(who) + ": NULL argument");
+  if (f->nn > 0 && !(winstepsize != 0.0))
+    return fail(ALDIC_EINVAL, std::string(who) + ": winstepsize must be nonzero with Neumann edges");
+  return ALDIC_OK;
+}
+
+constexpr double CG_TOL2 = 1e-26;  // (1e-13)^2, relative, per component
+
+}  // namespace
+
+extern "C" int aldic_subpb2_dev(aldic_ctx* ctx, aldic_fem* f, double beta, double mu, double alpha,
+                                double winstepsize, const double* d_U, const double* d_F,
+                                double* d_Uhat, int32_t max_iter, int32_t* iterations) {
+  RC_TRY(subpb2_check(ctx, f, winstepsize, d_U, d_F, d_Uhat, "subpb2_dev"));
+  if (max_iter <= 0) max_iter = (int32_t)std::min<int64_t>(std::max<int64_t>(2 * f->n, 100), 1 << 20);
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  StageTimer t(ctx, ALDIC_STAGE_SUBPB2);
+  subpb2_setup(f, beta, mu, alpha, winstepsize, d_U, d_F, s);
+  const bool graph = cg_chunk_graph(f, s, CG_TOL2, max_iter);
   int done = 0, iters = 0;
   while (!done) {
     if (graph)
       HIP_TRY(hipGraphLaunch(f->cg_exec, s));
     else
-      launch_cg_chunk(f, s, tol2, max_iter);
+      launch_cg_chunk(f, s, CG_TOL2, max_iter);
     int h[4];
     HIP_TRY(hipMemcpyAsync(h, f->ctr, sizeof(h), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     done = h[1];
     iters = h[2];
   }
-  out_kernel<<<f->nblk, FEM_T, 0, s>>>(n, f->kind, f->x, d_U, d_Uhat);
+  out_kernel<<<f->nblk, FEM_T, 0, s>>>(f->n, f->kind, f->x, d_U, d_Uhat);
   HIP_TRY(hipGetLastError());
   if (iterations) *iterations = iters;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_subpb2_dev_async(aldic_ctx* ctx, aldic_fem* f, double beta, double mu,
+                                      double alpha, double winstepsize, const double* d_U,
+                                      const double* d_F, double* d_Uhat, int32_t max_iter,
+                                      int32_t* d_iterations) {
+  RC_TRY(subpb2_check(ctx, f, winstepsize, d_U, d_F, d_Uhat, "subpb2_dev_async"));
+  if (max_iter <= 0 || max_iter > (1 << 16))
+    return fail(ALDIC_EINVAL, "subpb2_dev_async: max_iter must be in 1 .. 65536");
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  HIP_TRY(hipStreamIsCapturing(s, &cs));
+  const bool capturing = cs != hipStreamCaptureStatusNone;
+  StageTimer t(ctx, ALDIC_STAGE_SUBPB2);
+  subpb2_setup(f, beta, mu, alpha, winstepsize, d_U, d_F, s);
+  // every chunk the limit may need; those after convergence return at once.
+  // Under a caller's capture the chunks are recorded as plain launches (the
+  // mesh's own chunk graph is for eager calls).
+  const bool graph = !capturing && cg_chunk_graph(f, s, CG_TOL2, max_iter);
+  for (int done = 0; done < max_iter; done += CG_CHUNK) {
+    if (graph)
+      HIP_TRY(hipGraphLaunch(f->cg_exec, s));
+    else
+      launch_cg_chunk(f, s, CG_TOL2, max_iter);
+  }
+  out_kernel<<<f->nblk, FEM_T, 0, s>>>(f->n, f->kind, f->x, d_U, d_Uhat);
+  if (d_iterations) iters_out_kernel<<<1, 1, 0, s>>>(f->ctr, d_iterations);
+  HIP_TRY(hipGetLastError());
   return ALDIC_OK;
 }
 

@@ -369,6 +369,16 @@ int aldic_subpb2(aldic_ctx* ctx, aldic_fem* fem, double beta, double mu, double 
 int aldic_subpb2_dev(aldic_ctx* ctx, aldic_fem* fem, double beta, double mu, double alpha,
                      double winstepsize, const double* d_U, const double* d_F, double* d_Uhat,
                      int32_t max_iter, int32_t* iterations);
+/* The same solve with no host synchronisation, so that it can be recorded
+ * in a graph with the local step around it (the ALM iteration of
+ * TemporalMatch_quadtree_ST1.m:337-380): all ceil(max_iter / 16) chunks of
+ * 16 CG iterations the limit allows are launched (1 <= max_iter <= 65536;
+ * the chunks after convergence return at once), and the iteration count is
+ * written to the device int d_iterations (may be NULL) when the solve
+ * ends.  No counterpart in the reference (its solve is a host `\`). */
+int aldic_subpb2_dev_async(aldic_ctx* ctx, aldic_fem* fem, double beta, double mu, double alpha,
+                           double winstepsize, const double* d_U, const double* d_F,
+                           double* d_Uhat, int32_t max_iter, int32_t* d_iterations);
 
 /* ---- host post-processing of the drivers (no device work) ---- */
 

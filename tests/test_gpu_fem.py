@@ -117,3 +117,51 @@ def test_subpb2_iteration_limit_and_replays(pkg):
     ref = fo.subpb2(coords, el, bnd, None, 2, 1e-3, 1e-3, U, F, 0.0, 8.0)
     assert np.max(np.abs(conv - ref)) / np.max(np.abs(U)) < 1e-9
     mesh.close()
+
+
+def test_subpb2_async_graph_replays(pkg):
+    """The solve without host synchronisation: eager it gives the
+    synchronous solve's result bit for bit and the same iteration count (in
+    a device int); recorded as a graph and replayed on new U (written
+    between replays, with other device work in between) every replay equals
+    the synchronous solve of that U; a limit below the count stops there."""
+    from aldic_amd import api
+    coords, el, bnd = quadtree_mesh(8, 6, 4.0, refine=[(2, 2), (3, 2), (5, 4)])
+    n = len(coords)
+    ctx = pkg.default_context()
+    mesh = pkg.FEMesh(coords, el, bnd, None, 2)
+    DA = api.DeviceArray
+    U, F = _fields(coords, 21)
+    dU = DA.from_host(ctx, U, np.float64)
+    dF = DA.from_host(ctx, F, np.float64)
+    dS = DA(ctx, 2 * n, np.float64)
+    dA = DA(ctx, 2 * n, np.float64)
+    dIt = DA(ctx, 1, np.int32)
+    args = (2e-3, 1e-3)
+    it = mesh.solve_dev(ctx, *args, dU.ptr, dF.ptr, dS.ptr, 0.0, 8.0)
+    mesh.solve_dev_async(ctx, *args, dU.ptr, dF.ptr, dA.ptr, 0.0, 8.0, max_iter=64,
+                         d_iterations=dIt.ptr)
+    assert 3 < it < 64 and dIt.to_host()[0] == it
+    assert np.array_equal(dA.to_host(), dS.to_host())
+    dG = DA(ctx, 2 * n, np.float64)
+    g = ctx.capture(lambda: mesh.solve_dev_async(ctx, *args, dU.ptr, dF.ptr, dG.ptr, 0.0, 8.0,
+                                                 max_iter=64, d_iterations=dIt.ptr))
+    other = DA(ctx, 2 * n, np.float64)
+    rng = np.random.default_rng(4)
+    for rep in range(3):
+        Ur = U + rng.normal(scale=0.05, size=2 * n)
+        pkg.lib().aldic_memcpy_htod(ctx.ptr, dU.ptr, Ur.ctypes.data, Ur.nbytes)
+        g.launch(ctx)
+        ctx.synchronize()
+        want, itw = mesh.solve(*args, Ur, F, 0.0, 8.0, return_iterations=True)
+        assert np.array_equal(dG.to_host(), want), rep
+        assert dIt.to_host()[0] == itw
+        # eager work on the same mesh between replays
+        mesh.solve_dev(ctx, *args, dU.ptr, dF.ptr, other.ptr, 0.0, 8.0)
+    g.close()
+    mesh.solve_dev_async(ctx, *args, dU.ptr, dF.ptr, dA.ptr, 0.0, 8.0, max_iter=5,
+                         d_iterations=dIt.ptr)
+    assert dIt.to_host()[0] == 5
+    with pytest.raises(pkg.AldicError):
+        mesh.solve_dev_async(ctx, *args, dU.ptr, dF.ptr, dA.ptr, 0.0, 8.0, max_iter=0)
+    mesh.close()
