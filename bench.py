@@ -383,18 +383,31 @@ def main():
                  F=DA(ctx, 4 * n, np.float32), conv=DA(ctx, n, np.int32),
                  z=DA(ctx, n, np.float32)) for _ in range(2)]
 
-    def step_search(k):
-        b = sets[k % 2]
-        ctx.wait(ctx2, k % 2)  # the IC-GN of step k-2 has released b
+    def search_stage(sb):  # on ctx
         api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst,
                                [r, r, r, r], *[o.ptr for o in s_out], *[p.ptr for p in s_pk])
         api.grid_to_nodes_dev(ctx, s_out[2].ptr, s_out[3].ptr, nX, nY, x0, x0, sst, d_coords.ptr,
-                              n, b["U0"].ptr)
-        ctx.record(2 + k % 2)
-        ctx2.wait(ctx, 2 + k % 2)
+                              n, sets[sb]["U0"].ptr)
+
+    def icgn_stage(sb):  # on ctx2
+        b = sets[sb]
         api.local_icgn_dev(im, plan2, d_coords.ptr, b["U0"].ptr, d_ws.ptr, 1e-3, b["U"].ptr,
                            b["F"].ptr, None, b["conv"].ptr, None, b["z"].ptr, ctx=ctx2)
-        ctx2.record(k % 2)
+
+    def step_search(k, graphs=None):
+        sb = k % 2
+        ctx.wait(ctx2, sb)  # the IC-GN of step k-2 has released set sb
+        if graphs:
+            graphs[0][sb].launch(ctx)
+        else:
+            search_stage(sb)
+        ctx.record(2 + sb)
+        ctx2.wait(ctx, 2 + sb)
+        if graphs:
+            graphs[1][sb].launch(ctx2)
+        else:
+            icgn_stage(sb)
+        ctx2.record(sb)
 
     # plan-time autotune of the IC-GN launch shapes on this node set (not timed)
     api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst, [r, r, r, r],
@@ -442,9 +455,22 @@ def main():
     if args.init in ("both", "search"):
         dt, stages, last = timed([ctx, ctx2], api, step_search, args.steps, args.warmup)
         progress(f"search-init step {dt * 1e3:.3f} ms")
+        # the same pipeline with each stage replayed from a HIP graph (one
+        # per stage and output set)
+        sgraphs = [[ctx.capture(lambda sb=sb: search_stage(sb)) for sb in (0, 1)],
+                   [ctx2.capture(lambda sb=sb: icgn_stage(sb)) for sb in (0, 1)]]
+        dtg, _, lastg = timed([ctx, ctx2], api, lambda k: step_search(k, sgraphs), args.steps,
+                              args.warmup)
+        progress(f"search-init step, graphs {dtg * 1e3:.3f} ms")
+        sgraph_ms = round(dtg * 1e3, 4)
+        if dtg < dt:
+            dt, last = dtg, lastg
+        for gs in sgraphs:
+            for gr in gs:
+                gr.close()
         b = sets[last % 2]
         record("search", dt, stages, b["U"].to_host().astype(np.float64), b["conv"].to_host(),
-               b["z"].to_host())
+               b["z"].to_host(), dict(graph_ms=sgraph_ms))
 
     if args.init in ("both", "learned"):
         # ---- train the network on the device (not timed) ----
@@ -539,11 +565,11 @@ def main():
                "rms_ok": bool(d["rms"] <= 0.01), "by_subset_size": d["by_size"],
                "stage_ms_per_step": {k: round(v[0] / args.steps, 4)
                                      for k, v in d["stages"].items()}}
+        out["graph_ms_per_step"] = d["graph_ms"]
         if name == "learned":
             out["fallback"] = d["fallback"]
             out["fallback_rate"] = round(d["fallback"] / n, 5)
             out["prior_search_points"] = d["prior_points"]
-            out["graph_ms_per_step"] = d["graph_ms"]
             out["first_pass_65px"] = d["first_pass"]
             out["train"] = d["train"]
         return out
