@@ -84,6 +84,10 @@ struct aldic_plan {
   int full[5] = {0, 0, 0, 0, 0};
   int* defer_cnt = nullptr;
   int* defer_list = nullptr;
+  // hand-on list of the classes that run on the side stream (solved there
+  // while the largest class still runs on the context stream)
+  int* defer_cnt2 = nullptr;
+  int* defer_list2 = nullptr;
   // per-class lists of a node subset (IC-GN of listed nodes): capacity cnt[c]
   int* sub_lists[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int* sub_cnt = nullptr;  // 5 counts

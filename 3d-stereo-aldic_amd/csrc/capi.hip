@@ -108,6 +108,8 @@ int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, 
   }
   HIP_TRY(hipMalloc(&p->defer_cnt, sizeof(int)));
   HIP_TRY(hipMalloc(&p->defer_list, sizeof(int) * std::max<int64_t>(n, 1)));
+  HIP_TRY(hipMalloc(&p->defer_cnt2, sizeof(int)));
+  HIP_TRY(hipMalloc(&p->defer_list2, sizeof(int) * std::max<int64_t>(n, 1)));
   for (int c = 0; c < 5; c++)
     if (p->cnt[c] > 0) HIP_TRY(hipMalloc(&p->sub_lists[c], sizeof(int) * p->cnt[c]));
   HIP_TRY(hipMalloc(&p->sub_cnt, sizeof(int) * 5));
@@ -122,12 +124,17 @@ void free_plan(aldic_plan* p) {
   if (p->sub_cnt) (void)hipFree(p->sub_cnt);
   if (p->defer_cnt) (void)hipFree(p->defer_cnt);
   if (p->defer_list) (void)hipFree(p->defer_list);
+  if (p->defer_cnt2) (void)hipFree(p->defer_cnt2);
+  if (p->defer_list2) (void)hipFree(p->defer_list2);
 }
 
 // Launch sequence shared by IC-GN (mode 0) and Subpb1 (mode 1).
-// The size classes of one solve are independent until the hand-on pass:
-// the largest class runs on the context stream, the others on the side
-// stream at the same time, joined before the handed-on subsets.
+// The size classes of one solve are independent: the largest class runs on
+// the context stream, the others on the side stream at the same time.  Each
+// stream hands its subsets on to a list of its own and runs the workgroup
+// kernel over it, so the side stream's handed-on subsets (the small classes
+// near masked regions) are solved while the largest class still runs; the
+// side stream joins the context stream at the end.
 int side_stream(aldic_ctx* ctx) {
   if (ctx->side) return ALDIC_OK;
   HIP_TRY(hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
@@ -147,17 +154,21 @@ int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
   hipStream_t s = ctx->stream;
   RC_TRY(side_stream(ctx));
   a.n = (int)p->n;
-  a.defer_cnt = p->defer_cnt;
-  a.defer_list = p->defer_list;
   const int base = mode == 0 ? ALDIC_STAGE_ICGN_C0 : ALDIC_STAGE_SUBPB1_C0;
   zero_ints<<<1, 64, 0, s>>>(p->defer_cnt, 1);
+  zero_ints<<<1, 64, 0, s>>>(p->defer_cnt2, 1);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(ctx->fork_ev, s));
   HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->fork_ev, 0));
   const int big = largest_class(p->cnt);
+  int64_t side_total = 0;
   for (int c = 0; c < 5; c++) {
     if (p->cnt[c] == 0) continue;
-    hipStream_t on = c == big ? s : ctx->side;
+    const bool main = c == big;
+    hipStream_t on = main ? s : ctx->side;
+    if (!main) side_total += p->cnt[c];
+    a.defer_cnt = main ? p->defer_cnt : p->defer_cnt2;
+    a.defer_list = main ? p->defer_list : p->defer_list2;
     StageTimer t(ctx, base + c, on);
     if (c < 4) {
       a.list = p->lists[c];
@@ -169,12 +180,19 @@ int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
       HIP_TRY(aldic::launch_generic(mode, a, p->lists[4], nullptr, p->cnt[4], grid, on));
     }
   }
+  // subsets handed on by the fast kernels, per stream; the grid-stride loop
+  // reads the count on the device, so no host round trip is needed.  The
+  // side stream's pass runs beside the largest class: a smaller grid keeps
+  // its idle blocks (64 KB of LDS each) from taking that class's slots.
+  if (side_total > 0) {
+    StageTimer t(ctx, base + 5, ctx->side);
+    const int g2 = (int)std::min<int64_t>(side_total, 256);
+    HIP_TRY(aldic::launch_generic(mode, a, p->defer_list2, p->defer_cnt2, 0, g2, ctx->side));
+  }
   HIP_TRY(hipEventRecord(ctx->join_ev, ctx->side));
   HIP_TRY(hipStreamWaitEvent(s, ctx->join_ev, 0));
-  // subsets handed on by the fast kernels; the grid-stride loop reads the
-  // count on the device, so no host round trip is needed
-  const int grid = (int)std::min<int64_t>(std::max<int64_t>(p->n, 1), 1024);
   StageTimer t(ctx, base + 5);
+  const int grid = (int)std::min<int64_t>(std::max<int64_t>(p->cnt[big], 1), 1024);
   HIP_TRY(aldic::launch_generic(mode, a, p->defer_list, p->defer_cnt, 0, grid, s));
   return ALDIC_OK;
 }
