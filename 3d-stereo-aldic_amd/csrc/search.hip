@@ -168,6 +168,20 @@ __global__ __launch_bounds__(256) void search_prep_kernel(SearchArgs A) {
 // ---------------------------------------------------------------------------
 // tiles: one 256-thread workgroup per (point, lag tile)
 // ---------------------------------------------------------------------------
+// NT < RB template taps of the last, partial block of a template column:
+// the FMAs of the full block with the zero taps past s left out (the same
+// sums: a zero tap only adds +0).
+template <int NT>
+__device__ __forceinline__ void tail_fmas(const float* Ta, const float (&win)[2 * RB],
+                                          float (&num)[RB]) {
+#pragma unroll
+  for (int k = 0; k < NT; k++) {
+    const float tv = Ta[k];
+#pragma unroll
+    for (int r = 0; r < RB; r++) num[r] = fmaf(tv, win[k + r], num[r]);
+  }
+}
+
 __global__ __launch_bounds__(256) void search_tile_kernel(SearchArgs A,
                                                           const float* __restrict__ tmpl,
                                                           const float* __restrict__ g) {
@@ -229,23 +243,42 @@ __global__ __launch_bounds__(256) void search_tile_kernel(SearchArgs A,
       float num[RB];
 #pragma unroll
       for (int r = 0; r < RB; r++) num[r] = 0.0f;
-      for (int a = 0; a < s; a++) {
+      // a wave whose dim-2 lags all lie past the lag window (the last tile
+      // column) has nothing to sum; it still meets the barrier below
+      const bool busy = q0 + w * RB < K2;
+      const int sfull = s - s % RB, stail = s % RB;
+      for (int a = 0; a < s && busy; a++) {
         const float* colp = sub + (w * RB) * SW + lane + a;
         const float* Ta = T + a * A.sp;
         float win[2 * RB];
 #pragma unroll
         for (int k = 0; k < RB; k++) win[k] = colp[k * SW];
-        for (int b0 = 0; b0 < s; b0 += RB) {
+        for (int b0 = 0; b0 < sfull; b0 += RB) {
 #pragma unroll
           for (int k = 0; k < RB; k++) win[RB + k] = colp[(b0 + RB + k) * SW];
 #pragma unroll
           for (int k = 0; k < RB; k++) {
-            const float tv = Ta[b0 + k];  // uniform: scalar load (zero past s)
+            const float tv = Ta[b0 + k];  // uniform: scalar load
 #pragma unroll
             for (int r = 0; r < RB; r++) num[r] = fmaf(tv, win[k + r], num[r]);
           }
 #pragma unroll
           for (int k = 0; k < RB; k++) win[k] = win[RB + k];
+        }
+        if (stail) {
+#pragma unroll
+          for (int k = 0; k < RB; k++) win[RB + k] = colp[(sfull + RB + k) * SW];
+          const float* Tt = Ta + sfull;
+          static_assert(RB == 8, "the tail cases cover 1 .. RB - 1 taps");
+          switch (stail) {
+            case 1: tail_fmas<1>(Tt, win, num); break;
+            case 2: tail_fmas<2>(Tt, win, num); break;
+            case 3: tail_fmas<3>(Tt, win, num); break;
+            case 4: tail_fmas<4>(Tt, win, num); break;
+            case 5: tail_fmas<5>(Tt, win, num); break;
+            case 6: tail_fmas<6>(Tt, win, num); break;
+            default: tail_fmas<7>(Tt, win, num); break;
+          }
         }
       }
       __syncthreads();  // CB ready
