@@ -17,10 +17,11 @@ It is written to be independent of the library's C++ (natural.cpp):
   * the hull is a Jarvis march (the library: monotone chain);
   * q's cell is clipped from a box by the bisectors of EVERY site, and the
     area taken from site i is that cell clipped by the bisectors of i and
-    every other site nearby (the library: a growing search radius, then
-    only the natural neighbours);
-  * the mask regions are labelled by a breadth-first flood fill (the
-    library: a depth-first one).
+    every other site nearby (the library: sites visited bucket ring by
+    bucket ring until no further site can cut the cell, then only the
+    natural neighbours);
+  * the mask regions are labelled by a breadth-first flood fill over pixels
+    (the library: union-find over the runs of each column).
 
 Pinned by tests/test_natural.py: linear precision (a linear field is
 reproduced exactly, a property of Sibson's coordinates), symmetric
