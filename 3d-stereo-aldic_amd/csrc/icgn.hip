@@ -537,24 +537,32 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
       ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
       if constexpr (NW > 1) __syncthreads();
-      // the tile's loads go out in groups of TG before their LDS stores, so
-      // a group costs one memory latency (compile-time trip counts)
-      constexpr int TR = (TS * TS + NT - 1) / NT;
+      // thread t < CG * TS owns tile row t % TS and the columns t / TS +
+      // CG k: each load is the launch-uniform column base plus the thread's
+      // fixed offset, each LDS store a fixed offset from the thread's slot,
+      // so the staging needs no per-element index arithmetic.  The loads go
+      // out in groups of TG before their stores (one memory latency a group).
+      constexpr int CG = NT / TS;                // column groups
+      constexpr int KC = (TS + CG - 1) / CG;     // columns per thread
       constexpr int TG = 8;
-      const float* gt = a.g + (int64_t)tx0 + (int64_t)ty0 * M;
+      static_assert(CG >= 1, "a tile row per thread");
+      if (tid < CG * TS) {
+        const int rr = tid % TS, cg = tid / TS;
+        const float* gt = a.g + (int64_t)(tx0 + rr) + (int64_t)(ty0 + cg) * M;
+        float* tt = tile + rr + cg * TS;
 #pragma unroll 1
-      for (int i0 = 0; i0 < TR; i0 += TG) {
-        float tv[TG];
+        for (int k0 = 0; k0 < KC; k0 += TG) {
+          float tv[TG];
 #pragma unroll
-        for (int i = 0; i < TG; i++) {
-          const int e = tid + (i0 + i) * NT;
-          const int rr = e % TS, cc = e / TS;
-          tv[i] = (i0 + i < TR && e < TS * TS) ? gt[rr + (int64_t)cc * M] : 0.0f;
-        }
+          for (int k = 0; k < TG; k++) {
+            const bool in = k0 + k < KC && cg + CG * (k0 + k) < TS;
+            tv[k] = in ? gt[(int64_t)(CG * (k0 + k)) * M] : 0.0f;
+          }
 #pragma unroll
-        for (int i = 0; i < TG; i++) {
-          const int e = tid + (i0 + i) * NT;
-          if (i0 + i < TR && e < TS * TS) tile[e] = tv[i];
+          for (int k = 0; k < TG; k++) {
+            const bool in = k0 + k < KC && cg + CG * (k0 + k) < TS;
+            if (in) tt[CG * (k0 + k) * TS] = tv[k];
+          }
         }
       }
       if constexpr (NW > 1)
