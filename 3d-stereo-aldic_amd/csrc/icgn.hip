@@ -48,9 +48,11 @@ __device__ __forceinline__ bool chol6(const float h[21], float L[21]) {
 #pragma unroll
     for (int k = 0; k < j; k++) d -= L[tri(j, k)] * L[tri(j, k)];
     if (!(d > 0.0f)) return false;
-    const float ljj = sqrtf(d);
+    // hardware square root and reciprocal (1 ulp): no denormal rescaling
+    // or division fix-up sequences in the uniform setup
+    const float ljj = __builtin_amdgcn_sqrtf(d);
     L[tri(j, j)] = ljj;
-    const float inv = 1.0f / ljj;
+    const float inv = __builtin_amdgcn_rcpf(ljj);
 #pragma unroll
     for (int i = j + 1; i < 6; i++) {
       float s = h[tri(i, j)];
@@ -160,7 +162,7 @@ __device__ __forceinline__ void solve2(const float H[4], float b0, float b1, flo
 __device__ __forceinline__ bool ic_update(float P[6], const float d[6]) {
   const float det = (1.0f + d[0]) * (1.0f + d[3]) - d[1] * d[2];
   if (det == 0.0f) return false;
-  const float id = 1.0f / det;
+  const float id = __builtin_amdgcn_rcpf(det);
   const float t1 = (-d[0] - d[0] * d[3] + d[1] * d[2]) * id;
   const float t2 = -d[1] * id;
   const float t3 = -d[2] * id;
@@ -444,7 +446,6 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   // ---- Hessian ----
   float h21[21];
   float L[21];
-  bool spd = true;
   float H2[4] = {0, 0, 0, 0};
   if (MODE == MODE_ICGN) {
 #pragma unroll
@@ -462,7 +463,16 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     group_sum<NW, 21>(h21, red);
 #pragma unroll
     for (int k = 0; k < 21; k++) h21[k] = uni(h21[k]);
-    spd = chol6(h21, L);
+    const bool spd = chol6(h21, L);
+    if (wave_any(!spd)) {
+      // an indefinite H takes MATLAB's LU fallback: the generic kernel
+      // carries it, and the fast kernel keeps no LU factors in registers
+      if (tid == 0) {
+        const int q = atomicAdd(a.defer_cnt, 1);
+        a.defer_list[q] = node;
+      }
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < 21; k++) L[k] = uni(L[k]);
   } else {
@@ -692,17 +702,14 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     normNew = bnorm;
     normAbs = bnorm;
     if (step == 1) normInit = bnorm;
-    normNew = (normInit > tol) ? normNew / normInit : 0.0f;
+    normNew = (normInit > tol) ? normNew * __builtin_amdgcn_rcpf(normInit) : 0.0f;
     if (normNew < tol || normAbs < tol) break;
     float dP[6];
     if (MODE == MODE_ICGN) {
       float nb[6];
 #pragma unroll
       for (int k = 0; k < 6; k++) nb[k] = -bvec[k];
-      if (spd)
-        chol6_solve(L, nb, dP);
-      else
-        lu6_solve(h21, nb, dP);
+      chol6_solve(L, nb, dP);
     } else {
       const float dmax = fmaxf(H2[0], H2[3]);
       const float tH[4] = {H2[0] + 1e-3f * dmax, H2[1], H2[2], H2[3] + 1e-3f * dmax};
