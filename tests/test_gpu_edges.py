@@ -86,3 +86,45 @@ def test_largest_subsets(pkg, orc):
     assert conv[2] == -1 and np.all(np.isnan(U[4:]))
     assert convo[2] >= 1  # the FP64 oracle has no such limit
     im.close()
+
+
+def _one_dimensional_band(f, g, cols=60, shift=0.4):
+    """Columns below `cols` replaced by a texture that varies along dim 1
+    only (g shifted by `shift` px): subsets there have no y gradient."""
+    M = f.shape[0]
+    x = np.arange(1, M + 1, dtype=np.float64)
+
+    def prof(t):
+        return np.sin(0.7 * t) + 0.5 * np.sin(0.23 * t + 1.0)
+
+    f = np.array(f)
+    g = np.array(g)
+    f[:, :cols] = prof(x)[:, None]
+    g[:, :cols] = prof(x - shift)[:, None]
+    return np.asfortranarray(f), np.asfortranarray(g)
+
+
+def test_singular_hessian_subsets(pkg, orc):
+    """Subsets whose texture varies along one axis only have a numerically
+    singular H: its y entries are the gradient filter's rounding residue
+    (about 1e-29).  The FP64 oracle and the FP32 fast kernels both factor it,
+    take a step that leaves the image and report status 101; a textured
+    subset in the same launch matches the oracle.  (An H whose Cholesky
+    fails in FP32 goes to the workgroup kernel's LU; no natural input here
+    reaches that branch.)"""
+    f0, g0, _, _ = small_pair(shift=(0.4, -0.3))
+    f, g = _one_dimensional_band(f0, g0)
+    im = pkg.Images(f, g)
+    oim = orc.Images(f, g)
+    coords = np.array([[60.0, 30.0], [80.0, 28.0], [80.0, 100.0]])
+    U0 = np.zeros(6)
+    for ws in (30, 16):  # the 33 px and 17 px fast-kernel classes
+        U, F, H, conv, hole, z = pkg.local_icgn_raw(im, coords, U0, ws, 1e-3)
+        Uo, Fo, Ho, convo, holeo, zo = orc.local_icgn(oim, coords, U0, np.full(3, ws, np.int32),
+                                                      1e-3)
+        failed = lambda c: (c < 1) | (c > 100)
+        assert failed(conv[:2]).all() and failed(convo[:2]).all(), (ws, conv, convo)
+        assert (conv[:2] == 101).all() and (convo[:2] == 101).all(), (ws, conv, convo)
+        assert 1 <= conv[2] <= 100 and 1 <= convo[2] <= 100
+        assert np.abs(U[4:6] - Uo[4:6]).max() <= 1e-3
+    im.close()
