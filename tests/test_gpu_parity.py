@@ -12,6 +12,9 @@ Tolerances (floating point, the device computes in FP32 except where noted):
     FP32 round-off of b near convergence); deformation gradients
     |dF| <= 1e-5 where both stop after the same number of evaluations and
     <= 1e-3 (one converged Gauss-Newton step) where they stop one apart;
+    the Hessian H within 1e-5 of the node's largest entry on every subset
+    both solve (FP32 sums, about 1e-7 measured), and zero in both on the
+    -1 and 102 statuses;
   * the drivers' NaN fill (host code): 1e-9 against the oracle's from the
     same pre-fill U;
   * integer search: integer ZNCC peaks bit-exact; sub-pixel u, v within
@@ -58,6 +61,19 @@ def _cmp_icgn(got, want, conv_tol_frac=0.03, du_tol=1e-3, df_tol=1e-5, ws=None):
     same = okb & ~diff
     dF = np.abs(F - Fo).reshape(-1, 4)
     assert dF[same].max(initial=0) <= df_tol, dF[same].max()
+    # H (the Gauss-Newton Hessian the reference returns per node, built once
+    # before the iterations): FP32 sums over the subset's pixels against FP64,
+    # relative to the node's largest entry, on every subset both solved
+    both = (conv >= 1) & (conv <= 101) & (convo >= 1) & (convo <= 101)
+    Hg, Hw = np.asarray(H).reshape(len(conv), 21), np.asarray(Ho).reshape(len(conv), 21)
+    scale = np.maximum(np.abs(Hw[both]).max(axis=1, initial=0), 1e-300)[:, None]
+    dH = np.abs(Hg[both] - Hw[both]) / scale
+    print(f"  max relative |dH| {dH.max(initial=0):.2e} on {both.sum()} subsets")
+    assert dH.max(initial=0) <= 1e-5, dH.max()
+    # the failure statuses carry a zero H in both
+    for code in (-1, 102):
+        sel = conv == code
+        assert np.all(Hg[sel] == 0) and np.all(Hw[sel] == 0), code
     assert dF[okb & diff].max(initial=0) <= 1e-3, dF[okb & diff].max()
     # failed subsets report NaN exactly where the oracle does
     assert np.array_equal(np.isnan(U), np.isnan(Uo))
