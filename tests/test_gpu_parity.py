@@ -14,7 +14,8 @@ Tolerances (floating point, the device computes in FP32 except where noted):
     <= 1e-3 (one converged Gauss-Newton step) where they stop one apart;
     the Hessian H within 1e-5 of the node's largest entry on every subset
     both solve (FP32 sums, about 1e-7 measured), and zero in both on the
-    -1 and 102 statuses;
+    -1 and 102 statuses; the last evaluation's ZNCC within 1e-5 at equal
+    iteration counts;
   * the drivers' NaN fill (host code): 1e-9 against the oracle's from the
     same pre-fill U;
   * integer search: integer ZNCC peaks bit-exact; sub-pixel u, v within
@@ -70,6 +71,11 @@ def _cmp_icgn(got, want, conv_tol_frac=0.03, du_tol=1e-3, df_tol=1e-5, ws=None):
     dH = np.abs(Hg[both] - Hw[both]) / scale
     print(f"  max relative |dH| {dH.max(initial=0):.2e} on {both.sum()} subsets")
     assert dH.max(initial=0) <= 1e-5, dH.max()
+    # ZNCC of the last evaluation (1 - sum r^2 / 2) where both stop after the
+    # same number of evaluations
+    dz = np.abs(np.asarray(z) - np.asarray(zo))[same]
+    print(f"  max |dZNCC| {dz.max(initial=0):.2e} on {same.sum()} subsets")
+    assert dz.max(initial=0) <= 1e-5, dz.max()
     # the failure statuses carry a zero H in both
     for code in (-1, 102):
         sel = conv == code
