@@ -209,6 +209,31 @@ def test_icgn_zero_motion_and_translation(orc):
         assert np.all(np.linalg.eigvalsh(Hm) > 0)
 
 
+def test_icgn_hessian_is_the_packed_steepest_descent_gram(orc):
+    """H per node is sum over the subset of sd sd^T with sd = (dx fx, dx fy,
+    dy fx, dy fy, fx, fy) (dx, dy: the pixel's offsets from the node; fx, fy:
+    the gradient planes, which start 3 px into the image), packed as the
+    reference's HGlobal: the lower triangle column by column
+    (funICGNQuadtree.m:91-103, :451).  Built here with numpy from the
+    oracle's own gradient planes; a reordered sd or a shifted gradient index
+    changes it."""
+    f, g, _, _ = small_pair(shift=(0.4, -0.3))
+    im = orc.Images(f, g)
+    for (x0, y0, ws) in ((70, 64, 20), (90, 80, 30)):
+        U, F, H, conv, hole, z = orc.local_icgn(im, np.array([[float(x0), float(y0)]]),
+                                                np.zeros(2), np.array([ws], np.int32), 1e-3)
+        h = ws // 2
+        ii = np.arange(x0 - h, x0 + h + 1)
+        jj = np.arange(y0 - h, y0 + h + 1)
+        DX, DY = np.meshgrid(ii - x0, jj - y0, indexing="ij")
+        fx = im.DfDx[np.ix_(ii - 4, jj - 4)]
+        fy = im.DfDy[np.ix_(ii - 4, jj - 4)]
+        sd = np.stack([DX * fx, DX * fy, DY * fx, DY * fy, fx, fy]).reshape(6, -1)
+        Hf = sd @ sd.T
+        packed = np.concatenate([Hf[c:, c] for c in range(6)])
+        assert np.allclose(H[0], packed, rtol=1e-10, atol=0), np.abs(H[0] - packed).max()
+
+
 def test_icgn_affine_intensity_invariance(orc):
     f, g, _, _ = small_pair(shift=(1.3, -0.7))
     coords = np.asfortranarray(np.array([[60.0, 60.0], [90.0, 80.0]]))
