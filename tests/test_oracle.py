@@ -234,6 +234,33 @@ def test_icgn_hessian_is_the_packed_steepest_descent_gram(orc):
         assert np.allclose(H[0], packed, rtol=1e-10, atol=0), np.abs(H[0] - packed).max()
 
 
+def test_icgn_zncc_is_the_correlation_at_the_returned_warp(orc):
+    """The ZNCC returned per node is the zero-normalised correlation of the
+    reference subset with the deformed image interpolated at the returned
+    warp, x' = x0 + (1 + F1) dx + F3 dy + u, y' = y0 + F2 dx + (1 + F4) dy + v
+    (F = [du/dx dv/dx du/dy dv/dy], funICGNQuadtree.m:149-152), i.e. at the
+    last evaluation, which the convergence test ends without an update.
+    Computed here with numpy and ba_interp2_spline (whose first coordinate
+    runs along dim 2); a reordered F or an update after the test changes it."""
+    f, g, _, _ = small_pair(shift=(0.4, -0.3), strain=0.002)
+    im = orc.Images(f, g)
+    for (x0, y0, ws) in ((70, 64, 20), (90, 80, 30)):
+        U, F, H, conv, hole, z = orc.local_icgn(im, np.array([[float(x0), float(y0)]]),
+                                                np.zeros(2), np.array([ws], np.int32), 1e-3)
+        assert 1 < conv[0] <= 100
+        h = ws // 2
+        ii = np.arange(x0 - h, x0 + h + 1)
+        jj = np.arange(y0 - h, y0 + h + 1)
+        DX, DY = np.meshgrid((ii - x0).astype(float), (jj - y0).astype(float), indexing="ij")
+        fs = f[np.ix_(ii - 1, jj - 1)]
+        X = x0 + DX + F[0] * DX + F[2] * DY + U[0]
+        Y = y0 + DY + F[1] * DX + F[3] * DY + U[1]
+        gs = orc.interp_spline(g, Y, X)
+        fc, gc = fs - fs.mean(), gs - gs.mean()
+        zz = (fc * gc).sum() / np.sqrt((fc ** 2).sum() * (gc ** 2).sum())
+        assert abs(zz - z[0]) <= 1e-12, (zz, z[0])
+
+
 def test_icgn_affine_intensity_invariance(orc):
     f, g, _, _ = small_pair(shift=(1.3, -0.7))
     coords = np.asfortranarray(np.array([[60.0, 60.0], [90.0, 80.0]]))
