@@ -923,6 +923,13 @@ class DeviceArray:
         check(lib().aldic_memcpy_htod(ctx.ptr, d.ptr, a.ctypes.data, a.nbytes))
         return d
 
+    def upload(self, a):
+        """Copy host data (count elements, column-major flattened) in."""
+        a = np.ascontiguousarray(np.asarray(a, dtype=self.dtype).reshape(-1, order="F"))
+        if a.size != self.count:
+            raise ValueError("upload: size differs from the device array")
+        check(lib().aldic_memcpy_htod(self.ctx.ptr, self.ptr, a.ctypes.data, a.nbytes))
+
     def to_host(self):
         out = np.empty(self.count, dtype=self.dtype)
         check(lib().aldic_memcpy_dtoh(self.ctx.ptr, out.ctypes.data, self.ptr, out.nbytes))

@@ -88,9 +88,11 @@ struct aldic_plan {
   // while the largest class still runs on the context stream)
   int* defer_cnt2 = nullptr;
   int* defer_list2 = nullptr;
-  // per-class lists of a node subset (IC-GN of listed nodes): capacity cnt[c]
+  // per-class lists of a node subset (IC-GN of listed nodes): capacity
+  // cnt[c] for c < 4, n for list 4 (the workgroup kernel, which also takes
+  // the overflow of a full class list)
   int* sub_lists[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  int* sub_cnt = nullptr;  // 5 counts
+  int* sub_cnt = nullptr;  // 5 counts + the number of entries dropped (list 4 full)
   // launch shape per size class and solver (0 IC-GN, 1 Subpb1)
   int var[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 };

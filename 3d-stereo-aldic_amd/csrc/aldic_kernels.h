@@ -48,14 +48,20 @@ int size_class(int S1, int S2);
 int class_side(int cls);  // the largest subset side of a fast-kernel class
 hipError_t launch_fast(int mode, int cls, int var, const SolveArgs& a, hipStream_t s);
 int num_variants();
+// The workgroup kernel over list[0 .. n): n = fixed_count, or the device
+// count *count (when count is not null) clamped to the capacity fixed_count.
 hipError_t launch_generic(int mode, const SolveArgs& a, const int* list, const int* count,
                           int fixed_count, int grid, hipStream_t s);
 int generic_max_pixels();
-// Sort the nodes list[0 .. *count) into per-size-class lists (appended at
-// counts[c]; counts must start at zero).
+// Sort the nodes list[0 .. min(*count, max_count)) into per-size-class
+// lists (appended at counts[c]; counts must start at zero).  List c holds
+// at most cap[c] entries: a node that finds its class list full goes to
+// list 4 (the workgroup kernel, any size), and one that finds list 4 full
+// is counted in counts[5] and dropped.  Readers clamp counts[c] to cap[c].
 struct ClassLists {
   int* list[5];
-  int* counts;
+  int cap[5];
+  int* counts;  // 6 ints
 };
 hipError_t launch_bucket(const SolveArgs& a, const int* list, const int* count, int max_count,
                          ClassLists out, hipStream_t s);

@@ -110,9 +110,13 @@ int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, 
   HIP_TRY(hipMalloc(&p->defer_list, sizeof(int) * std::max<int64_t>(n, 1)));
   HIP_TRY(hipMalloc(&p->defer_cnt2, sizeof(int)));
   HIP_TRY(hipMalloc(&p->defer_list2, sizeof(int) * std::max<int64_t>(n, 1)));
-  for (int c = 0; c < 5; c++)
+  // the per-class lists of a node subset hold at most the class's plan
+  // count; list 4 (the workgroup kernel) holds every node, so that a listed
+  // node whose class list is full still has a place
+  for (int c = 0; c < 4; c++)
     if (p->cnt[c] > 0) HIP_TRY(hipMalloc(&p->sub_lists[c], sizeof(int) * p->cnt[c]));
-  HIP_TRY(hipMalloc(&p->sub_cnt, sizeof(int) * 5));
+  HIP_TRY(hipMalloc(&p->sub_lists[4], sizeof(int) * std::max<int64_t>(n, 1)));
+  HIP_TRY(hipMalloc(&p->sub_cnt, sizeof(int) * 6));
   return ALDIC_OK;
 }
 
@@ -187,13 +191,14 @@ int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
   if (side_total > 0) {
     StageTimer t(ctx, base + 5, ctx->side);
     const int g2 = (int)std::min<int64_t>(side_total, 256);
-    HIP_TRY(aldic::launch_generic(mode, a, p->defer_list2, p->defer_cnt2, 0, g2, ctx->side));
+    HIP_TRY(aldic::launch_generic(mode, a, p->defer_list2, p->defer_cnt2, (int)p->n, g2,
+                                  ctx->side));
   }
   HIP_TRY(hipEventRecord(ctx->join_ev, ctx->side));
   HIP_TRY(hipStreamWaitEvent(s, ctx->join_ev, 0));
   StageTimer t(ctx, base + 5);
   const int grid = (int)std::min<int64_t>(std::max<int64_t>(p->cnt[big], 1), 1024);
-  HIP_TRY(aldic::launch_generic(mode, a, p->defer_list, p->defer_cnt, 0, grid, s));
+  HIP_TRY(aldic::launch_generic(mode, a, p->defer_list, p->defer_cnt, (int)p->n, grid, s));
   return ALDIC_OK;
 }
 
@@ -884,7 +889,10 @@ extern "C" int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im,
   a.defer_list = plan->defer_list;
   StageTimer t(ctx, ALDIC_STAGE_FALLBACK);
   aldic::ClassLists cl;
-  for (int c = 0; c < 5; c++) cl.list[c] = plan->sub_lists[c];
+  for (int c = 0; c < 5; c++) {
+    cl.list[c] = plan->sub_lists[c];
+    cl.cap[c] = c < 4 ? plan->cnt[c] : (int)plan->n;
+  }
   cl.counts = plan->sub_cnt;
   HIP_TRY(aldic::launch_bucket(a, d_list, d_count, (int)plan->n, cl, s));
   zero_ints<<<1, 64, 0, s>>>(plan->defer_cnt, 1);
@@ -900,11 +908,12 @@ extern "C" int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im,
     HIP_TRY(aldic::launch_fast(0, c, plan->var[0][c], a, s));
   }
   a.dcnt = nullptr;
-  if (plan->cnt[4] > 0)
-    HIP_TRY(aldic::launch_generic(0, a, plan->sub_lists[4], plan->sub_cnt + 4, 0,
-                                  std::min(plan->cnt[4], 2048), s));
+  // list 4: the plan's > 65 px subsets and any node whose class list was
+  // full (the count is read on the device, clamped to the capacity n)
+  HIP_TRY(aldic::launch_generic(0, a, plan->sub_lists[4], plan->sub_cnt + 4, (int)plan->n,
+                                std::max(1, std::min(plan->cnt[4], 2048)), s));
   const int grid = (int)std::min<int64_t>(std::max<int64_t>(plan->n, 1), 1024);
-  HIP_TRY(aldic::launch_generic(0, a, plan->defer_list, plan->defer_cnt, 0, grid, s));
+  HIP_TRY(aldic::launch_generic(0, a, plan->defer_list, plan->defer_cnt, (int)plan->n, grid, s));
   return ALDIC_OK;
 }
 

@@ -310,6 +310,15 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   const int hx = FS > 0 ? (FS - 1) / 2 : wx / 2, hy = FS > 0 ? (FS - 1) / 2 : wy / 2;
   const int S1 = FS > 0 ? FS : wx + 1, S2 = FS > 0 ? FS : wy + 1;
   const int NS = S1 * S2;
+  if (FS == 0 && (NS > NT * PPL || S1 + 8 > TS || S2 + 8 > TS)) {
+    // a window larger than this launch shape holds (a winsize array that
+    // is not the one the plan was built from): the generic kernel takes it
+    if (tid == 0) {
+      const int q = atomicAdd(a.defer_cnt, 1);
+      a.defer_list[q] = node;
+    }
+    return;
+  }
 
   float P[6];
   if (MODE == MODE_ICGN) {
@@ -390,8 +399,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     // >= 40% masked: status 102, P stays P0, H = 0 (:440-444)
     if (tid == 0) {
       if (MODE == MODE_ICGN) {
-        float Pz[6] = {0, 0, 0, 0, P[4], P[5]};
-        write_icgn_noh(a, node, ST_MASKED, Pz, __builtin_nanf(""));
+        write_icgn_noh(a, node, ST_MASKED, P, __builtin_nanf(""));
       } else {
         write_subpb1(a, node, ST_MASKED, P);
       }
@@ -752,7 +760,8 @@ __global__ __launch_bounds__(fast_block<NW>(), WALK && NW > 1 ? 2 : 1) void fast
   __shared__ float tiles[SUBS][TS * TS + TS + 1];
   __shared__ float red[NW * 32];
   const int sub = threadIdx.x / NT;
-  const int cnt = a.dcnt ? *a.dcnt : a.cnt;
+  // a device count is clamped to the list's capacity a.cnt
+  const int cnt = a.dcnt ? min(*a.dcnt, a.cnt) : a.cnt;
   // uniform per subset group (SUBS == 1 when NW > 1); the group's barriers
   // and the reduction's barrier keep one entry's LDS use apart from the next
   if constexpr (WALK) {
@@ -846,7 +855,7 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
   __shared__ uint8_t stmp[GEN_MAXPIX];  // logical image fed to bwselect
   __shared__ uint8_t sfsel[GEN_MAXPIX]; // Subpb1: f selection after its mask block
   __shared__ float red[4 * 32];
-  const int total = count ? *count : fixed_count;
+  const int total = count ? min(*count, fixed_count) : fixed_count;
   const int M = a.M, N = a.N;
   const float nan = __builtin_nanf("");
   for (int it = blockIdx.x; it < total; it += gridDim.x) {
@@ -1372,9 +1381,9 @@ int generic_max_pixels() { return GEN_MAXPIX; }
 }  // namespace aldic
 
 namespace {
-__global__ void bucket_kernel(SolveArgs a, const int* list, const int* count,
+__global__ void bucket_kernel(SolveArgs a, const int* list, const int* count, int max_count,
                               aldic::ClassLists out) {
-  const int total = *count;
+  const int total = min(*count, max_count);
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
     const int node = list[i];
     int x0, y0, wx, wy;
@@ -1383,7 +1392,16 @@ __global__ void bucket_kernel(SolveArgs a, const int* list, const int* count,
     int c = 4;
     for (int k = 3; k >= 0; k--)
       if (side <= aldic::CLS_SIDE[k]) c = k;
-    out.list[c][atomicAdd(&out.counts[c], 1)] = node;
+    int q = c < 4 ? atomicAdd(&out.counts[c], 1) : out.cap[c];
+    if (q >= out.cap[c]) {  // class list full (or not allocated): workgroup kernel
+      c = 4;
+      q = atomicAdd(&out.counts[4], 1);
+      if (q >= out.cap[4]) {
+        atomicAdd(&out.counts[5], 1);
+        continue;
+      }
+    }
+    out.list[c][q] = node;
   }
 }
 }  // namespace
@@ -1397,9 +1415,9 @@ namespace aldic {
 
 hipError_t launch_bucket(const SolveArgs& a, const int* list, const int* count, int max_count,
                          ClassLists out, hipStream_t s) {
-  zero_ints_kernel<<<1, 64, 0, s>>>(out.counts, 5);
+  zero_ints_kernel<<<1, 64, 0, s>>>(out.counts, 6);
   const int grid = std::max(1, std::min((max_count + 255) / 256, 1024));
-  bucket_kernel<<<grid, 256, 0, s>>>(a, list, count, out);
+  bucket_kernel<<<grid, 256, 0, s>>>(a, list, count, max_count, out);
   return hipGetLastError();
 }
 

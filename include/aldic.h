@@ -195,7 +195,11 @@ int aldic_plan_destroy(aldic_plan* plan);
 
 /* Device-resident LocalICGN: d_coords n x 2 (double), d_U0 2n (float);
  * outputs float/int device arrays as aldic_local_icgn (d_F, d_H21, d_hole,
- * d_zncc may be NULL).  Asynchronous on the context stream. */
+ * d_zncc may be NULL).  Asynchronous on the context stream.  d_winsize
+ * must hold the winsize values the plan was built from (the plan's size
+ * classes are fixed on the host); a node whose device value is larger than
+ * its class holds is solved by the workgroup kernel instead, never with a
+ * clipped window. */
 int aldic_local_icgn_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
                          const double* d_coords, const float* d_U0, const int32_t* d_winsize,
                          double tol, float* d_U, float* d_F, float* d_H21, int32_t* d_convIt,

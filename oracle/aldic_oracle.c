@@ -476,9 +476,22 @@ static int grow_winsize(double A, double B, int w) {
 int orc_icgn_subset(const orc_images* im, int64_t x0, int64_t y0, int winsize, double tol,
                     const double U0[2], double U[2], double F[4], double H21[21],
                     double* zncc) {
-  const int maxIterNum = 100;
+  return orc_icgn_subset_ex(im, x0, y0, winsize, tol, 100, U0, NULL, U, F, H21, zncc);
+}
+
+/* The loop of :135 runs while k <= maxIterNum (k counts evaluations, so at
+ * most maxIterNum + 1 of them; maxit >= 0 is that cap, the reference's is
+ * 100); the status codes 101/102/103 are the reference's (maxIterNum + 1,
+ * + 2, + 3 with maxIterNum = 100) whatever cap is passed.  F0 (NULL =
+ * zeros) starts P at [F0 U0] instead of [0 0 0 0 U0] (:41). */
+int orc_icgn_subset_ex(const orc_images* im, int64_t x0, int64_t y0, int winsize, double tol,
+                       int maxit, const double U0[2], const double F0[4], double U[2],
+                       double F[4], double H21[21], double* zncc) {
+  const int maxIterNum = maxit;
   const int winsize0 = winsize;
   double P[6] = {0, 0, 0, 0, U0[0], U0[1]};
+  if (F0)
+    for (int k = 0; k < 4; k++) P[k] = F0[k];
   double H[36];
   memset(H, 0, sizeof(H));
   for (int k = 0; k < 21; k++) H21[k] = 0.0;
@@ -666,17 +679,17 @@ int orc_icgn_subset(const orc_images* im, int64_t x0, int64_t y0, int winsize, d
       if (!ic_update(P, dP)) break;
     }
     status = step;
-    if (!(normNew < tol || normAbs < tol)) status = maxIterNum + 1;
-    if (isnan(normNew)) status = maxIterNum + 1;
+    if (!(normNew < tol || normAbs < tol)) status = ORC_NOCONV;
+    if (isnan(normNew)) status = ORC_NOCONV;
     {
       double sa = 0.0;
       for (int64_t k = 0; k < NS; k++) sa += fabs(f[k]);
-      if (sa < 1e-6) status = maxIterNum + 3;
+      if (sa < 1e-6) status = ORC_EMPTY;
     }
     *zncc = last_zncc;
   } else {
     memset(H, 0, sizeof(H));
-    status = maxIterNum + 2;
+    status = ORC_MASKED;
   }
   U[0] = P[4];
   U[1] = P[5];
@@ -919,6 +932,15 @@ void orc_local_icgn(const orc_images* im, const double* coords, const double* U0
                     const int32_t* winsize, int winsize_per_node, double tol, int nthreads,
                     double* U, double* F, double* H, int32_t* convIt, uint8_t* markHole,
                     double* zncc) {
+  orc_local_icgn_ex(im, coords, U0, NULL, n, winsize, winsize_per_node, tol, 100, nthreads, U,
+                    F, H, convIt, markHole, zncc);
+}
+
+void orc_local_icgn_ex(const orc_images* im, const double* coords, const double* U0,
+                       const double* F0, int64_t n, const int32_t* winsize,
+                       int winsize_per_node, double tol, int maxit, int nthreads, double* U,
+                       double* F, double* H, int32_t* convIt, uint8_t* markHole,
+                       double* zncc) {
   coef_init();
 #ifdef _OPENMP
   if (nthreads < 1) nthreads = 1;
@@ -948,7 +970,8 @@ void orc_local_icgn(const orc_images* im, const double* coords, const double* U0
     }
     double Ut[2], Ft[4], Ht[21], zt;
     const double u0[2] = {U0[2 * t], U0[2 * t + 1]};
-    const int st = orc_icgn_subset(im, x0, y0, ws, tol, u0, Ut, Ft, Ht, &zt);
+    const int st = orc_icgn_subset_ex(im, x0, y0, ws, tol, maxit, u0, F0 ? F0 + 4 * t : NULL,
+                                      Ut, Ft, Ht, &zt);
     convIt[t] = st;
     U[2 * t] = Ut[0];
     U[2 * t + 1] = Ut[1];

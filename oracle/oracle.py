@@ -63,6 +63,9 @@ def lib():
         L.orc_subpb1_subset.restype = C.c_int
         L.orc_local_icgn.argtypes = [C.POINTER(_Images), _dp, _dp, _i64, _i32, C.c_int,
                                      C.c_double, C.c_int, _dp, _dp, _dp, _i32, _u8, _dp]
+        L.orc_local_icgn_ex.argtypes = [C.POINTER(_Images), _dp, _dp, C.c_void_p, _i64, _i32,
+                                        C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
+                                        _i32, _u8, _dp]
         L.orc_subpb1.argtypes = [C.POINTER(_Images), _dp, _dp, _dp, _dp, _dp, _i64, _i32,
                                  C.c_double, C.c_double, C.c_int, _dp, _i32]
         L.orc_normxcorr2_crop.argtypes = [_dp, _i64, _i64, _dp, _i64, _i64, _dp]
@@ -157,9 +160,12 @@ def subpb1_subset(im: Images, x0, y0, wx, wy, mu, udual, vdual, UOld, FOld, tol)
     return U, st
 
 
-def local_icgn(im: Images, coords, U0, winsize, tol, nthreads=1):
+def local_icgn(im: Images, coords, U0, winsize, tol, nthreads=1, F0=None, max_iter=None):
     """LocalICGNQuadtree's per-node loop (no bad-point rule, no NaN fill).
 
+    F0 (4n, optional) starts each node at P0 = [F0 U0]; max_iter (>= 0)
+    runs the loop while k <= max_iter (None: the reference's 100), i.e. at
+    most max_iter + 1 evaluations.
     Returns U (2n), F (4n), H (n x 21), convIt (n), markHole (n), zncc (n)."""
     coords = _F(coords)
     n = coords.shape[0]
@@ -172,8 +178,17 @@ def local_icgn(im: Images, coords, U0, winsize, tol, nthreads=1):
     conv = np.zeros(n, dtype=np.int32, order="F")
     hole = np.zeros(n, dtype=np.uint8, order="F")
     z = np.zeros(n, order="F")
-    lib().orc_local_icgn(C.byref(im.s), coords, _F(U0), n, ws, per, float(tol), int(nthreads),
-                         U, Fo, H, conv, hole, z)
+    if F0 is None and max_iter is None:
+        lib().orc_local_icgn(C.byref(im.s), coords, _F(U0), n, ws, per, float(tol),
+                             int(nthreads), U, Fo, H, conv, hole, z)
+    else:
+        f0 = None if F0 is None else _F(np.asarray(F0).reshape(-1))
+        if f0 is not None and f0.size != 4 * n:
+            raise ValueError("F0 must hold 4 values per node")
+        lib().orc_local_icgn_ex(C.byref(im.s), coords, _F(U0), None if f0 is None else
+                                f0.ctypes.data, n, ws, per, float(tol),
+                                100 if max_iter is None else int(max_iter),
+                                int(nthreads), U, Fo, H, conv, hole, z)
     return U, Fo, H, conv, hole, z
 
 

@@ -388,3 +388,117 @@ def test_bad_point_rules(pkg):
     bad1, num1 = pkg.subpb1_bad_points(conv)
     thr1 = max(good.mean() + 0.25 * good.std(ddof=1), 10)
     assert set(bad1) == {5, 6, 7} | set(np.flatnonzero(conv > thr1))
+
+
+def _b_at(orc, im, f, g, x0, y0, ws, P):
+    """b = bottomf * sum SD^T r at the warp P = [ux vx uy vy u v] of an
+    unmasked subset (funICGNQuadtree.m:149-152, :365-374), from the image
+    planes with numpy and ba_interp2_spline."""
+    h = ws // 2
+    ii = np.arange(x0 - h, x0 + h + 1)
+    jj = np.arange(y0 - h, y0 + h + 1)
+    DX, DY = np.meshgrid((ii - x0).astype(float), (jj - y0).astype(float), indexing="ij")
+    fs = f[np.ix_(ii - 1, jj - 1)]
+    fx = im.DfDx[np.ix_(ii - 4, jj - 4)]
+    fy = im.DfDy[np.ix_(ii - 4, jj - 4)]
+    X = x0 + (1 + P[0]) * DX + P[2] * DY + P[4]
+    Y = y0 + P[1] * DX + (1 + P[3]) * DY + P[5]
+    gs = orc.interp_spline(g, Y, X)
+    bf = np.sqrt(((fs - fs.mean()) ** 2).sum())
+    bg = np.sqrt(((gs - gs.mean()) ** 2).sum())
+    r = (fs - fs.mean()) / bf - (gs - gs.mean()) / bg
+    sd = np.stack([DX * fx, DX * fy, DY * fx, DY * fy, fx, fy]).reshape(6, -1)
+    return bf * (sd @ r.reshape(-1))
+
+
+def test_icgn_stopping_rule_is_relative_to_the_first_b(orc):
+    """The returned convIt is the first evaluation k with ||b_k|| / ||b_1||
+    < tol or ||b_k|| < tol, and the returned warp is the one that evaluation
+    saw, not updated (funICGNQuadtree.m:376-389).  The warps P_j after j
+    updates come from evaluation-capped runs (a cap of j - 1 leaves P_j);
+    ||b|| at each is computed here with numpy from the image planes.  A
+    rule relative to the previous evaluation's ||b||, a step-size rule
+    ||dP|| < tol, or an update after the test would pick other counts on
+    some of these nodes."""
+    f, g, _, _ = small_pair(shift=(0.45, -0.3), strain=0.003)
+    im = orc.Images(f, g)
+    tol = 1e-3
+    nodes = [(70, 64, 20), (90, 80, 30), (60, 90, 16), (100, 70, 24), (80, 100, 20),
+             (110, 60, 30)]
+    prev_rule_differs = dp_rule_differs = 0
+    for (x0, y0, ws) in nodes:
+        c = np.array([[float(x0), float(y0)]])
+        wsa = np.array([ws], np.int32)
+        U0 = np.array([0.0, 0.0])
+        U, F, H, conv, hole, z = orc.local_icgn(im, c, U0, wsa, tol)
+        s = int(conv[0])
+        assert 2 < s <= 100, s
+        Ps = [np.array([0, 0, 0, 0, 0.0, 0.0])]
+        for j in range(1, s):
+            Uj, Fj, _, cj, _, _ = orc.local_icgn(im, c, U0, wsa, tol, max_iter=j - 1)
+            assert cj[0] == 101, (j, cj[0])  # not converged within j evaluations
+            Ps.append(np.concatenate([Fj, Uj]))
+        nb = np.array([np.linalg.norm(_b_at(orc, im, f, g, x0, y0, ws, P)) for P in Ps])
+        rel = nb / nb[0]
+        stop = [k for k in range(len(nb)) if rel[k] < tol or nb[k] < tol]
+        assert stop and stop[0] == s - 1, (rel, s)  # evaluation s is index s - 1
+        # the returned warp is the last evaluated one (no update after the test)
+        assert np.allclose(np.concatenate([F, U]), Ps[s - 1], atol=1e-12)
+        # what the alternatives would have returned
+        prev = [k for k in range(1, len(nb)) if nb[k] / nb[k - 1] < tol]
+        prev_rule_differs += (not prev) or prev[0] != s - 1
+        dps = [k for k in range(1, len(Ps)) if np.linalg.norm(Ps[k] - Ps[k - 1]) < tol]
+        dp_rule_differs += (not dps) or dps[0] + 1 != s
+    assert prev_rule_differs > 0 and dp_rule_differs > 0
+
+
+def test_icgn_mask_branch_window_growth(orc):
+    """A partly masked subset: the window grows to 2 ceil(sqrt(S^2 / (S^2 -
+    c0)) ws / 2) with c0 masked pixels in the nominal S x S window, and f,
+    DfDx, DfDy keep only the 4-connected mask component of the centre
+    (funICGNQuadtree.m:66-88).  With ImgDef = ImgRef and a zero start the
+    first evaluation converges, so H (HtempPar) is returned; it must equal
+    the Gram matrix of the steepest-descent images over that component of
+    the grown window, built here with numpy and scipy's labelling.  A zero
+    stripe 3 px wide cuts the window, so the pixels beyond it are masked
+    out by the component rule although the mask keeps them."""
+    from scipy.ndimage import label
+    f, _, _, _ = small_pair(seed=21)
+    M, N = f.shape
+    x0, y0, ws = 80, 70, 20
+    S = ws + 1
+    mask = np.ones((M, N))
+    mask[:, y0 + 5 - 1:y0 + 7] = 0  # MATLAB columns y0+5 .. y0+7
+    im = orc.Images(f, f, mask)
+    U, F, H, conv, hole, z = orc.local_icgn(im, np.array([[float(x0), float(y0)]]),
+                                            np.zeros(2), np.array([ws], np.int32), 1e-3)
+    assert conv[0] == 1, conv
+    c0 = 3 * S
+    wn = int(2 * np.ceil(np.sqrt(S * S / (S * S - c0)) * ws / 2))
+    assert wn == 22  # the grown window differs from the nominal one
+    h = wn // 2
+    ii = np.arange(x0 - h, x0 + h + 1)
+    jj = np.arange(y0 - h, y0 + h + 1)
+    m = (f * mask)[np.ix_(ii - 1, jj - 1)] != 0
+    lab, _ = label(m)  # 4-connectivity
+    seed = (wn + 1) // 2  # bwselect(..., floor((ws+1)/2), floor((ws+1)/2), 4)
+    comp = lab == lab[seed - 1, seed - 1]
+    assert comp.sum() < m.sum()  # the stripe cuts some of the mask off
+    DX, DY = np.meshgrid((ii - x0).astype(float), (jj - y0).astype(float), indexing="ij")
+
+    def packed_gram(keep, lo, hi):
+        """HGlobal packing of sum sd sd^T over rows/cols lo:hi of the grown
+        window, gradients kept where `keep` holds."""
+        gx = (im.DfDx[np.ix_(ii - 4, jj - 4)] * keep)[lo:hi, lo:hi]
+        gy = (im.DfDy[np.ix_(ii - 4, jj - 4)] * keep)[lo:hi, lo:hi]
+        dx, dy = DX[lo:hi, lo:hi], DY[lo:hi, lo:hi]
+        a = np.stack([dx * gx, dx * gy, dy * gx, dy * gy, gx, gy]).reshape(6, -1)
+        G = a @ a.T
+        return np.concatenate([G[c:, c] for c in range(6)])
+
+    packed = packed_gram(comp, 0, 2 * h + 1)
+    assert np.allclose(H[0], packed, rtol=1e-10, atol=0), np.abs(H[0] - packed).max()
+    # the nominal 21 x 21 window, or the whole mask instead of the component,
+    # give other Hessians
+    for other in (packed_gram(comp, 1, 2 * h), packed_gram(m, 0, 2 * h + 1)):
+        assert np.abs(other - packed).max() > 1e-3 * np.abs(packed).max()
