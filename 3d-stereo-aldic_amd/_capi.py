@@ -86,6 +86,10 @@ SIGNATURES = {
     "aldic_integer_search_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.POINTER(_i64),
                                            C.POINTER(_i64), C.c_int32, C.c_int32,
                                            C.POINTER(_i64), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "aldic_integer_search_noalloc_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.POINTER(_i64),
+                                                   C.POINTER(_i64), C.c_int32, C.c_int32,
+                                                   C.POINTER(_i64), _vp, _vp, _vp, _vp, _vp, _vp,
+                                                   _vp]),
     "aldic_local_icgn_f0_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
                                           C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "aldic_cnn_create": (C.c_int, [_vp, C.c_int32, _pp]),

@@ -828,6 +828,18 @@ def integer_search_dev(ctx: Context, d_f, d_g, M, N, gridx, gridy, winsize, wins
                                          d_ipr, d_ipc))
 
 
+def integer_search_noalloc_dev(ctx: Context, d_f, d_g, M, N, gridx, gridy, winsize,
+                               winstepsize, region, d_x, d_y, d_u, d_v, d_ccmax, d_ipr, d_ipc):
+    """integer_search_dev with its scratch in the library's fixed per-GPU
+    array (no device allocation; calls that overlap on one GPU race)."""
+    gx = (C.c_int64 * 2)(int(gridx[0]), int(gridx[-1]))
+    gy = (C.c_int64 * 2)(int(gridy[0]), int(gridy[-1]))
+    rg = (C.c_int64 * 4)(*[int(v) for v in region])
+    check(lib().aldic_integer_search_noalloc_dev(ctx.ptr, d_f, d_g, int(M), int(N), gx, gy,
+                                                 int(winsize), int(winstepsize), rg, d_x, d_y,
+                                                 d_u, d_v, d_ccmax, d_ipr, d_ipc))
+
+
 def grid_to_nodes_dev(ctx: Context, d_u, d_v, nX, nY, x0, y0, step, d_coords, n, d_U0):
     check(lib().aldic_grid_to_nodes_dev(ctx.ptr, d_u, d_v, int(nX), int(nY), float(x0), float(y0),
                                         float(step), d_coords, int(n), d_U0))

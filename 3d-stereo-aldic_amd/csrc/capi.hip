@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,39 @@ int set_error(int code, const std::string& msg) {
 static int fail(int code, const std::string& msg) { return aldic::set_error(code, msg); }
 
 extern "C" const char* aldic_last_error(void) { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation search: a file-scope device array, one copy
+// per GPU, held until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the
+// symbol): its device address on each GPU is looked up once, with that GPU
+// current, and kept in scratch_dev[device].
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t NOALLOC_SCRATCH_BYTES = ALDIC_NOALLOC_SCRATCH_BYTES;  // 32 MiB per GPU
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+std::mutex scratch_mu;
+void* scratch_dev[NOALLOC_MAX_DEVICES] = {nullptr};
+
+// device address of `scratch` on `device`, which must be current
+int scratch_address(int device, void** out) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+    return fail(ALDIC_EINVAL, "no-allocation scratch: device index beyond " +
+                                  std::to_string(NOALLOC_MAX_DEVICES - 1));
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ALDIC_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(e));
+    scratch_dev[device] = p;
+  }
+  *out = scratch_dev[device];
+  return ALDIC_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -224,6 +258,10 @@ extern "C" int aldic_ctx_create(int device, aldic_ctx** out) {
     delete c;
     return fail(ALDIC_EDEVICE, std::string("aldic_ctx_create: ") + hipGetErrorString(e));
   }
+  // set-up of the no-allocation entry point's scratch on this device (its
+  // first call retries and reports the error if this lookup failed)
+  void* sp = nullptr;
+  (void)scratch_address(device, &sp);
   *out = c;
   return ALDIC_OK;
 }
@@ -1312,6 +1350,45 @@ extern "C" int aldic_grid_to_nodes_dev(aldic_ctx* ctx, const double* d_u, const 
   StageTimer t(ctx, ALDIC_STAGE_GRID2NODES);
   HIP_TRY(aldic::launch_grid_to_nodes(d_u, d_v, (int)nX, (int)nY, x0, y0, step, d_coords, (int)n,
                                       d_U0, ctx->stream));
+  return ALDIC_OK;
+}
+
+// The same search with its per-point scratch in the file-scope array
+// `scratch` of this GPU instead of a context buffer: no allocation at all.
+extern "C" int aldic_integer_search_noalloc_dev(aldic_ctx* ctx, const float* d_f,
+                                                const float* d_g, int64_t M, int64_t N,
+                                                const int64_t gridx[2], const int64_t gridy[2],
+                                                int32_t winsize, int32_t winstepsize,
+                                                const int64_t region[4], double* d_x,
+                                                double* d_y, double* d_u, double* d_v,
+                                                double* d_ccmax, int32_t* d_ipr,
+                                                int32_t* d_ipc) {
+  if (!ctx || !d_f || !d_g) return fail(ALDIC_EINVAL, "integer search (no alloc): NULL argument");
+  RC_TRY(search_checks(M, N, gridx, gridy, winsize, winstepsize, region));
+  const int64_t nX = grid_count(gridx[0], gridx[1], winsize, winstepsize);
+  const int64_t nY = grid_count(gridy[0], gridy[1], winsize, winstepsize);
+  if (nX * nY == 0) return ALDIC_OK;
+  if (!d_x || !d_y || !d_u || !d_v || !d_ccmax || !d_ipr || !d_ipc)
+    return fail(ALDIC_EINVAL, "integer search (no alloc): NULL output");
+  const int gx[2] = {(int)gridx[0], (int)gridx[1]}, gy[2] = {(int)gridy[0], (int)gridy[1]};
+  const int rg[4] = {(int)region[0], (int)region[1], (int)region[2], (int)region[3]};
+  const size_t need = aldic::search_scratch_bytes((int)(nX * nY), winsize, rg);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(ALDIC_EINVAL,
+                "integer search (no alloc): " + std::to_string(nX * nY) + " points at winsize " +
+                    std::to_string(winsize) + " need " + std::to_string(need) +
+                    " bytes of scratch, more than the " +
+                    std::to_string(NOALLOC_SCRATCH_BYTES) +
+                    " this entry point holds; split the grid or call aldic_integer_search_dev");
+  RC_TRY(set_device(ctx));
+  void* sp = nullptr;
+  RC_TRY(scratch_address(ctx->device, &sp));
+  // the prep kernel writes every point's clamps and template and the tile
+  // kernel every (point, tile) best before they are read: nothing in the
+  // scratch needs clearing between calls
+  HIP_TRY(aldic::launch_search(d_f, d_g, (int)M, (int)N, (int)nX, (int)nY, gx, gy, winsize,
+                               winstepsize, rg, sp, d_x, d_y, d_u, d_v, d_ccmax, d_ipr, d_ipc,
+                               ctx->stream));
   return ALDIC_OK;
 }
 

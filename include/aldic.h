@@ -266,6 +266,24 @@ int aldic_integer_search_dev(aldic_ctx* ctx, const float* d_f, const float* d_g,
                              double* d_x, double* d_y, double* d_u, double* d_v,
                              double* d_ccmax, int32_t* d_ipeak_row, int32_t* d_ipeak_col);
 
+/* aldic_integer_search_dev without any allocation: the per-point scratch
+ * (clamps, mean-removed template, per-tile bests) lives in a file-scope
+ * device array of the library, ALDIC_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends.  A grid whose scratch would exceed it is
+ * ALDIC_EINVAL (aldic_last_error gives the size needed).  Outputs as
+ * aldic_integer_search_dev.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (two contexts, two threads) race.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (work under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). */
+#define ALDIC_NOALLOC_SCRATCH_BYTES (32 << 20)
+int aldic_integer_search_noalloc_dev(aldic_ctx* ctx, const float* d_f, const float* d_g,
+                                     int64_t M, int64_t N, const int64_t gridx[2],
+                                     const int64_t gridy[2], int32_t winsize,
+                                     int32_t winstepsize, const int64_t region[4], double* d_x,
+                                     double* d_y, double* d_u, double* d_v, double* d_ccmax,
+                                     int32_t* d_ipeak_row, int32_t* d_ipeak_col);
+
 /* ---------------------------------------------------------------------------
  * Learned starting point (no counterpart in the reference: it stands beside
  * IntegerSearchQuadtree as the source of IC-GN's P0) and its fallback.
