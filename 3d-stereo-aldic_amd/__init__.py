@@ -26,7 +26,8 @@ from .api import (  # noqa: E402
     icgn_bad_points,
     local_icgn_raw,
     nan_fill,
-    remove_outliers,
+    funRemoveOutliers,
+    inpaint_nans,
     scattered_natural,
     subpb1_bad_points,
     subpb1_raw,
@@ -36,5 +37,5 @@ __all__ = [
     "AldicError", "Context", "FEMesh", "Images", "IntegerSearchQuadtree",
     "LocalICGNQuadtree", "Subpb1Quadtree", "Subpb2Quadtree", "ba_interp2_spline", "default_context", "funImgGradient",
     "funIntegerSearch", "funNormalizeImg", "icgn_bad_points", "local_icgn_raw",
-    "nan_fill", "remove_outliers", "scattered_natural", "subpb1_bad_points", "subpb1_raw", "lib",
+    "nan_fill", "funRemoveOutliers", "inpaint_nans", "scattered_natural", "subpb1_bad_points", "subpb1_raw", "lib",
 ]

@@ -23,10 +23,7 @@ Differences that a caller sees, all stated where they occur:
   libaldic (:func:`nan_fill`); where sites tie for the nearest one outside
   the hull it takes the first, and degenerate (collinear) sites in the
   global pass raise an error;
-* ``cc["qfactors"]`` (PCE/PPE) is not computed: it is unused downstream of
-  IntegerSearchQuadtree with qDICOrNot = 0 (IntegerSearchQuadtree.m:159);
-* funRemoveOutliers' median test and inpaint_nans are applied on the host
-  only where they change something (see :func:`remove_outliers`).
+* funRemoveOutliers and inpaint_nans run on the host (:mod:`.outliers`).
 """
 from __future__ import annotations
 
@@ -37,6 +34,7 @@ import numpy as np
 
 from . import _capi
 from ._capi import check, lib
+from .outliers import funMedRemoveOutliers, funRemoveOutliers, inpaint_nans, medFilt2  # noqa: F401
 
 
 def _F(a, dt=np.float64):
@@ -635,24 +633,49 @@ def funIntegerSearch(f, g, tempSizeOfSearchRegion, gridx, gridy, winsize, winste
     return out["x"], out["y"], out["u"], out["v"], cc
 
 
+def _preset(DICpara, key):
+    """DICpara.(key) when the field exists and is not empty (isfield and
+    ~isempty), else None."""
+    v = DICpara.get(key)
+    if v is None or np.size(v) == 0:
+        return None
+    return v
+
+
 def IntegerSearchQuadtree(ImgRef, ImgDef, file_name, DICpara, ImgSeqNum,
                           ctx: Context | None = None):
     """[DICpara,x0,y0,u,v,cc] = IntegerSearchQuadtree(ImgRef,ImgDef,file_name,DICpara,ImgSeqNum)
-    with InitFFTSearchMethod = 1 (IntegerSearchQuadtree.m:47-160).  The search
-    radius is chosen as :54-71 (stereo: StereoSearchDistance; temporal:
-    TemporalSearchDistance; else NewFFTSearchDistance); there is no
-    interactive prompt, a missing radius raises.  Points whose rounded
-    centre is masked become NaN (:106-115); funRemoveOutliers(u,v,cc,0,100)
-    (:158-160) runs on the host."""
+    with InitFFTSearchMethod = 1 (IntegerSearchQuadtree.m:47-160, :226-228).
+
+    * The search radius (:52-90): stereo (ImgSeqNum == -1) takes
+      StereoSearchDistance, else NewFFTSearchDistance; temporal (ImgSeqNum
+      >= 2) TemporalSearchDistance, else NewFFTSearchDistance, and sets
+      fixSearchDistanceOrNot = 0 when the field is absent.  Where the
+      reference would prompt with input() (no preset), this raises.  A
+      scalar becomes [s s].
+    * funIntegerSearch over gridxyROIRange with winsize/winstepsize (:93).
+    * Points whose rounded centre is 0 in ImgRefMask become NaN (:106-115);
+      as in the reference's try/catch, a missing ImgSize/ImgRefMask or a
+      centre outside ImgSize leaves u, v unmasked.
+    * cc.ccThreshold = 1.25 and funRemoveOutliers(u,v,cc,0,100) (:158-160).
+    cc also carries q-factors (PCE, PPE per point, normalised over the grid,
+    funIntegerSearch.m:162-163, :184-187)."""
     method = DICpara.get("InitFFTSearchMethod", 1)
     if method != 1:
         raise NotImplementedError("only InitFFTSearchMethod = 1 (whole field) is built")
+    DICpara = dict(DICpara)
     r = None
     if ImgSeqNum == -1:
-        r = DICpara.get("StereoSearchDistance") or DICpara.get("NewFFTSearchDistance")
+        r = _preset(DICpara, "StereoSearchDistance")
+        if r is None:
+            r = _preset(DICpara, "NewFFTSearchDistance")
     elif ImgSeqNum >= 2:
-        r = DICpara.get("TemporalSearchDistance") or DICpara.get("NewFFTSearchDistance")
-    if r is None or (np.size(r) == 0):
+        r = _preset(DICpara, "TemporalSearchDistance")
+        if r is None:
+            r = _preset(DICpara, "NewFFTSearchDistance")
+        if r is not None and "fixSearchDistanceOrNot" not in DICpara:
+            DICpara["fixSearchDistanceOrNot"] = 0
+    if r is None:
         raise ValueError("search distance not set (the reference would prompt with input())")
     r = np.asarray(r).reshape(-1)
     if r.size == 1:
@@ -661,114 +684,25 @@ def IntegerSearchQuadtree(ImgRef, ImgDef, file_name, DICpara, ImgSeqNum,
     x0, y0, u, v, cc = funIntegerSearch(ImgRef, ImgDef, r, roi["gridx"], roi["gridy"],
                                         DICpara["winsize"], DICpara["winstepsize"], 0, ctx=ctx)
     mask = DICpara.get("ImgRefMask")
-    if mask is not None:
+    size = DICpara.get("ImgSize")
+    if mask is not None and size is not None:
         mask = np.asarray(mask)
-        xi = np.round(x0).astype(np.int64) - 1
-        yi = np.round(y0).astype(np.int64) - 1
-        hole = mask[xi, yi] == 0
-        u = np.where(hole, np.nan, u)
-        v = np.where(hole, np.nan, v)
+        xi = np.round(x0).astype(np.int64)
+        yi = np.round(y0).astype(np.int64)
+        M, N = int(size[0]), int(size[1])
+        # sub2ind(ImgSize, ...) throws outside ImgSize, and so does an index
+        # beyond the mask: the catch leaves u, v as they are
+        if (xi.size == 0 or (xi.min() >= 1 and yi.min() >= 1 and xi.max() <= M and
+                             yi.max() <= N and M * N <= mask.size)):
+            lin = (xi - 1) + (yi - 1) * M
+            hole = mask.reshape(-1, order="F")[lin] == 0
+            u = np.where(hole, np.nan, u)
+            v = np.where(hole, np.nan, v)
     cc["ccThreshold"] = 1.25
-    u, v, cc = remove_outliers(u, v, cc, 0, 100)
-    DICpara = dict(DICpara)
+    u, v, cc, _ = funRemoveOutliers(u, v, cc, 0, 100)
     DICpara["InitFFTSearchMethod"] = 1
     DICpara["SizeOfFFTSearchRegion"] = r
     return DICpara, x0, y0, u, v, cc
-
-
-# ---------------------------------------------------------------------------
-# funRemoveOutliers (host post-step of the search)
-# ---------------------------------------------------------------------------
-def _medfilt2(V0):
-    """medFilt2(V0,[2 2],'symmetric',2) (func/funRemoveOutliers.m:393-444):
-    pad one row and one column after the data by mirroring, take the 2x2
-    window anchored at each point, drop its second element (the right
-    neighbour) and return the median of the remaining three -- the point,
-    the one below it along dim 1 and the diagonal one -- in single
-    precision."""
-    V = np.asarray(V0, dtype=np.float32)
-    P = np.pad(V, ((0, 1), (0, 1)), mode="symmetric")
-    a = P[:-1, :-1]
-    b = P[1:, :-1]
-    c = P[1:, 1:]
-    return np.median(np.stack([a, b, c], axis=2), axis=2).astype(np.float32)
-
-
-def _med_fluct(u, epsilon=0.1):
-    """funMedRemoveOutliers (func/funRemoveOutliers.m:374-390): normalised
-    fluctuation |u - med| / (med(|u - med|) + epsilon), single precision as
-    the reference's mixed double/single arithmetic yields.  NaNs are filled
-    first (inpaint_nans(.,3) in the reference; the spring fill here)."""
-    uf = inpaint_springs(np.asarray(u, dtype=np.float64))
-    med = _medfilt2(uf)
-    fluct = (uf.astype(np.float32) - med).astype(np.float32)
-    res = _medfilt2(np.abs(fluct))
-    return med, np.abs(fluct / (res + np.float32(epsilon)))
-
-
-def remove_outliers(u, v, cc, qDICOrNot=0, Thr0=100):
-    """funRemoveOutliers(u,v,cc,0,Thr0) restricted to what changes the grid:
-    the median test with threshold Thr0 and, where it (or a NaN) asks for it,
-    the fill of the flagged points.  With no NaN and no outlier the grid is
-    returned unchanged, as in the reference.  The fill itself is
-    inpaint_nans(.,4) in the reference (spring model); here it is the same
-    spring model solved by least squares on the host (parity unpinned: the
-    reference's inpaint_nans is MATLAB code this build does not run)."""
-    u = np.array(u, dtype=np.float64)
-    v = np.array(v, dtype=np.float64)
-    hole = np.isnan(u)
-    _, fu = _med_fluct(u)
-    _, fv = _med_fluct(v)
-    mag = np.sqrt(np.nan_to_num(fu) ** 2 + np.nan_to_num(fv) ** 2)
-    out = mag > Thr0
-    if not out.any() and not hole.any():
-        return u, v, cc
-    u2 = u.copy()
-    v2 = v.copy()
-    u2[out] = np.nan
-    v2[out] = np.nan
-    u2 = inpaint_springs(u2)
-    v2 = inpaint_springs(v2)
-    u2[hole] = np.nan
-    v2[hole] = np.nan
-    return u2, v2, cc
-
-
-def inpaint_springs(a):
-    """Spring-model fill of NaNs (the method of inpaint_nans(.,4)): minimise
-    the sum of squared differences across every horizontal, vertical and
-    diagonal neighbour pair with the known values fixed."""
-    a = np.array(a, dtype=np.float64)
-    nanm = np.isnan(a)
-    if not nanm.any() or nanm.all():
-        return a
-    import scipy.sparse as sp
-    import scipy.sparse.linalg as spl
-    M, N = a.shape
-    idx = np.arange(M * N).reshape(M, N, order="F")
-    rows, cols, vals = [], [], []
-    r = 0
-    for di, dj in ((1, 0), (0, 1), (1, 1), (1, -1)):
-        i0, i1 = max(0, -di), M - max(0, di)
-        j0, j1 = max(0, -dj), N - max(0, dj)
-        p = idx[i0:i1, j0:j1].reshape(-1)
-        q = idx[i0 + di:i1 + di, j0 + dj:j1 + dj].reshape(-1)
-        k = np.arange(p.size) + r
-        rows += [k, k]
-        cols += [p, q]
-        vals += [np.ones(p.size), -np.ones(p.size)]
-        r += p.size
-    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                      shape=(r, M * N))
-    flat = a.reshape(-1, order="F")
-    unk = np.flatnonzero(np.isnan(flat))
-    kn = np.flatnonzero(~np.isnan(flat))
-    rhs = -A[:, kn] @ flat[kn]
-    Au = A[:, unk]
-    sol = spl.lsqr(Au, rhs, atol=1e-12, btol=1e-12)[0]
-    flat = flat.copy()
-    flat[unk] = sol
-    return flat.reshape(M, N, order="F")
 
 
 # ---------------------------------------------------------------------------
