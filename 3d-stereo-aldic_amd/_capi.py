@@ -81,7 +81,7 @@ SIGNATURES = {
     "aldic_plan_set_variant": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32]),
     "aldic_integer_search": (C.c_int, [_vp, _dp, _dp, _i64, _i64, C.POINTER(_i64),
                                        C.POINTER(_i64), C.c_int32, C.c_int32, C.POINTER(_i64),
-                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i64),
+                                       _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i64),
                                        C.POINTER(_i64)]),
     "aldic_integer_search_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.POINTER(_i64),
                                            C.POINTER(_i64), C.c_int32, C.c_int32,

@@ -597,7 +597,8 @@ def funIntegerSearch(f, g, tempSizeOfSearchRegion, gridx, gridy, winsize, winste
     """[x,y,u,v,cc] = funIntegerSearch(f,g,tempSizeOfSearchRegion,gridx,gridy,winsize,
     winstepsize,0) -- whole-field branch (func/funIntegerSearch.m:7-46, :100-203).
 
-    Grids are nY x nX.  cc = {"max": ..., "A": []}; q-factors are not computed."""
+    Grids are nY x nX.  cc = {"max": ..., "A": [], "qfactors": (nX*nY) x 2}
+    with the q-factors in the reference's point order (x index fastest)."""
     if tempNoOfInitPt != 0:
         raise NotImplementedError("only the whole-field search (tempNoOfInitPt = 0) is built")
     ctx = ctx or default_context()
@@ -616,18 +617,19 @@ def funIntegerSearch(f, g, tempSizeOfSearchRegion, gridx, gridy, winsize, winste
     nX, nY = C.c_int64(), C.c_int64()
     L = lib()
     check(L.aldic_integer_search(ctx.ptr, f, g, M, N, gx, gy, ws, st, rg, None, None, None, None,
-                                 None, None, None, C.byref(nX), C.byref(nY)))
+                                 None, None, None, None, C.byref(nX), C.byref(nY)))
     shp = (nY.value, nX.value)
     out = {k: np.zeros(shp, order="F") for k in ("x", "y", "u", "v", "max")}
     pr = np.zeros(shp, dtype=np.int32, order="F")
     pc = np.zeros(shp, dtype=np.int32, order="F")
+    q = np.zeros((shp[0] * shp[1], 2), order="F")
     if shp[0] * shp[1] > 0:
         check(L.aldic_integer_search(ctx.ptr, f, g, M, N, gx, gy, ws, st, rg,
                                      out["x"].ctypes.data, out["y"].ctypes.data,
                                      out["u"].ctypes.data, out["v"].ctypes.data,
                                      out["max"].ctypes.data, pr.ctypes.data, pc.ctypes.data,
-                                     C.byref(nX), C.byref(nY)))
-    cc = {"max": out["max"], "A": []}
+                                     q.ctypes.data, C.byref(nX), C.byref(nY)))
+    cc = {"max": out["max"], "A": [], "qfactors": q}
     if return_peaks:
         return out["x"], out["y"], out["u"], out["v"], cc, pr, pc
     return out["x"], out["y"], out["u"], out["v"], cc

@@ -89,10 +89,13 @@ hipError_t launch_gradient_out_f64(const double* ref, const double* mask, int64_
 // integer search (search.hip); scratch of search_scratch_bytes(points, ...)
 size_t search_smem_bytes(int ws);
 size_t search_scratch_bytes(int maxpts, int ws, const int reg[4]);
+// q (may be null): raw (PCE, PPE) per grid point, 2 doubles, point order
+// ci + cj * nX (compute_qFactor of the full normxcorr2 map)
 hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, int nY,
                          const int gridx[2], const int gridy[2], int ws, int step,
                          const int reg[4], void* scratch, double* x, double* y, double* u,
-                         double* v, double* ccmax, int* ipr, int* ipc, hipStream_t s);
+                         double* v, double* ccmax, int* ipr, int* ipc, hipStream_t s,
+                         double* q = nullptr);
 hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
                                const double* coords, int n, const int* list, const int* count,
                                int max_count, int ws, const int reg[4], void* scratch, float* U0,

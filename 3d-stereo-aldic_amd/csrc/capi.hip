@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -1261,12 +1262,13 @@ int search_checks(int64_t M, int64_t N, const int64_t gridx[2], const int64_t gr
 
 }  // namespace
 
-extern "C" int aldic_integer_search_dev(aldic_ctx* ctx, const float* d_f, const float* d_g,
-                                        int64_t M, int64_t N, const int64_t gridx[2],
-                                        const int64_t gridy[2], int32_t winsize,
-                                        int32_t winstepsize, const int64_t region[4], double* d_x,
-                                        double* d_y, double* d_u, double* d_v, double* d_ccmax,
-                                        int32_t* d_ipr, int32_t* d_ipc) {
+// the device search of the grid; d_q (may be null) receives the raw
+// (PCE, PPE) pair of every point
+static int search_dev(aldic_ctx* ctx, const float* d_f, const float* d_g, int64_t M, int64_t N,
+                      const int64_t gridx[2], const int64_t gridy[2], int32_t winsize,
+                      int32_t winstepsize, const int64_t region[4], double* d_x, double* d_y,
+                      double* d_u, double* d_v, double* d_ccmax, int32_t* d_ipr, int32_t* d_ipc,
+                      double* d_q) {
   if (!ctx || !d_f || !d_g) return fail(ALDIC_EINVAL, "integer search: NULL argument");
   RC_TRY(search_checks(M, N, gridx, gridy, winsize, winstepsize, region));
   const int64_t nX = grid_count(gridx[0], gridx[1], winsize, winstepsize);
@@ -1283,8 +1285,18 @@ extern "C" int aldic_integer_search_dev(aldic_ctx* ctx, const float* d_f, const 
   StageTimer t(ctx, ALDIC_STAGE_SEARCH);
   HIP_TRY(aldic::launch_search(d_f, d_g, (int)M, (int)N, (int)nX, (int)nY, gx, gy, winsize,
                                winstepsize, rg, scratch, d_x, d_y, d_u, d_v, d_ccmax, d_ipr,
-                               d_ipc, ctx->stream));
+                               d_ipc, ctx->stream, d_q));
   return ALDIC_OK;
+}
+
+extern "C" int aldic_integer_search_dev(aldic_ctx* ctx, const float* d_f, const float* d_g,
+                                        int64_t M, int64_t N, const int64_t gridx[2],
+                                        const int64_t gridy[2], int32_t winsize,
+                                        int32_t winstepsize, const int64_t region[4], double* d_x,
+                                        double* d_y, double* d_u, double* d_v, double* d_ccmax,
+                                        int32_t* d_ipr, int32_t* d_ipc) {
+  return search_dev(ctx, d_f, d_g, M, N, gridx, gridy, winsize, winstepsize, region, d_x, d_y,
+                    d_u, d_v, d_ccmax, d_ipr, d_ipc, nullptr);
 }
 
 // The search-init start of the listed nodes: the grid points their
@@ -1396,8 +1408,8 @@ extern "C" int aldic_integer_search(aldic_ctx* ctx, const double* f, const doubl
                                     int64_t N, const int64_t gridx[2], const int64_t gridy[2],
                                     int32_t winsize, int32_t winstepsize, const int64_t region[4],
                                     double* x, double* y, double* u, double* v, double* ccmax,
-                                    int32_t* ipr, int32_t* ipc, int64_t* nX_out,
-                                    int64_t* nY_out) {
+                                    int32_t* ipr, int32_t* ipc, double* qfactors,
+                                    int64_t* nX_out, int64_t* nY_out) {
   if (!ctx) return fail(ALDIC_EINVAL, "integer search: NULL context");
   RC_TRY(search_checks(M, N, gridx, gridy, winsize, winstepsize, region));
   const int64_t nX = grid_count(gridx[0], gridx[1], winsize, winstepsize);
@@ -1425,12 +1437,19 @@ extern "C" int aldic_integer_search(aldic_ctx* ctx, const double* f, const doubl
   RC_TRY(stage(ctx, 8, (size_t)np, &dc));
   RC_TRY(stage(ctx, 9, (size_t)np, &dpr));
   RC_TRY(stage(ctx, 10, (size_t)np, &dpc));
+  double* dq = nullptr;
+  if (qfactors) RC_TRY(stage(ctx, 11, (size_t)(2 * np), &dq));
   hipStream_t s = ctx->stream;
   HIP_TRY(hipMemcpyAsync(df, f, sizeof(double) * MN, hipMemcpyHostToDevice, s));
   HIP_TRY(hipMemcpyAsync(dg, g, sizeof(double) * MN, hipMemcpyHostToDevice, s));
   HIP_TRY(aldic::launch_prep_planes(df, dg, nullptr, M, N, ff, fg, nullptr, s));
-  RC_TRY(aldic_integer_search_dev(ctx, ff, fg, M, N, gridx, gridy, winsize, winstepsize, region,
-                                  dx, dy, du, dv, dc, dpr, dpc));
+  RC_TRY(search_dev(ctx, ff, fg, M, N, gridx, gridy, winsize, winstepsize, region, dx, dy, du,
+                    dv, dc, dpr, dpc, dq));
+  std::vector<double> raw;
+  if (qfactors) {
+    raw.resize(2 * np);
+    HIP_TRY(hipMemcpyAsync(raw.data(), dq, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, s));
+  }
   HIP_TRY(hipMemcpyAsync(x, dx, sizeof(double) * np, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(y, dy, sizeof(double) * np, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(u, du, sizeof(double) * np, hipMemcpyDeviceToHost, s));
@@ -1439,5 +1458,22 @@ extern "C" int aldic_integer_search(aldic_ctx* ctx, const double* f, const doubl
   if (ipr) HIP_TRY(hipMemcpyAsync(ipr, dpr, sizeof(int32_t) * np, hipMemcpyDeviceToHost, s));
   if (ipc) HIP_TRY(hipMemcpyAsync(ipc, dpc, sizeof(int32_t) * np, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  if (qfactors) {
+    // cc.qfactors(:,k) = (q - min q) / max(q - min q) over the grid, NaN
+    // left out of min and max (funIntegerSearch.m:184-187); an Inf (a point
+    // whose normxcorr2 threw) makes the others 0 and itself NaN, as there
+    for (int k = 0; k < 2; k++) {
+      double mn = NAN, mx = NAN;
+      for (int64_t t = 0; t < np; t++) {
+        const double q = raw[2 * t + k];
+        if (!std::isnan(q) && (std::isnan(mn) || q < mn)) mn = q;
+      }
+      for (int64_t t = 0; t < np; t++) {
+        const double q = raw[2 * t + k] - mn;
+        if (!std::isnan(q) && (std::isnan(mx) || q > mx)) mx = q;
+      }
+      for (int64_t t = 0; t < np; t++) qfactors[t + k * np] = (raw[2 * t + k] - mn) / mx;
+    }
+  }
   return ALDIC_OK;
 }

@@ -469,6 +469,153 @@ __global__ __launch_bounds__(256) void search_final_kernel(SearchArgs A) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// q-factors (funIntegerSearch.m:162-163, compute_qFactor :211-264): one
+// 256-thread workgroup per point over the FULL normxcorr2 map, (W1 + s - 1)
+// x (W2 + s - 1) lags, template origins -(s-1) .. W-1 in the window.  Pass
+// 1 finds the map's min and max; pass 2 recomputes each coefficient (the
+// same arithmetic, so the same value) and accumulates sum (c - min)^2 and a
+// 30-bin histogram with the bin edges of MATLAB's histcounts.  The window
+// and the template are staged in LDS when they fit (WIN_LDS), else the
+// window is read from the image.
+// ---------------------------------------------------------------------------
+constexpr int QBINS = 30;
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ double eps_of(double x) {
+  x = fabs(x);
+  if (x == 0.0) return 4.9406564584124654e-324;
+  int e;
+  frexp(x, &e);
+  return ldexp(1.0, e - 53);
+}
+
+// histcounts(x, 30) bin width and right edge for data in [0, xmax] (its
+// bin picker with the number of bins given; the left edge is 0 here);
+// returns false for nearly constant data (one bin holds everything)
+__device__ bool q_bins(double xmax, double* bw, double* right) {
+  const double xscale = xmax, xrange = xmax;
+  if (!(xrange > fmax(sqrt(eps_of(xscale)), 2.2250738585072014e-308))) return false;
+  const double ll = xrange / QBINS, ul = xrange / (QBINS - 1);
+  const double q10 = pow(10.0, floor(log10(ul - ll)));
+  *bw = q10 * ceil(ll / q10);
+  *right = fmax(QBINS * *bw, xmax);
+  return true;
+}
+
+template <bool WIN_LDS>
+__device__ float q_coef(const SearchArgs& A, const float* Tl, const float* win, int s, int W1,
+                        int W2, int g_r0, int g_c0, int o1, int o2, float denT) {
+  const int a0 = max(0, -o1), a1 = min(s, W1 - o1);
+  const int b0 = max(0, -o2), b1 = min(s, W2 - o2);
+  float num = 0.0f, sa = 0.0f, sa2 = 0.0f;
+  for (int b = b0; b < b1; b++) {
+    for (int a = a0; a < a1; a++) {
+      const int wr = o1 + a, wc = o2 + b;
+      const float v = WIN_LDS ? win[wr + wc * W1]
+                              : A.g[(int64_t)(g_r0 + wr) + (int64_t)(g_c0 + wc) * A.M];
+      num = fmaf(Tl[a + b * s], v, num);
+      sa += v;
+      sa2 = fmaf(v, v, sa2);
+    }
+  }
+  // the tile kernel's rule: a window of (numerically) zero variance, or a
+  // coefficient outside [-1, 1] beyond FP32 round-off, gives 0
+  const float varA = sa2 - sa * sa / (float)(s * s);
+  float gam = 0.0f;
+  if (varA > 1e-6f * sa2 && varA > 0.0f) {
+    gam = num / (denT * sqrtf(varA));
+    if (fabsf(gam) - 1.0f > 1e-5f) gam = 0.0f;
+  }
+  return gam;
+}
+
+template <bool WIN_LDS>
+__global__ __launch_bounds__(256) void search_qfactor_kernel(SearchArgs A, double* q) {
+  extern __shared__ float qsm[];
+  __shared__ float rmin[4], rmax[4];
+  __shared__ double rsum[4];
+  __shared__ int hist[QBINS];
+  const int tid = threadIdx.x, lane = tid % 64, w = tid / 64;
+  const int total = num_points(A);
+  const int s = A.ws + 1;
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    const PointPrm P = A.prm[t];
+    if (!P.valid) {  // normxcorr2 throws: [Inf Inf] (funIntegerSearch.m:173)
+      if (tid == 0) q[2 * t] = q[2 * t + 1] = __builtin_inf();
+      continue;
+    }
+    const int W1 = s + P.L + P.R, W2 = s + P.U + P.D;
+    const int F1 = W1 + s - 1, nF = F1 * (W2 + s - 1);
+    const int g_r0 = P.ii - P.L - 1, g_c0 = P.jj - P.U - 1;
+    float* Tl = qsm;        // s x s, Tl[a + b*s] = T(a, b) - mean
+    float* win = Tl + s * s;  // W1 x W2 (WIN_LDS)
+    const float* T = A.tmpl + (int64_t)t * s * A.sp;
+    for (int e = tid; e < s * s; e += 256) Tl[e] = T[(e % s) * A.sp + e / s];
+    if (WIN_LDS)
+      for (int e = tid; e < W1 * W2; e += 256)
+        win[e] = A.g[(int64_t)(g_r0 + e % W1) + (int64_t)(g_c0 + e / W1) * A.M];
+    if (tid < QBINS) hist[tid] = 0;
+    __syncthreads();
+    float mn = 1e30f, mx = -1e30f;
+    for (int e = tid; e < nF; e += 256) {
+      const float c = q_coef<WIN_LDS>(A, Tl, win, s, W1, W2, g_r0, g_c0, e % F1 - (s - 1),
+                                      e / F1 - (s - 1), P.denT);
+      mn = fminf(mn, c);
+      mx = fmaxf(mx, c);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      mn = fminf(mn, __shfl_xor(mn, off));
+      mx = fmaxf(mx, __shfl_xor(mx, off));
+    }
+    if (lane == 0) {
+      rmin[w] = mn;
+      rmax[w] = mx;
+    }
+    __syncthreads();
+    mn = fminf(fminf(rmin[0], rmin[1]), fminf(rmin[2], rmin[3]));
+    mx = fmaxf(fmaxf(rmax[0], rmax[1]), fmaxf(rmax[2], rmax[3]));
+    const double cmin = mn, peak = (double)mx - (double)mn;
+    double bw = 0.0, right = 0.0;
+    const bool spread = q_bins(peak, &bw, &right);
+    double e2 = 0.0;
+    for (int e = tid; e < nF; e += 256) {
+      const float c = q_coef<WIN_LDS>(A, Tl, win, s, W1, W2, g_r0, g_c0, e % F1 - (s - 1),
+                                      e / F1 - (s - 1), P.denT);
+      const double v = (double)c - cmin;
+      e2 += v * v;
+      int b = 0;
+      if (spread) {
+        // bin j holds edges(j) <= v < edges(j+1), edges(j) = j bw, the last
+        // bin also its right edge
+        b = min(max((int)floor(v / bw), 0), QBINS - 1);
+        if (b > 0 && v < b * bw) b--;
+        if (b < QBINS - 1 && v >= (b + 1) * bw) b++;
+      }
+      atomicAdd(&hist[b], 1);
+    }
+    e2 = wave_sum_d(e2);
+    if (lane == 0) rsum[w] = e2;
+    __syncthreads();
+    if (tid == 0) {
+      const double E2 = rsum[0] + rsum[1] + rsum[2] + rsum[3];
+      double ent = 0.0;
+      for (int j = 0; j < QBINS; j++)
+        if (hist[j] > 0) {
+          const double p = (double)hist[j] / (double)nF;
+          ent += p * log(1.0 / p);
+        }
+      q[2 * t] = peak * peak / (E2 / (double)nF);
+      q[2 * t + 1] = 1.0 / ent;
+    }
+    __syncthreads();
+  }
+}
+
 void tile_counts(const int reg[4], int* nt1, int* ntiles) {
   const int K1 = reg[0] + reg[1] + 3, K2 = reg[2] + reg[3] + 3;
   *nt1 = (K1 + KT1 - 1) / KT1;
@@ -490,18 +637,43 @@ void carve(SearchArgs& A, void* scratch, int maxpts) {
   A.tbi = reinterpret_cast<int*>(p);
 }
 
-hipError_t run(const SearchArgs& A, int maxpts, hipStream_t s) {
+// the q-factor kernel's LDS: the template, plus the largest clamped window
+// when that fits in QWIN_LDS_MAX bytes
+constexpr size_t QWIN_LDS_MAX = 150 * 1024;
+
+hipError_t run(const SearchArgs& A, int maxpts, hipStream_t s, double* q = nullptr) {
   const size_t smem = aldic::search_smem_bytes(A.ws);
   if (smem > 160 * 1024) return hipErrorInvalidValue;
-  static bool attr = false;
-  if (smem > 64 * 1024 && !attr) {
-    hipError_t e = hipFuncSetAttribute((const void*)search_tile_kernel,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  // dynamic LDS beyond the 64 KB default is asked for, per launch size
+  if (smem > 64 * 1024) {
+    const hipError_t e = hipFuncSetAttribute((const void*)search_tile_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)smem);
     if (e != hipSuccess) return e;
-    attr = true;
   }
   if (maxpts <= 0) return hipSuccess;
   search_prep_kernel<<<std::min(maxpts, 2048), 256, 0, s>>>(A);
+  if (q) {
+    const int sd = A.ws + 1;
+    const size_t tb = sizeof(float) * sd * sd;
+    const size_t wb = sizeof(float) * (size_t)(sd + A.reg[0] + A.reg[1]) *
+                      (size_t)(sd + A.reg[2] + A.reg[3]);
+    const int grid = std::min(maxpts, 4096);
+    const bool lds_win = tb + wb <= QWIN_LDS_MAX;
+    const size_t qsm = lds_win ? tb + wb : tb;
+    if (qsm > QWIN_LDS_MAX) return hipErrorInvalidValue;
+    if (qsm > 64 * 1024) {
+      const hipError_t e = hipFuncSetAttribute(
+          lds_win ? (const void*)search_qfactor_kernel<true>
+                  : (const void*)search_qfactor_kernel<false>,
+          hipFuncAttributeMaxDynamicSharedMemorySize, (int)qsm);
+      if (e != hipSuccess) return e;
+    }
+    if (lds_win)
+      search_qfactor_kernel<true><<<grid, 256, qsm, s>>>(A, q);
+    else
+      search_qfactor_kernel<false><<<grid, 256, qsm, s>>>(A, q);
+  }
   const int64_t items = (int64_t)maxpts * A.ntiles;
   search_tile_kernel<<<(int)std::min<int64_t>(items, 4096), 256, smem, s>>>(A, A.tmpl, A.g);
   search_final_kernel<<<std::min((maxpts + 3) / 4, 1024), 256, 0, s>>>(A);
@@ -586,7 +758,8 @@ hipError_t launch_search_points(const float* f, const float* g, int M, int N, in
 hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, int nY,
                          const int gridx[2], const int gridy[2], int ws, int step,
                          const int reg[4], void* scratch, double* x, double* y, double* u,
-                         double* v, double* ccmax, int* ipr, int* ipc, hipStream_t s) {
+                         double* v, double* ccmax, int* ipr, int* ipc, hipStream_t s,
+                         double* q) {
   SearchArgs A{};
   A.f = f;
   A.g = g;
@@ -611,7 +784,7 @@ hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, i
   A.ipr = ipr;
   A.ipc = ipc;
   carve(A, scratch, nX * nY);
-  return run(A, nX * nY, s);
+  return run(A, nX * nY, s, q);
 }
 
 }  // namespace aldic

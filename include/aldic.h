@@ -241,12 +241,16 @@ int aldic_subpb1_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* p
  * InitFFTSearchMethod = 1.  region = [L R U D] after the repmat of
  * funIntegerSearch.m:12.  Grid outputs are nY x nX column-major; pass NULL
  * outputs to query nX, nY.  ipeak_row/col are the 1-based integer peak in
- * the cropped lag window (0 where normxcorr2 would throw, u = v = NaN). */
+ * the cropped lag window (0 where normxcorr2 would throw, u = v = NaN).
+ * qfactors (may be NULL) receives cc.qfactors, nX*nY x 2 column-major
+ * (PCE, PPE of each point's full normxcorr2 map, compute_qFactor :211-264,
+ * normalised over the grid :184-187), rows in the reference's point order
+ * (x index fastest). */
 int aldic_integer_search(aldic_ctx* ctx, const double* f, const double* g, int64_t M, int64_t N,
                          const int64_t gridx[2], const int64_t gridy[2], int32_t winsize,
                          int32_t winstepsize, const int64_t region[4], double* x, double* y,
                          double* u, double* v, double* ccmax, int32_t* ipeak_row,
-                         int32_t* ipeak_col, int64_t* nX, int64_t* nY);
+                         int32_t* ipeak_col, double* qfactors, int64_t* nX, int64_t* nY);
 
 /* Initial guess of arbitrary nodes from the search grid (as the reference
  * carries U0 from the search grid onto quadtree nodes,

@@ -1015,8 +1015,12 @@ static double eps_of(double x) {
   return ldexp(1.0, e - 53);
 }
 
-int orc_normxcorr2_crop(const double* T, int64_t s1, int64_t s2, const double* A, int64_t W1,
-                        int64_t W2, double* out) {
+/* normxcorr2 (Lewis 1995, zero-padded) at the lags whose template origin
+ * in A is (o1 + p, o2 + q), p < K1, q < K2 (0-based); coefficients whose
+ * denominator is at most sqrt(eps) of the largest one over these lags, or
+ * which leave [-1, 1] by more than sqrt(eps), are 0. */
+static int ncc_lags(const double* T, int64_t s1, int64_t s2, const double* A, int64_t W1,
+                    int64_t W2, int64_t o1, int64_t K1, int64_t o2, int64_t K2, double* out) {
   const int64_t n = s1 * s2;
   double st = 0.0;
   for (int64_t k = 0; k < n; k++) st += T[k];
@@ -1030,19 +1034,18 @@ int orc_normxcorr2_crop(const double* T, int64_t s1, int64_t s2, const double* A
   }
   if (allsame) return -1; /* "The values of TEMPLATE cannot all be the same." */
   const double denT = sqrt(ss);
-  const int64_t K1 = W1 - s1 + 3, K2 = W2 - s2 + 3;
   double* den = (double*)malloc(sizeof(double) * K1 * K2);
   double* num = (double*)malloc(sizeof(double) * K1 * K2);
   double dmax = 0.0;
   for (int64_t q = 0; q < K2; q++)
     for (int64_t p = 0; p < K1; p++) {
-      const int64_t o1 = p - 1, o2 = q - 1; /* template top-left in A */
+      const int64_t t1 = o1 + p, t2 = o2 + q; /* template top-left in A */
       double sa = 0.0, sa2 = 0.0, nm = 0.0;
       for (int64_t b = 0; b < s2; b++) {
-        const int64_t j = o2 + b;
+        const int64_t j = t2 + b;
         if (j < 0 || j >= W2) continue;
         for (int64_t a = 0; a < s1; a++) {
-          const int64_t i = o1 + a;
+          const int64_t i = t1 + a;
           if (i < 0 || i >= W1) continue;
           const double v = A[i + j * W1];
           sa += v;
@@ -1068,6 +1071,94 @@ int orc_normxcorr2_crop(const double* T, int64_t s1, int64_t s2, const double* A
   free(den);
   free(num);
   return 0;
+}
+
+int orc_normxcorr2_crop(const double* T, int64_t s1, int64_t s2, const double* A, int64_t W1,
+                        int64_t W2, double* out) {
+  return ncc_lags(T, s1, s2, A, W1, W2, -1, W1 - s1 + 3, -1, W2 - s2 + 3, out);
+}
+
+int orc_normxcorr2_full(const double* T, int64_t s1, int64_t s2, const double* A, int64_t W1,
+                        int64_t W2, double* out) {
+  return ncc_lags(T, s1, s2, A, W1, W2, -(s1 - 1), W1 + s1 - 1, -(s2 - 1), W2 + s2 - 1, out);
+}
+
+/* Bin edges of MATLAB's histcounts(x, nbins) for data in [xmin, xmax]: the
+ * "nice" edges of its bin picker with the number of bins given (a left edge
+ * on a multiple of a rounded raw width, then the width rounded up at the
+ * decimal place of the slack between (xmax - left)/nbins and
+ * (xmax - left)/(nbins - 1)), or a unit-wide cover for nearly constant data.
+ * Parity unpinned beyond the cases tests/test_oracle.py constructs:
+ * MATLAB's bin picker is not vendored. */
+static void hist_edges(double xmin, double xmax, int nbins, double* edges) {
+  const double xscale = fmax(fabs(xmin), fabs(xmax));
+  const double xrange = xmax - xmin;
+  double raw = xrange / nbins;
+  raw = fmax(raw, eps_of(xscale));
+  double left, right, bw;
+  if (xrange > fmax(sqrt(eps_of(xscale)), DBL_MIN)) {
+    const double p10 = pow(10.0, floor(log10(raw)));
+    const double rel = raw / p10;
+    bw = p10 * floor(rel);
+    left = fmax(fmin(bw * floor(xmin / bw), xmin), -DBL_MAX);
+    if (nbins > 1) {
+      const double ll = (xmax - left) / nbins;
+      const double ul = (xmax - left) / (nbins - 1);
+      const double q10 = pow(10.0, floor(log10(ul - ll)));
+      bw = q10 * ceil(ll / q10);
+    }
+    right = fmin(fmax(left + nbins * bw, xmax), DBL_MAX);
+  } else {
+    const double brange = fmax(1.0, ceil(nbins * eps_of(xscale)));
+    left = floor(2.0 * (xmin - brange / 4.0)) / 2.0;
+    right = ceil(2.0 * (xmax + brange / 4.0)) / 2.0;
+    bw = (right - left) / nbins;
+  }
+  edges[0] = left;
+  for (int k = 1; k < nbins; k++) edges[k] = left + k * bw;
+  edges[nbins] = right;
+}
+
+void orc_hist_edges(double xmin, double xmax, int nbins, double* edges) {
+  hist_edges(xmin, xmax, nbins, edges);
+}
+
+/* compute_qFactor of funIntegerSearch.m:211-264 on one correlation map:
+ * cc_min = cc - min(cc), peak = max(cc_min); PCE = peak^2 / mean(cc_min^2);
+ * PPE = 1 / entropy of histcounts(cc_min, 30). */
+void orc_qfactor(const double* cc, int64_t n, double q[2]) {
+  double mn = INFINITY, mx = -INFINITY;
+  for (int64_t k = 0; k < n; k++) {
+    mn = fmin(mn, cc[k]);
+    mx = fmax(mx, cc[k]);
+  }
+  const double peak = mx - mn;
+  double e2 = 0.0;
+  for (int64_t k = 0; k < n; k++) e2 += (cc[k] - mn) * (cc[k] - mn);
+  q[0] = (peak * peak) / (e2 / (double)n);
+  double edges[31];
+  hist_edges(0.0, peak, 30, edges);
+  int64_t cnt[30] = {0};
+  int64_t tot = 0;
+  for (int64_t k = 0; k < n; k++) {
+    const double v = cc[k] - mn;
+    if (v < edges[0] || v > edges[30]) continue;
+    int b = 29;
+    for (int j = 1; j <= 30; j++)
+      if (v < edges[j]) {
+        b = j - 1;
+        break;
+      }
+    cnt[b]++;
+    tot++;
+  }
+  double ent = 0.0;
+  for (int j = 0; j < 30; j++)
+    if (cnt[j] > 0) {
+      const double p = (double)cnt[j] / (double)tot;
+      ent += p * log(1.0 / p);
+    }
+  q[1] = 1.0 / ent;
 }
 
 void orc_findpeak(const double* f, int64_t m, int64_t n, double* xpeak, double* ypeak,
@@ -1135,6 +1226,15 @@ int orc_integer_search(const double* f, const double* g, int64_t M, int64_t N,
                        const int64_t region[4], int nthreads, double* x, double* y, double* u,
                        double* v, double* ccmax, int32_t* ipeak_row, int32_t* ipeak_col,
                        int64_t* nX_out, int64_t* nY_out) {
+  return orc_integer_search_q(f, g, M, N, gridx, gridy, ws, step, region, nthreads, x, y, u, v,
+                              ccmax, ipeak_row, ipeak_col, NULL, nX_out, nY_out);
+}
+
+int orc_integer_search_q(const double* f, const double* g, int64_t M, int64_t N,
+                         const int64_t gridx[2], const int64_t gridy[2], int ws, int step,
+                         const int64_t region[4], int nthreads, double* x, double* y, double* u,
+                         double* v, double* ccmax, int32_t* ipeak_row, int32_t* ipeak_col,
+                         double* qfactors, int64_t* nX_out, int64_t* nY_out) {
   /* XList = gridx(1):step:gridx(2)-ws-step (funIntegerSearch.m:106-107) */
   int64_t nX = 0, nY = 0;
   for (int64_t i = gridx[0]; i <= gridx[1] - ws - step; i += step) nX++;
@@ -1171,6 +1271,17 @@ int orc_integer_search(const double* f, const double* g, int64_t M, int64_t N,
     const int64_t K1 = W1 - s + 3, K2 = W2 - s + 3;
     double* cc = (double*)malloc(sizeof(double) * K1 * K2);
     const int64_t o = cj + ci * nY; /* u(cj, ci), nY x nX column-major */
+    if (qfactors) {
+      /* q-factors of the full normxcorr2 map (:162-163); [Inf Inf] where
+       * normxcorr2 throws (:173) */
+      const int64_t F1 = W1 + s - 1, F2 = W2 + s - 1;
+      double* full = (double*)malloc(sizeof(double) * F1 * F2);
+      double q[2] = {INFINITY, INFINITY};
+      if (orc_normxcorr2_full(C, s, s, Dw, W1, W2, full) == 0) orc_qfactor(full, F1 * F2, q);
+      qfactors[t] = q[0];
+      qfactors[t + tot] = q[1];
+      free(full);
+    }
     if (orc_normxcorr2_crop(C, s, s, Dw, W1, W2, cc) != 0) {
       u[o] = NAN;
       v[o] = NAN;
@@ -1192,6 +1303,20 @@ int orc_integer_search(const double* f, const double* g, int64_t M, int64_t N,
     free(C);
     free(Dw);
     free(cc);
+  }
+  if (qfactors && tot > 0) {
+    /* qf_ = q - min(q); cc.qfactors = qf_ / max(qf_) per column, NaN
+     * ignored by min and max (:184-187) */
+    for (int k = 0; k < 2; k++) {
+      double* qc = qfactors + k * tot;
+      double mn = NAN, mx = NAN;
+      for (int64_t t = 0; t < tot; t++)
+        if (!isnan(qc[t]) && (isnan(mn) || qc[t] < mn)) mn = qc[t];
+      for (int64_t t = 0; t < tot; t++) qc[t] -= mn;
+      for (int64_t t = 0; t < tot; t++)
+        if (!isnan(qc[t]) && (isnan(mx) || qc[t] > mx)) mx = qc[t];
+      for (int64_t t = 0; t < tot; t++) qc[t] /= mx;
+    }
   }
   return 0;
 }

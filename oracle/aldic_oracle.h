@@ -110,6 +110,19 @@ void orc_subpb1(const orc_images* im, const double* coords, const double* UOld,
 int orc_normxcorr2_crop(const double* T, int64_t s1, int64_t s2, const double* A,
                         int64_t W1, int64_t W2, double* out);
 
+/* The full normxcorr2(T, A) map: (W1 + s1 - 1) x (W2 + s2 - 1), the same
+ * arithmetic as orc_normxcorr2_crop at every lag. */
+int orc_normxcorr2_full(const double* T, int64_t s1, int64_t s2, const double* A,
+                        int64_t W1, int64_t W2, double* out);
+
+/* compute_qFactor (funIntegerSearch.m:211-264) of one map of n values:
+ * q[0] = PCE, q[1] = PPE (histcounts with 30 bins). */
+void orc_qfactor(const double* cc, int64_t n, double q[2]);
+
+/* The 31 bin edges histcounts(x, 30)-style for data in [xmin, xmax]
+ * (nbins + 1 values). */
+void orc_hist_edges(double xmin, double xmax, int nbins, double* edges);
+
 /* findpeak.m:1-67 with subpixel=1.  f: m x n column-major.
  * xpeak is the column (dim 2) and ypeak the row (dim 1), both 1-based. */
 void orc_findpeak(const double* f, int64_t m, int64_t n, double* xpeak,
@@ -127,6 +140,16 @@ int orc_integer_search(const double* f, const double* g, int64_t M, int64_t N,
                        int nthreads, double* x, double* y, double* u, double* v,
                        double* ccmax, int32_t* ipeak_row, int32_t* ipeak_col,
                        int64_t* nX_out, int64_t* nY_out);
+
+/* orc_integer_search plus cc.qfactors (NULL = not computed): nX*nY x 2
+ * column-major, rows in the reference's point order (x index fastest),
+ * normalised over the grid as funIntegerSearch.m:184-187. */
+int orc_integer_search_q(const double* f, const double* g, int64_t M, int64_t N,
+                         const int64_t gridx[2], const int64_t gridy[2],
+                         int winsize, int winstepsize, const int64_t region[4],
+                         int nthreads, double* x, double* y, double* u, double* v,
+                         double* ccmax, int32_t* ipeak_row, int32_t* ipeak_col,
+                         double* qfactors, int64_t* nX_out, int64_t* nY_out);
 
 /* bwselect(BW, c, r, n) on a m1 x m2 column-major logical image: the
  * n-connected (4 or 8) component of BW containing pixel (r, c) (1-based).

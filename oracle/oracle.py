@@ -79,6 +79,13 @@ def lib():
                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.orc_integer_search.restype = C.c_int
+        L.orc_integer_search_q.argtypes = (L.orc_integer_search.argtypes[:17] + [C.c_void_p] +
+                                           L.orc_integer_search.argtypes[17:])
+        L.orc_integer_search_q.restype = C.c_int
+        L.orc_normxcorr2_full.argtypes = [_dp, _i64, _i64, _dp, _i64, _i64, _dp]
+        L.orc_normxcorr2_full.restype = C.c_int
+        L.orc_qfactor.argtypes = [_dp, _i64, _dp]
+        L.orc_hist_edges.argtypes = [C.c_double, C.c_double, C.c_int, _dp]
         L.orc_bwselect.argtypes = [_u8, _i64, _i64, _i64, _i64, C.c_int, _u8]
         _lib = L
     return _lib
@@ -214,6 +221,32 @@ def normxcorr2_crop(T, A):
     return out
 
 
+def normxcorr2_full(T, A):
+    """The full (W1 + s1 - 1) x (W2 + s2 - 1) normxcorr2 map."""
+    T = _F(T)
+    A = _F(A)
+    s1, s2 = T.shape
+    W1, W2 = A.shape
+    out = np.zeros((W1 + s1 - 1, W2 + s2 - 1), order="F")
+    if lib().orc_normxcorr2_full(T, s1, s2, A, W1, W2, out) != 0:
+        raise ValueError("normxcorr2: template values are all the same")
+    return out
+
+
+def qfactor(cc):
+    """(PCE, PPE) of one correlation map (compute_qFactor)."""
+    cc = _F(np.asarray(cc).reshape(-1))
+    q = np.zeros(2, order="F")
+    lib().orc_qfactor(cc, cc.size, q)
+    return q
+
+
+def hist_edges(xmin, xmax, nbins):
+    e = np.zeros(nbins + 1, order="F")
+    lib().orc_hist_edges(float(xmin), float(xmax), int(nbins), e)
+    return e
+
+
 def findpeak(f):
     f = _F(f)
     xp, yp, mf = C.c_double(), C.c_double(), C.c_double()
@@ -223,10 +256,12 @@ def findpeak(f):
     return xp.value, yp.value, mf.value, pr.value, pc.value
 
 
-def integer_search(f, g, gridx, gridy, winsize, winstepsize, region, nthreads=1):
+def integer_search(f, g, gridx, gridy, winsize, winstepsize, region, nthreads=1,
+                   qfactors=False):
     """funIntegerSearchWholeField with region = [L R U D] already expanded.
 
-    Returns dict of nY x nX arrays x, y, u, v, ccmax, ipeak_row, ipeak_col."""
+    Returns dict of nY x nX arrays x, y, u, v, ccmax, ipeak_row, ipeak_col,
+    and with qfactors=True also "qfactors" (nX*nY x 2, cc.qfactors)."""
     f = _F(f)
     g = _F(g)
     M, N = f.shape
@@ -241,11 +276,14 @@ def integer_search(f, g, gridx, gridy, winsize, winstepsize, region, nthreads=1)
     out = {k: np.zeros(shp, order="F") for k in ("x", "y", "u", "v", "ccmax")}
     out["ipeak_row"] = np.zeros(shp, dtype=np.int32, order="F")
     out["ipeak_col"] = np.zeros(shp, dtype=np.int32, order="F")
-    L.orc_integer_search(f, g, M, N, gx, gy, int(winsize), int(winstepsize), rg, int(nthreads),
-                         out["x"].ctypes.data, out["y"].ctypes.data, out["u"].ctypes.data,
-                         out["v"].ctypes.data, out["ccmax"].ctypes.data,
-                         out["ipeak_row"].ctypes.data, out["ipeak_col"].ctypes.data,
-                         C.byref(nX), C.byref(nY))
+    q = np.zeros((shp[0] * shp[1], 2), order="F") if qfactors else None
+    L.orc_integer_search_q(f, g, M, N, gx, gy, int(winsize), int(winstepsize), rg,
+                           int(nthreads), out["x"].ctypes.data, out["y"].ctypes.data,
+                           out["u"].ctypes.data, out["v"].ctypes.data, out["ccmax"].ctypes.data,
+                           out["ipeak_row"].ctypes.data, out["ipeak_col"].ctypes.data,
+                           None if q is None else q.ctypes.data, C.byref(nX), C.byref(nY))
+    if qfactors:
+        out["qfactors"] = q
     return out
 
 

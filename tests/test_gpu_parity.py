@@ -321,3 +321,58 @@ def test_integer_search_edges_and_radius(pkg, orc):
         assert np.array_equal(pr, s["ipeak_row"]) and np.array_equal(pc, s["ipeak_col"])
         assert np.max(np.abs(u - s["u"])) <= 1e-3 + 1e-9
         assert np.max(np.abs(v - s["v"])) <= 1e-3 + 1e-9
+
+
+def test_integer_search_qfactors(pkg, orc):
+    """cc.qfactors (PCE, PPE of each point's full normxcorr2 map, normalised
+    over the grid) against the oracle: PCE within 1e-4 and PPE within 2e-3
+    after normalisation (the device's FP32 coefficients can put a value
+    within 1e-6 of a histogram edge in the other bin).  A point with a
+    constant template (normxcorr2 throws) is NaN and turns every other
+    point's q-factors to 0, in both."""
+    f, g, uf, vf = c1_pair()
+    cases = [(f, g, [1, 512], [1, 512], 30, 16, [10, 10]),
+             (f, g, [1, 300], [1, 280], 20, 12, [5, 9])]
+    fc = np.array(f)
+    fc[100:140, 100:140] = 0.5  # one grid point's template is constant
+    cases.append((np.asfortranarray(fc), g, [1, 300], [1, 280], 20, 12, [6, 6]))
+    for k, (ff, gg, gx, gy, ws, st, r) in enumerate(cases):
+        x, y, u, v, cc = pkg.funIntegerSearch(ff, gg, r, gx, gy, ws, st)
+        s = orc.integer_search(ff, gg, gx, gy, ws, st, np.concatenate([r, r]), nthreads=8,
+                               qfactors=True)
+        q, qo = cc["qfactors"], s["qfactors"]
+        assert q.shape == qo.shape == (u.size, 2)
+        assert np.array_equal(np.isnan(q), np.isnan(qo))
+        fin = ~np.isnan(qo[:, 0])
+        d0 = np.abs(q[fin, 0] - qo[fin, 0]).max()
+        d1 = np.abs(q[fin, 1] - qo[fin, 1]).max()
+        print(f"case {k}: {u.size} points, NaN {np.isnan(qo[:, 0]).sum()}, "
+              f"max |dPCE| {d0:.2e}, max |dPPE| {d1:.2e}")
+        assert d0 <= 1e-4 and d1 <= 2e-3
+        if k == 2:
+            assert np.isnan(qo[:, 0]).sum() >= 1 and np.all(q[fin] == 0)
+
+
+def test_integer_search_quadtree_driver(pkg, orc):
+    """IntegerSearchQuadtree end to end (stereo radius, mask -> NaN, the
+    median test and the springs fill of funRemoveOutliers) against the
+    oracle's search followed by the dense restatement of the outlier pass."""
+    import inpaint_oracle
+    f, g, uf, vf = c1_pair()
+    M, N = f.shape
+    mask = hole_mask(M, N, (256, 256), 60)
+    DICpara = {"gridxyROIRange": {"gridx": [1, M], "gridy": [1, N]}, "winsize": 30,
+               "winstepsize": 16, "StereoSearchDistance": 10, "ImgRefMask": mask,
+               "ImgSize": (M, N), "showImgOrNot": 0}
+    P, x0, y0, u, v, cc = pkg.IntegerSearchQuadtree(f, g, None, DICpara, -1)
+    s = orc.integer_search(f, g, [1, M], [1, N], 30, 16, [10, 10, 10, 10], nthreads=8)
+    hole = mask[np.round(s["x"]).astype(int) - 1, np.round(s["y"]).astype(int) - 1] == 0
+    uo = np.where(hole, np.nan, s["u"])
+    vo = np.where(hole, np.nan, s["v"])
+    uo, vo, out = inpaint_oracle.funRemoveOutliers(uo, vo, 100.0)
+    assert hole.sum() > 20 and np.array_equal(np.isnan(u), np.isnan(uo))
+    fin = ~np.isnan(uo)
+    # sub-pixel peaks within the 1e-3 rounding of findpeak, as the search test
+    assert np.abs(u[fin] - uo[fin]).max() <= 1e-3 + 1e-9
+    assert np.abs(v[fin] - vo[fin]).max() <= 1e-3 + 1e-9
+    assert list(P["SizeOfFFTSearchRegion"]) == [10, 10] and cc["ccThreshold"] == 1.25

@@ -502,3 +502,97 @@ def test_icgn_mask_branch_window_growth(orc):
     # give other Hessians
     for other in (packed_gram(comp, 1, 2 * h), packed_gram(m, 0, 2 * h + 1)):
         assert np.abs(other - packed).max() > 1e-3 * np.abs(packed).max()
+
+
+def _ncc_definition(T, A):
+    """normxcorr2 by its definition on the zero-padded image: Pearson-type
+    coefficient of the mean-removed template with every template-sized
+    window of A padded by s - 1 zeros on each side (n = template size in
+    the window's moments)."""
+    s1, s2 = T.shape
+    W1, W2 = A.shape
+    P = np.zeros((W1 + 2 * (s1 - 1), W2 + 2 * (s2 - 1)))
+    P[s1 - 1:s1 - 1 + W1, s2 - 1:s2 - 1 + W2] = A
+    Tc = T - T.mean()
+    n = T.size
+    out = np.zeros((W1 + s1 - 1, W2 + s2 - 1))
+    for p in range(out.shape[0]):
+        for q in range(out.shape[1]):
+            w = P[p:p + s1, q:q + s2]
+            den = np.sqrt((Tc ** 2).sum() * max((w ** 2).sum() - w.sum() ** 2 / n, 0.0))
+            out[p, q] = (Tc * w).sum() / den if den > 1e-12 else 0.0
+    return out
+
+
+def test_normxcorr2_full_map_and_its_crop(orc):
+    rng = np.random.default_rng(41)
+    T = rng.normal(size=(7, 7))
+    A = rng.normal(size=(15, 12))
+    full = orc.normxcorr2_full(T, A)
+    want = _ncc_definition(T, A)
+    assert full.shape == (21, 18)
+    assert np.abs(full - want).max() < 1e-12
+    # the crop funIntegerSearch.m:166 takes: (ws : end - ws + 1), ws = s - 1
+    ws = 6
+    assert np.array_equal(orc.normxcorr2_crop(T, A), full[ws - 1:full.shape[0] - ws + 1,
+                                                          ws - 1:full.shape[1] - ws + 1])
+
+
+def test_qfactor_closed_forms(orc):
+    """PCE = peak^2 / mean((cc - min)^2); PPE = 1 / entropy of the 30-bin
+    histogram of cc - min, with histcounts' nice bin width: for a peak of 1
+    the bins are 0.034 wide (ll = 1/30, ul = 1/29: the slack 0.00115 puts the
+    rounding at 1e-3), not 1/30."""
+    e = orc.hist_edges(0.0, 1.0, 30)
+    assert np.allclose(e[:-1], 0.034 * np.arange(30)) and e[-1] == pytest.approx(1.02)
+    e = orc.hist_edges(0.0, 0.7346, 30)
+    assert e[1] == pytest.approx(0.0245) and e[-1] == pytest.approx(0.735)
+    # values at the 0.034-bin centres with known counts, plus 0 and 1
+    counts = {0: 5, 29: 3}
+    vals = [0.0] * 5 + [1.0] * 3
+    for k in range(1, 29):
+        vals += [(k + 0.5) * 0.034] * k
+        counts[k] = k
+    cc = np.array(vals) - 0.25  # the minimum is removed first
+    pce, ppe = orc.qfactor(cc)
+    c = np.array(list(counts.values()), dtype=float)
+    p = c / c.sum()
+    assert ppe == pytest.approx(1.0 / np.sum(p * np.log(1 / p)), rel=1e-12)
+    cm = cc - cc.min()
+    assert pce == pytest.approx(1.0 / np.mean(cm ** 2), rel=1e-12)
+    # with 1/30-wide bins some of those centres would share a bin: another
+    # entropy
+    k_alt = np.minimum(np.floor(np.array(vals) * 30), 29)
+    pa = np.unique(k_alt, return_counts=True)[1] / len(vals)
+    assert abs(1.0 / np.sum(pa * np.log(1 / pa)) - ppe) > 1e-3
+    # a two-valued map: entropy of the two bins
+    two = np.array([0.2] * 7 + [0.9] * 3)
+    assert orc.qfactor(two)[1] == pytest.approx(1.0 / -(0.7 * np.log(0.7) + 0.3 * np.log(0.3)))
+
+
+def test_search_qfactors_order_and_normalisation(orc):
+    """cc.qfactors rows follow the reference's point order (x index fastest),
+    each point's pair is compute_qFactor of its full normxcorr2 map (window
+    clamps of funIntegerSearch.m:139-152), and each column is shifted and
+    scaled to [0, 1] over the grid (:184-187)."""
+    f, g, _, _ = small_pair(shift=(1.3, -0.7))
+    M, N = f.shape
+    ws, step, r = 20, 16, 6
+    s = orc.integer_search(f, g, [1, M], [1, N], ws, step, [r] * 4, qfactors=True)
+    nY, nX = s["u"].shape
+    q = s["qfactors"]
+    assert q.shape == (nX * nY, 2)
+    raw = np.zeros((nX * nY, 2))
+    for t in range(nX * nY):
+        ci, cj = t % nX, t // nX
+        ii, jj = 1 + ci * step, 1 + cj * step
+        L = min(r, ii - 1)
+        U = min(r, jj - 1)
+        R = r if ii + ws + r <= M else max(M - ii - ws - 1, 0)
+        D = r if jj + ws + r <= N else max(N - jj - ws - 1, 0)
+        C = f[ii - 1:ii + ws, jj - 1:jj + ws]
+        Dw = g[ii - L - 1:ii + ws + R, jj - U - 1:jj + ws + D]
+        raw[t] = orc.qfactor(_ncc_definition(C, Dw))
+    want = (raw - raw.min(axis=0)) / (raw - raw.min(axis=0)).max(axis=0)
+    assert np.abs(q - want).max() < 1e-9
+    assert np.allclose(q.min(axis=0), 0) and np.allclose(q.max(axis=0), 1)
