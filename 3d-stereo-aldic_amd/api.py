@@ -729,6 +729,11 @@ class Plan:
         """Fix the launch shape of one size class (solver 0 IC-GN, 1 Subpb1)."""
         check(lib().aldic_plan_set_variant(self.ptr, int(solver), int(cls), int(variant)))
 
+    def set_stopping(self, rule: int, tol_dp: float = 0.0):
+        """IC-GN stopping rule of this plan's solves: 0 the reference's
+        relative ||b|| rule, 1 the step-size rule ||dP|| < tol_dp px."""
+        check(lib().aldic_plan_set_stopping(self.ptr, int(rule), float(tol_dp)))
+
     def close(self):
         if getattr(self, "ptr", None):
             lib().aldic_plan_destroy(self.ptr)

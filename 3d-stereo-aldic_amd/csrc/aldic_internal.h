@@ -95,6 +95,9 @@ struct aldic_plan {
   int* sub_cnt = nullptr;  // 5 counts + the number of entries dropped (list 4 full)
   // launch shape per size class and solver (0 IC-GN, 1 Subpb1)
   int var[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  // IC-GN stopping rule of the plan's solves (aldic_plan_set_stopping)
+  int stop_rule = 0;
+  float tol_dp = 0.0f;
 };
 
 // Stage timing: a pair of events around each launch of a stage on the

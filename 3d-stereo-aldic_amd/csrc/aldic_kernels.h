@@ -31,6 +31,8 @@ struct SolveArgs {
   float mu;
   float tol;
   int maxit;            // evaluation cap k <= maxit (0: the reference's 100)
+  int stop_rule;        // IC-GN: 0 the reference's relative ||b|| rule, 1 the step-size rule
+  float tol_dp;         // step-size rule threshold (px)
   const int* dcnt;      // fast kernels: list length read on the device (cnt = capacity)
   int fixed_side;       // fast kernels: every listed subset has the class's full side
   float* U;             // 2n

@@ -193,6 +193,10 @@ int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
   hipStream_t s = ctx->stream;
   RC_TRY(side_stream(ctx));
   a.n = (int)p->n;
+  if (mode == 0) {
+    a.stop_rule = p->stop_rule;
+    a.tol_dp = p->tol_dp;
+  }
   const int base = mode == 0 ? ALDIC_STAGE_ICGN_C0 : ALDIC_STAGE_SUBPB1_C0;
   zero_ints<<<1, 64, 0, s>>>(p->defer_cnt, 1);
   zero_ints<<<1, 64, 0, s>>>(p->defer_cnt2, 1);
@@ -926,6 +930,8 @@ extern "C" int aldic_local_icgn_list_dev(aldic_ctx* ctx, const aldic_images* im,
                           d_convIt, d_hole, d_zncc);
   a.defer_cnt = plan->defer_cnt;
   a.defer_list = plan->defer_list;
+  a.stop_rule = plan->stop_rule;
+  a.tol_dp = plan->tol_dp;
   StageTimer t(ctx, ALDIC_STAGE_FALLBACK);
   aldic::ClassLists cl;
   for (int c = 0; c < 5; c++) {
@@ -1025,6 +1031,8 @@ extern "C" int aldic_local_icgn_autotune_dev(aldic_ctx* ctx, const aldic_images*
   a.zncc = d_zncc;
   a.defer_cnt = plan->defer_cnt;
   a.defer_list = plan->defer_list;
+  a.stop_rule = plan->stop_rule;
+  a.tol_dp = plan->tol_dp;
   hipStream_t s = ctx->stream;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   HIP_TRY(hipEventCreate(&e0));
@@ -1077,6 +1085,16 @@ extern "C" int aldic_plan_set_variant(aldic_plan* plan, int32_t solver, int32_t 
       variant >= aldic::num_variants())
     return fail(ALDIC_EINVAL, "plan_set_variant: out of range");
   plan->var[solver][cls] = variant;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_plan_set_stopping(aldic_plan* plan, int32_t rule, double tol_dp) {
+  if (!plan || rule < 0 || rule > 1)
+    return fail(ALDIC_EINVAL, "plan_set_stopping: rule must be 0 (reference) or 1 (step size)");
+  if (rule == 1 && !(tol_dp > 0.0 && tol_dp < 1e3))
+    return fail(ALDIC_EINVAL, "plan_set_stopping: the step-size threshold must be positive");
+  plan->stop_rule = rule;
+  plan->tol_dp = rule == 1 ? (float)tol_dp : 0.0f;
   return ALDIC_OK;
 }
 

@@ -180,6 +180,18 @@ __device__ __forceinline__ bool ic_update(float P[6], const float d[6]) {
   return true;
 }
 
+// The step-size stopping rule: the update dP = [du/dx dv/dx du/dy dv/dy du
+// dv] moves the subset centre by (du, dv) and a point (dx, dy) from it by
+// (du/dx dx + du/dy dy, dv/dx dx + dv/dy dy); the rule bounds the root sum
+// of squares of the centre shift and the gradient terms at the half widths
+// (hx, hy): sqrt(du^2 + dv^2 + (du/dx^2 + dv/dx^2) hx^2 + (du/dy^2 +
+// dv/dy^2) hy^2) < tol_dp px.
+__device__ __forceinline__ bool dp_small(const float d[6], float hx, float hy, float tol_dp) {
+  const float m = d[4] * d[4] + d[5] * d[5] + (d[0] * d[0] + d[1] * d[1]) * hx * hx +
+                  (d[2] * d[2] + d[3] * d[3]) * hy * hy;
+  return m < tol_dp * tol_dp;
+}
+
 // ---------------------------------------------------------------------------
 // Group reductions: NW waves cooperate on one subset.
 // ---------------------------------------------------------------------------
@@ -504,6 +516,10 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   const int maxit = a.maxit > 0 ? a.maxit : 100;
   float normNew = 1.0f, normAbs = 1.0f, normInit = 0.0f;
   int step = 0;
+  // step-size rule (IC-GN only): converged once the applied update moves
+  // the subset by less than a.tol_dp px (centre and corners, see dp_small)
+  const bool dp_rule = MODE == MODE_ICGN && a.stop_rule == 1;
+  bool dp_conv = false;
   int tx0 = -1000000, ty0 = -1000000;  // tile origin (0-based image index)
   float last_zncc = __builtin_nanf("");
   bool err = false;
@@ -513,7 +529,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   const float lo_u = 3.0f - (float)x0, hi_u = (float)(M - 2 - x0);
   const float lo_v = 3.0f - (float)y0, hi_v = (float)(N - 2 - y0);
 
-  while (step <= maxit && normNew > tol && normAbs > tol) {
+  while (step <= maxit && (dp_rule || (normNew > tol && normAbs > tol))) {
     step++;
     const float A11 = 1.0f + P[0], A12 = P[2], A21 = P[1], A22 = 1.0f + P[3];
     // warp (:149-152) in displacement-relative form.  The warp is affine,
@@ -711,7 +727,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     normAbs = bnorm;
     if (step == 1) normInit = bnorm;
     normNew = (normInit > tol) ? normNew * __builtin_amdgcn_rcpf(normInit) : 0.0f;
-    if (normNew < tol || normAbs < tol) break;
+    if (!dp_rule && (normNew < tol || normAbs < tol)) break;
     float dP[6];
     if (MODE == MODE_ICGN) {
       float nb[6];
@@ -727,13 +743,17 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     if (!ic_update(P, dP)) break;
 #pragma unroll
     for (int k = 0; k < 6; k++) P[k] = uni(P[k]);
+    if (dp_rule && dp_small(dP, (float)hx, (float)hy, a.tol_dp)) {
+      dp_conv = true;
+      break;
+    }
   }
   int st;
   if (err) {
     st = ST_ERR;
   } else {
     st = step;
-    if (!(normNew < tol || normAbs < tol)) st = 101;
+    if (dp_rule ? !dp_conv : !(normNew < tol || normAbs < tol)) st = 101;
     if (normNew != normNew) st = 101;
     if (sabs < 1e-6f) st = 103;
   }
@@ -1032,11 +1052,13 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
       const int maxit = a.maxit > 0 ? a.maxit : 100;
       float normNew = 1.0f, normAbs = 1.0f, normInit = 0.0f;
       int step = 0;
+      const bool dp_rule = MODE == MODE_ICGN && a.stop_rule == 1;
+      bool dp_conv = false;
       bool have_bwg = false, err = false, defer_oob = false;
       float sum_bwg = 0.0f;
       const int ws0 = wx0;
       int cur_w = wx;
-      while (step <= maxit && normNew > tol && normAbs > tol) {
+      while (step <= maxit && (dp_rule || (normNew > tol && normAbs > tol))) {
         step++;
         if (MODE == MODE_ICGN && step > 1 && c0 > 0.0f) {
           // :139-145 -- a change of size throws at the next elementwise op
@@ -1220,7 +1242,7 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
         normAbs = bnorm;
         if (step == 1) normInit = bnorm;
         normNew = (normInit > tol) ? normNew / normInit : 0.0f;
-        if (normNew < tol || normAbs < tol) break;
+        if (!dp_rule && (normNew < tol || normAbs < tol)) break;
         float dP[6];
         if (MODE == MODE_ICGN) {
           float nb[6];
@@ -1237,6 +1259,10 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
         }
         if (!ic_update(P, dP)) break;
         __syncthreads();
+        if (dp_rule && dp_small(dP, 0.5f * (float)(S1 - 1), 0.5f * (float)(S2 - 1), a.tol_dp)) {
+          dp_conv = true;
+          break;
+        }
       }
       (void)defer_oob;
       if (err) {
@@ -1244,7 +1270,7 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
         break;
       }
       st = step;
-      if (!(normNew < tol || normAbs < tol)) st = 101;
+      if (dp_rule ? !dp_conv : !(normNew < tol || normAbs < tol)) st = 101;
       if (normNew != normNew) st = 101;
       // sum |tempf| with the current f selection
       float sa[1] = {0.0f};

@@ -217,6 +217,17 @@ int aldic_local_icgn_autotune_dev(aldic_ctx* ctx, const aldic_images* im, aldic_
                                   int32_t* d_convIt, float* d_zncc, int32_t reps, int32_t* chosen);
 /* Fix the launch shape of one size class (solver 0: IC-GN, 1: Subpb1). */
 int aldic_plan_set_variant(aldic_plan* plan, int32_t solver, int32_t cls, int32_t variant);
+/* Stopping rule of the plan's IC-GN solves (the _dev entry points; Subpb1
+ * keeps the reference's).  rule 0 (default): the reference's, ||b_k|| /
+ * ||b_1|| < tol or ||b_k|| < tol, tested before the update, which is then
+ * not applied (funICGNQuadtree.m:376-389).  rule 1: the step-size rule for
+ * throughput runs, no counterpart in the reference: the update is applied
+ * and the subset is converged when it moved the subset by less than tol_dp
+ * px, sqrt(du^2 + dv^2 + (du/dx^2 + dv/dx^2) hx^2 + (du/dy^2 + dv/dy^2)
+ * hy^2) < tol_dp with (hx, hy) the subset's half widths; the status is the
+ * number of evaluations, 101 if the cap comes first.  The ZNCC output is
+ * that of the last evaluated warp, before the last update. */
+int aldic_plan_set_stopping(aldic_plan* plan, int32_t rule, double tol_dp);
 
 /* Same as aldic_local_icgn_dev with a starting guess of the gradients:
  * P0 = [F0 U0] instead of [0 0 0 0 U0] (funICGNQuadtree.m:41).  d_F0 is 4n

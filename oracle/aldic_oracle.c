@@ -476,7 +476,7 @@ static int grow_winsize(double A, double B, int w) {
 int orc_icgn_subset(const orc_images* im, int64_t x0, int64_t y0, int winsize, double tol,
                     const double U0[2], double U[2], double F[4], double H21[21],
                     double* zncc) {
-  return orc_icgn_subset_ex(im, x0, y0, winsize, tol, 100, U0, NULL, U, F, H21, zncc);
+  return orc_icgn_subset_ex(im, x0, y0, winsize, tol, 100, 0, 0.0, U0, NULL, U, F, H21, zncc);
 }
 
 /* The loop of :135 runs while k <= maxIterNum (k counts evaluations, so at
@@ -485,8 +485,9 @@ int orc_icgn_subset(const orc_images* im, int64_t x0, int64_t y0, int winsize, d
  * + 2, + 3 with maxIterNum = 100) whatever cap is passed.  F0 (NULL =
  * zeros) starts P at [F0 U0] instead of [0 0 0 0 U0] (:41). */
 int orc_icgn_subset_ex(const orc_images* im, int64_t x0, int64_t y0, int winsize, double tol,
-                       int maxit, const double U0[2], const double F0[4], double U[2],
-                       double F[4], double H21[21], double* zncc) {
+                       int maxit, int stop_rule, double tol_dp, const double U0[2],
+                       const double F0[4], double U[2], double F[4], double H21[21],
+                       double* zncc) {
   const int maxIterNum = maxit;
   const int winsize0 = winsize;
   double P[6] = {0, 0, 0, 0, U0[0], U0[1]};
@@ -561,10 +562,12 @@ int orc_icgn_subset_ex(const orc_images* im, int64_t x0, int64_t y0, int winsize
 
     double normNew = 1.0, normAbs = 1.0, normInit = 0.0;
     int step = 0;
+    const int dp_rule = (stop_rule == 1);
+    int dp_conv = 0;
     int have_bwg = 0;
     int64_t sum_bwg = 0;
     double last_zncc = NAN;
-    while (step <= maxIterNum && normNew > tol && normAbs > tol) {
+    while (step <= maxIterNum && (dp_rule || (normNew > tol && normAbs > tol))) {
       step++;
       if (step > 1 && c0 > 0) {
         /* :139-145: the window is re-sized from the last g mask, but f,
@@ -672,14 +675,26 @@ int orc_icgn_subset_ex(const orc_images* im, int64_t x0, int64_t y0, int winsize
         normNew = normNew / normInit;
       else
         normNew = 0.0;
-      if (normNew < tol || normAbs < tol) break;
+      if (!dp_rule && (normNew < tol || normAbs < tol)) break;
       double dP[6], nbv[6];
       for (int p = 0; p < 6; p++) nbv[p] = -bvec[p];
       solve_sym(6, H, nbv, dP);
       if (!ic_update(P, dP)) break;
+      if (dp_rule) {
+        /* the step-size rule (no counterpart in the reference): the update
+         * just applied moves the centre by (dP5, dP6) and the half-width
+         * points by the gradient terms */
+        const double hh = (double)(winsize / 2);
+        const double m = dP[4] * dP[4] + dP[5] * dP[5] + (dP[0] * dP[0] + dP[1] * dP[1]) * hh * hh +
+                         (dP[2] * dP[2] + dP[3] * dP[3]) * hh * hh;
+        if (m < tol_dp * tol_dp) {
+          dp_conv = 1;
+          break;
+        }
+      }
     }
     status = step;
-    if (!(normNew < tol || normAbs < tol)) status = ORC_NOCONV;
+    if (dp_rule ? !dp_conv : !(normNew < tol || normAbs < tol)) status = ORC_NOCONV;
     if (isnan(normNew)) status = ORC_NOCONV;
     {
       double sa = 0.0;
@@ -932,15 +947,15 @@ void orc_local_icgn(const orc_images* im, const double* coords, const double* U0
                     const int32_t* winsize, int winsize_per_node, double tol, int nthreads,
                     double* U, double* F, double* H, int32_t* convIt, uint8_t* markHole,
                     double* zncc) {
-  orc_local_icgn_ex(im, coords, U0, NULL, n, winsize, winsize_per_node, tol, 100, nthreads, U,
-                    F, H, convIt, markHole, zncc);
+  orc_local_icgn_ex(im, coords, U0, NULL, n, winsize, winsize_per_node, tol, 100, 0, 0.0,
+                    nthreads, U, F, H, convIt, markHole, zncc);
 }
 
 void orc_local_icgn_ex(const orc_images* im, const double* coords, const double* U0,
                        const double* F0, int64_t n, const int32_t* winsize,
-                       int winsize_per_node, double tol, int maxit, int nthreads, double* U,
-                       double* F, double* H, int32_t* convIt, uint8_t* markHole,
-                       double* zncc) {
+                       int winsize_per_node, double tol, int maxit, int stop_rule,
+                       double tol_dp, int nthreads, double* U, double* F, double* H,
+                       int32_t* convIt, uint8_t* markHole, double* zncc) {
   coef_init();
 #ifdef _OPENMP
   if (nthreads < 1) nthreads = 1;
@@ -970,8 +985,8 @@ void orc_local_icgn_ex(const orc_images* im, const double* coords, const double*
     }
     double Ut[2], Ft[4], Ht[21], zt;
     const double u0[2] = {U0[2 * t], U0[2 * t + 1]};
-    const int st = orc_icgn_subset_ex(im, x0, y0, ws, tol, maxit, u0, F0 ? F0 + 4 * t : NULL,
-                                      Ut, Ft, Ht, &zt);
+    const int st = orc_icgn_subset_ex(im, x0, y0, ws, tol, maxit, stop_rule, tol_dp, u0,
+                                      F0 ? F0 + 4 * t : NULL, Ut, Ft, Ht, &zt);
     convIt[t] = st;
     U[2 * t] = Ut[0];
     U[2 * t + 1] = Ut[1];

@@ -62,14 +62,18 @@ int orc_icgn_subset(const orc_images* im, int64_t x0, int64_t y0, int winsize,
                     double tol, const double U0[2], double U[2], double F[4],
                     double H21[21], double* zncc);
 
-/* orc_icgn_subset with the evaluation cap and the start as parameters: the
- * loop runs while k <= maxit (maxit >= 0; the reference's cap is 100, so
- * at most maxit + 1 evaluations), and F0 (NULL = zeros) starts P at
- * [F0 U0].  Used to read the warp after k updates and
- * to check the learned starting point's IC-GN; statuses as above. */
+/* orc_icgn_subset with the evaluation cap, the stopping rule and the start
+ * as parameters: the loop runs while k <= maxit (maxit >= 0; the
+ * reference's cap is 100, so at most maxit + 1 evaluations); stop_rule 0
+ * is the reference's relative ||b|| rule, 1 the step-size rule (the
+ * applied update moves the subset by less than tol_dp px, the same measure
+ * as the library's aldic_plan_set_stopping); F0 (NULL = zeros) starts P at
+ * [F0 U0].  Used to read the warp after k updates and to check the learned
+ * starting point's IC-GN; statuses as above. */
 int orc_icgn_subset_ex(const orc_images* im, int64_t x0, int64_t y0, int winsize,
-                       double tol, int maxit, const double U0[2], const double F0[4],
-                       double U[2], double F[4], double H21[21], double* zncc);
+                       double tol, int maxit, int stop_rule, double tol_dp,
+                       const double U0[2], const double F0[4], double U[2], double F[4],
+                       double H21[21], double* zncc);
 
 /* funICGN_Subpb1quadtree.m:1-476 for one node.  Returns the status (ORC_ERR on
  * a MATLAB error). */
@@ -89,12 +93,12 @@ void orc_local_icgn(const orc_images* im, const double* coords, const double* U0
                     int32_t* convIt, uint8_t* markHole, double* zncc);
 
 /* orc_local_icgn from P0 = [F0 U0] (F0 4n, NULL = zeros) with the
- * evaluation cap maxit (>= 0) of orc_icgn_subset_ex. */
+ * evaluation cap maxit (>= 0) and the stopping rule of orc_icgn_subset_ex. */
 void orc_local_icgn_ex(const orc_images* im, const double* coords, const double* U0,
                        const double* F0, int64_t n, const int32_t* winsize,
-                       int winsize_per_node, double tol, int maxit, int nthreads, double* U,
-                       double* F, double* H, int32_t* convIt, uint8_t* markHole,
-                       double* zncc);
+                       int winsize_per_node, double tol, int maxit, int stop_rule,
+                       double tol_dp, int nthreads, double* U, double* F, double* H,
+                       int32_t* convIt, uint8_t* markHole, double* zncc);
 
 /* Subpb1Quadtree.m:43-165 loop over n nodes.  winsize_xy is n x 2
  * column-major (DICpara.winsize_List). */

@@ -64,8 +64,8 @@ def lib():
         L.orc_local_icgn.argtypes = [C.POINTER(_Images), _dp, _dp, _i64, _i32, C.c_int,
                                      C.c_double, C.c_int, _dp, _dp, _dp, _i32, _u8, _dp]
         L.orc_local_icgn_ex.argtypes = [C.POINTER(_Images), _dp, _dp, C.c_void_p, _i64, _i32,
-                                        C.c_int, C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
-                                        _i32, _u8, _dp]
+                                        C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
+                                        C.c_int, _dp, _dp, _dp, _i32, _u8, _dp]
         L.orc_subpb1.argtypes = [C.POINTER(_Images), _dp, _dp, _dp, _dp, _dp, _i64, _i32,
                                  C.c_double, C.c_double, C.c_int, _dp, _i32]
         L.orc_normxcorr2_crop.argtypes = [_dp, _i64, _i64, _dp, _i64, _i64, _dp]
@@ -167,12 +167,14 @@ def subpb1_subset(im: Images, x0, y0, wx, wy, mu, udual, vdual, UOld, FOld, tol)
     return U, st
 
 
-def local_icgn(im: Images, coords, U0, winsize, tol, nthreads=1, F0=None, max_iter=None):
+def local_icgn(im: Images, coords, U0, winsize, tol, nthreads=1, F0=None, max_iter=None,
+               stop_rule=0, tol_dp=0.0):
     """LocalICGNQuadtree's per-node loop (no bad-point rule, no NaN fill).
 
     F0 (4n, optional) starts each node at P0 = [F0 U0]; max_iter (>= 0)
     runs the loop while k <= max_iter (None: the reference's 100), i.e. at
-    most max_iter + 1 evaluations.
+    most max_iter + 1 evaluations; stop_rule 1 replaces the reference's
+    relative ||b|| rule by the step-size rule ||dP|| < tol_dp px.
     Returns U (2n), F (4n), H (n x 21), convIt (n), markHole (n), zncc (n)."""
     coords = _F(coords)
     n = coords.shape[0]
@@ -185,7 +187,7 @@ def local_icgn(im: Images, coords, U0, winsize, tol, nthreads=1, F0=None, max_it
     conv = np.zeros(n, dtype=np.int32, order="F")
     hole = np.zeros(n, dtype=np.uint8, order="F")
     z = np.zeros(n, order="F")
-    if F0 is None and max_iter is None:
+    if F0 is None and max_iter is None and stop_rule == 0:
         lib().orc_local_icgn(C.byref(im.s), coords, _F(U0), n, ws, per, float(tol),
                              int(nthreads), U, Fo, H, conv, hole, z)
     else:
@@ -194,7 +196,8 @@ def local_icgn(im: Images, coords, U0, winsize, tol, nthreads=1, F0=None, max_it
             raise ValueError("F0 must hold 4 values per node")
         lib().orc_local_icgn_ex(C.byref(im.s), coords, _F(U0), None if f0 is None else
                                 f0.ctypes.data, n, ws, per, float(tol),
-                                100 if max_iter is None else int(max_iter),
+                                100 if max_iter is None else int(max_iter), int(stop_rule),
+                                float(tol_dp),
                                 int(nthreads), U, Fo, H, conv, hole, z)
     return U, Fo, H, conv, hole, z
 

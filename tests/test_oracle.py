@@ -596,3 +596,46 @@ def test_search_qfactors_order_and_normalisation(orc):
     want = (raw - raw.min(axis=0)) / (raw - raw.min(axis=0)).max(axis=0)
     assert np.abs(q - want).max() < 1e-9
     assert np.allclose(q.min(axis=0), 0) and np.allclose(q.max(axis=0), 1)
+
+
+def _warp_matrix(P):
+    """x' = W(P) [dx dy 1]^T of funICGNQuadtree.m:149-152, P = [ux vx uy vy u v]."""
+    return np.array([[1 + P[0], P[2], P[4]], [P[1], 1 + P[3], P[5]], [0.0, 0.0, 1.0]])
+
+
+def test_icgn_step_size_rule(orc):
+    """Stop rule 1 (the throughput rule, no counterpart in the reference):
+    the returned count is the first evaluation whose applied update dP moves
+    the subset by less than tol_dp px, sqrt(du^2 + dv^2 + (du/dx^2 +
+    dv/dx^2) h^2 + (du/dy^2 + dv/dy^2) h^2) with h the half width, and the
+    returned warp includes that update.  dP is recovered here from the warps
+    of capped runs through the inverse-compositional relation W(P_j) =
+    W(P_{j-1}) W(dP_j)^-1 (funICGNQuadtree.m:396-414)."""
+    f, g, _, _ = small_pair(shift=(0.45, -0.3), strain=0.003)
+    im = orc.Images(f, g)
+    tol_dp = 1e-3
+    for (x0, y0, ws) in ((70, 64, 20), (90, 80, 30), (100, 70, 24)):
+        c = np.array([[float(x0), float(y0)]])
+        wsa = np.array([ws], np.int32)
+        U0 = np.array([0.0, 0.0])
+        U, F, H, conv, hole, z = orc.local_icgn(im, c, U0, wsa, 1e-3, stop_rule=1, tol_dp=tol_dp)
+        s = int(conv[0])
+        assert 2 < s <= 100, s
+        Ps = [np.zeros(6)]
+        for j in range(1, s):
+            Uj, Fj, _, cj, _, _ = orc.local_icgn(im, c, U0, wsa, 1e-3, max_iter=j - 1,
+                                                 stop_rule=1, tol_dp=tol_dp)
+            assert cj[0] == 101
+            Ps.append(np.concatenate([Fj, Uj]))
+        Ps.append(np.concatenate([F, U]))
+        h = ws // 2
+        moves = []
+        for j in range(1, s + 1):
+            Wd = np.linalg.solve(_warp_matrix(Ps[j]), _warp_matrix(Ps[j - 1]))
+            d = np.array([Wd[0, 0] - 1, Wd[1, 0], Wd[0, 1], Wd[1, 1] - 1, Wd[0, 2], Wd[1, 2]])
+            moves.append(np.sqrt(d[4] ** 2 + d[5] ** 2 + (d[0] ** 2 + d[1] ** 2) * h * h +
+                                 (d[2] ** 2 + d[3] ** 2) * h * h))
+        assert moves[-1] < tol_dp and all(mv >= tol_dp for mv in moves[:-1]), moves
+        # the reference rule on the same node stops elsewhere or at another warp
+        Ur, Fr, _, cr, _, _ = orc.local_icgn(im, c, U0, wsa, 1e-3)
+        assert cr[0] != s or np.abs(Ur - U).max() > 1e-9
