@@ -26,13 +26,18 @@ def uniform_grid(gridx, gridy, winsize, winstepsize):
     return np.asfortranarray(coords), len(XList), len(YList)
 
 
-def hole_nodes(M, N, centre, radius, step=8, ws_min=8, ws_max=64, margin=None):
+def hole_nodes(M, N, centre, radius, step=8, ws_min=8, ws_max=64, margin=None,
+               extra=(0, 0, 0, 0)):
     """Returns (coords n x 2, winsize n) for nodes outside the hole whose
-    subsets (grown up to 1.3x for masked pixels) stay inside the image."""
+    subsets (grown up to 1.3x for masked pixels) stay inside the image.
+    ``extra`` widens the margin at the low-x, high-x, low-y and high-y
+    edges, e.g. by the largest disparity towards that edge, so that the
+    deformed subsets stay inside the image as well (the ROI a user sets
+    for a stereo pair)."""
     if margin is None:
         margin = int(0.65 * ws_max) + 8
-    xs = np.arange(margin, M - margin + 1, step, dtype=np.float64)
-    ys = np.arange(margin, N - margin + 1, step, dtype=np.float64)
+    xs = np.arange(margin + extra[0], M - margin - extra[1] + 1, step, dtype=np.float64)
+    ys = np.arange(margin + extra[2], N - margin - extra[3] + 1, step, dtype=np.float64)
     X, Y = np.meshgrid(xs, ys, indexing="ij")
     d = np.hypot(X - centre[0], Y - centre[1]) - radius
     keep = d > 0

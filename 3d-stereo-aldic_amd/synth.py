@@ -134,3 +134,18 @@ def field_stereo(M, N, shift=40.0, spread=6.0, curvature=0.005):
 def field_translation_stretch(stretch=0.001, ty=0.7, M=2048):
     xc = (M + 1) / 2.0
     return (lambda X, Y: stretch * (X - xc)), (lambda X, Y: ty + 0.0 * X)
+
+
+def field_bulge(M, N, shift=25.0, peak=15.0, sigma=600.0):
+    """Disparity of a plane with a Gaussian bulge seen by a second camera:
+    a uniform x-shift plus ``peak`` px at the centre decaying with
+    ``sigma`` px (up to shift + peak = 40 px), no y component."""
+    xc, yc = (M + 1) / 2.0, (N + 1) / 2.0
+
+    def u(X, Y):
+        return shift + peak * np.exp(-((X - xc) ** 2 + (Y - yc) ** 2) / (2.0 * sigma ** 2))
+
+    def v(X, Y):
+        return 0.0 * X
+
+    return u, v

@@ -1,42 +1,52 @@
 """Benchmark of ALDIC's local matching path on one MI355X.
 
-Workload (the stereo configuration the metric is quoted on): a 2048 x 2048
-12-bit speckle pair (0.015 spots/px^2, seed 2); the right view is the left
-one under a tilted-plane disparity (40 px x-shift +/- 6 px foreshortening);
-~60k nodes on an 8 px grid outside a centred circular hole of radius
-300 px, each node with its own subset size from 8 px at the hole edge up to
-64 px away from it.  One step = the whole local matching pass over that
-batch with inputs resident in HBM:
+Headline workload (the large stereo configuration the metric is quoted on,
+learned vs search start head to head): a 4096 x 3000 12-bit speckle pair
+(0.01 spots/px^2, seed 5); the right view is the left one under a 25 px
+x-disparity plus a Gaussian bulge of 15 px peak (sigma 600 px), so up to
+40 px; ~175k nodes on an 8 px grid outside a centred circular hole of
+radius 300 px, each node with its own subset size from 8 px at the hole
+edge up to 64 px away from it (the ROI's high-x margin widened by the
+largest disparity, so that deformed subsets stay inside the image).  The
+2048 x 2048 tilted-plane pair (0.015 spots/px^2, seed 2, 40 px +/- 6 px
+disparity, ~55k nodes) is reported beside it.
 
-  integer search (ZNCC, radius 50, 33 px templates on a 32 px grid)
-  -> initial guess of every node from the search grid
-  -> first-order IC-GN of every node (tol 1e-3, <= 101 evaluations).
+One step = the whole local matching pass over one pair with inputs resident
+in HBM:
 
-The learned-init step replaces the dense search:
+  search start:  integer search (ZNCC, radius 50, 33 px templates on a
+                 32 px grid) -> initial guess of every node from the search
+                 grid -> first-order IC-GN of every node (tol 1e-3, <= 101
+                 evaluations);
+  learned start: sparse integer search (same templates and radius, 256 px
+                 grid) -> coarse prior at every node -> CNN on the 48 x 48
+                 patch pair at the node and at node + round(prior) -> [F0 U0]
+                 -> IC-GN (at most 31 evaluations) -> gate (more than 30
+                 evaluations or ZNCC < 0.9) -> the search start of the failed
+                 nodes (only the grid points they read) -> IC-GN rerun.
 
-  sparse integer search (same templates and radius, 256 px grid)
-  -> coarse prior at every node -> CNN on the 48 x 48 patch pair at the
-  node and at node + round(prior) -> [F0 U0] -> IC-GN
-  -> gate (more than 30 evaluations or ZNCC < 0.9) -> the search-init
-  start of the failed nodes (only the 32 px grid points they read) ->
-  IC-GN rerun of those nodes.
+Each start runs with two IC-GN stopping rules: the reference's relative
+||b|| rule (parity) and the step-size rule (the update moves the subset by
+less than 1e-3 px; aldic_plan_set_stopping).  The network is trained first
+(not timed) on the device, on 2M patch pairs labelled by IC-GN on seeded
+speckle mosaics.
 
-The network is trained first (not timed) on the device, on 2M patch pairs
-labelled by IC-GN on seeded speckle mosaics.
-
-Converged = status 1..100, ZNCC >= 0.9.  The JSON line reports converged
-subsets/s of the learned-init step (value) and of the search-init step
-(fft_init), the RMS error of the converged subsets against the known field,
-the roofline of the dominant kernel (its HIP-event time, algorithmic bytes)
-and, on rank 0, the FP64 CPU restatement (oracle, OpenMP) timed on the same
-workload on the host cores.
+Converged = status 1..100, ZNCC >= 0.9 and not a bad point by the
+reference's rule (ConvIt > 99 or > max(mean + std, 6) over the good ones,
+LocalICGNQuadtree.m:167-192).  The JSON line's value is the converged
+subsets/s of the learned start with the step-size rule on the headline
+workload; the other starts, rules and the second workload sit beside it,
+each with its RMS error against the known field.  The roofline is the
+dominant kernel of that step (its HIP-event time, algorithmic bytes); on
+rank 0 the FP64 CPU restatement (oracle, OpenMP) runs the search-start path
+with the reference's rule on the same pair.
 """
 from __future__ import annotations
 
 import argparse
-import importlib.util
 import json
 import os
+import platform
 import sys
 import time
 
@@ -50,12 +60,12 @@ import __graft_entry__ as ge  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 ROOFLINE_REPS = 5
-# HBM bytes per launch of the dominant IC-GN kernel (65 px class, this
-# workload, search start), rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in KiB per
-# dispatch, each scaled by the same counter's reading on a 16 MiB
-# device-to-device copy (FETCH 8207.2 KiB -> x1.996, WRITE 16384 KiB -> x1);
-# measured with this bench (profiles/icgn_class3_hbm_traffic.md)
-ICGN_C3_TRAFFIC = (246617.7 * 16384.0 / 8207.2 + 6690.0 * 1.0) * 1024.0
+TOL_DP = 1e-3  # step-size rule threshold (px)
+# HBM bytes per launch of the dominant IC-GN kernel (65 px class, learned
+# start, step-size rule, headline workload), rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE in KiB per dispatch, each scaled by the same counter's reading
+# on a device-to-device copy of known size; None until measured
+ICGN_C3_TRAFFIC = None
 
 
 def load(name, rel):
@@ -78,24 +88,78 @@ def icgn_bytes(side):
     return 12.0 * s * s + 4.0 * (s + 8) ** 2 + 32.0
 
 
-def make_workload(seed=2):
+WORKLOADS = {
+    "bulge4096x3000": dict(M=4096, N=3000, density=0.01, seed=5, field="bulge", max_disp=40),
+    "tilt2048": dict(M=2048, N=2048, density=0.015, seed=2, field="stereo", max_disp=46),
+}
+
+
+def make_workload(name):
     synth = load("aldic_synth_b", "3d-stereo-aldic_amd/synth.py")
     mesh = load("aldic_mesh_b", "3d-stereo-aldic_amd/mesh.py")
-    M = N = 2048
-    uf, vf = synth.field_stereo(M, N)
-    f12, g12, _ = synth.make_pair(M, N, uf, vf, density=0.015, seed=seed, bits=12)
+    spec = WORKLOADS[name]
+    M, N = spec["M"], spec["N"]
+    uf, vf = (synth.field_bulge(M, N) if spec["field"] == "bulge" else synth.field_stereo(M, N))
+    f12, g12, _ = synth.make_pair(M, N, uf, vf, density=spec["density"], seed=spec["seed"],
+                                  bits=12)
     f = (f12 - f12.mean()) / f12.std(ddof=1)
     g = (g12 - g12.mean()) / g12.std(ddof=1)
-    coords, ws = mesh.hole_nodes(M, N, (1024.5, 1024.5), 300.0)
-    return dict(M=M, N=N, f=np.asfortranarray(f), g=np.asfortranarray(g), coords=coords, ws=ws,
-                uf=uf, vf=vf, search_ws=32, search_step=32, radius=50)
+    centre = ((M + 1) / 2.0, (N + 1) / 2.0)
+    coords, ws = mesh.hole_nodes(M, N, centre, 300.0, extra=(0, spec["max_disp"] + 4, 0, 0))
+    return dict(name=name, M=M, N=N, f=np.asfortranarray(f), g=np.asfortranarray(g),
+                coords=coords, ws=ws, uf=uf, vf=vf, search_ws=32, search_step=32, radius=50,
+                spec=spec)
+
+
+def window_mean_truth(W, fun):
+    """The known field averaged over each node's square subset (a 9 x 9
+    Gauss-Legendre rule): what a first-order subset can represent of a
+    curved field, beside the pointwise value."""
+    x, wq = np.polynomial.legendre.leggauss(9)
+    cx, cy = W["coords"][:, 0], W["coords"][:, 1]
+    h = W["ws"] / 2.0
+    acc = np.zeros(len(cx))
+    for a, wa in zip(x, wq):
+        for b, wb in zip(x, wq):
+            acc += wa * wb * fun(cx + a * h, cy + b * h)
+    return acc / 4.0
+
+
+def host_info(threads):
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": avail,
+            "threads": threads}
+
+
+def cpu_threads():
+    """Every core this process may run on, at most the 16 of the host's
+    share for one GPU (OMP_NUM_THREADS when it says fewer)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    t = min(avail, 16)
+    return min(t, env) if env > 0 else t
 
 
 def cpu_baseline(W, n_sample, threads, U_gpu=None):
     """The FP64 oracle (C + OpenMP, test infrastructure) on the same pair:
-    the whole search grid, then IC-GN of a random sample of nodes.  Also
-    returns the oracle's error against the known field on that sample and
-    its distance to the GPU result there."""
+    the whole search grid, then IC-GN (reference rule) of a sample of nodes
+    (all when n_sample is 0).  Also returns the oracle's accuracy on that
+    sample and its distance to the GPU's search-start result there."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     f, g = W["f"], W["g"]
@@ -106,12 +170,11 @@ def cpu_baseline(W, n_sample, threads, U_gpu=None):
                            nthreads=threads)
     t_search = time.perf_counter() - t0
     nY, nX = s["u"].shape
-    rng = np.random.default_rng(0)
     n = len(W["ws"])
     if n_sample <= 0 or n_sample >= n:
         idx = np.arange(n)
     else:
-        idx = np.sort(rng.choice(n, size=n_sample, replace=False))
+        idx = np.sort(np.random.default_rng(0).choice(n, size=n_sample, replace=False))
     coords = np.asfortranarray(W["coords"][idx])
     x0 = W["search_ws"] / 2 + 1
     fx = np.clip((coords[:, 0] - x0) / W["search_step"], 0, nX - 1)
@@ -128,23 +191,28 @@ def cpu_baseline(W, n_sample, threads, U_gpu=None):
     t0 = time.perf_counter()
     U, F, H, conv, hole, z = orc.local_icgn(im, coords, U0, W["ws"][idx], 1e-3, nthreads=threads)
     t_icgn = time.perf_counter() - t0
-    conv_ok = (conv >= 1) & (conv <= 100) & (z >= 0.9)
-    # the search cost scales with the share of nodes in the sample
-    t_total = t_icgn + t_search * len(idx) / n
+    pkg = ge.load_package()
+    bad, _ = pkg.icgn_bad_points(conv)
+    ok = (conv >= 1) & (conv <= 100) & (z >= 0.9)
+    ok[bad] = False
+    t_total = t_icgn + t_search * len(idx) / n  # the search pro-rated to the sample
     e = np.hypot(U[0::2] - W["uf"](coords[:, 0], coords[:, 1]),
                  U[1::2] - W["vf"](coords[:, 0], coords[:, 1]))
-    diag = {"oracle_rms_px": float(np.sqrt(np.mean(e[conv_ok] ** 2))),
-            "oracle_converged_frac": float(conv_ok.mean())}
+    diag = {"oracle_rms_px": float(np.sqrt(np.mean(e[ok] ** 2))),
+            "oracle_converged_frac": float(ok.mean()), "nodes": int(len(idx))}
     if U_gpu is not None:
         du = np.abs(U_gpu.reshape(-1, 2)[idx] - U.reshape(-1, 2)).max(axis=1)
-        diag["gpu_vs_oracle_max_du_px"] = float(np.nanmax(du[conv_ok]))
-    return diag, {"value": float(conv_ok.sum() / t_total), "unit": "converged subsets/s",
-            "cores": threads, "kind": "port",
-            "sample": (f"FP64 oracle, search-init path: integer search over the full "
-                       f"{nX}x{nY} grid ({t_search:.1f} s"
-                       + (", pro-rated to the sample" if len(idx) < n else "")
-                       + f") + IC-GN of {len(idx)} of the {n} nodes ({t_icgn:.1f} s), "
-                       f"{threads} OpenMP threads")}
+        both = ok & np.isfinite(du)
+        diag["gpu_vs_oracle_max_du_px"] = float(np.nanmax(du[both]))
+        diag["gpu_vs_oracle_p99_du_px"] = float(np.quantile(du[both], 0.99))
+    info = host_info(threads)
+    return diag, {"value": float(ok.sum() / t_total), "unit": "converged subsets/s",
+                  "cores": threads, "kind": "port", **info,
+                  "sample": (f"FP64 oracle, search-start path, reference stopping rule: integer "
+                             f"search over the full {nX}x{nY} grid ({t_search:.1f} s"
+                             + (", pro-rated to the sample" if len(idx) < n else "")
+                             + f") + IC-GN of {len(idx)} of the {n} nodes ({t_icgn:.1f} s), "
+                             f"{threads} OpenMP threads")}
 
 
 def progress(msg):
@@ -156,24 +224,31 @@ def subset_class(side):
                     np.where(side <= 65, 3, 4))))
 
 
-def accuracy(W, U, conv, z):
-    """Converged mask, overall RMS and per-size statistics against the field."""
+def accuracy(api, W, U, conv, z, truth_mean):
+    """Converged mask (status 1..100, ZNCC >= 0.9, not a bad point by the
+    reference's rule), RMS against the pointwise field and against its
+    subset means, and per-size statistics."""
     cx, cy = W["coords"][:, 0], W["coords"][:, 1]
-    ok = (conv >= 1) & (conv <= 100) & (z >= 0.9)
-    eu = U[0::2] - W["uf"](cx, cy)
-    ev = U[1::2] - W["vf"](cx, cy)
-    e = np.hypot(eu, ev)
-    rms = float(np.sqrt(np.mean(e[ok] ** 2))) if ok.any() else float("nan")
+    bad, _ = api.icgn_bad_points(conv)
+    status_ok = (conv >= 1) & (conv <= 100) & (z >= 0.9)
+    ok = status_ok.copy()
+    ok[bad] = False
+    e = np.hypot(U[0::2] - W["uf"](cx, cy), U[1::2] - W["vf"](cx, cy))
+    em = np.hypot(U[0::2] - truth_mean[0], U[1::2] - truth_mean[1])
+
+    def rms(a, sel):
+        return float(np.sqrt(np.mean(a[sel] ** 2))) if sel.any() else float("nan")
+
     by_size = {}
     for w in np.unique(W["ws"]):
         sel = W["ws"] == w
         oks = ok & sel
         by_size[int(w)] = {"nodes": int(sel.sum()), "converged": int(oks.sum()),
-                           "rms_px": round(float(np.sqrt(np.mean(e[oks] ** 2))), 5) if oks.any()
-                           else None,
-                           "median_px": round(float(np.median(e[oks])), 5) if oks.any() else None,
-                           "mean_iters": round(float(conv[oks].mean()), 2) if oks.any() else None}
-    return ok, rms, by_size
+                           "rms_px": round(rms(e, oks), 5),
+                           "rms_vs_subset_mean_px": round(rms(em, oks), 5),
+                           "mean_evals": round(float(conv[oks].mean()), 3) if oks.any() else None}
+    return dict(ok=ok, status_ok=int(status_ok.sum()), bad_points=int(len(bad)),
+                rms=rms(e, ok), rms_mean=rms(em, ok), by_size=by_size)
 
 
 def timed(ctxs, api, step, steps, warmup):
@@ -311,51 +386,24 @@ def global_step_mode(pkg, args):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=0,
-                    help="IC-GN nodes of the CPU baseline (0 = all)")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--train-samples", type=int, default=2_000_000)
-    ap.add_argument("--init", choices=["both", "search", "learned"], default="both")
-    ap.add_argument("--global-step", action="store_true",
-                    help="diagnostic: time the rows around the local step on the stereo "
-                         "workload (quadtree mesh, Subpb2, NaN fill, raw decode) and exit "
-                         "(no metric line)")
-    ap.add_argument("--cap-sweep", default="",
-                    help="diagnostic: comma-separated IC-GN evaluation caps; times the "
-                         "search-start IC-GN at each cap and exits (no metric line)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: the local matching path is "
-                          "measured on one MI355X (see DESIGN.md for how it would shard)",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
 
-    pkg = ge.load_package()
+
+RULES = {"reference": (0, 0.0), "step": (1, TOL_DP)}
+
+
+def run_workload(pkg, W, params, args, headline):
+    """Every start x stopping rule on one workload; returns the summaries,
+    the roofline of the headline configuration's dominant kernel and the
+    search start's U (reference rule) for the CPU comparison."""
     api = pkg.api
-    if args.global_step:
-        global_step_mode(pkg, args)
-        return
     from aldic_amd import learned
-    t0 = time.perf_counter()
-    W = make_workload()
-    t_gen = time.perf_counter() - t0
-    progress(f"workload ready ({t_gen:.1f} s, {len(W['ws'])} nodes)")
     M, N = W["M"], W["N"]
     n = len(W["ws"])
-    ctx = pkg.default_context()
+    ctx = api.Context(0)
     DA = api.DeviceArray
-
-    # ---- inputs resident in HBM ----
     d_f = DA.from_host(ctx, W["f"], np.float32)
     d_g = DA.from_host(ctx, W["g"], np.float32)
     im = api.Images.from_device(d_f.ptr, d_g.ptr, M, N, ctx=ctx)
-    plan = api.Plan(W["ws"], n, ctx=ctx)
     d_coords = DA.from_host(ctx, W["coords"], np.float64)  # n x 2 column-major
     d_ws = DA.from_host(ctx, W["ws"], np.int32)
     sws, sst, r = W["search_ws"], W["search_step"], W["radius"]
@@ -364,264 +412,259 @@ def main():
     ng = nX * nY
     s_out = [DA(ctx, ng, np.float64) for _ in range(5)]  # x y u v cc
     s_pk = [DA(ctx, ng, np.int32) for _ in range(2)]
-    d_U0 = DA(ctx, 2 * n, np.float32)
-    d_U = DA(ctx, 2 * n, np.float32)
-    d_F = DA(ctx, 4 * n, np.float32)
-    d_conv = DA(ctx, n, np.int32)
-    d_z = DA(ctx, n, np.float32)
     x0 = sws / 2 + 1
-    # a second context (stream) pipelines consecutive frames: the IC-GN of
-    # step k runs on ctx2 while the search of step k+1 runs on ctx; outputs
-    # are double-buffered by k % 2
-    ctx2 = api.Context(0)
+    # three contexts (streams) pipeline consecutive frames; each solver
+    # context has its own plan of the same nodes
+    ctx2, ctx3 = api.Context(0), api.Context(0)
+    plan = api.Plan(W["ws"], n, ctx=ctx)
     plan2 = api.Plan(W["ws"], n, ctx=ctx2)
-    # the learned-init step pipelines over three: network (ctx), IC-GN and
-    # gate (ctx3, plan3), fallback (ctx2, plan2)
-    ctx3 = api.Context(0)
     plan3 = api.Plan(W["ws"], n, ctx=ctx3)
     sets = [dict(U0=DA(ctx, 2 * n, np.float32), U=DA(ctx, 2 * n, np.float32),
                  F=DA(ctx, 4 * n, np.float32), conv=DA(ctx, n, np.int32),
                  z=DA(ctx, n, np.float32)) for _ in range(2)]
+    truth_mean = (window_mean_truth(W, W["uf"]), window_mean_truth(W, W["vf"]))
 
     def search_stage(sb):  # on ctx
         api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst,
                                [r, r, r, r], *[o.ptr for o in s_out], *[p.ptr for p in s_pk])
-        api.grid_to_nodes_dev(ctx, s_out[2].ptr, s_out[3].ptr, nX, nY, x0, x0, sst, d_coords.ptr,
-                              n, sets[sb]["U0"].ptr)
+        api.grid_to_nodes_dev(ctx, s_out[2].ptr, s_out[3].ptr, nX, nY, x0, x0, sst,
+                              d_coords.ptr, n, sets[sb]["U0"].ptr)
 
     def icgn_stage(sb):  # on ctx2
         b = sets[sb]
         api.local_icgn_dev(im, plan2, d_coords.ptr, b["U0"].ptr, d_ws.ptr, 1e-3, b["U"].ptr,
                            b["F"].ptr, None, b["conv"].ptr, None, b["z"].ptr, ctx=ctx2)
 
-    def step_search(k, graphs=None):
+    def step_search(k):
         sb = k % 2
         ctx.wait(ctx2, sb)  # the IC-GN of step k-2 has released set sb
-        if graphs:
-            graphs[0][sb].launch(ctx)
-        else:
-            search_stage(sb)
+        search_stage(sb)
         ctx.record(2 + sb)
         ctx2.wait(ctx, 2 + sb)
-        if graphs:
-            graphs[1][sb].launch(ctx2)
-        else:
-            icgn_stage(sb)
+        icgn_stage(sb)
         ctx2.record(sb)
 
-    # plan-time autotune of the IC-GN launch shapes on this node set (not timed)
-    api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst, [r, r, r, r],
-                           *[o.ptr for o in s_out], *[p.ptr for p in s_pk])
-    api.grid_to_nodes_dev(ctx, s_out[2].ptr, s_out[3].ptr, nX, nY, x0, x0, sst, d_coords.ptr, n,
-                          d_U0.ptr)
-    chosen = api.local_icgn_autotune_dev(im, plan, d_coords.ptr, d_U0.ptr, d_ws.ptr, 1e-3, d_U.ptr,
-                                         d_F.ptr, d_conv.ptr, d_z.ptr, reps=3)
-    res = {}
-    side = W["ws"] + 1
-    cls = subset_class(side)
-
-    def record(name, dt, stages, U, conv, z, extra=None):
-        ok, rms, by_size = accuracy(W, U, conv, z)
-        kern = {}
-        for c in range(4):
-            key = f"icgn_c{c}"
-            if key in stages:
-                ms, cnt = stages[key]
-                kern[key] = (ms / cnt, float(icgn_bytes(side[cls == c]).sum()))
-        res[name] = dict(dt=dt, stages=stages, ok=ok, rms=rms, by_size=by_size, kern=kern,
-                         U=U, **(extra or {}))
-
-    progress(f"autotuned launch shapes {list(chosen)}")
-    if args.cap_sweep:
-        for cap in [int(c) for c in args.cap_sweep.split(",")]:
-            api.local_icgn_f0_dev(im, plan, d_coords.ptr, d_U0.ptr, None, d_ws.ptr, 1e-3, d_U.ptr,
-                                  d_F.ptr, d_conv=d_conv.ptr, d_zncc=d_z.ptr, max_iter=cap)
-            ctx.synchronize()
-            api.set_timing(ctx, True)
-            api.stage_times(ctx, reset=True)
-            for _ in range(ROOFLINE_REPS):
-                api.local_icgn_f0_dev(im, plan, d_coords.ptr, d_U0.ptr, None, d_ws.ptr, 1e-3,
-                                      d_U.ptr, d_F.ptr, d_conv=d_conv.ptr, d_zncc=d_z.ptr,
-                                      max_iter=cap)
-            st = api.stage_times(ctx, reset=True)
-            api.set_timing(ctx, False)
-            print(json.dumps({"cap": cap, "stage_ms": {k: round(v[0] / v[1], 4)
-                                                       for k, v in st.items()}}), flush=True)
-        return
+    # plan-time autotune of the launch shapes on this node set (not timed)
+    search_stage(0)
+    b0 = sets[0]
+    chosen = api.local_icgn_autotune_dev(im, plan, d_coords.ptr, b0["U0"].ptr, d_ws.ptr, 1e-3,
+                                         b0["U"].ptr, b0["F"].ptr, b0["conv"].ptr, b0["z"].ptr,
+                                         reps=3)
     for c in range(4):
         for pl in (plan2, plan3):
             pl.set_variant(0, c, chosen[c])
             pl.set_variant(1, c, chosen[c])
-    if args.init in ("both", "search"):
-        dt, stages, last = timed([ctx, ctx2], api, step_search, args.steps, args.warmup)
-        progress(f"search-init step {dt * 1e3:.3f} ms")
-        # the same pipeline with each stage replayed from a HIP graph (one
-        # per stage and output set)
-        sgraphs = [[ctx.capture(lambda sb=sb: search_stage(sb)) for sb in (0, 1)],
-                   [ctx2.capture(lambda sb=sb: icgn_stage(sb)) for sb in (0, 1)]]
-        dtg, _, lastg = timed([ctx, ctx2], api, lambda k: step_search(k, sgraphs), args.steps,
-                              args.warmup)
-        progress(f"search-init step, graphs {dtg * 1e3:.3f} ms")
-        sgraph_ms = round(dtg * 1e3, 4)
-        if dtg < dt:
-            dt, last = dtg, lastg
-        for gs in sgraphs:
-            for gr in gs:
-                gr.close()
+    progress(f"{W['name']}: autotuned launch shapes {list(chosen)}")
+    inet = learned.Network(ctx, max_batch=(n + 15) // 16 * 16, params=params)
+    li = learned.LearnedInit(inet, im, d_f.ptr, d_g.ptr, plan3, d_coords.ptr, n, d_ws.ptr,
+                             prior_grid=([1, M], [1, N], sws, 256),
+                             fallback_grid=([1, M], [1, N], sws, sst), region=(r, r, r, r),
+                             max_iter=30, min_zncc=0.9, tol=1e-3, ctx_fb=ctx2, plan_fb=plan2,
+                             ctx_icgn=ctx3)
+    out = {}
+    U_search_ref = None
+    for rule, (code, tol_dp) in RULES.items():
+        for pl in (plan2, plan3):
+            pl.set_stopping(code, tol_dp)
+        dt, st, last = timed([ctx, ctx2], api, step_search, args.steps, args.warmup)
         b = sets[last % 2]
-        record("search", dt, stages, b["U"].to_host().astype(np.float64), b["conv"].to_host(),
-               b["z"].to_host(), dict(graph_ms=sgraph_ms))
-
-    if args.init in ("both", "learned"):
-        # ---- train the network on the device (not timed) ----
-        t0 = time.perf_counter()
-        net = learned.Network(ctx, max_batch=256, seed=0)
-        hist = learned.train(net, samples=args.train_samples, batch=256, lr=2e-3, tiles=64,
-                             seed=4, log_every=1000, log=progress)
-        ev = learned.evaluate(net, seed=12345, tiles=32)
-        t_train = time.perf_counter() - t0
-        # inference copy of the trained weights, one batch for all nodes
-        inet = learned.Network(ctx, max_batch=(n + 15) // 16 * 16, params=net.get_params())
-        net.close()
-        li = learned.LearnedInit(inet, im, d_f.ptr, d_g.ptr, plan3, d_coords.ptr, n, d_ws.ptr,
-                                 prior_grid=([1, M], [1, N], sws, 256),
-                                 fallback_grid=([1, M], [1, N], sws, sst),
-                                 region=(r, r, r, r), max_iter=30, min_zncc=0.9, tol=1e-3,
-                                 ctx_fb=ctx2, plan_fb=plan2, ctx_icgn=ctx3)
-        dt, stages, last = timed([ctx, ctx2, ctx3], api, li.run, args.steps, args.warmup)
-        progress(f"learned-init step {dt * 1e3:.3f} ms")
-        # the same pipeline with every stage replayed from a HIP graph
-        graphs = li.capture()
-        dtg, _, lastg = timed([ctx, ctx2, ctx3], api, lambda k: li.run(k, graphs), args.steps,
-                              args.warmup)
-        progress(f"learned-init step, graphs {dtg * 1e3:.3f} ms")
-        if dtg < dt:
-            dt, last = dtg, lastg
+        U = b["U"].to_host().astype(np.float64)
+        acc = accuracy(api, W, U, b["conv"].to_host(), b["z"].to_host(), truth_mean)
+        out[("search", rule)] = dict(dt=dt, stages=st, acc=acc)
+        if rule == "reference":
+            U_search_ref = U
+        progress(f"{W['name']} search start, {rule} rule: {dt * 1e3:.3f} ms, "
+                 f"{acc['ok'].sum()} converged, rms {acc['rms']:.4f} px")
+        dt, st, last = timed([ctx, ctx2, ctx3], api, li.run, args.steps, args.warmup)
         lU, lF, lconv, lz = li.outputs(last)
-        train = {"samples": hist["samples"], "steps": hist["steps"],
-                 "kept_frac": round(hist["kept"] / max(hist["total"], 1), 5),
-                 "seconds": round(t_train, 1), "final_loss": round(hist["loss"][-1], 5)
-                 if hist["loss"] else None,
-                 "heldout_rms_u_px": round(ev["rms_u_px"], 4),
-                 "heldout_rms_F": round(ev["rms_F"], 5)}
-        # the first IC-GN pass alone (network start, before the gate and the
-        # fallback) on the 65 px class: evaluations per subset
-        for c in (ctx, ctx2, ctx3):
-            c.synchronize()
+        acc = accuracy(api, W, lU.to_host().astype(np.float64), lconv.to_host(), lz.to_host(),
+                       truth_mean)
+        out[("learned", rule)] = dict(dt=dt, stages=st, acc=acc, fallback=li.fallback_count(last))
+        progress(f"{W['name']} learned start, {rule} rule: {dt * 1e3:.3f} ms, "
+                 f"{acc['ok'].sum()} converged, rms {acc['rms']:.4f} px, fallback "
+                 f"{li.fallback_count(last)}")
+    for c in (ctx, ctx2, ctx3):
+        c.synchronize()
+    # ---- roofline of the headline step's dominant kernel: the learned
+    # start's first IC-GN pass (step-size rule), ROOFLINE_REPS launches
+    # alone on one stream, HIP events on that stream ----
+    roof = None
+    if headline:
+        plan.set_stopping(*RULES["step"])
         li._stage_net(0)
         ctx.synchronize()
-        li._stage_icgn(0)
-        ctx3.synchronize()
-        conv1 = li.bufs[0]["conv"].to_host()
-        big = W["ws"] == 64
-        c1 = conv1[big]
-        first = {"mean_evals_converged": round(float(c1[(c1 >= 1) & (c1 <= 30)].mean()), 3),
-                 "over_30": int((c1 > 30).sum()), "status_101": int((c1 == 101).sum()),
-                 "subsets": int(big.sum())}
-        record("learned", dt, stages, lU.to_host().astype(np.float64), lconv.to_host(),
-               lz.to_host(), dict(fallback=li.fallback_count(last), train=train,
-                                  prior_points=li.nX * li.nY,
-                                  graph_ms=round(dtg * 1e3, 4), first_pass=first))
+        bb = li.bufs[0]
+        ctx.synchronize()
+        api.set_timing(ctx, True)
+        api.stage_times(ctx, reset=True)
+        for _ in range(ROOFLINE_REPS):
+            api.local_icgn_f0_dev(im, plan, d_coords.ptr, bb["U0"].ptr, bb["F0"].ptr, d_ws.ptr,
+                                  1e-3, bb["U"].ptr, bb["F"].ptr, d_conv=bb["conv"].ptr,
+                                  d_zncc=bb["z"].ptr, max_iter=30)
+        rst = api.stage_times(ctx, reset=True)
+        api.set_timing(ctx, False)
+        rconv = bb["conv"].to_host()
+        side = W["ws"] + 1
+        cls = subset_class(side)
+        kern = {}
+        for c in range(4):
+            key = f"icgn_c{c}"
+            if key in rst:
+                ms, cnt = rst[key]
+                sel = cls == c
+                it = np.clip(rconv[sel], 1, 31)
+                kern[key] = (ms / cnt, float(icgn_bytes(side[sel]).sum()),
+                             float(icgn_flops(side[sel], it).sum()), int(sel.sum()),
+                             float(it.mean()))
+        dom = max(kern, key=lambda k: kern[k][0])
+        ms, nbytes, flops, units, evals = kern[dom]
+        achieved = nbytes / (ms * 1e-3) / 1e9
+        tflops = flops / (ms * 1e-3) / 1e12
+        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": ICGN_C3_TRAFFIC if dom == "icgn_c3" else None,
+                "kernel_ms": round(ms, 4), "launches": ROOFLINE_REPS,
+                "algorithmic_bytes": nbytes, "subsets": units, "mean_evaluations": round(evals, 3),
+                "fp32": {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4)},
+                "pass": "learned start, first IC-GN pass (<= 31 evaluations), step-size rule"}
+    inet.close()
+    for pl in (plan, plan2, plan3):
+        pl.close()
+    im.close()
+    return out, roof, U_search_ref, dict(nodes=n, search_points=ng, grid=(nX, nY),
+                                         shapes=[int(c) for c in chosen])
 
-    head = res.get("learned") or res["search"]
-    n_conv = int(head["ok"].sum())
-    value = n_conv / head["dt"]
-    # ---- roofline of the dominant IC-GN kernel: ROOFLINE_REPS launches
-    # alone on one stream (the timed steps overlap it with the other
-    # stream's work), HIP events on that stream ----
-    progress("roofline pass")
-    api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst, [r, r, r, r],
-                           *[o.ptr for o in s_out], *[p.ptr for p in s_pk])
-    api.grid_to_nodes_dev(ctx, s_out[2].ptr, s_out[3].ptr, nX, nY, x0, x0, sst, d_coords.ptr, n,
-                          d_U0.ptr)
-    ctx.synchronize()
-    api.set_timing(ctx, True)
-    api.stage_times(ctx, reset=True)
-    for _ in range(ROOFLINE_REPS):
-        api.local_icgn_dev(im, plan, d_coords.ptr, d_U0.ptr, d_ws.ptr, 1e-3, d_U.ptr, d_F.ptr,
-                           None, d_conv.ptr, None, d_z.ptr)
-    rstages = api.stage_times(ctx, reset=True)
-    api.set_timing(ctx, False)
-    rconv = d_conv.to_host()
-    kern = {}
-    for c in range(4):
-        key = f"icgn_c{c}"
-        if key in rstages:
-            ms, cnt = rstages[key]
-            sel = cls == c
-            it = np.clip(rconv[sel], 1, 101)
-            kern[key] = (ms / cnt, float(icgn_bytes(side[sel]).sum()),
-                         float(icgn_flops(side[sel], it).sum()))
-    dom = max(kern, key=lambda k: kern[k][0])
-    dom_ms, dom_bytes, dom_flops = kern[dom]
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    tflops = dom_flops / (dom_ms * 1e-3) / 1e12
 
-    def summary(name):
-        d = res[name]
-        nc = int(d["ok"].sum())
-        out = {"value": round(nc / d["dt"], 1), "ms_per_step": round(d["dt"] * 1e3, 4),
-               "converged": nc, "converged_frac": round(nc / n, 5), "rms_px": round(d["rms"], 5),
-               "rms_ok": bool(d["rms"] <= 0.01), "by_subset_size": d["by_size"],
-               "stage_ms_per_step": {k: round(v[0] / args.steps, 4)
-                                     for k, v in d["stages"].items()}}
-        out["graph_ms_per_step"] = d["graph_ms"]
-        if name == "learned":
-            out["fallback"] = d["fallback"]
-            out["fallback_rate"] = round(d["fallback"] / n, 5)
-            out["prior_search_points"] = d["prior_points"]
-            out["first_pass_65px"] = d["first_pass"]
-            out["train"] = d["train"]
-        return out
+def summary(d, n, steps):
+    acc = d["acc"]
+    nc = int(acc["ok"].sum())
+    s = {"value": round(nc / d["dt"], 1), "ms_per_step": round(d["dt"] * 1e3, 4),
+         "converged": nc, "converged_frac": round(nc / n, 5),
+         "status_ok_before_bad_point_rule": acc["status_ok"], "bad_points": acc["bad_points"],
+         "rms_px": round(acc["rms"], 5), "rms_ok": bool(acc["rms"] <= 0.01),
+         "rms_vs_subset_mean_px": round(acc["rms_mean"], 5), "by_subset_size": acc["by_size"],
+         "stage_ms_per_step": {k: round(v[0] / steps, 4) for k, v in d["stages"].items()}}
+    if "fallback" in d:
+        s["fallback"] = d["fallback"]
+        s["fallback_rate"] = round(d["fallback"] / n, 5)
+    return s
 
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="IC-GN nodes of the CPU baseline (0 = all)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core of the share")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--train-samples", type=int, default=2_000_000)
+    ap.add_argument("--workloads", default="bulge4096x3000,tilt2048",
+                    help="comma-separated; the first is the headline")
+    ap.add_argument("--global-step", action="store_true",
+                    help="diagnostic: time the rows around the local step on the 2048 x 2048 "
+                         "workload (quadtree mesh, Subpb2, NaN fill, raw decode) and exit "
+                         "(no metric line)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: the local matching path is "
+                          "measured on one MI355X (see DESIGN.md for how it would shard)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    pkg = ge.load_package()
+    if args.global_step:
+        global_step_mode(pkg, args)
+        return
+    from aldic_amd import learned
+    # ---- train the network on the device (not timed) ----
+    t0 = time.perf_counter()
+    tctx = pkg.api.Context(0)
+    net = learned.Network(tctx, max_batch=256, seed=0)
+    hist = learned.train(net, samples=args.train_samples, batch=256, lr=2e-3, tiles=64, seed=4,
+                         log_every=2000, log=progress)
+    ev = learned.evaluate(net, seed=12345, tiles=32)
+    params = net.get_params()
+    net.close()
+    tctx.close()
+    train = {"samples": hist["samples"], "steps": hist["steps"],
+             "kept_frac": round(hist["kept"] / max(hist["total"], 1), 5),
+             "seconds": round(time.perf_counter() - t0, 1),
+             "final_loss": round(hist["loss"][-1], 5) if hist["loss"] else None,
+             "heldout_rms_u_px": round(ev["rms_u_px"], 4), "heldout_rms_F": round(ev["rms_F"], 5)}
+    progress(f"network trained ({train['seconds']} s)")
+    names = [w for w in args.workloads.split(",") if w]
+    results = {}
+    for i, name in enumerate(names):
+        t0 = time.perf_counter()
+        W = make_workload(name)
+        gen_s = time.perf_counter() - t0
+        progress(f"workload {name} ready ({gen_s:.1f} s, {len(W['ws'])} nodes)")
+        out, roof, Uref, meta = run_workload(pkg, W, params, args, headline=(i == 0))
+        results[name] = dict(W=W, out=out, roof=roof, Uref=Uref, meta=meta, gen_s=gen_s)
+        if i > 0:
+            W["f"] = W["g"] = None  # keep the headline pair only
+    head_name = names[0]
+    H = results[head_name]
+    n = H["meta"]["nodes"]
+    headd = H["out"][("learned", "step")]
+    value = headd["acc"]["ok"].sum() / headd["dt"]
+
+    def block(res):
+        nn = res["meta"]["nodes"]
+        blk = {}
+        for (start, rule), d in res["out"].items():
+            blk[f"{start}_start_{rule}_rule"] = summary(d, nn, args.steps)
+        for rule in RULES:
+            a = res["out"][("learned", rule)]
+            b = res["out"][("search", rule)]
+            blk[f"learned_vs_search_{rule}_rule"] = round(
+                (a["acc"]["ok"].sum() / a["dt"]) / (b["acc"]["ok"].sum() / b["dt"]), 3)
+        blk["nodes"] = nn
+        blk["search_points"] = res["meta"]["search_points"]
+        blk["icgn_launch_shapes"] = res["meta"]["shapes"]
+        blk["gen_s"] = round(res["gen_s"], 1)
+        return blk
+
+    spec = H["W"]["spec"]
     line = {
         "metric": "converged IC-GN subsets/s at <=0.01 px RMS, learned vs FFT init, 1 MI355X",
-        "value": round(value, 1),
+        "value": round(float(value), 1),
         "unit": "converged subsets/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(head["dt"] * 1e3, 4),
+        "ms_per_step": round(headd["dt"] * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "stereo pair 2048x2048 12-bit speckle, hole r=300, "
-                               f"{n} nodes with 8-64 px subsets; init: "
-                               + ("learned (CNN on a 256 px sparse-search prior, gate + "
-                                  "search fallback)" if "learned" in res else
-                                  f"ZNCC search r=50 on a {nX}x{nY} grid")
-                               + "; IC-GN tol 1e-3",
-                   "init": "learned" if "learned" in res else "search", "nodes": n,
-                   "search_points": ng},
-        "converged": n_conv,
-        "converged_frac": round(n_conv / n, 5),
-        "rms_px": round(head["rms"], 5),
-        "rms_ok": bool(head["rms"] <= 0.01),
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": round(ICGN_C3_TRAFFIC) if dom == "icgn_c3" else None,
-                     "kernel_ms": round(dom_ms, 4), "launches": ROOFLINE_REPS,
-                     "algorithmic_bytes": dom_bytes,
-                     "fp32": {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
-                              "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4)}},
-        "gen_s": round(t_gen, 1),
-        "icgn_launch_shapes": [int(c) for c in chosen],
+        "config": {"workload": (f"large stereo pair {spec['M']}x{spec['N']} 12-bit speckle "
+                                f"(0.01 spots/px^2, seed 5), 25 px disparity + 15 px Gaussian "
+                                f"bulge (sigma 600 px), hole r=300, {n} nodes with 8-64 px "
+                                f"subsets; learned start (CNN on a 256 px sparse-search prior, "
+                                f"gate + search fallback) vs ZNCC search start (r=50, "
+                                f"{H['meta']['grid'][0]}x{H['meta']['grid'][1]} grid); IC-GN "
+                                f"tol 1e-3"),
+                   "init": "learned", "stop": f"step size < {TOL_DP} px", "nodes": n},
+        "converged": int(headd["acc"]["ok"].sum()),
+        "converged_frac": round(float(headd["acc"]["ok"].sum()) / n, 5),
+        "rms_px": round(headd["acc"]["rms"], 5),
+        "rms_ok": bool(headd["acc"]["rms"] <= 0.01),
+        "roofline": H["roof"],
+        "network": train,
     }
-    for name in ("learned", "search"):
-        if name in res:
-            line["learned_init" if name == "learned" else "fft_init"] = summary(name)
-    if "learned" in res and "search" in res:
-        line["learned_vs_fft"] = round((res["learned"]["ok"].sum() / res["learned"]["dt"]) /
-                                       (res["search"]["ok"].sum() / res["search"]["dt"]), 3)
+    line[head_name] = block(H)
+    for name in names[1:]:
+        line[name] = block(results[name])
+    line["learned_vs_fft"] = line[head_name]["learned_vs_search_step_rule"]
     if not args.no_cpu:
-        progress("CPU baseline (FP64 oracle)")
-        # the oracle runs the search-init path: compare it with the GPU's
-        ref = res.get("search")
-        diag, line["cpu_baseline"] = cpu_baseline(W, args.cpu_sample, args.cpu_threads,
-                                                  ref["U"] if ref else None)
+        threads = args.cpu_threads or cpu_threads()
+        progress(f"CPU baseline (FP64 oracle, {threads} threads)")
+        diag, line["cpu_baseline"] = cpu_baseline(H["W"], args.cpu_sample, threads, H["Uref"])
         line["oracle_sample"] = diag
     else:
         line["cpu_baseline"] = None
