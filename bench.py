@@ -64,8 +64,9 @@ TOL_DP = 1e-3  # step-size rule threshold (px)
 # HBM bytes per launch of the dominant IC-GN kernel (65 px class, learned
 # start, step-size rule, headline workload), rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE in KiB per dispatch, each scaled by the same counter's reading
-# on a device-to-device copy of known size; None until measured
-ICGN_C3_TRAFFIC = None
+# on a device-to-device copy of known size (FETCH x 1.9996, WRITE x 1);
+# measured with this bench (profiles/icgn_c3_bulge4096_hbm_traffic.md)
+ICGN_C3_TRAFFIC = (3367983.1 * 1.9996 + 21790.4) * 1024.0
 
 
 def load(name, rel):
