@@ -9,13 +9,14 @@
 
 QrefineR is the red refinement with the 1-irregular rule of Funken and
 Schmidt ("Adaptive mesh refinement in 2D: an efficient implementation in
-Matlab", CMAM 20:459-479, 2020).  Its helper ``provideGeometricData`` is not
-part of the reference; :func:`geometric_data` restates it: undirected edges
-numbered by first appearance, scanning the elements' edges 1-2, 2-3, 3-4,
-4-1 (all elements' first edges, then all second edges, ...) and then the
-edges (a, b), (a, h), (h, b) of each hanging node h on an edge (a, b).  New
-nodes are numbered in edge order as in QrefineR, so the mesh is the
-reference's up to the node numbering (which depends on that absent helper).
+Matlab", CMAM 20:459-479, 2020); the reference vendors it with its helper
+``provideGeometricData`` (func_quadtree/refinement/provideGeometricData.m),
+which :func:`geometric_data` restates: the irregular list [a b h] is taken
+as triangles (edges a-b, b-h, h-a), the elements as quadrilaterals (edges
+1-2, 2-3, 3-4, 4-1), and the edges are numbered in the lexicographic order
+of their sorted node pairs (``unique(sort(edges,2),'rows')``).  The steps
+of QrefineR and that numbering are the published algorithm's, kept alike so
+that nodes and elements come out in the reference's order.
 
 The per-element mask tests read one summed-area table of the mask instead
 of slicing the image per element; the rules (range of the mask over the
@@ -55,27 +56,24 @@ def MeshSetUp(x, y, DICpara):
 # refinement
 # ---------------------------------------------------------------------------
 def geometric_data(irregular, elements):
-    """edge2nodes (E x 2, 1-based), irregular2edges (nirr x 3),
-    element2edges (ne x 4), 0-based edge numbers."""
-    el = np.asarray(elements, dtype=np.int64)
+    """[edge2nodes, irregular2edges, element2edges] =
+    provideGeometricData(irregular, elements) as QrefineR.m:43-45 calls it:
+    the edges of the irregular triangles (a-b, b-h, h-a for each row, all
+    first edges, then all second, then all third) followed by those of the
+    quadrilaterals (1-2, 2-3, 3-4, 4-1, likewise), numbered by
+    unique(sort(edges,2),'rows') (provideGeometricData.m:44-57).
+    edge2nodes is E x 2 (sorted pairs, 1-based nodes); edge numbers are
+    0-based."""
+    el = np.asarray(elements, dtype=np.int64).reshape(-1, 4)
     irr = np.asarray(irregular, dtype=np.int64).reshape(-1, 3)
-    I = el[:, [0, 1, 2, 3]].T.reshape(-1)
-    J = el[:, [1, 2, 3, 0]].T.reshape(-1)
-    if len(irr):
-        I = np.concatenate([I, irr[:, 0], irr[:, 0], irr[:, 2]])
-        J = np.concatenate([J, irr[:, 1], irr[:, 2], irr[:, 1]])
-    lo, hi = np.minimum(I, J), np.maximum(I, J)
-    key = lo * (max(int(hi.max(initial=0)), 1) + 1) + hi
-    _, first, inv = np.unique(key, return_index=True, return_inverse=True)
-    # renumber by first appearance
-    order = np.argsort(first, kind="stable")
-    rank = np.empty_like(order)
-    rank[order] = np.arange(len(order))
-    num = rank[inv]
-    edge2nodes = np.stack([I[first[order]], J[first[order]]], axis=1)
-    ne = len(el)
-    element2edges = num[:4 * ne].reshape(4, ne).T
-    irregular2edges = num[4 * ne:].reshape(3, -1).T if len(irr) else np.zeros((0, 3), np.int64)
+    tri = np.concatenate([irr[:, [0, 1]], irr[:, [1, 2]], irr[:, [2, 0]]])
+    quad = np.concatenate([el[:, [0, 1]], el[:, [1, 2]], el[:, [2, 3]], el[:, [3, 0]]])
+    edges = np.sort(np.concatenate([tri, quad]), axis=1)
+    edge2nodes, ie = np.unique(edges, axis=0, return_inverse=True)
+    ie = ie.reshape(-1)
+    nt, ne = len(irr), len(el)
+    irregular2edges = ie[:3 * nt].reshape(3, nt).T
+    element2edges = ie[3 * nt:3 * nt + 4 * ne].reshape(4, ne).T
     return edge2nodes, irregular2edges, element2edges
 
 

@@ -147,3 +147,44 @@ def test_oracle_pins():
         for side in range(4):
             for f in t.finer_along(c, side):
                 assert f[0] == c[0] + 1, (c, f)
+
+
+def test_geometric_data_numbering():
+    """provideGeometricData as QrefineR calls it (provideGeometricData.m:
+    44-57): irregular rows are triangles a-b, b-h, h-a; element edges 1-2,
+    2-3, 3-4, 4-1; edges numbered in the lexicographic order of their
+    sorted node pairs, edge2nodes holding those sorted pairs."""
+    from aldic_amd import quadtree as q
+    #  4---3---6
+    #  |   |   |
+    #  1---2---5
+    el = np.array([[1, 2, 3, 4], [2, 5, 6, 3]])
+    irr = np.array([[5, 6, 7]])  # a hanging node 7 on edge 5-6
+    e2n, i2e, el2e = q.geometric_data(irr, el)
+    want = [(1, 2), (1, 4), (2, 3), (2, 5), (3, 4), (3, 6), (5, 6), (5, 7), (6, 7)]
+    assert [tuple(r) for r in e2n] == want
+    idx = {p: k for k, p in enumerate(want)}
+    assert list(el2e[0]) == [idx[(1, 2)], idx[(2, 3)], idx[(3, 4)], idx[(1, 4)]]
+    assert list(el2e[1]) == [idx[(2, 5)], idx[(5, 6)], idx[(3, 6)], idx[(2, 3)]]
+    assert list(i2e[0]) == [idx[(5, 6)], idx[(6, 7)], idx[(5, 7)]]
+
+
+def test_qrefiner_node_order():
+    """One red refinement: the new edge nodes are numbered in edge order
+    (the lexicographic numbering above: edges 1-2, 1-4, 2-3, 3-4), then the
+    midpoint, and the four children follow QrefineR.m's layout."""
+    from aldic_amd import quadtree as q
+    c = np.array([[0.0, 0.0], [8.0, 0.0], [8.0, 8.0], [0.0, 8.0]])
+    el = np.array([[1, 2, 3, 4]])
+    c2, el2, irr2 = q.QrefineR(c, el, np.zeros((0, 3), np.int64), [1])
+    assert np.allclose(c2[4:], [[4, 0], [0, 4], [8, 4], [4, 8], [4, 4]])
+    assert el2.tolist() == [[1, 5, 9, 6], [2, 7, 9, 5], [3, 8, 9, 7], [4, 6, 9, 8]]
+    assert irr2.shape == (0, 3)
+    # refining one of two neighbours leaves a hanging node on the shared edge,
+    # listed with the edge's sorted node pair
+    c = np.array([[0.0, 0.0], [8.0, 0.0], [8.0, 8.0], [0.0, 8.0], [16.0, 0.0], [16.0, 8.0]])
+    el = np.array([[1, 2, 3, 4], [2, 5, 6, 3]])
+    c2, el2, irr2 = q.QrefineR(c, el, np.zeros((0, 3), np.int64), [1])
+    # new nodes 7, 8, 9, 10 on edges 1-2, 1-4, 2-3, 3-4 (in edge order)
+    assert irr2.tolist() == [[2, 3, 9]]
+    assert np.allclose(c2[6:10], [[4, 0], [0, 4], [8, 4], [4, 8]])
