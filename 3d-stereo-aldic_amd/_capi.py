@@ -80,6 +80,10 @@ SIGNATURES = {
                                                 _vp, _vp, _vp, C.c_int32, _vp]),
     "aldic_plan_set_variant": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32]),
     "aldic_plan_set_stopping": (C.c_int, [_vp, C.c_int32, C.c_double]),
+    "aldic_alm_iteration_dev": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_double, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp,
+                                          _vp, C.c_int32]),
     "aldic_integer_search": (C.c_int, [_vp, _dp, _dp, _i64, _i64, C.POINTER(_i64),
                                        C.POINTER(_i64), C.c_int32, C.c_int32, C.POINTER(_i64),
                                        _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.POINTER(_i64),

@@ -759,6 +759,24 @@ def subpb1_dev(images: Images, plan: Plan, d_coords, d_UOld, d_FOld, d_vdual, d_
                                  d_vdual, d_winsize_xy, float(mu), float(tol), d_U, d_conv))
 
 
+def alm_iteration_dev(images: Images, plan: Plan, fem: "FEMesh", d_coords, d_winsize_xy, mu, beta,
+                      tol, tol2, check_update, first, d_F1, d_hole, d_U1, d_U2, d_vdual, d_update,
+                      d_state, alpha=0.0, winstepsize=1.0, cg_max_iter=64,
+                      ctx: Context | None = None):
+    """One augmented-Lagrangian iteration on the device
+    (aldic_alm_iteration_dev, TemporalMatch_quadtree_ST1.m:337-387): d_U1,
+    d_U2, d_vdual (FP64, 2n) are updated in place; d_update receives
+    (UpdateY, UpdateY2), d_state (int32[2]) the done flag and the number of
+    bad Subpb1 nodes, which take the global estimate instead of the host's
+    natural-neighbour fill."""
+    c = ctx or images.ctx
+    check(lib().aldic_alm_iteration_dev(c.ptr, images.ptr, plan.ptr, fem.ptr, d_coords,
+                                        d_winsize_xy, float(mu), float(beta), float(alpha),
+                                        float(winstepsize), float(tol), float(tol2),
+                                        1 if check_update else 0, 1 if first else 0, d_F1, d_hole,
+                                        d_U1, d_U2, d_vdual, d_update, d_state, int(cg_max_iter)))
+
+
 def integer_search_dev(ctx: Context, d_f, d_g, M, N, gridx, gridy, winsize, winstepsize, region,
                        d_x, d_y, d_u, d_v, d_ccmax, d_ipr, d_ipc):
     gx = (C.c_int64 * 2)(int(gridx[0]), int(gridx[-1]))

@@ -14,6 +14,12 @@
 
 namespace aldic {
 int set_error(int code, const std::string& msg);
+// aldic_subpb2_dev_async with a device flag: nonzero *d_skip (may be null)
+// ends the CG before its first chunk (d_Uhat then holds the start)
+int subpb2_async(aldic_ctx* ctx, aldic_fem* f, double beta, double mu, double alpha,
+                 double winstepsize, const double* d_U, const double* d_F, double* d_Uhat,
+                 int32_t max_iter, int32_t* d_iterations, const int* d_skip);
+int64_t fem_nodes(const aldic_fem* f);
 }
 
 #define HIP_TRY(expr)                                                                  \

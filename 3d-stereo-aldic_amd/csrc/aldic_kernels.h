@@ -43,6 +43,7 @@ struct SolveArgs {
   uint8_t* hole;        // n (IC-GN markCoordHoleStrain flags), may be null
   int* defer_cnt;       // subsets handed from the fast to the generic kernel
   int* defer_list;
+  const int* skip;      // may be null: nonzero on the device -> the launch does nothing
 };
 
 namespace aldic {

@@ -1495,3 +1495,230 @@ extern "C" int aldic_integer_search(aldic_ctx* ctx, const double* f, const doubl
   }
   return ALDIC_OK;
 }
+
+// ---------------------------------------------------------------------------
+// One iteration of the augmented-Lagrangian loop on the device
+// (TemporalMatch_quadtree_ST1.m:337-387): Subpb1 from the global estimate,
+// the bad-point rule of Subpb1Quadtree.m:167-182, Subpb2, the hole-edge
+// copy, the UpdateY norms, the duals and the early exit, with no host
+// read-back, so that a frame's iterations can be recorded as one graph.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int SLOT_ALM = 30;  // .. 36
+
+__global__ void alm_start_kernel(int first, int32_t* state, double* update) {
+  if (first) {
+    state[0] = 0;           // done
+    update[0] = 1e4;        // UpdateY before any is computed (:333)
+    update[1] = __builtin_nan("");
+  }
+  if (!state[0]) state[1] = 0;  // bad Subpb1 nodes of this iteration
+}
+
+__global__ void alm_prep_kernel(int64_t n, const int32_t* state, const double* U2,
+                                const double* F1, const double* vdual, float* UOld, float* FOld,
+                                float* vd) {
+  if (state[0]) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 4 * n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    FOld[i] = (float)F1[i];
+    if (i < 2 * n) {
+      UOld[i] = (float)U2[i];
+      vd[i] = (float)vdual[i];
+    }
+  }
+}
+
+// the Subpb1 bad-point threshold max(mean + 0.25 std, 10) over the counts in
+// [0, 99] (unbiased std; 10 when there are none), one workgroup
+__global__ __launch_bounds__(1024) void alm_badstat_kernel(int64_t n, const int32_t* state,
+                                                           const int32_t* conv, double* thr) {
+  __shared__ double s1[1024], s2[1024], sc[1024];
+  if (state[0]) return;
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 1024) {
+    const int v = conv[i];
+    if (v >= 0 && v <= 99) {
+      a += v;
+      b += (double)v * v;
+      c += 1.0;
+    }
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = b;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int h = 512; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      s1[threadIdx.x] += s1[threadIdx.x + h];
+      s2[threadIdx.x] += s2[threadIdx.x + h];
+      sc[threadIdx.x] += sc[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double cnt = sc[0];
+    double t = 10.0;
+    if (cnt > 0.0) {
+      const double m = s1[0] / cnt;
+      const double var = cnt > 1.0 ? fmax((s2[0] - cnt * m * m) / (cnt - 1.0), 0.0) : 0.0;
+      t = fmax(m + 0.25 * sqrt(var), 10.0);
+    }
+    thr[0] = t;
+  }
+}
+
+// USubpb1: the Subpb1 result in FP64; a bad or thrown (NaN) node takes the
+// global step's estimate it started from
+__global__ void alm_fill_kernel(int64_t n, int32_t* state, const int32_t* conv, const double* thr,
+                                const float* U1f, const double* U2, double* U1) {
+  if (state[0]) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = conv[i];
+    const float u = U1f[2 * i], v = U1f[2 * i + 1];
+    const bool bad = c < 0 || c > 99 || (double)c > thr[0] || u != u || v != v;
+    U1[2 * i] = bad ? U2[2 * i] : (double)u;
+    U1[2 * i + 1] = bad ? U2[2 * i + 1] : (double)v;
+    if (bad) atomicAdd(&state[1], 1);
+  }
+}
+
+__global__ void alm_hole_kernel(int64_t n, const int32_t* state, const uint8_t* hole,
+                                const double* U1, double* U2n) {
+  if (state[0]) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    if (hole[i]) {
+      U2n[2 * i] = U1[2 * i];
+      U2n[2 * i + 1] = U1[2 * i + 1];
+    }
+}
+
+// UpdateY = ||U2_prev - U2|| / sqrt(2n), UpdateY2 = ||U1_prev - U1|| / sqrt(2n)
+// (:364-366), one workgroup
+__global__ __launch_bounds__(1024) void alm_norm_kernel(int64_t n, const int32_t* state,
+                                                        int check, const double* U2p,
+                                                        const double* U2n, const double* U1p,
+                                                        const double* U1, double* update) {
+  __shared__ double sa[1024], sb[1024];
+  if (state[0] || !check) return;
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < 2 * n; i += 1024) {
+    const double d2 = U2p[i] - U2n[i], d1 = U1p[i] - U1[i];
+    a += d2 * d2;
+    b += d1 * d1;
+  }
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int h = 512; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+      sb[threadIdx.x] += sb[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    update[0] = sqrt(sa[0]) / sqrt((double)(2 * n));
+    update[1] = sqrt(sb[0]) / sqrt((double)(2 * n));
+  }
+}
+
+// the iteration's results and the duals vdual = U2 - U1 (:371, :379)
+__global__ void alm_commit_kernel(int64_t n, const int32_t* state, const double* U1,
+                                  const double* U2n, double* dU1, double* dU2, double* vdual) {
+  if (state[0]) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 2 * n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    dU1[i] = U1[i];
+    dU2[i] = U2n[i];
+    vdual[i] = U2n[i] - U1[i];
+  }
+}
+
+// the early exit of :381-384 (UpdateY2 exists only where UpdateY was
+// computed, so without the norms the loop does not stop)
+__global__ void alm_exit_kernel(int32_t* state, int check, double tol2, const double* update) {
+  if (state[0] || !check) return;
+  if (update[0] < tol2 || update[1] < tol2) state[0] = 1;
+}
+
+int grid_for(int64_t work) {
+  return (int)std::max<int64_t>(1, std::min<int64_t>((work + 255) / 256, 2048));
+}
+
+}  // namespace
+
+extern "C" int aldic_alm_iteration_dev(aldic_ctx* ctx, const aldic_images* im,
+                                       const aldic_plan* plan, aldic_fem* fem,
+                                       const double* d_coords, const int32_t* d_winsize_xy,
+                                       double mu, double beta, double alpha, double winstepsize,
+                                       double tol, double tol2, int32_t check_update,
+                                       int32_t first, const double* d_F1, const uint8_t* d_hole,
+                                       double* d_U1, double* d_U2, double* d_vdual,
+                                       double* d_update, int32_t* d_state, int32_t cg_max_iter) {
+  if (!ctx || !images_ok(im) || !plan || !plan->rect || !fem)
+    return fail(ALDIC_EINVAL, "alm_iteration_dev: bad context/images/plan (rect)/mesh");
+  const int64_t n = plan->n;
+  if (aldic::fem_nodes(fem) != n)
+    return fail(ALDIC_EINVAL, "alm_iteration_dev: the plan and the mesh hold different node counts");
+  if (!d_coords || !d_winsize_xy || !d_F1 || !d_U1 || !d_U2 || !d_vdual || !d_update || !d_state)
+    return fail(ALDIC_EINVAL, "alm_iteration_dev: NULL array");
+  if (cg_max_iter < 1 || cg_max_iter > (1 << 16))
+    return fail(ALDIC_EINVAL, "alm_iteration_dev: cg_max_iter must be in 1 .. 65536");
+  if (n == 0) return ALDIC_OK;
+  RC_TRY(set_device(ctx));
+  hipStream_t s = ctx->stream;
+  float *UOld, *FOld, *vd, *U1f;
+  int32_t* conv;
+  double *U1, *U2n, *thr;
+  RC_TRY(stage(ctx, SLOT_ALM, (size_t)(2 * n), &UOld));
+  RC_TRY(stage(ctx, SLOT_ALM + 1, (size_t)(4 * n), &FOld));
+  RC_TRY(stage(ctx, SLOT_ALM + 2, (size_t)(2 * n), &vd));
+  RC_TRY(stage(ctx, SLOT_ALM + 3, (size_t)(2 * n), &U1f));
+  RC_TRY(stage(ctx, SLOT_ALM + 4, (size_t)n, &conv));
+  RC_TRY(stage(ctx, SLOT_ALM + 5, (size_t)(4 * n + 1), &U1));
+  U2n = U1 + 2 * n;
+  thr = U1 + 4 * n;
+  const int g2 = grid_for(4 * n);
+  alm_start_kernel<<<1, 1, 0, s>>>(first ? 1 : 0, d_state, d_update);
+  alm_prep_kernel<<<g2, 256, 0, s>>>(n, d_state, d_U2, d_F1, d_vdual, UOld, FOld, vd);
+  HIP_TRY(hipGetLastError());
+  // Subpb1 from [FSubpb2 USubpb2] (Subpb1Quadtree.m:71-165); skipped once done
+  SolveArgs a{};
+  a.fm = im->fm;
+  a.fx = im->fx;
+  a.fy = im->fy;
+  a.g = im->g;
+  a.mask = im->mask;
+  a.M = (int)im->M;
+  a.N = (int)im->N;
+  a.coords = d_coords;
+  a.wsx = d_winsize_xy;
+  a.wsy = d_winsize_xy + n;
+  a.ws_per_node = 1;
+  a.UOld = UOld;
+  a.FOld = FOld;
+  a.vdual = vd;
+  a.mu = (float)mu;
+  a.tol = (float)tol;
+  a.U = U1f;
+  a.conv = conv;
+  a.skip = d_state;
+  RC_TRY(run_solver(ctx, 1, a, plan));
+  alm_badstat_kernel<<<1, 1024, 0, s>>>(n, d_state, conv, thr);
+  alm_fill_kernel<<<grid_for(n), 256, 0, s>>>(n, d_state, conv, thr, U1f, d_U2, U1);
+  HIP_TRY(hipGetLastError());
+  // Subpb2 from USubpb1 (FSubpb1 = FSubpb2, the duals' F part is zero)
+  RC_TRY(aldic::subpb2_async(ctx, fem, beta, mu, alpha, winstepsize, U1, d_F1, U2n, cg_max_iter,
+                             nullptr, d_state));
+  if (d_hole) alm_hole_kernel<<<grid_for(n), 256, 0, s>>>(n, d_state, d_hole, U1, U2n);
+  alm_norm_kernel<<<1, 1024, 0, s>>>(n, d_state, check_update ? 1 : 0, d_U2, U2n, d_U1, U1,
+                                     d_update);
+  alm_commit_kernel<<<grid_for(2 * n), 256, 0, s>>>(n, d_state, U1, U2n, d_U1, d_U2, d_vdual);
+  alm_exit_kernel<<<1, 1, 0, s>>>(d_state, check_update ? 1 : 0, tol2, d_update);
+  HIP_TRY(hipGetLastError());
+  return ALDIC_OK;
+}

@@ -868,10 +868,27 @@ extern "C" int aldic_subpb2_dev(aldic_ctx* ctx, aldic_fem* f, double beta, doubl
   return ALDIC_OK;
 }
 
+namespace {
+// a skipped solve: the CG's done flag is set before its first chunk
+__global__ void skip_cg_kernel(const int* skip, int* ctr) {
+  if (*skip) ctr[1] = 1;
+}
+}  // namespace
+
+int64_t aldic::fem_nodes(const aldic_fem* f) { return f ? f->n : -1; }
+
 extern "C" int aldic_subpb2_dev_async(aldic_ctx* ctx, aldic_fem* f, double beta, double mu,
                                       double alpha, double winstepsize, const double* d_U,
                                       const double* d_F, double* d_Uhat, int32_t max_iter,
                                       int32_t* d_iterations) {
+  return aldic::subpb2_async(ctx, f, beta, mu, alpha, winstepsize, d_U, d_F, d_Uhat, max_iter,
+                             d_iterations, nullptr);
+}
+
+int aldic::subpb2_async(aldic_ctx* ctx, aldic_fem* f, double beta, double mu, double alpha,
+                        double winstepsize, const double* d_U, const double* d_F,
+                        double* d_Uhat, int32_t max_iter, int32_t* d_iterations,
+                        const int* d_skip) {
   RC_TRY(subpb2_check(ctx, f, winstepsize, d_U, d_F, d_Uhat, "subpb2_dev_async"));
   if (max_iter <= 0 || max_iter > (1 << 16))
     return fail(ALDIC_EINVAL, "subpb2_dev_async: max_iter must be in 1 .. 65536");
@@ -882,6 +899,7 @@ extern "C" int aldic_subpb2_dev_async(aldic_ctx* ctx, aldic_fem* f, double beta,
   const bool capturing = cs != hipStreamCaptureStatusNone;
   StageTimer t(ctx, ALDIC_STAGE_SUBPB2);
   subpb2_setup(f, beta, mu, alpha, winstepsize, d_U, d_F, s);
+  if (d_skip) skip_cg_kernel<<<1, 1, 0, s>>>(d_skip, f->ctr);
   // every chunk the limit may need; those after convergence return at once.
   // Under a caller's capture the chunks are recorded as plain launches (the
   // mesh's own chunk graph is for eager calls).

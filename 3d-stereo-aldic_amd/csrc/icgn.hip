@@ -779,6 +779,7 @@ __global__ __launch_bounds__(fast_block<NW>(), WALK && NW > 1 ? 2 : 1) void fast
   static_assert(NW == 1 || SUBS == 1, "multi-wave subsets own their block");
   __shared__ float tiles[SUBS][TS * TS + TS + 1];
   __shared__ float red[NW * 32];
+  if (a.skip && *a.skip) return;  // uniform: the whole launch is skipped
   const int sub = threadIdx.x / NT;
   // a device count is clamped to the list's capacity a.cnt
   const int cnt = a.dcnt ? min(*a.dcnt, a.cnt) : a.cnt;
@@ -875,6 +876,7 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
   __shared__ uint8_t stmp[GEN_MAXPIX];  // logical image fed to bwselect
   __shared__ uint8_t sfsel[GEN_MAXPIX]; // Subpb1: f selection after its mask block
   __shared__ float red[4 * 32];
+  if (a.skip && *a.skip) return;  // uniform: the whole launch is skipped
   const int total = count ? min(*count, fixed_count) : fixed_count;
   const int M = a.M, N = a.N;
   const float nan = __builtin_nanf("");

@@ -417,6 +417,37 @@ int aldic_subpb2_dev_async(aldic_ctx* ctx, aldic_fem* fem, double beta, double m
                            double winstepsize, const double* d_U, const double* d_F,
                            double* d_Uhat, int32_t max_iter, int32_t* d_iterations);
 
+/* One iteration of the augmented-Lagrangian loop of
+ * TemporalMatch_quadtree_ST1.m:337-387 on the device, with no host
+ * synchronisation (the iterations of a frame can be recorded in one
+ * graph):
+ *   USubpb1 = Subpb1(UOld = USubpb2, FOld = FSubpb2 = d_F1, vdual) on the
+ *     plan's nodes (rect plan, winsize_List in d_winsize_xy), then the bad
+ *     points of Subpb1Quadtree.m:167-182 (ConvIt < 0 or > 99 or > max(mean
+ *     + 0.25 std, 10)) -- where the reference fills them by natural-
+ *     neighbour interpolation (host code, aldic_nan_fill), this step gives
+ *     a bad or thrown node the global estimate USubpb2 it started from;
+ *   USubpb2 = Subpb2(USubpb1, d_F1) on the mesh (Jacobi-PCG, at most
+ *     cg_max_iter iterations), hole-edge nodes (d_hole nonzero, may be
+ *     NULL) keep USubpb1 (:359);
+ *   when check_update: UpdateY = ||USubpb2_prev - USubpb2|| / sqrt(2n) and
+ *     UpdateY2 likewise for USubpb1 into d_update[0..1] (:363-366);
+ *   vdual = USubpb2 - USubpb1 (:379); when check_update and either norm is
+ *     below tol2 the loop is done (:381-384) and later iterations on the
+ *     same d_state do nothing.
+ * In/out (FP64 device arrays): d_U1 (USubpb1, 2n: on input the previous
+ * USubpb1, e.g. LocalICGN's), d_U2 (USubpb2, 2n), d_vdual (2n).  d_state:
+ * int32[2] = {done, bad Subpb1 nodes of the last executed iteration};
+ * first != 0 clears done and sets d_update to {1e4, NaN}.  The plan and
+ * the mesh must hold the same nodes.  No counterpart in the reference as
+ * a single call; it replaces the body of its ALM while loop. */
+int aldic_alm_iteration_dev(aldic_ctx* ctx, const aldic_images* im, const aldic_plan* plan,
+                            aldic_fem* fem, const double* d_coords, const int32_t* d_winsize_xy,
+                            double mu, double beta, double alpha, double winstepsize, double tol,
+                            double tol2, int32_t check_update, int32_t first, const double* d_F1,
+                            const uint8_t* d_hole, double* d_U1, double* d_U2, double* d_vdual,
+                            double* d_update, int32_t* d_state, int32_t cg_max_iter);
+
 /* ---- host post-processing of the drivers (no device work) ---- */
 
 /* scatteredInterpolant(xs, ys, vals, 'natural', 'nearest') evaluated at

@@ -1,5 +1,9 @@
-"""The two further GPU configurations of the workload set, at full size,
-through the library's own entry points.
+"""The GPU configurations of the workload set, at full size, through the
+library's own entry points.
+
+* Stereo pair 2048 x 2048 (the bench's second workload) with the network
+  trained on the full 2M patch pairs: search and learned starts, fallback
+  rate, accuracy by subset size, oracle sample.
 
 * Large stereo pair: 4096 x 3000 12-bit speckle at 0.01 spots/px^2; the
   right view is the left under a 25 px x-shift plus a Gaussian bulge of
@@ -26,6 +30,8 @@ stopping rule); accuracy against the known fields as stated at each
 assertion (the method's noise floor on these images, measured by the
 oracle on the same nodes).
 """
+import time
+
 import numpy as np
 import pytest
 
@@ -321,3 +327,92 @@ def test_incremental_sequence_alm(pkg, orc, mesh):
     assert worst["alm_gap"] <= 0.01
     fem.close()
     plan.close()
+
+
+def test_stereo2048_pair_with_full_training(pkg, orc, synth, mesh):
+    """The 2048 x 2048 tilted-plane pair of the bench (12-bit, 0.015
+    spots/px^2, seed 2, 40 px +/- 6 px disparity, ~54k nodes of 8-64 px,
+    the ROI's high-x margin widened by the largest disparity) with the
+    network trained on the full 2M pairs: search start and learned start,
+    the fallback rate (<= 5%), accuracy by subset size over the nodes the
+    reference's bad-point rule keeps (the method's noise floor on this
+    speckle, measured by the oracle: 64 px <= 0.01 px, 32 px <= 0.025,
+    16 px <= 0.06), and the search-start IC-GN against the FP64 oracle on
+    60 nodes of each size from the same U0 (|dU| <= 1e-3 px)."""
+    api = pkg.api
+    from aldic_amd import learned
+    M = N = 2048
+    uf, vf = synth.field_stereo(M, N)
+    f12, g12, _ = synth.make_pair(M, N, uf, vf, density=0.015, seed=2, bits=12)
+    f, g = _norm(f12), _norm(g12)
+    coords, ws = mesh.hole_nodes(M, N, (1024.5, 1024.5), 300.0, extra=(0, 50, 0, 0))
+    n = len(ws)
+    _say(f"stereo 2048: {n} nodes")
+    assert n > 50_000
+    ctx = api.default_context()
+    DA = api.DeviceArray
+    d_f = DA.from_host(ctx, f, np.float32)
+    d_g = DA.from_host(ctx, g, np.float32)
+    im = api.Images.from_device(d_f.ptr, d_g.ptr, M, N, ctx=ctx)
+    plan = api.Plan(ws, n, ctx=ctx)
+    d_coords = DA.from_host(ctx, coords, np.float64)
+    d_ws = DA.from_host(ctx, ws, np.int32)
+    sws, sst, r = 32, 32, 50
+    nX = len(range(1, M - sws - sst + 1, sst))
+    nY = len(range(1, N - sws - sst + 1, sst))
+    s_out = [DA(ctx, nX * nY, np.float64) for _ in range(5)]
+    s_pk = [DA(ctx, nX * nY, np.int32) for _ in range(2)]
+    d_U0, d_U = DA(ctx, 2 * n, np.float32), DA(ctx, 2 * n, np.float32)
+    d_conv, d_z = DA(ctx, n, np.int32), DA(ctx, n, np.float32)
+    api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], sws, sst, [r] * 4,
+                           *[o.ptr for o in s_out], *[p.ptr for p in s_pk])
+    api.grid_to_nodes_dev(ctx, s_out[2].ptr, s_out[3].ptr, nX, nY, sws / 2 + 1, sws / 2 + 1,
+                          sst, d_coords.ptr, n, d_U0.ptr)
+    api.local_icgn_dev(im, plan, d_coords.ptr, d_U0.ptr, d_ws.ptr, 1e-3, d_U.ptr, None, None,
+                       d_conv.ptr, None, d_z.ptr)
+    U0 = d_U0.to_host().astype(np.float64)
+    Us = d_U.to_host().astype(np.float64).reshape(-1, 2)
+    conv_s, z_s = d_conv.to_host(), d_z.to_host()
+    # the learned start with the network trained on the full 2M pairs
+    t0 = time.perf_counter()
+    net = learned.Network(ctx, max_batch=256, seed=0)
+    hist = learned.train(net, samples=2_000_000, batch=256, lr=2e-3, tiles=64, seed=4)
+    _say(f"trained on {hist['samples']} pairs in {time.perf_counter() - t0:.1f} s")
+    assert hist["samples"] >= 2_000_000
+    inet = learned.Network(ctx, max_batch=(n + 15) // 16 * 16, params=net.get_params())
+    net.close()
+    li = learned.LearnedInit(inet, im, d_f.ptr, d_g.ptr, plan, d_coords.ptr, n, d_ws.ptr,
+                             prior_grid=([1, M], [1, N], sws, 256),
+                             fallback_grid=([1, M], [1, N], sws, sst), region=(r, r, r, r),
+                             max_iter=30, min_zncc=0.9, tol=1e-3)
+    li.run(0)
+    ctx.synchronize()
+    Ul, _, conv_l, z_l = [a.to_host() for a in li.outputs(0)]
+    Ul = Ul.astype(np.float64).reshape(-1, 2)
+    nfb = li.fallback_count(0)
+    _say(f"fallback {nfb} ({100 * nfb / n:.2f}%)")
+    assert nfb / n <= 0.05
+    truth = np.stack([uf(coords[:, 0], coords[:, 1]), vf(coords[:, 0], coords[:, 1])], 1)
+    limits = {64: 0.01, 32: 0.025, 16: 0.06}
+    for name, U, conv, z in (("search", Us, conv_s, z_s), ("learned", Ul, conv_l, z_l)):
+        bad, _ = api.icgn_bad_points(conv)
+        ok = (conv >= 1) & (conv <= 100) & (z >= 0.9)
+        ok[bad] = False
+        for side, lim in limits.items():
+            sel = ok & (ws == side)
+            rms = np.sqrt(np.mean(np.sum((U[sel] - truth[sel]) ** 2, 1)))
+            _say(f"{name} {side} px: rms {rms:.4f} px over {sel.sum()} of {(ws == side).sum()}")
+            assert sel.sum() >= 0.5 * (ws == side).sum() and rms <= lim
+    rng = np.random.default_rng(5)
+    idx = np.sort(np.concatenate([rng.choice(np.flatnonzero(ws == s), 60, replace=False)
+                                  for s in (8, 16, 32, 64)]))
+    Uo, _, _, convo, _, zo = orc.local_icgn(orc.Images(f, g), np.asfortranarray(coords[idx]),
+                                            U0.reshape(-1, 2)[idx].reshape(-1), ws[idx], 1e-3,
+                                            nthreads=8)
+    both = (convo >= 1) & (convo <= 100) & (conv_s[idx] >= 1) & (conv_s[idx] <= 100)
+    du = np.abs(Uo.reshape(-1, 2) - Us[idx]).max(axis=1)[both]
+    _say(f"oracle sample: {both.sum()} of {len(idx)} converged in both, max |dU| {du.max():.2e}")
+    assert both.sum() >= 0.8 * len(idx) and du.max() <= 1e-3
+    inet.close()
+    plan.close()
+    im.close()
