@@ -36,6 +36,8 @@ reference's rule (ConvIt > 99 or > max(mean + std, 6) over the good ones,
 LocalICGNQuadtree.m:167-192).  The JSON line's value is the converged
 subsets/s of the learned start with the step-size rule on the headline
 workload; the other starts, rules and the second workload sit beside it,
+with one frame of the incremental sequence (temporal IC-GN and the two
+augmented-Lagrangian iterations replayed as one HIP graph, `seq2048_alm`),
 each with its RMS error against the known field.  The roofline is the
 dominant kernel of that step (its HIP-event time, algorithmic bytes); on
 rank 0 the FP64 CPU restatement (oracle, OpenMP) runs the search-start path
@@ -537,6 +539,123 @@ def run_workload(pkg, W, params, args, headline):
                                          shapes=[int(c) for c in chosen])
 
 
+def alm_block(pkg, args):
+    """The incremental sequence's per-frame work on the device: a 2048 x 2048
+    temporal pair (0.1% x-stretch, 0.7 px y-translation; 12-bit speckle,
+    0.015 spots/px^2, seed 3) over the quadtree mesh of a centred hole
+    (r = 300 px; 64 px elements refined to 8 px at the hole, 64 px
+    subsets).  Timed: the temporal IC-GN of every node (device, the plan's
+    size classes), and the two ALM iterations of TemporalMatch_quadtree_
+    ST1.m:337-387 (Subpb1, bad-point rule, Subpb2, hole-edge copy, UpdateY,
+    duals, early exit) replayed as one captured graph.  The starting
+    USubpb1/USubpb2 come from the host driver (LocalICGNQuadtree with its
+    NaN fill, then Subpb2), as the reference's loop receives them."""
+    api = pkg.api
+    from aldic_amd import quadtree
+    synth = load("aldic_synth_b", "3d-stereo-aldic_amd/synth.py")
+    mesh = load("aldic_mesh_b", "3d-stereo-aldic_amd/mesh.py")
+    M = N = 2048
+    xc = (M + 1) / 2.0
+    uf = lambda X, Y: 0.001 * (X - xc)  # noqa: E731
+    vf = lambda X, Y: 0.7 + 0.0 * X  # noqa: E731
+    f12, g12, _ = synth.make_pair(M, N, uf, vf, density=0.015, seed=3, bits=12)
+    f = np.asfortranarray((f12 - f12.mean()) / f12.std(ddof=1))
+    g = np.asfortranarray((g12 - g12.mean()) / g12.std(ddof=1))
+    mask = mesh.hole_mask(M, N, (1024.5, 1024.5), 300.0)
+    ws = step = 64
+    c0, nX, nY = mesh.uniform_grid([1, M], [1, N], ws, step)
+    DICmesh = quadtree.MeshSetUp(c0[:, 0].reshape(nY, nX), c0[:, 1].reshape(nY, nX),
+                                 {"ImgSize": (M, N), "winstepsize": step})
+    DICmesh["elementMinSize"] = 8
+    qm, _, _ = quadtree.GenerateQuadtreeMesh(np.zeros(2 * len(c0)), None, mask, DICmesh,
+                                             {"winstepsize": step, "winsize": ws})
+    coords = np.asfortranarray(qm["coordinatesFEM"])
+    n = len(coords)
+    ctx = api.Context(0)
+    DA = api.DeviceArray
+    DICpara = {"winsize": ws, "winstepsize": step, "ImgRefMask": mask}
+    Df = pkg.funImgGradient(f, g, mask, ctx=ctx)
+    im = api.Images(f, g, mask, Df, ctx=ctx)
+    U0 = np.empty(2 * n)
+    U0[0::2] = np.round(uf(coords[:, 0], coords[:, 1]))
+    U0[1::2] = np.round(vf(coords[:, 0], coords[:, 1]))
+    # the temporal IC-GN of the frame on the device
+    wsn = np.full(n, ws, np.int32)
+    plan = api.Plan(wsn, n, ctx=ctx)
+    d_c = DA.from_host(ctx, coords, np.float64)
+    d_ws = DA.from_host(ctx, wsn, np.int32)
+    d_U0 = DA.from_host(ctx, U0, np.float32)
+    d_Uf, d_Ff = DA(ctx, 2 * n, np.float32), DA(ctx, 4 * n, np.float32)
+    d_cv, d_z = DA(ctx, n, np.int32), DA(ctx, n, np.float32)
+    icgn = lambda: api.local_icgn_dev(im, plan, d_c.ptr, d_U0.ptr, d_ws.ptr, 1e-2,  # noqa: E731
+                                      d_Uf.ptr, d_Ff.ptr, None, d_cv.ptr, None, d_z.ptr)
+    icgn()
+    ctx.synchronize()
+    reps = max(args.steps, 5)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        icgn()
+    ctx.synchronize()
+    icgn_ms = (time.perf_counter() - t0) / reps * 1e3
+    # the ALM loop's start, as the reference's loop receives it
+    U1, F1, _, _, _, _, hole = pkg.LocalICGNQuadtree(U0, coords, Df, f, g, DICpara, "GaussNewton",
+                                                      1e-2, 1, ctx=ctx)
+    mu = 1e-3
+    beta = 1e-3 * step * step * mu
+    fem = pkg.FEMesh(coords, qm["elementsFEM"], qm["dirichlet"], None, 2, ctx=ctx)
+    U2 = fem.solve(beta, mu, U1, F1, 0.0, step)
+    U2.reshape(-1, 2)[hole] = U1.reshape(-1, 2)[hole]
+    vdual = U2 - U1
+    wl = np.full((n, 2), ws, np.int32)
+    plan1 = api.Plan(np.asfortranarray(wl), n, rect=True, ctx=ctx)
+    d_wl = DA.from_host(ctx, np.asfortranarray(wl), np.int32)
+    d_F1 = DA.from_host(ctx, F1, np.float64)
+    hole8 = np.zeros(n, np.uint8)
+    hole8[hole] = 1
+    d_hole = DA.from_host(ctx, hole8, np.uint8)
+    d_U1, d_U2, d_vd = (DA(ctx, 2 * n, np.float64) for _ in range(3))
+    d_upd, d_state = DA(ctx, 2, np.float64), DA(ctx, 2, np.int32)
+
+    def reload_inputs():
+        d_U1.upload(U1)
+        d_U2.upload(U2)
+        d_vd.upload(vdual)
+
+    def iterations():
+        for k in range(2):
+            api.alm_iteration_dev(im, plan1, fem, d_c.ptr, d_wl.ptr, mu, beta, 1e-2, 1e-2, True,
+                                  k == 0, d_F1.ptr, d_hole.ptr, d_U1.ptr, d_U2.ptr, d_vd.ptr,
+                                  d_upd.ptr, d_state.ptr, winstepsize=step, cg_max_iter=512)
+
+    reload_inputs()
+    iterations()
+    ctx.synchronize()
+    state = d_state.to_host()
+    graph = ctx.capture(iterations)
+    tot = 0.0
+    for _ in range(reps):
+        reload_inputs()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        graph.launch(ctx)
+        ctx.synchronize()
+        tot += time.perf_counter() - t0
+    alm_ms = tot / reps * 1e3
+    upd = d_upd.to_host()
+    graph.close()
+    for o in (plan, plan1, fem, im):
+        o.close()
+    return {"workload": "2048x2048 temporal pair, quadtree mesh of a r=300 hole (64 px "
+                        "elements to 8 px), 64 px subsets",
+            "nodes": n, "elements": int(len(qm["elementsFEM"])),
+            "temporal_icgn_ms": round(icgn_ms, 4),
+            "alm_two_iterations_graph_ms": round(alm_ms, 4),
+            "frame_ms": round(icgn_ms + alm_ms, 4),
+            "frames_per_s": round(1e3 / (icgn_ms + alm_ms), 1),
+            "alm_done_after_first": bool(state[0]), "bad_subpb1_nodes": int(state[1]),
+            "update_y": [float(upd[0]), float(upd[1])]}
+
+
 def summary(d, n, steps):
     acc = d["acc"]
     nc = int(acc["ok"].sum())
@@ -561,6 +680,7 @@ def main():
                     help="IC-GN nodes of the CPU baseline (0 = all)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: every core of the share")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-alm", action="store_true", help="skip the incremental-sequence frame")
     ap.add_argument("--train-samples", type=int, default=2_000_000)
     ap.add_argument("--workloads", default="bulge4096x3000,tilt2048",
                     help="comma-separated; the first is the headline")
@@ -662,6 +782,9 @@ def main():
     for name in names[1:]:
         line[name] = block(results[name])
     line["learned_vs_fft"] = line[head_name]["learned_vs_search_step_rule"]
+    if not args.no_alm:
+        progress("incremental-sequence frame: temporal IC-GN + ALM graph")
+        line["seq2048_alm"] = alm_block(pkg, args)
     if not args.no_cpu:
         threads = args.cpu_threads or cpu_threads()
         progress(f"CPU baseline (FP64 oracle, {threads} threads)")
