@@ -806,8 +806,16 @@ __device__ __forceinline__ void block_sum(float (&v)[K], float* red) {
 }
 
 // bwselect(BW, c, r, conn) on the S1 x S2 window held in LDS: out = the
-// conn-connected component of BW containing (r, c) (1-based), by monotone
-// label propagation until a full sweep changes nothing.
+// conn-connected component of BW containing (r, c) (1-based).  Rounds of
+// line sweeps: each thread owns a line along dim 1 (then along dim 2) and
+// walks it both ways, setting every BW pixel whose predecessor in the walk
+// is set or that touches a set pixel of the two neighbouring lines (4: the
+// same position; 8: also the diagonal ones), until a round changes
+// nothing.  A whole run of a line fills in one walk, so a round advances
+// the fill around one bend of the component.  Every pixel set is in BW and
+// touches a set pixel, so reading a neighbouring line while its owner
+// writes it only delays the fill; each pixel is written by one thread per
+// phase.
 __device__ void lds_bwselect(const uint8_t* bw, uint8_t* out, int S1, int S2, int r, int c,
                              int conn) {
   const int NS = S1 * S2;
@@ -818,26 +826,69 @@ __device__ void lds_bwselect(const uint8_t* bw, uint8_t* out, int S1, int S2, in
     out[s] = bw[s] ? 1 : 0;
   }
   __syncthreads();
+  const bool diag = conn == 8;
   bool again = true;
   while (again) {
     int changed = 0;
-    for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
-      if (!bw[p] || out[p]) continue;
-      const int pr = p % S1, pc = p / S1;
-      bool hit = false;
-      for (int dc = -1; dc <= 1 && !hit; dc++)
-        for (int dr = -1; dr <= 1; dr++) {
-          if ((dr == 0 && dc == 0) || (conn == 4 && dr != 0 && dc != 0)) continue;
-          const int qr = pr + dr, qc = pc + dc;
-          if (qr < 0 || qr >= S1 || qc < 0 || qc >= S2) continue;
-          if (out[qr + qc * S1]) {
-            hit = true;
-            break;
+    // lines along dim 1: column j, pixels i + j S1, neighbours in columns j +- 1
+    for (int j = threadIdx.x; j < S2; j += GEN_THREADS) {
+      for (int dir = 0; dir < 2; dir++) {
+        int prev = 0;
+        for (int k = 0; k < S1; k++) {
+          const int i = dir == 0 ? k : S1 - 1 - k;
+          const int q = i + j * S1;
+          if (!bw[q]) {
+            prev = 0;
+            continue;
           }
+          int v = out[q] | prev;
+          if (!v) {
+#pragma unroll
+            for (int dj = -1; dj <= 1; dj += 2) {
+              const int jj = j + dj;
+              if (jj < 0 || jj >= S2) continue;
+              const uint8_t* o = out + jj * S1;
+              v |= o[i];
+              if (diag) v |= (i > 0 ? o[i - 1] : 0) | (i + 1 < S1 ? o[i + 1] : 0);
+            }
+          }
+          if (v && !out[q]) {
+            out[q] = 1;
+            changed = 1;
+          }
+          prev = v;
         }
-      if (hit) {
-        out[p] = 1;
-        changed = 1;
+      }
+    }
+    __syncthreads();
+    // lines along dim 2: row i, pixels i + j S1, neighbours in rows i +- 1
+    for (int i = threadIdx.x; i < S1; i += GEN_THREADS) {
+      for (int dir = 0; dir < 2; dir++) {
+        int prev = 0;
+        for (int k = 0; k < S2; k++) {
+          const int j = dir == 0 ? k : S2 - 1 - k;
+          const int q = i + j * S1;
+          if (!bw[q]) {
+            prev = 0;
+            continue;
+          }
+          int v = out[q] | prev;
+          if (!v) {
+#pragma unroll
+            for (int di = -1; di <= 1; di += 2) {
+              const int ii = i + di;
+              if (ii < 0 || ii >= S1) continue;
+              v |= out[ii + j * S1];
+              if (diag)
+                v |= (j > 0 ? out[ii + (j - 1) * S1] : 0) | (j + 1 < S2 ? out[ii + (j + 1) * S1] : 0);
+            }
+          }
+          if (v && !out[q]) {
+            out[q] = 1;
+            changed = 1;
+          }
+          prev = v;
+        }
       }
     }
     again = __syncthreads_or(changed) != 0;
@@ -875,6 +926,7 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
   __shared__ uint8_t sbwg[GEN_MAXPIX];  // current g mask
   __shared__ uint8_t stmp[GEN_MAXPIX];  // logical image fed to bwselect
   __shared__ uint8_t sfsel[GEN_MAXPIX]; // Subpb1: f selection after its mask block
+  __shared__ uint8_t sbwf8[GEN_MAXPIX]; // 8-connected component of the f mask (on first use)
   __shared__ float red[4 * 32];
   if (a.skip && *a.skip) return;  // uniform: the whole launch is skipped
   const int total = count ? min(*count, fixed_count) : fixed_count;
@@ -1056,7 +1108,7 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
       int step = 0;
       const bool dp_rule = MODE == MODE_ICGN && a.stop_rule == 1;
       bool dp_conv = false;
-      bool have_bwg = false, err = false, defer_oob = false;
+      bool have_bwg = false, err = false, defer_oob = false, have_f8 = false;
       float sum_bwg = 0.0f;
       const int ws0 = wx0;
       int cur_w = wx;
@@ -1133,22 +1185,36 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
           break;
         }
         if (c0 > 0.0f || zg > 0.0f) {
-          // :170-214 (ICGN) / :158-192 (Subpb1)
-          for (int p = threadIdx.x; p < NS; p += GEN_THREADS) stmp[p] = sg[p] != 0.0f;
-          __syncthreads();
-          lds_bwselect(stmp, sbwg, S1, S2, seed_r, seed_c, MODE == MODE_ICGN ? 8 : 4);
-          __syncthreads();
-          float nz = 0.0f;
-          for (int p = threadIdx.x; p < NS; p += GEN_THREADS) nz += sbwg[p] ? 0.0f : 1.0f;
-          float vv[1] = {nz};
-          block_sum<1>(vv, red);
-          if (vv[0] == 0.0f) {
-            for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
-              const int64_t idx = (int64_t)(ix0 + p % S1) + (int64_t)(iy0 + p / S1) * M;
-              stmp[p] = a.mask[idx] != 0.0f;
-            }
+          // :170-214 (ICGN) / :158-192 (Subpb1): the component of g ~= 0 at
+          // the centre; where it is the whole window, the 8-connected
+          // component of the f mask instead.  With no zero in g that
+          // component is the whole window without any fill, and the f
+          // mask's component does not change over the iterations: it is
+          // computed once per subset.
+          bool allone = true;
+          if (zg > 0.0f) {
+            for (int p = threadIdx.x; p < NS; p += GEN_THREADS) stmp[p] = sg[p] != 0.0f;
             __syncthreads();
-            lds_bwselect(stmp, sbwg, S1, S2, seed_r, seed_c, 8);
+            lds_bwselect(stmp, sbwg, S1, S2, seed_r, seed_c, MODE == MODE_ICGN ? 8 : 4);
+            __syncthreads();
+            float nz = 0.0f;
+            for (int p = threadIdx.x; p < NS; p += GEN_THREADS) nz += sbwg[p] ? 0.0f : 1.0f;
+            float vv[1] = {nz};
+            block_sum<1>(vv, red);
+            allone = vv[0] == 0.0f;
+          }
+          if (allone) {
+            if (!have_f8) {
+              for (int p = threadIdx.x; p < NS; p += GEN_THREADS) {
+                const int64_t idx = (int64_t)(ix0 + p % S1) + (int64_t)(iy0 + p / S1) * M;
+                stmp[p] = a.mask[idx] != 0.0f;
+              }
+              __syncthreads();
+              lds_bwselect(stmp, sbwf8, S1, S2, seed_r, seed_c, 8);
+              __syncthreads();
+              have_f8 = true;
+            }
+            for (int p = threadIdx.x; p < NS; p += GEN_THREADS) sbwg[p] = sbwf8[p];
             __syncthreads();
           }
           float sb = 0.0f;
@@ -1171,9 +1237,9 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
             f_moments(sfsel);
             hessian(sbwg);
             fsel = sfsel;
-          } else {
-            f_moments(sbwf);
           }
+          // IC-GN recomputes the f moments here over the same selection
+          // (sbwf): the values of the setup, kept
           __syncthreads();
         }
         // g moments over |g| > 1e-10 (two passes as in the reference)
