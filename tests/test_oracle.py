@@ -639,3 +639,23 @@ def test_icgn_step_size_rule(orc):
         # the reference rule on the same node stops elsewhere or at another warp
         Ur, Fr, _, cr, _, _ = orc.local_icgn(im, c, U0, wsa, 1e-3)
         assert cr[0] != s or np.abs(Ur - U).max() > 1e-9
+
+
+def test_normalize_roi_moments_and_the_host_version(orc, pkg):
+    """funNormalizeImg.m:22-40: (Img - mean) / std over the ROI (clamped to
+    the image), applied to the whole image.  Pinned by the ROI's moments
+    (mean 0, sample std 1), by invariance to an affine change of intensity,
+    and by an ROI that only the clamp keeps inside the image; the host
+    funNormalizeImg must agree with it."""
+    rng = np.random.default_rng(11)
+    img = rng.uniform(0, 4095, size=(37, 29))
+    for gx, gy in (([5, 30], [3, 20]), ([-4, 60], [0, 29])):
+        out = orc.normalize(img, gx, gy)
+        a0, a1 = max(gx[0], 1) - 1, min(gx[1], 37)
+        b0, b1 = max(gy[0], 1) - 1, min(gy[1], 29)
+        roi = out[a0:a1, b0:b1]
+        assert abs(roi.mean()) < 1e-12 and abs(roi.std(ddof=1) - 1.0) < 1e-12
+        assert np.allclose(orc.normalize(3.5 * img - 7.0, gx, gy), out, atol=1e-12)
+        host, rng_out = pkg.funNormalizeImg([img, img], {"gridx": gx, "gridy": gy})
+        assert np.allclose(host[0], out, atol=1e-12) and np.allclose(host[1], out, atol=1e-12)
+        assert rng_out["gridx"] == [a0 + 1, a1] and rng_out["gridy"] == [b0 + 1, b1]
