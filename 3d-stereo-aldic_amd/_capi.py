@@ -58,6 +58,8 @@ SIGNATURES = {
     "aldic_subpb2_dev_async": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_double, C.c_double,
                                          _vp, _vp, _vp, C.c_int32, _vp]),
     "aldic_images_planes": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "aldic_images_set_interp": (C.c_int, [_vp, C.c_int32]),
+    "aldic_bspline_prefilter_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64]),
     "aldic_images_create_raw": (C.c_int, [_vp, _vp, _vp, C.c_int32, _vp, _i64, _i64, _vp, _vp,
                                           _pp, _vp]),
     "aldic_normalize_raw_dev": (C.c_int, [_vp, _vp, C.c_int32, _i64, _i64, _vp, _vp, _vp, _vp]),

@@ -221,6 +221,7 @@ class Images:
         self._keep = None
         self.ptr = p
         self.M, self.N = M, N
+        self.interp = "local"
 
     @classmethod
     def from_device(cls, fm_ptr, g_ptr, M, N, mask_ptr=None, fx_ptr=None, fy_ptr=None,
@@ -232,6 +233,7 @@ class Images:
                                             M, N, C.byref(p)))
         self.ptr = p
         self.M, self.N = M, N
+        self.interp = "local"
         return self
 
     @classmethod
@@ -270,7 +272,22 @@ class Images:
                                             stats.ctypes.data))
         self.ptr = p
         self.M, self.N = M, N
+        self.interp = "local"
         self.stats = tuple(stats)
+        return self
+
+    def set_interp(self, mode):
+        """Interpolant of ImgDef for every solver on these images: "local"
+        (the reference's kernel, ba_interp2_spline.cpp:310-325; the default
+        and the only mode with reference parity) or "bspline" (the
+        interpolating cubic B-spline on prefiltered coefficients; no
+        reference counterpart).  Calling it again remakes the
+        coefficients."""
+        modes = {"local": 0, "bspline": 1}
+        if mode not in modes:
+            raise ValueError(f"interp {mode!r}: 'local' or 'bspline'")
+        check(lib().aldic_images_set_interp(self.ptr, modes[mode]))
+        self.interp = mode
         return self
 
     def planes(self):
@@ -296,11 +313,19 @@ class Images:
             pass
 
 
-def _images_for(Df, ImgRef, ImgDef, ctx):
+def _images_for(Df, ImgRef, ImgDef, ctx, DICpara=None):
+    """Resident images for a driver call.  ``DICpara["InterpMethod"]``
+    ("local" by default, or "bspline": see Images.set_interp; no counterpart
+    in the reference's DICpara) picks the interpolant of images built here;
+    an Images argument keeps its own."""
     if isinstance(ImgRef, Images):
         return ImgRef
     mask = None if Df is None else Df.get("ImgRefMask")
-    return Images(ImgRef, ImgDef, mask, Df, ctx=ctx)
+    im = Images(ImgRef, ImgDef, mask, Df, ctx=ctx)
+    interp = (DICpara or {}).get("InterpMethod", "local")
+    if interp != "local":
+        im.set_interp(interp)
+    return im
 
 
 # ---------------------------------------------------------------------------
@@ -542,7 +567,7 @@ def LocalICGNQuadtree(U0, coordinatesFEM, Df, ImgRef, ImgDef, DICpara, ICGNmetho
         raise NotImplementedError("only first-order IC-GN (shapeFunctionOrder = 1) is built")
     if ICGNmethod != "GaussNewton":
         raise NotImplementedError("only ICGNmethod = 'GaussNewton' is used by the reference")
-    images = _images_for(Df, ImgRef, ImgDef, ctx)
+    images = _images_for(Df, ImgRef, ImgDef, ctx, DICpara)
     t0 = time.perf_counter()
     U, F, H, conv, hole, _z = local_icgn_raw(images, coordinatesFEM, U0, DICpara["winsize"], tol)
     LocalTime = time.perf_counter() - t0
@@ -572,7 +597,7 @@ def Subpb1Quadtree(UOld, FOld, udual, vdual, coordinatesFEM, Df, ImgRef, ImgDef,
     fill_nans=False returns U with the bad nodes still NaN."""
     if ALSolveStep == 1:
         raise ValueError("ALSolveStep == 1 is not reachable in the reference (undefined Htemp)")
-    images = _images_for(Df, ImgRef, ImgDef, ctx)
+    images = _images_for(Df, ImgRef, ImgDef, ctx, DICpara)
     n = np.asarray(coordinatesFEM).shape[0]
     wl = DICpara.get("winsize_List")
     if wl is None:

@@ -70,3 +70,17 @@ __device__ __forceinline__ void spline_weights(float t, float w[4]) {
   w[2] = t * (45.0f + t * (75.0f - 64.0f * t)) * s;
   w[3] = t * (-9.0f + t * (-15.0f + 24.0f * t)) * s;
 }
+
+// The interpolating cubic B-spline (an option beside the reference's
+// kernel, evaluated on prefiltered coefficients): 6 times the basis at the
+// taps -1, 0, +1, +2 of the cell, (1-t)^3, 3t^3 - 6t^2 + 4,
+// -3t^3 + 3t^2 + 3t + 1, t^3.  A value built from them is 36 times the
+// spline.
+__device__ __forceinline__ void bspline_weights6(float t, float w[4]) {
+  const float u = 1.0f - t;
+  const float t2 = t * t, t3 = t2 * t;
+  w[0] = u * u * u;
+  w[1] = fmaf(3.0f, t3, fmaf(-6.0f, t2, 4.0f));
+  w[2] = fmaf(-3.0f, t3, fmaf(3.0f, t2, fmaf(3.0f, t, 1.0f)));
+  w[3] = t3;
+}

@@ -77,6 +77,10 @@ struct aldic_images {
   float* mask = nullptr;
   float* fx = nullptr;
   float* fy = nullptr;
+  // interpolant of g: 0 the reference's local kernel on g, 1 the
+  // interpolating cubic B-spline on gc (its coefficients, made on request)
+  int interp = 0;
+  float* gc = nullptr;
 };
 
 struct aldic_plan {

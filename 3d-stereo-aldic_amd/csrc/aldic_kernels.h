@@ -14,9 +14,11 @@ struct SolveArgs {
   const float* fm;
   const float* fx;
   const float* fy;
-  const float* g;
+  const float* g;       // ImgDef, or its B-spline coefficients when basis is 1
   const float* mask;
   int M, N;
+  int basis;            // 0: the reference's local kernel on ImgDef; 1: the
+                        // interpolating cubic B-spline on its coefficients
   int n;                // number of nodes (stride of H21)
   const int* list;      // node indices handled by this launch
   int cnt;              // length of list
@@ -77,6 +79,10 @@ hipError_t launch_raw_sums(const void* raw, int bits, int64_t M, int64_t x0, int
                            int64_t y0, int64_t y1, unsigned long long* sums, hipStream_t s);
 hipError_t launch_raw_normalize(const void* raw, int bits, int64_t n, double mean, double sd,
                                 double* out, hipStream_t s);
+// interpolating cubic B-spline coefficients of an M x N FP32 plane (mirror
+// boundaries); in and out must not overlap
+hipError_t launch_bspline_prefilter(const float* in, float* out, int64_t M, int64_t N,
+                                   hipStream_t s);
 hipError_t launch_prep_planes(const double* ref, const double* def, const double* mask,
                               int64_t M, int64_t N, float* fm, float* g, float* maskf,
                               hipStream_t s);

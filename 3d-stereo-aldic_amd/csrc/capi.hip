@@ -528,7 +528,7 @@ static int alloc_planes(aldic_images* im) {
 extern "C" int aldic_images_destroy(aldic_images* im) {
   if (!im) return ALDIC_OK;
   (void)hipSetDevice(im->device);
-  float* ps[5] = {im->fm, im->g, im->mask, im->fx, im->fy};
+  float* ps[6] = {im->fm, im->g, im->mask, im->fx, im->fy, im->gc};
   for (float* p : ps)
     if (p) (void)hipFree(p);
   delete im;
@@ -651,6 +651,35 @@ extern "C" int aldic_images_planes(const aldic_images* im, const float** fm, con
   if (mask) *mask = im->mask;
   if (fx) *fx = im->fx;
   if (fy) *fy = im->fy;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_bspline_prefilter_dev(aldic_ctx* ctx, const float* d_in, float* d_out,
+                                           int64_t M, int64_t N) {
+  if (!ctx || !d_in || !d_out) return fail(ALDIC_EINVAL, "bspline_prefilter_dev: NULL argument");
+  if (M < 1 || N < 1 || M > INT32_MAX / 2 || N > INT32_MAX / 2)
+    return fail(ALDIC_EINVAL, "bspline_prefilter_dev: bad size");
+  const int64_t MN = M * N;
+  if (d_in < d_out + MN && d_out < d_in + MN)
+    return fail(ALDIC_EINVAL, "bspline_prefilter_dev: input and output overlap");
+  RC_TRY(set_device(ctx));
+  HIP_TRY(aldic::launch_bspline_prefilter(d_in, d_out, M, N, ctx->stream));
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_images_set_interp(aldic_images* im, int32_t mode) {
+  if (!im || !im->ctx) return fail(ALDIC_EINVAL, "images_set_interp: NULL images");
+  if (mode != ALDIC_INTERP_LOCAL && mode != ALDIC_INTERP_BSPLINE)
+    return fail(ALDIC_EINVAL, "images_set_interp: mode must be ALDIC_INTERP_LOCAL or _BSPLINE");
+  RC_TRY(set_device(im->ctx));
+  if (mode == ALDIC_INTERP_BSPLINE) {
+    if (!im->gc) HIP_TRY(hipMalloc(&im->gc, sizeof(float) * (size_t)(im->M * im->N)));
+    HIP_TRY(aldic::launch_bspline_prefilter(im->g, im->gc, im->M, im->N, im->ctx->stream));
+    // solvers may run on another context's stream: the plane is complete
+    // when this returns
+    HIP_TRY(hipStreamSynchronize(im->ctx->stream));
+  }
+  im->interp = mode;
   return ALDIC_OK;
 }
 
@@ -804,17 +833,24 @@ extern "C" int aldic_plan_destroy(aldic_plan* plan) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// The resident planes a solver reads; g is the B-spline coefficient plane
+// when the images were switched to that interpolant.
+void image_args(SolveArgs& a, const aldic_images* im) {
+  a.fm = im->fm;
+  a.fx = im->fx;
+  a.fy = im->fy;
+  a.g = im->interp ? im->gc : im->g;
+  a.basis = im->interp;
+  a.mask = im->mask;
+  a.M = (int)im->M;
+  a.N = (int)im->N;
+}
+
 SolveArgs icgn_args(const aldic_images* im, int64_t n, const double* d_coords, const float* d_U0,
                     const float* d_F0, const int32_t* d_winsize, double tol, float* d_U,
                     float* d_F, float* d_H21, int32_t* d_convIt, uint8_t* d_hole, float* d_zncc) {
   SolveArgs a{};
-  a.fm = im->fm;
-  a.fx = im->fx;
-  a.fy = im->fy;
-  a.g = im->g;
-  a.mask = im->mask;
-  a.M = (int)im->M;
-  a.N = (int)im->N;
+  image_args(a, im);
   a.n = (int)n;
   a.coords = d_coords;
   a.wsx = d_winsize;
@@ -974,13 +1010,7 @@ extern "C" int aldic_local_icgn_dev(aldic_ctx* ctx, const aldic_images* im, cons
     return fail(ALDIC_EINVAL, "local_icgn_dev: NULL array");
   RC_TRY(set_device(ctx));
   SolveArgs a{};
-  a.fm = im->fm;
-  a.fx = im->fx;
-  a.fy = im->fy;
-  a.g = im->g;
-  a.mask = im->mask;
-  a.M = (int)im->M;
-  a.N = (int)im->N;
+  image_args(a, im);
   a.coords = d_coords;
   a.wsx = d_winsize;
   a.wsy = nullptr;
@@ -1012,13 +1042,7 @@ extern "C" int aldic_local_icgn_autotune_dev(aldic_ctx* ctx, const aldic_images*
     return fail(ALDIC_EINVAL, "autotune: NULL array");
   RC_TRY(set_device(ctx));
   SolveArgs a{};
-  a.fm = im->fm;
-  a.fx = im->fx;
-  a.fy = im->fy;
-  a.g = im->g;
-  a.mask = im->mask;
-  a.M = (int)im->M;
-  a.N = (int)im->N;
+  image_args(a, im);
   a.n = (int)plan->n;
   a.coords = d_coords;
   a.wsx = d_winsize;
@@ -1109,13 +1133,7 @@ extern "C" int aldic_subpb1_dev(aldic_ctx* ctx, const aldic_images* im, const al
     return fail(ALDIC_EINVAL, "subpb1_dev: NULL array");
   RC_TRY(set_device(ctx));
   SolveArgs a{};
-  a.fm = im->fm;
-  a.fx = im->fx;
-  a.fy = im->fy;
-  a.g = im->g;
-  a.mask = im->mask;
-  a.M = (int)im->M;
-  a.N = (int)im->N;
+  image_args(a, im);
   a.coords = d_coords;
   a.wsx = d_winsize_xy;
   a.wsy = d_winsize_xy + plan->n;
@@ -1688,13 +1706,7 @@ extern "C" int aldic_alm_iteration_dev(aldic_ctx* ctx, const aldic_images* im,
   HIP_TRY(hipGetLastError());
   // Subpb1 from [FSubpb2 USubpb2] (Subpb1Quadtree.m:71-165); skipped once done
   SolveArgs a{};
-  a.fm = im->fm;
-  a.fx = im->fx;
-  a.fy = im->fy;
-  a.g = im->g;
-  a.mask = im->mask;
-  a.M = (int)im->M;
-  a.N = (int)im->N;
+  image_args(a, im);
   a.coords = d_coords;
   a.wsx = d_winsize_xy;
   a.wsy = d_winsize_xy + n;

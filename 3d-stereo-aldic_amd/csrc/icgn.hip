@@ -285,7 +285,7 @@ constexpr int fast_block() {
 // (-1, -1) tap offset is a non-negative base address.
 // FS > 0: every subset of the launch is FS x FS (the plan checked it), so
 // the pixel layout, the partial slot and the divisions are compile-time.
-template <int MODE, int NW, int PPL, int TS, int FS, bool WALK>
+template <int MODE, int NW, int PPL, int TS, int FS, bool WALK, int BS>
 __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float* tileb,
                                             float* red) {
   constexpr int NT = ALDIC_WAVE * NW;
@@ -605,11 +605,11 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
         __builtin_amdgcn_wave_barrier();
     }
     // gather + interpolate.  The value is kept 3136 times the interpolant
-    // (unscaled 56x weights on both axes): the residual below only sees
-    // (g - mean g) / sqrt(sum (g - mean g)^2), which is scale free, and the
-    // magnitude test of the reference (|g| <= 1e-10, :272-273) is scaled
-    // with it.
-    constexpr float TINY = 1e-10f * 3136.0f;
+    // (unscaled 56x weights on both axes; 36x for the B-spline option): the
+    // residual below only sees (g - mean g) / sqrt(sum (g - mean g)^2),
+    // which is scale free, and the magnitude test of the reference
+    // (|g| <= 1e-10, :272-273) is scaled with it.
+    constexpr float TINY = 1e-10f * (BS ? 36.0f : 3136.0f);
     const float cx = (float)(x0 - 1 - tx0) + P[4], cy = (float)(y0 - 1 - ty0) + P[5];
     int tiny = 0;
     float s1[2] = {0.0f, 0.0f};  // sum, sum of squares
@@ -624,8 +624,13 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       const float dxf = __builtin_amdgcn_fractf(tx), dyf = __builtin_amdgcn_fractf(ty);
       const int ix = (int)tx, iy = (int)ty;
       float wr[4], wc[4];
-      spline_weights56(dxf, wr);
-      spline_weights56(dyf, wc);
+      if constexpr (BS) {
+        bspline_weights6(dxf, wr);
+        bspline_weights6(dyf, wc);
+      } else {
+        spline_weights56(dxf, wr);
+        spline_weights56(dyf, wc);
+      }
       const float* base = tileb + __umul24(iy, TS) + ix;
       float acc = 0.0f;
 #pragma unroll
@@ -771,7 +776,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
 // group takes one entry (a separate instance: the loop costs the one-entry
 // kernel registers; a walking multi-wave one is held to two waves per
 // SIMD).
-template <int MODE, int NW, int PPL, int TS, int FS, bool WALK>
+template <int MODE, int NW, int PPL, int TS, int FS, bool WALK, int BS>
 __global__ __launch_bounds__(fast_block<NW>(), WALK && NW > 1 ? 2 : 1) void fast_kernel(
     SolveArgs a) {
   constexpr int NT = ALDIC_WAVE * NW;
@@ -787,10 +792,10 @@ __global__ __launch_bounds__(fast_block<NW>(), WALK && NW > 1 ? 2 : 1) void fast
   // and the reduction's barrier keep one entry's LDS use apart from the next
   if constexpr (WALK) {
     for (int slot = blockIdx.x * SUBS + sub; slot < cnt; slot += gridDim.x * SUBS)
-      fast_subset<MODE, NW, PPL, TS, FS, true>(a, slot, tiles[sub], red);
+      fast_subset<MODE, NW, PPL, TS, FS, true, BS>(a, slot, tiles[sub], red);
   } else {
     const int slot = blockIdx.x * SUBS + sub;
-    if (slot < cnt) fast_subset<MODE, NW, PPL, TS, FS, false>(a, slot, tiles[sub], red);
+    if (slot < cnt) fast_subset<MODE, NW, PPL, TS, FS, false, BS>(a, slot, tiles[sub], red);
   }
 }
 
@@ -895,12 +900,24 @@ __device__ void lds_bwselect(const uint8_t* bw, uint8_t* out, int S1, int S2, in
   }
 }
 
+template <int BS>
 __device__ __forceinline__ float interp_global(const float* g, int M, float u22, float v22) {
-  // u22, v22 are 1-based (dim-1, dim-2) coordinates inside [3, M-2] x [3, N-2]
+  // u22, v22 are 1-based (dim-1, dim-2) coordinates inside [3, M-2] x [3, N-2];
+  // BS: g holds B-spline coefficients and the weights are the basis'
   const float fu = floorf(u22), fv = floorf(v22);
   float wr[4], wc[4];
-  spline_weights(u22 - fu, wr);
-  spline_weights(v22 - fv, wc);
+  if constexpr (BS) {
+    bspline_weights6(u22 - fu, wr);
+    bspline_weights6(v22 - fv, wc);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      wr[k] *= 1.0f / 6.0f;
+      wc[k] *= 1.0f / 6.0f;
+    }
+  } else {
+    spline_weights(u22 - fu, wr);
+    spline_weights(v22 - fv, wc);
+  }
   const int ix = (int)fu - 2, iy = (int)fv - 2;  // 0-based first tap
   float acc = 0.0f;
 #pragma unroll
@@ -918,7 +935,7 @@ __device__ __forceinline__ int grow_ws(float A, float B, int w) {
   return (int)v;
 }
 
-template <int MODE>
+template <int MODE, int BS>
 __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const int* list,
                                                               const int* count, int fixed_count) {
   __shared__ float sg[GEN_MAXPIX];
@@ -1165,7 +1182,7 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
           oob += o ? 1.0f : 0.0f;
           nanw += nn ? 1.0f : 0.0f;
           float gv = 0.0f;
-          if (!o && !nn) gv = interp_global(a.g, M, (float)x0 + u, (float)y0 + v);
+          if (!o && !nn) gv = interp_global<BS>(a.g, M, (float)x0 + u, (float)y0 + v);
           sg[p] = gv;
           zg += (gv == 0.0f) ? 1.0f : 0.0f;
         }
@@ -1397,7 +1414,7 @@ int num_variants() { return NVAR; }
 
 int class_side(int cls) { return cls >= 0 && cls < 4 ? CLS_SIDE[cls] : 0; }
 
-template <int MODE, int C, int V>
+template <int MODE, int C, int V, int BS>
 static void launch_shape(const SolveArgs& a, hipStream_t s) {
   constexpr int NW = VAR_NW[C][V];
   constexpr int block = fast_block<NW>();
@@ -1409,29 +1426,37 @@ static void launch_shape(const SolveArgs& a, hipStream_t s) {
   if (a.dcnt) {
     const int g = std::min(grid, LIST_GRID);
     if (a.fixed_side)
-      fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], CLS_SIDE[C], true><<<g, block, 0, s>>>(a);
+      fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], CLS_SIDE[C], true, BS><<<g, block, 0, s>>>(a);
     else
-      fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], 0, true><<<g, block, 0, s>>>(a);
+      fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], 0, true, BS><<<g, block, 0, s>>>(a);
   } else if (a.fixed_side) {
-    fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], CLS_SIDE[C], false><<<grid, block, 0, s>>>(a);
+    fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], CLS_SIDE[C], false, BS><<<grid, block, 0, s>>>(a);
   } else {
-    fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], 0, false><<<grid, block, 0, s>>>(a);
+    fast_kernel<MODE, NW, VAR_PPL[C][V], CLS_TS[C], 0, false, BS><<<grid, block, 0, s>>>(a);
+  }
+}
+
+template <int MODE, int C, int BS>
+static void launch_variant(int var, const SolveArgs& a, hipStream_t s) {
+  switch (var) {
+    case 1:
+      launch_shape<MODE, C, 1, BS>(a, s);
+      break;
+    case 2:
+      launch_shape<MODE, C, 2, BS>(a, s);
+      break;
+    default:
+      launch_shape<MODE, C, 0, BS>(a, s);
+      break;
   }
 }
 
 template <int MODE, int C>
 static void launch_class(int var, const SolveArgs& a, hipStream_t s) {
-  switch (var) {
-    case 1:
-      launch_shape<MODE, C, 1>(a, s);
-      break;
-    case 2:
-      launch_shape<MODE, C, 2>(a, s);
-      break;
-    default:
-      launch_shape<MODE, C, 0>(a, s);
-      break;
-  }
+  if (a.basis)
+    launch_variant<MODE, C, 1>(var, a, s);
+  else
+    launch_variant<MODE, C, 0>(var, a, s);
 }
 
 template <int MODE>
@@ -1463,10 +1488,14 @@ hipError_t launch_fast(int mode, int cls, int var, const SolveArgs& a, hipStream
 
 hipError_t launch_generic(int mode, const SolveArgs& a, const int* list, const int* count,
                           int fixed_count, int grid, hipStream_t s) {
-  if (mode == 0)
-    generic_kernel<MODE_ICGN><<<grid, GEN_THREADS, 0, s>>>(a, list, count, fixed_count);
+  if (mode == 0 && a.basis)
+    generic_kernel<MODE_ICGN, 1><<<grid, GEN_THREADS, 0, s>>>(a, list, count, fixed_count);
+  else if (mode == 0)
+    generic_kernel<MODE_ICGN, 0><<<grid, GEN_THREADS, 0, s>>>(a, list, count, fixed_count);
+  else if (a.basis)
+    generic_kernel<MODE_SUBPB1, 1><<<grid, GEN_THREADS, 0, s>>>(a, list, count, fixed_count);
   else
-    generic_kernel<MODE_SUBPB1><<<grid, GEN_THREADS, 0, s>>>(a, list, count, fixed_count);
+    generic_kernel<MODE_SUBPB1, 0><<<grid, GEN_THREADS, 0, s>>>(a, list, count, fixed_count);
   return hipGetLastError();
 }
 

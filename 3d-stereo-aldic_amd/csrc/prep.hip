@@ -134,6 +134,124 @@ __global__ void zero_u64_kernel(unsigned long long* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// Interpolating cubic B-spline coefficients (not the reference's interpolant,
+// whose MEX evaluates a local 4 x 4 kernel on raw pixels): c with
+// (c[k-1] + 4 c[k] + c[k+1]) / 6 = s[k] along each line, on the whole-sample
+// mirror extension of the image, by the separable causal / anticausal
+// recursion with pole z = sqrt(3) - 2 and gain 6 per axis (Unser, Aldroubi,
+// Eden, IEEE Trans. Signal Process. 41(2), 1993).
+//
+// One workgroup per 64 x 64 output tile.  It loads the tile with a 16-pixel
+// apron on every side (mirror-reflected at the image border, coalesced along
+// dim 1) into LDS, runs the recursion along dim 1 on all 96 columns of the
+// apron tile, then along dim 2 on the 64 core rows, and stores the core.  A
+// recursion started 16 samples before (and ended 16 after) the core, from a
+// zero state, differs from the exact infinite one by at most |z|^16 = 7e-10
+// of the signal's scale; no pass over a whole line is needed, so every tile
+// is independent and the image is read about 2.25 times (from L2) and
+// written once.
+//
+// Zero rule: a coefficient whose pixel lies in a 4 x 4 window of all-zero
+// samples inside the image is set to zero, so that the spline vanishes
+// exactly on the cells whose 16 samples are zero, as the reference's kernel
+// does there (a zeroed, masked region of ImgDef is what its g == 0 test,
+// funICGNQuadtree.m:166-170, detects).  The flags come from the raw apron
+// tile before the recursion: all-zero runs of 4 along dim 1, then windows
+// of 4 such runs along dim 2, then each core pixel ORs the 16 windows that
+// cover it.
+// ---------------------------------------------------------------------------
+constexpr int BS_CORE = 64, BS_APRON = 16, BS_T = BS_CORE + 2 * BS_APRON, BS_PITCH = BS_T + 1;
+constexpr int BS_ZS = BS_CORE + 3;  // window starts per axis: core offsets -3 .. 63
+constexpr int BS_ZC = BS_CORE + 6;  // columns the runs are needed on: -3 .. 66
+
+__device__ __forceinline__ int reflect_index(int k, int n) {
+  if (n == 1) return 0;
+  const int P = 2 * n - 2;
+  k %= P;
+  if (k < 0) k += P;
+  return k < n ? k : P - k;
+}
+
+__global__ __launch_bounds__(256) void bspline_prefilter_kernel(const float* __restrict__ in,
+                                                                float* __restrict__ out, int M,
+                                                                int N) {
+  __shared__ float tile[BS_T * BS_PITCH];  // tile[j * BS_PITCH + i], i along dim 1
+  __shared__ uint8_t run0[BS_ZC * BS_ZS];   // [column][start]: 4 zero samples down dim 1
+  __shared__ uint8_t win0[BS_ZS * BS_ZS];   // [start col][start row]: 4 x 4 zero window
+  const int i0 = blockIdx.x * BS_CORE, j0 = blockIdx.y * BS_CORE;
+  for (int t = threadIdx.x; t < BS_T * BS_T; t += 256) {
+    const int i = t % BS_T, j = t / BS_T;
+    const int gi = reflect_index(i0 + i - BS_APRON, M), gj = reflect_index(j0 + j - BS_APRON, N);
+    tile[j * BS_PITCH + i] = in[(int64_t)gi + (int64_t)gj * M];
+  }
+  __syncthreads();
+  // zero flags (windows must lie inside the image: the apron's reflected
+  // samples do not count)
+  for (int t = threadIdx.x; t < BS_ZC * BS_ZS; t += 256) {
+    const int a = t % BS_ZS, c = t / BS_ZS;  // start row i0 - 3 + a, column j0 - 3 + c
+    const int gi = i0 - 3 + a;
+    const float* q = tile + (BS_APRON - 3 + c) * BS_PITCH + BS_APRON - 3 + a;
+    run0[t] = gi >= 0 && gi + 3 < M && q[0] == 0.0f && q[1] == 0.0f && q[2] == 0.0f &&
+              q[3] == 0.0f;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < BS_ZS * BS_ZS; t += 256) {
+    const int a = t % BS_ZS, b = t / BS_ZS;  // start row i0 - 3 + a, column j0 - 3 + b
+    const int gj = j0 - 3 + b;
+    win0[t] = gj >= 0 && gj + 3 < N && run0[b * BS_ZS + a] && run0[(b + 1) * BS_ZS + a] &&
+              run0[(b + 2) * BS_ZS + a] && run0[(b + 3) * BS_ZS + a];
+  }
+  __syncthreads();
+  // each thread keeps the flags of the core pixels it stores (the same
+  // t -> (i, j) map as the store below)
+  uint32_t kill = 0;
+  for (int t = threadIdx.x, k = 0; t < BS_CORE * BS_CORE; t += 256, k++) {
+    const int i = t % BS_CORE, j = t / BS_CORE;  // covered by starts a in i..i+3, b in j..j+3
+    bool z = false;
+#pragma unroll
+    for (int db = 0; db < 4; db++)
+#pragma unroll
+      for (int da = 0; da < 4; da++) z |= win0[(j + db) * BS_ZS + i + da] != 0;
+    kill |= (uint32_t)z << k;
+  }
+  const float z = -0.26794919243112270f;  // sqrt(3) - 2
+  if (threadIdx.x < BS_T) {  // dim 1: one column of the apron tile per thread
+    float* line = tile + threadIdx.x * BS_PITCH;
+    float c = 0.0f;
+    for (int e = 0; e < BS_T; e++) {
+      c = fmaf(z, c, line[e]);
+      line[e] = c;
+    }
+    c = 0.0f;
+    for (int e = BS_T - 1; e >= 0; e--) {
+      c = z * (c - line[e]);
+      line[e] = 6.0f * c;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < BS_CORE) {  // dim 2: one core row per thread
+    float* line = tile + BS_APRON + threadIdx.x;
+    float c = 0.0f;
+    for (int e = 0; e < BS_T; e++) {
+      c = fmaf(z, c, line[e * BS_PITCH]);
+      line[e * BS_PITCH] = c;
+    }
+    c = 0.0f;
+    for (int e = BS_T - 1; e >= 0; e--) {
+      c = z * (c - line[e * BS_PITCH]);
+      line[e * BS_PITCH] = 6.0f * c;
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x, k = 0; t < BS_CORE * BS_CORE; t += 256, k++) {
+    const int i = t % BS_CORE, j = t / BS_CORE;
+    if (i0 + i < M && j0 + j < N)
+      out[(int64_t)(i0 + i) + (int64_t)(j0 + j) * M] =
+          (kill >> k) & 1u ? 0.0f : tile[(j + BS_APRON) * BS_PITCH + i + BS_APRON];
+  }
+}
+
 int grid_for(int64_t n) {
   int64_t g = (n + 255) / 256;
   if (g > 16384) g = 16384;
@@ -191,6 +309,13 @@ hipError_t launch_gradient_out_f64(const double* ref, const double* mask, int64_
                                    double* DfDx, double* DfDy, hipStream_t s) {
   gradient_kernel<double, double><<<grid_for((M - 6) * (N - 6)), 256, 0, s>>>(ref, mask, M, N, DfDx,
                                                                               DfDy, 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_bspline_prefilter(const float* in, float* out, int64_t M, int64_t N,
+                                   hipStream_t s) {
+  const dim3 grid((unsigned)((M + BS_CORE - 1) / BS_CORE), (unsigned)((N + BS_CORE - 1) / BS_CORE));
+  bspline_prefilter_kernel<<<grid, 256, 0, s>>>(in, out, (int)M, (int)N);
   return hipGetLastError();
 }
 

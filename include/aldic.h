@@ -158,6 +158,30 @@ int aldic_normalize_raw_dev(aldic_ctx* ctx, const void* d_raw, int32_t bits, int
  * belong to the handle. */
 int aldic_images_planes(const aldic_images* im, const float** fm, const float** g,
                         const float** mask, const float** fx, const float** fy);
+
+/* Interpolant of ImgDef used by every solver on these images.  No
+ * counterpart on the reference's path: its MEX evaluates a local 4 x 4
+ * kernel on raw pixels (ba_interp2_spline.cpp:310-325), which is
+ * ALDIC_INTERP_LOCAL, the default and the only mode with reference parity.
+ * ALDIC_INTERP_BSPLINE is the interpolating cubic B-spline (the exact spline
+ * whose coefficients the reference's kernel approximates cell by cell, cf.
+ * func/BicubicBsplineBuildFullTable.m:1-24): its coefficient plane is made
+ * here by a separable recursive prefilter with mirror boundaries, and the
+ * solvers evaluate the spline on it.  It removes the local kernel's
+ * position bias (up to 0.0069 px on a ramp) and changes results by about
+ * that much; the reference's "g == 0" mask tests then see the spline's
+ * values.  Calling it again (e.g. after writing g through
+ * aldic_images_planes) remakes the coefficients; returns when they are
+ * complete. */
+#define ALDIC_INTERP_LOCAL 0
+#define ALDIC_INTERP_BSPLINE 1
+int aldic_images_set_interp(aldic_images* im, int32_t mode);
+/* The prefilter alone: interpolating cubic B-spline coefficients of an
+ * M x N FP32 device plane (column-major, mirror boundaries) into d_out,
+ * stream-ordered on ctx.  d_in and d_out must not overlap.  No counterpart
+ * in the reference. */
+int aldic_bspline_prefilter_dev(aldic_ctx* ctx, const float* d_in, float* d_out, int64_t M,
+                                int64_t N);
 int aldic_images_destroy(aldic_images* im);
 
 /* Per-node part of LocalICGNQuadtree (func_quadtree/LocalICGNQuadtree.m:

@@ -28,6 +28,12 @@
  *                         affine invariance.
  *   orc_subpb1_subset     mu -> 0 reproduces the translation-only IC-GN;
  *                         large mu pins U to UOld + vdual.
+ *   orc_bspline_prefilter against scipy.ndimage.spline_filter (order 3,
+ *                         mirror); (c[k-1] + 4c[k] + c[k+1])/6 = s[k]; the
+ *                         zero rule by brute force on an image with a hole.
+ *   orc_interp_bspline_d  against scipy.ndimage.map_coordinates (order 3,
+ *                         mirror) inside the image; reproduces samples at
+ *                         nodes and linear ramps exactly.
  *   orc_bwselect          brute force on small random masks (components by
  *                         union-find), 4- vs 8-connectivity cases.  MATLAB's
  *                         bwselect itself is not vendored: parity unpinned
@@ -210,6 +216,105 @@ void orc_interp_spline_f(const float* F, int64_t M, int64_t N, int64_t P, const 
     }
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Optional interpolant: the interpolating cubic B-spline (global prefilter)  */
+/* ------------------------------------------------------------------------ */
+
+/* Not the reference's interpolant: its MEX evaluates the local kernel above
+ * (ba_interp2_spline.cpp:310-325), whose 4-point collocation inverse
+ * approximates the B-spline coefficients cell by cell.  This is the exact
+ * interpolating cubic B-spline: coefficients c with (c[k-1] + 4 c[k] +
+ * c[k+1]) / 6 = s[k] on the whole-sample mirror extension of each line
+ * (s[-k] = s[k], s[n-1+k] = s[n-1-k]), by the causal / anticausal recursion
+ * with pole z = sqrt(3) - 2 and gain 6 (Unser, Aldroubi, Eden, IEEE Trans.
+ * Signal Process. 41(2), 1993; Unser, IEEE Signal Process. Mag. 16(6), 1999),
+ * with the exact initial values of that extension: the causal start sums one
+ * period 2n-2 and divides by 1 - z^(2n-2). */
+static void bspline_line(double* s, int64_t n, int64_t stride) {
+  if (n < 2) return;
+  const double z = sqrt(3.0) - 2.0;
+  const int64_t P = 2 * n - 2;
+  double zk = 1.0, c0 = 0.0;
+  for (int64_t k = 0; k < P; k++) {
+    c0 += zk * s[(k < n ? k : P - k) * stride];
+    zk *= z;
+  }
+  c0 /= 1.0 - zk;
+  for (int64_t k = 0; k < n; k++) s[k * stride] *= 6.0;
+  s[0] = 6.0 * c0;
+  for (int64_t k = 1; k < n; k++) s[k * stride] += z * s[(k - 1) * stride];
+  s[(n - 1) * stride] =
+      (z / (z * z - 1.0)) * (s[(n - 1) * stride] + z * s[(n - 2) * stride]);
+  for (int64_t k = n - 2; k >= 0; k--) s[k * stride] = z * (s[(k + 1) * stride] - s[k * stride]);
+}
+
+/* Then every coefficient whose pixel lies in a 4 x 4 window of all-zero
+ * samples inside the image is set to zero: the spline then vanishes exactly
+ * on the cells whose 16 samples are all zero, where the reference's kernel
+ * does (a masked, zeroed region of ImgDef, which funICGNQuadtree.m:166-170
+ * detects by g == 0), instead of decaying as 0.268^d into it. */
+void orc_bspline_prefilter(const double* in, int64_t M, int64_t N, double* out) {
+  double* c = (double*)malloc(sizeof(double) * (size_t)(M * N));
+  memcpy(c, in, sizeof(double) * (size_t)(M * N));
+  for (int64_t j = 0; j < N; j++) bspline_line(c + j * M, M, 1);
+  for (int64_t i = 0; i < M; i++) bspline_line(c + i, N, M);
+  for (int64_t j = 0; j < N; j++)
+    for (int64_t i = 0; i < M; i++) {
+      int kill = 0;
+      for (int64_t b = j - 3; b <= j && !kill; b++)
+        for (int64_t a = i - 3; a <= i && !kill; a++) {
+          if (a < 0 || b < 0 || a + 3 >= M || b + 3 >= N) continue;
+          int all0 = 1;
+          for (int64_t q = 0; q < 4 && all0; q++)
+            for (int64_t p = 0; p < 4 && all0; p++) all0 = in[(a + p) + (b + q) * M] == 0.0;
+          kill = all0;
+        }
+      out[i + j * M] = kill ? 0.0 : c[i + j * M];
+    }
+  free(c);
+}
+
+/* The spline at (X, Y) (X along dim 2, Y along dim 1, 1-based) from its
+ * coefficients: the same 4 x 4 cell and edge clamp as the MEX, weighted by
+ * the cubic B-spline basis (1/6)[(1-t)^3, 3t^3-6t^2+4, -3t^3+3t^2+3t+1, t^3]
+ * at the fractional parts. */
+static inline void bspline_weights(double t, double w[4]) {
+  const double u = 1.0 - t;
+  w[0] = u * u * u / 6.0;
+  w[1] = (3.0 * t * t * t - 6.0 * t * t + 4.0) / 6.0;
+  w[2] = (-3.0 * t * t * t + 3.0 * t * t + 3.0 * t + 1.0) / 6.0;
+  w[3] = t * t * t / 6.0;
+}
+
+void orc_interp_bspline_d(const double* C, int64_t M, int64_t N, const double* X, const double* Y,
+                          int64_t ND, double* Z) {
+  for (int64_t i = 0; i < ND; i++) {
+    double wr[4], wc[4];
+    bspline_weights(Y[i] - floor(Y[i]), wr);
+    bspline_weights(X[i] - floor(X[i]), wc);
+    int64_t idx[4][4];
+    cell_index(M, N, X[i], Y[i], idx);
+    double out = 0.0;
+    for (int c = 0; c < 4; c++) {
+      double col = 0.0;
+      for (int r = 0; r < 4; r++) col += wr[r] * C[idx[r][c]];
+      out += wc[c] * col;
+    }
+    Z[i] = out;
+  }
+}
+
+/* g at the warped points: the reference's kernel on ImgDef, or the spline
+ * on ImgDefCoef when the images carry it. */
+static void interp_def(const orc_images* im, const double* X, const double* Y, int64_t n,
+                       double* g) {
+  if (im->ImgDefCoef)
+    orc_interp_bspline_d(im->ImgDefCoef, im->M, im->N, X, Y, n, g);
+  else
+    orc_interp_spline_d(im->ImgDef, im->M, im->N, 1, X, Y, n, g);
+}
+
 
 /* ------------------------------------------------------------------------ */
 /* funImgGradient.m:49-78 and funNormalizeImg.m:36-40                        */
@@ -616,7 +721,7 @@ int orc_icgn_subset_ex(const orc_images* im, int64_t x0, int64_t y0, int winsize
         normNew = 1e6;
         break;
       }
-      orc_interp_spline_d(im->ImgDef, im->M, im->N, 1, X, Y, NS, g);
+      interp_def(im, X, Y, NS, g);
       if (any_nan(g, NS)) goto done; /* logical(NaN) throws (:166) */
       const int64_t cg = count_zero(g, NS);
       if (c0 > 0 || cg > 0) {
@@ -840,7 +945,7 @@ int orc_subpb1_subset(const orc_images* im, int64_t x0, int64_t y0, int wx, int 
         normNew = 1e6;
         break;
       }
-      orc_interp_spline_d(im->ImgDef, im->M, im->N, 1, X, Y, NS, g);
+      interp_def(im, X, Y, NS, g);
       if (any_nan(g, NS)) goto done;
       const int64_t cg = count_zero(g, NS);
       if (c0 > 0 || cg > 0) {

@@ -35,6 +35,15 @@ void orc_interp_spline_d(const double* F, int64_t M, int64_t N, int64_t P,
 void orc_interp_spline_f(const float* F, int64_t M, int64_t N, int64_t P,
                          const float* X, const float* Y, int64_t ND, float* Z);
 
+/* Interpolating cubic B-spline coefficients of an M x N image (mirror
+ * boundaries; zero for pixels covered by a 4 x 4 window of all-zero samples
+ * inside the image), and the spline's values at (X, Y) as
+ * orc_interp_spline_d addresses them.  No counterpart on the reference's path: the exact spline
+ * beside its local kernel. */
+void orc_bspline_prefilter(const double* in, int64_t M, int64_t N, double* out);
+void orc_interp_bspline_d(const double* C, int64_t M, int64_t N, const double* X, const double* Y,
+                          int64_t ND, double* Z);
+
 /* funImgGradient.m:49-78: 7-tap gradients, cropped by 3, times the mask.
  * DfDx, DfDy are (M-6) x (N-6) column-major.  mask may be NULL (all ones). */
 void orc_img_gradient(const double* img, const double* mask, int64_t M, int64_t N,
@@ -51,6 +60,10 @@ typedef struct {
   const double* DfDx;    /* (M-6) x (N-6) */
   const double* DfDy;    /* (M-6) x (N-6) */
   int64_t M, N;
+  /* NULL: g is interpolated by the reference's local kernel; else the cubic
+   * B-spline coefficients of ImgDef (orc_bspline_prefilter), and g is that
+   * spline.  The second is not the reference's method. */
+  const double* ImgDefCoef;
 } orc_images;
 
 /* funICGNQuadtree.m:1-454 for one node (x0, y0 already rounded).

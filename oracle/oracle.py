@@ -36,6 +36,7 @@ class _Images(C.Structure):
         ("DfDy", C.c_void_p),
         ("M", C.c_int64),
         ("N", C.c_int64),
+        ("ImgDefCoef", C.c_void_p),
     ]
 
 
@@ -55,6 +56,8 @@ def lib():
         L.orc_interp_spline_f.argtypes = [_fp, _i64, _i64, _i64, _fp, _fp, _i64, _fp]
         L.orc_img_gradient.argtypes = [_dp, C.c_void_p, _i64, _i64, _dp, _dp]
         L.orc_normalize.argtypes = [_dp, _i64, _i64, _i64, _i64, _i64, _i64, _dp]
+        L.orc_bspline_prefilter.argtypes = [_dp, _i64, _i64, _dp]
+        L.orc_interp_bspline_d.argtypes = [_dp, _i64, _i64, _dp, _dp, _i64, _dp]
         L.orc_icgn_subset.argtypes = [C.POINTER(_Images), _i64, _i64, C.c_int, C.c_double,
                                       _dp, _dp, _dp, _dp, C.POINTER(C.c_double)]
         L.orc_icgn_subset.restype = C.c_int
@@ -133,10 +136,31 @@ def normalize(img, gridx, gridy):
     return out
 
 
+def bspline_prefilter(img):
+    """Interpolating cubic B-spline coefficients (mirror boundaries)."""
+    img = _F(img)
+    out = np.zeros_like(img, order="F")
+    lib().orc_bspline_prefilter(img, img.shape[0], img.shape[1], out)
+    return out
+
+
+def interp_bspline(coef, X, Y):
+    """The spline of coefficients coef at (X, Y): X along dim 2, Y along
+    dim 1, 1-based (the addressing of interp_spline)."""
+    coef = _F(coef)
+    X = np.ascontiguousarray(X, dtype=np.float64).ravel()
+    Y = np.ascontiguousarray(Y, dtype=np.float64).ravel()
+    Z = np.zeros(X.size)
+    lib().orc_interp_bspline_d(coef, coef.shape[0], coef.shape[1], X, Y, X.size, Z)
+    return Z
+
+
 class Images:
     """Holds the arrays an orc_images struct points to (keeps them alive)."""
 
-    def __init__(self, ImgRef, ImgDef, mask=None, DfDx=None, DfDy=None):
+    def __init__(self, ImgRef, ImgDef, mask=None, DfDx=None, DfDy=None, interp="local"):
+        """interp "local": the reference's kernel; "bspline": the
+        interpolating cubic B-spline of ImgDef (not the reference's)."""
         self.ImgRef = _F(ImgRef)
         self.ImgDef = _F(ImgDef)
         M, N = self.ImgRef.shape
@@ -146,8 +170,12 @@ class Images:
         self.DfDx = _F(DfDx)
         self.DfDy = _F(DfDy)
         self.M, self.N = M, N
+        if interp not in ("local", "bspline"):
+            raise ValueError(f"interp {interp!r}: 'local' or 'bspline'")
+        self.coef = bspline_prefilter(self.ImgDef) if interp == "bspline" else None
         self.s = _Images(self.ImgRef.ctypes.data, self.ImgDef.ctypes.data, self.mask.ctypes.data,
-                         self.DfDx.ctypes.data, self.DfDy.ctypes.data, M, N)
+                         self.DfDx.ctypes.data, self.DfDy.ctypes.data, M, N,
+                         None if self.coef is None else self.coef.ctypes.data)
 
 
 def icgn_subset(im: Images, x0, y0, winsize, tol, U0):

@@ -659,3 +659,108 @@ def test_normalize_roi_moments_and_the_host_version(orc, pkg):
         host, rng_out = pkg.funNormalizeImg([img, img], {"gridx": gx, "gridy": gy})
         assert np.allclose(host[0], out, atol=1e-12) and np.allclose(host[1], out, atol=1e-12)
         assert rng_out["gridx"] == [a0 + 1, a1] and rng_out["gridy"] == [b0 + 1, b1]
+
+
+def test_bspline_prefilter_and_spline_against_scipy(orc):
+    """The optional interpolant (no counterpart on the reference's path):
+    coefficients against scipy.ndimage.spline_filter (order 3, mirror), the
+    spline against scipy.ndimage.map_coordinates inside the image (where the
+    MEX's cell never reaches the clamp), samples reproduced at nodes and
+    linear ramps reproduced everywhere inside; the reference's local kernel
+    misses a ramp by t(2t-1)(t-1)/14."""
+    from scipy import ndimage
+    rng = np.random.default_rng(21)
+    for shape in ((37, 29), (5, 7), (2, 9), (64, 3)):
+        A = rng.normal(size=shape)
+        c = orc.bspline_prefilter(A)
+        assert np.abs(c - ndimage.spline_filter(A, order=3, mode="mirror")).max() < 1e-12
+        # the defining relation, (c[k-1] + 4 c[k] + c[k+1]) / 6 = s[k] along
+        # both axes, in the interior; the two axes commute
+        if min(shape) >= 3:
+            r = (c[:-2] + 4 * c[1:-1] + c[2:]) / 6
+            r = (r[:, :-2] + 4 * r[:, 1:-1] + r[:, 2:]) / 6
+            assert np.abs(r - A[1:-1, 1:-1]).max() < 1e-12
+            assert np.allclose(orc.bspline_prefilter(A.T).T, c, atol=1e-12)
+    A = rng.normal(size=(37, 29))
+    c = orc.bspline_prefilter(A)
+    X = rng.uniform(2, 28 - 1e-9, 500)
+    Y = rng.uniform(2, 36 - 1e-9, 500)
+    z = orc.interp_bspline(c, X, Y)
+    assert np.abs(z - ndimage.map_coordinates(A, [Y - 1, X - 1], order=3, mode="mirror")).max() \
+        < 1e-12
+    I, J = np.meshgrid(np.arange(2, 37), np.arange(2, 29), indexing="ij")
+    assert np.abs(orc.interp_bspline(c, J.ravel(), I.ravel()) - A[I - 1, J - 1].ravel()).max() \
+        < 1e-12
+    # a ramp: its mirror extension kinks at the border, whose effect on the
+    # coefficients decays as |sqrt(3) - 2|^d = 0.268^d with the distance d
+    ramp = np.asfortranarray(0.37 * np.arange(1, 65)[:, None] - 0.11 * np.arange(1, 61)[None, :])
+    cr = orc.bspline_prefilter(ramp)
+    X = rng.uniform(16, 45, 300)
+    Y = rng.uniform(16, 49, 300)
+    assert np.abs(orc.interp_bspline(cr, X, Y) - (0.37 * Y - 0.11 * X)).max() < 1e-8
+    t = Y - np.floor(Y)
+    local = orc.interp_spline(ramp, X, Y) - (0.37 * Y - 0.11 * X)
+    bias = 0.37 * t * (2 * t - 1) * (t - 1) / 14 - 0.11 * (X - np.floor(X)) * \
+        (2 * (X - np.floor(X)) - 1) * ((X - np.floor(X)) - 1) / 14
+    assert np.abs(local - bias).max() < 1e-6 and np.abs(local).max() > 1e-3
+
+
+def test_icgn_bspline_mode_removes_the_kernel_bias(orc, synth):
+    """IC-GN on a noise-free, analytically rendered sub-pixel translation:
+    with the reference's kernel the displacement carries the kernel's bias
+    (several 1e-3 px); with the interpolating spline it is an order of
+    magnitude smaller.  The same oracle IC-GN runs both; only the images'
+    interpolant differs."""
+    M = N = 160
+    u0, v0 = 0.3, -0.2
+    f8, g8, _ = synth.make_pair(M, N, lambda X, Y: u0 + 0 * X, lambda X, Y: v0 + 0 * X,
+                                density=0.02, seed=4, noise=0.0, bits=12)
+    f = np.asfortranarray((f8 - f8.mean()) / f8.std())
+    g = np.asfortranarray((g8 - g8.mean()) / g8.std())
+    xs = np.arange(40.0, 121.0, 20.0)
+    X, Y = np.meshgrid(xs, xs, indexing="ij")
+    coords = np.asfortranarray(np.stack([X.ravel(), Y.ravel()], 1))
+    U0 = np.zeros(2 * len(coords))
+    err = {}
+    for mode in ("local", "bspline"):
+        U, F, H, conv, *_ = orc.local_icgn(orc.Images(f, g, interp=mode), coords, U0, 40, 1e-6)
+        assert np.all((conv >= 1) & (conv <= 100))
+        err[mode] = np.abs(np.stack([U[0::2] - u0, U[1::2] - v0])).max()
+    assert err["local"] > 1e-3 and err["bspline"] < err["local"] / 8, err
+
+
+def test_bspline_zero_rule(orc):
+    """Pixels covered by a 4 x 4 window of all-zero samples get a zero
+    coefficient, so the spline is exactly zero on the cells whose 16 samples
+    are zero (as the reference's kernel is there) and the other coefficients
+    are the plain spline's.  An isolated zero sample changes nothing."""
+    from numpy.lib.stride_tricks import sliding_window_view
+    from scipy import ndimage
+    rng = np.random.default_rng(22)
+    M, N = 48, 40
+    A = rng.normal(size=(M, N))
+    I, J = np.meshgrid(np.arange(M), np.arange(N), indexing="ij")
+    A[(I - 20) ** 2 + (J - 18) ** 2 < 64] = 0.0  # a hole
+    A[40, 5] = 0.0  # a lone zero
+    A[0:5, 30:36] = 0.0  # a zero block on the border
+    A = np.asfortranarray(A)
+    c = orc.bspline_prefilter(A)
+    plain = ndimage.spline_filter(A, order=3, mode="mirror")
+    win0 = np.all(sliding_window_view(A == 0.0, (4, 4)), axis=(2, 3))  # window starts
+    kill = np.zeros((M, N), bool)
+    for a, b in zip(*np.nonzero(win0)):
+        kill[a:a + 4, b:b + 4] = True
+    assert kill.sum() > 100 and not kill[40, 5]
+    assert np.all(c[kill] == 0.0)
+    assert np.abs(c[~kill] - plain[~kill]).max() < 1e-12
+    # the spline on cells (1-based X, Y in [2, M-1)): exactly zero iff the
+    # cell's 16 samples are zero
+    X = rng.uniform(2, N - 1 - 1e-9, 4000)
+    Y = rng.uniform(2, M - 1 - 1e-9, 4000)
+    z = orc.interp_bspline(c, X, Y)
+    a = np.floor(Y).astype(int) - 2  # 0-based first tap row
+    b = np.floor(X).astype(int) - 2
+    cell0 = win0[a, b]
+    assert cell0.sum() > 50
+    assert np.all(z[cell0] == 0.0) and np.all(z[~cell0] != 0.0)
+    assert np.array_equal(orc.interp_spline(A, X, Y) == 0.0, cell0)
