@@ -824,6 +824,13 @@ def integer_search_noalloc_dev(ctx: Context, d_f, d_g, M, N, gridx, gridy, winsi
                                                  d_u, d_v, d_ccmax, d_ipr, d_ipc))
 
 
+def bspline_prefilter_dev(ctx: Context, d_in, d_out, M, N):
+    """Interpolating cubic B-spline coefficients of an M x N FP32 device
+    plane into d_out (aldic_bspline_prefilter_dev; stream-ordered on ctx; no
+    reference counterpart)."""
+    check(lib().aldic_bspline_prefilter_dev(ctx.ptr, d_in, d_out, int(M), int(N)))
+
+
 def grid_to_nodes_dev(ctx: Context, d_u, d_v, nX, nY, x0, y0, step, d_coords, n, d_U0):
     check(lib().aldic_grid_to_nodes_dev(ctx.ptr, d_u, d_v, int(nX), int(nY), float(x0), float(y0),
                                         float(step), d_coords, int(n), d_U0))

@@ -173,6 +173,31 @@ __device__ __forceinline__ int reflect_index(int k, int n) {
   return k < n ? k : P - k;
 }
 
+// The recursion along one line of the apron tile (BS_T samples STRIDE
+// apart in LDS), held in registers; only the core's coefficients are
+// written back (the other pass and the store read nothing else).
+template <int STRIDE>
+__device__ __forceinline__ void bspline_line(float* line) {
+  const float z = -0.26794919243112270f;  // sqrt(3) - 2
+  float v[BS_T];
+#pragma unroll
+  for (int e = 0; e < BS_T; e++) v[e] = line[e * STRIDE];
+  float c = 0.0f;
+#pragma unroll
+  for (int e = 0; e < BS_T; e++) {
+    c = fmaf(z, c, v[e]);
+    v[e] = c;
+  }
+  c = 0.0f;
+#pragma unroll
+  for (int e = BS_T - 1; e >= 0; e--) {
+    c = z * (c - v[e]);
+    v[e] = 6.0f * c;
+  }
+#pragma unroll
+  for (int e = BS_APRON; e < BS_APRON + BS_CORE; e++) line[e * STRIDE] = v[e];
+}
+
 __global__ __launch_bounds__(256) void bspline_prefilter_kernel(const float* __restrict__ in,
                                                                 float* __restrict__ out, int M,
                                                                 int N) {
@@ -215,34 +240,10 @@ __global__ __launch_bounds__(256) void bspline_prefilter_kernel(const float* __r
       for (int da = 0; da < 4; da++) z |= win0[(j + db) * BS_ZS + i + da] != 0;
     kill |= (uint32_t)z << k;
   }
-  const float z = -0.26794919243112270f;  // sqrt(3) - 2
-  if (threadIdx.x < BS_T) {  // dim 1: one column of the apron tile per thread
-    float* line = tile + threadIdx.x * BS_PITCH;
-    float c = 0.0f;
-    for (int e = 0; e < BS_T; e++) {
-      c = fmaf(z, c, line[e]);
-      line[e] = c;
-    }
-    c = 0.0f;
-    for (int e = BS_T - 1; e >= 0; e--) {
-      c = z * (c - line[e]);
-      line[e] = 6.0f * c;
-    }
-  }
+  // dim 1: one column of the apron tile per thread; dim 2: one core row
+  if (threadIdx.x < BS_T) bspline_line<1>(tile + threadIdx.x * BS_PITCH);
   __syncthreads();
-  if (threadIdx.x < BS_CORE) {  // dim 2: one core row per thread
-    float* line = tile + BS_APRON + threadIdx.x;
-    float c = 0.0f;
-    for (int e = 0; e < BS_T; e++) {
-      c = fmaf(z, c, line[e * BS_PITCH]);
-      line[e * BS_PITCH] = c;
-    }
-    c = 0.0f;
-    for (int e = BS_T - 1; e >= 0; e--) {
-      c = z * (c - line[e * BS_PITCH]);
-      line[e * BS_PITCH] = 6.0f * c;
-    }
-  }
+  if (threadIdx.x < BS_CORE) bspline_line<BS_PITCH>(tile + BS_APRON + threadIdx.x);
   __syncthreads();
   for (int t = threadIdx.x, k = 0; t < BS_CORE * BS_CORE; t += 256, k++) {
     const int i = t % BS_CORE, j = t / BS_CORE;

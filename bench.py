@@ -486,6 +486,33 @@ def run_workload(pkg, W, params, args, headline):
         progress(f"{W['name']} learned start, {rule} rule: {dt * 1e3:.3f} ms, "
                  f"{acc['ok'].sum()} converged, rms {acc['rms']:.4f} px, fallback "
                  f"{li.fallback_count(last)}")
+    # ---- the interpolating cubic B-spline option (not the reference's
+    # interpolant; no parity with it): both starts under the step-size rule.
+    # Each frame needs its coefficient plane: the prefilter is timed on its
+    # own and added to the step ----
+    d_gc = DA(ctx, M * N, np.float32)
+    dtp, _, _ = timed([ctx], api, lambda k: api.bspline_prefilter_dev(ctx, d_g.ptr, d_gc.ptr, M, N),
+                      args.steps, args.warmup)
+    d_gc.free()
+    im.set_interp("bspline")
+    for pl in (plan2, plan3):
+        pl.set_stopping(*RULES["step"])
+    dt, st, last = timed([ctx, ctx2], api, step_search, args.steps, args.warmup)
+    b = sets[last % 2]
+    acc = accuracy(api, W, b["U"].to_host().astype(np.float64), b["conv"].to_host(),
+                   b["z"].to_host(), truth_mean)
+    out[("search", "step_bspline")] = dict(dt=dt + dtp, stages=st, acc=acc, prefilter_dt=dtp)
+    progress(f"{W['name']} search start, step rule, B-spline: {(dt + dtp) * 1e3:.3f} ms "
+             f"(prefilter {dtp * 1e3:.3f} ms), rms {acc['rms']:.4f} px")
+    dt, st, last = timed([ctx, ctx2, ctx3], api, li.run, args.steps, args.warmup)
+    lU, lF, lconv, lz = li.outputs(last)
+    acc = accuracy(api, W, lU.to_host().astype(np.float64), lconv.to_host(), lz.to_host(),
+                   truth_mean)
+    out[("learned", "step_bspline")] = dict(dt=dt + dtp, stages=st, acc=acc, prefilter_dt=dtp,
+                                           fallback=li.fallback_count(last))
+    progress(f"{W['name']} learned start, step rule, B-spline: {(dt + dtp) * 1e3:.3f} ms, "
+             f"rms {acc['rms']:.4f} px, fallback {li.fallback_count(last)}")
+    im.set_interp("local")
     for c in (ctx, ctx2, ctx3):
         c.synchronize()
     # ---- roofline of the headline step's dominant kernel: the learned
@@ -668,6 +695,8 @@ def summary(d, n, steps):
     if "fallback" in d:
         s["fallback"] = d["fallback"]
         s["fallback_rate"] = round(d["fallback"] / n, 5)
+    if "prefilter_dt" in d:
+        s["prefilter_ms_in_step"] = round(d["prefilter_dt"] * 1e3, 4)
     return s
 
 
@@ -738,7 +767,7 @@ def main():
         blk = {}
         for (start, rule), d in res["out"].items():
             blk[f"{start}_start_{rule}_rule"] = summary(d, nn, args.steps)
-        for rule in RULES:
+        for rule in (*RULES, "step_bspline"):
             a = res["out"][("learned", rule)]
             b = res["out"][("search", rule)]
             blk[f"learned_vs_search_{rule}_rule"] = round(
@@ -782,6 +811,15 @@ def main():
     for name in names[1:]:
         line[name] = block(results[name])
     line["learned_vs_fft"] = line[head_name]["learned_vs_search_step_rule"]
+    bsp = H["out"][("learned", "step_bspline")]
+    line["interp_bspline"] = {
+        "note": "option, not the reference's interpolant: the interpolating cubic B-spline on "
+                "prefiltered coefficients; learned start, step-size rule, prefilter included",
+        "value": round(float(bsp["acc"]["ok"].sum() / bsp["dt"]), 1),
+        "ms_per_step": round(bsp["dt"] * 1e3, 4),
+        "prefilter_ms": round(bsp["prefilter_dt"] * 1e3, 4),
+        "rms_px": round(bsp["acc"]["rms"], 5), "rms_ok": bool(bsp["acc"]["rms"] <= 0.01),
+        "learned_vs_search": line[head_name]["learned_vs_search_step_bspline_rule"]}
     if not args.no_alm:
         progress("incremental-sequence frame: temporal IC-GN + ALM graph")
         line["seq2048_alm"] = alm_block(pkg, args)
