@@ -35,6 +35,7 @@ SIGNATURES = {
     "aldic_memcpy_htod": (C.c_int, [_vp, _vp, _vp, _i64]),
     "aldic_memcpy_dtoh": (C.c_int, [_vp, _vp, _vp, _i64]),
     "aldic_ctx_set_timing": (C.c_int, [_vp, C.c_int32]),
+    "aldic_ctx_set_search_method": (C.c_int, [_vp, C.c_int32]),
     "aldic_ctx_record": (C.c_int, [_vp, C.c_int32]),
     "aldic_ctx_capture_begin": (C.c_int, [_vp]),
     "aldic_ctx_capture_end": (C.c_int, [_vp, _pp]),

@@ -57,6 +57,16 @@ class Context:
     def synchronize(self):
         check(lib().aldic_ctx_synchronize(self.ptr))
 
+    def set_search_method(self, method):
+        """Integer-search peak finding on this context: "direct" (default)
+        or "fft" (aldic_ctx_set_search_method)."""
+        modes = {"direct": 0, "fft": 1}
+        if method not in modes:
+            raise ValueError(f"search method {method!r}: 'direct' or 'fft'")
+        check(lib().aldic_ctx_set_search_method(self.ptr, modes[method]))
+        self.search_method = method
+        return self
+
     def record(self, mark: int):
         """Mark the point reached by the work submitted so far."""
         check(lib().aldic_ctx_record(self.ptr, int(mark)))

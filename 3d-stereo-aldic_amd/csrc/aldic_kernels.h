@@ -99,23 +99,26 @@ hipError_t launch_gradient_out_f64(const double* ref, const double* mask, int64_
 size_t search_smem_bytes(int ws);
 size_t search_scratch_bytes(int maxpts, int ws, const int reg[4]);
 // q (may be null): raw (PCE, PPE) per grid point, 2 doubles, point order
-// ci + cj * nX (compute_qFactor of the full normxcorr2 map)
+// ci + cj * nX (compute_qFactor of the full normxcorr2 map).  method: 0
+// direct lag tiles, 1 FFT lag blocks where the template side is <= 65
+// (direct otherwise); the sub-pixel fit is direct either way.
 hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, int nY,
                          const int gridx[2], const int gridy[2], int ws, int step,
                          const int reg[4], void* scratch, double* x, double* y, double* u,
                          double* v, double* ccmax, int* ipr, int* ipc, hipStream_t s,
-                         double* q = nullptr);
+                         double* q = nullptr, int method = 0);
 hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
                                const double* coords, int n, const int* list, const int* count,
                                int max_count, int ws, const int reg[4], void* scratch, float* U0,
-                               float* F0, hipStream_t s);
+                               float* F0, hipStream_t s, int method = 0);
 
 // grid search over a device list of grid points (indices ci + cj*nX)
 hipError_t launch_search_points(const float* f, const float* g, int M, int N, int nX, int nY,
                                 const int gridx[2], const int gridy[2], int ws, int step,
                                 const int reg[4], const int* points, const int* count,
                                 void* scratch, double* x, double* y, double* u, double* v,
-                                double* ccmax, int* ipr, int* ipc, hipStream_t s);
+                                double* ccmax, int* ipr, int* ipc, hipStream_t s,
+                                int method = 0);
 hipError_t launch_grid_to_listed_nodes(const double* gu, const double* gv, int nX, int nY,
                                        double x0, double y0, double step, const double* coords,
                                        int n, const int* list, const int* count, float* U0,

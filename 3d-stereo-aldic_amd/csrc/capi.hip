@@ -277,6 +277,14 @@ extern "C" int aldic_ctx_set_timing(aldic_ctx* ctx, int32_t enable) {
   return ALDIC_OK;
 }
 
+extern "C" int aldic_ctx_set_search_method(aldic_ctx* ctx, int32_t method) {
+  if (!ctx) return fail(ALDIC_EINVAL, "NULL context");
+  if (method != ALDIC_SEARCH_DIRECT && method != ALDIC_SEARCH_FFT)
+    return fail(ALDIC_EINVAL, "search method must be ALDIC_SEARCH_DIRECT or ALDIC_SEARCH_FFT");
+  ctx->search_method = method;
+  return ALDIC_OK;
+}
+
 extern "C" int aldic_ctx_stage_times(aldic_ctx* ctx, double* ms, int32_t* launches, int32_t reset) {
   if (!ctx) return fail(ALDIC_EINVAL, "NULL context");
   RC_TRY(set_device(ctx));
@@ -940,7 +948,7 @@ extern "C" int aldic_search_nodes_dev(aldic_ctx* ctx, const aldic_images* im,
   StageTimer t(ctx, ALDIC_STAGE_FALLBACK);
   HIP_TRY(aldic::launch_search_nodes(im->fm, im->g, (int)im->M, (int)im->N, d_coords, (int)n,
                                      d_list, d_count, (int)max_count, winsize, rg, scratch, d_U0,
-                                     d_F0, ctx->stream));
+                                     d_F0, ctx->stream, ctx->search_method));
   return ALDIC_OK;
 }
 
@@ -1321,7 +1329,7 @@ static int search_dev(aldic_ctx* ctx, const float* d_f, const float* d_g, int64_
   StageTimer t(ctx, ALDIC_STAGE_SEARCH);
   HIP_TRY(aldic::launch_search(d_f, d_g, (int)M, (int)N, (int)nX, (int)nY, gx, gy, winsize,
                                winstepsize, rg, scratch, d_x, d_y, d_u, d_v, d_ccmax, d_ipr,
-                               d_ipc, ctx->stream, d_q));
+                               d_ipc, ctx->stream, d_q, ctx->search_method));
   return ALDIC_OK;
 }
 
@@ -1380,7 +1388,7 @@ extern "C" int aldic_search_grid_nodes_dev(aldic_ctx* ctx, const float* d_f, con
                                    (int)n, d_list, d_count, flags, points, npoints, s));
   HIP_TRY(aldic::launch_search_points(d_f, d_g, (int)M, (int)N, (int)nX, (int)nY, gx, gy, winsize,
                                       winstepsize, rg, points, npoints, scratch, x, y, u, v, cc,
-                                      ipr, ipc, s));
+                                      ipr, ipc, s, ctx->search_method));
   HIP_TRY(aldic::launch_grid_to_listed_nodes(u, v, (int)nX, (int)nY, x0, y0,
                                              (double)winstepsize, d_coords, (int)n, d_list,
                                              d_count, d_U0, d_F0, s));
@@ -1436,7 +1444,7 @@ extern "C" int aldic_integer_search_noalloc_dev(aldic_ctx* ctx, const float* d_f
   // scratch needs clearing between calls
   HIP_TRY(aldic::launch_search(d_f, d_g, (int)M, (int)N, (int)nX, (int)nY, gx, gy, winsize,
                                winstepsize, rg, sp, d_x, d_y, d_u, d_v, d_ccmax, d_ipr, d_ipc,
-                               ctx->stream));
+                               ctx->stream, nullptr, ctx->search_method));
   return ALDIC_OK;
 }
 

@@ -47,6 +47,7 @@ struct SearchArgs {
   int ws, sp;      // template side s = ws + 1, padded row length sp
   int reg[4];      // L R U D
   int ntiles, nt1; // lag tiles per point (max over points), along dim 1
+  int fft;         // lag tiles are FFT blocks (search_fft_kernel) instead of direct tiles
   // grid mode
   int nX, nY, gx0, gx1, gy0, gy1, step;
   // list mode (list != nullptr): nodes; grid-list mode (glist != nullptr):
@@ -182,6 +183,55 @@ __device__ __forceinline__ void tail_fmas(const float* Ta, const float (&win)[2 
   }
 }
 
+// Window sums of a lag tile, shared by the direct and the FFT tiles so that
+// both give the same denominators: CB(i, q) = sum_{b<s} sub(i, q+b) for
+// q = qa .. qa + CB_SEG - 1 (sub(i, j) at sub[j * ld + i]), by a running
+// sum restarted every CB_SEG lags; CB is stored with row length SW.
+constexpr int CB_SEG = 16;
+
+__device__ __forceinline__ void col_box_sums(const float* sub, int ld, int SW, int s, int qa,
+                                             int i, float* CB, float* CB2) {
+  float c1 = 0.0f, c2 = 0.0f;
+  for (int b = 0; b < s; b++) {
+    const float v = sub[(qa + b) * ld + i];
+    c1 += v;
+    c2 = fmaf(v, v, c2);
+  }
+  CB[qa * SW + i] = c1;
+  CB2[qa * SW + i] = c2;
+  for (int q = qa + 1; q < qa + CB_SEG; q++) {
+    const float vo = sub[(q - 1) * ld + i], vn = sub[(q - 1 + s) * ld + i];
+    c1 += vn - vo;
+    c2 += fmaf(vn, vn, -(vo * vo));
+    CB[q * SW + i] = c1;
+    CB2[q * SW + i] = c2;
+  }
+}
+
+// normxcorr2's denominator at tile lag (p, q) from the column sums:
+// denT * sqrt(var) of the s x s window, or 0 for a window of (numerically)
+// zero variance (its coefficient is 0)
+__device__ __forceinline__ float lag_den(const float* CB, const float* CB2, int SW, int s, int q,
+                                         int p, float inv_n, float denT) {
+  float sa = 0.0f, sa2 = 0.0f;
+  for (int a2 = 0; a2 < s; a2++) {
+    sa += CB[q * SW + p + a2];
+    sa2 += CB2[q * SW + p + a2];
+  }
+  const float varA = sa2 - sa * sa * inv_n;
+  return (varA > 1e-6f * sa2 && varA > 0.0f) ? denT * sqrtf(varA) : 0.0f;
+}
+
+// the coefficient; one outside [-1, 1] beyond FP32 round-off is 0
+__device__ __forceinline__ float zncc_of(float num, float den) {
+  float gam = 0.0f;
+  if (den > 0.0f) {
+    gam = num / den;
+    if (fabsf(gam) - 1.0f > 1e-5f) gam = 0.0f;
+  }
+  return gam;
+}
+
 __global__ __launch_bounds__(256) void search_tile_kernel(SearchArgs A,
                                                           const float* __restrict__ tmpl,
                                                           const float* __restrict__ g) {
@@ -221,22 +271,7 @@ __global__ __launch_bounds__(256) void search_tile_kernel(SearchArgs A,
       // column sums along dim 2: CB(i, q) = sum_{b<s} sub(i, q+b)
       for (int task = tid; task < SW * 2; task += 256) {
         const int i = task % SW, half = task / SW;
-        const int qa = half * (KT2 / 2);
-        float c1 = 0.0f, c2 = 0.0f;
-        for (int b = 0; b < s; b++) {
-          const float v = sub[(qa + b) * SW + i];
-          c1 += v;
-          c2 += v * v;
-        }
-        CB[qa * SW + i] = c1;
-        CB2[qa * SW + i] = c2;
-        for (int q = qa + 1; q < qa + KT2 / 2; q++) {
-          const float vo = sub[(q - 1) * SW + i], vn = sub[(q - 1 + s) * SW + i];
-          c1 += vn - vo;
-          c2 += vn * vn - vo * vo;
-          CB[q * SW + i] = c1;
-          CB2[q * SW + i] = c2;
-        }
+        col_box_sums(sub, SW, SW, s, half * CB_SEG, i, CB, CB2);
       }
       // numerator: lane -> dim-1 lag p0+lane, wave w -> dim-2 lags q0+8w..+7
       const float* T = tmpl + (int64_t)t * s * A.sp;
@@ -286,21 +321,7 @@ __global__ __launch_bounds__(256) void search_tile_kernel(SearchArgs A,
 #pragma unroll
       for (int r = 0; r < RB; r++) {
         const int q = q0 + w * RB + r;
-        float sa = 0.0f, sa2 = 0.0f;
-        const int qq = w * RB + r;
-        for (int a2 = 0; a2 < s; a2++) {
-          sa += CB[qq * SW + lane + a2];
-          sa2 += CB2[qq * SW + lane + a2];
-        }
-        // normxcorr2 denominator; a window of (numerically) zero variance
-        // gives 0, and so does a coefficient outside [-1, 1] beyond FP32
-        // round-off
-        const float varA = sa2 - sa * sa * inv_n;
-        float gam = 0.0f;
-        if (varA > 1e-6f * sa2 && varA > 0.0f) {
-          gam = num[r] / (P.denT * sqrtf(varA));
-          if (fabsf(gam) - 1.0f > 1e-5f) gam = 0.0f;
-        }
+        const float gam = zncc_of(num[r], lag_den(CB, CB2, SW, s, w * RB + r, lane, inv_n, P.denT));
         if (p < K1 && q < K2) {
           const int idx = p + q * K1;
           if (gam > my_best || (gam == my_best && idx < my_idx)) {
@@ -337,6 +358,335 @@ __global__ __launch_bounds__(256) void search_tile_kernel(SearchArgs A,
       }
     }
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FFT tiles: one 576-thread workgroup per (point, 64 x 64 lag block).
+//
+// The block's numerators are one circular cross-correlation of size
+// 128 x 128: the window region R (rows p0-1 .. p0+126, columns q0-1 ..
+// q0+126, zero outside the clamped window) against the mean-removed
+// template T zero-padded to 128 x 128; c(p, q) = sum_ab T(a, b) R(p+a, q+b)
+// = IFFT2(FFT2(R) . conj(FFT2(T))) has no wrap-around for p, q < 64 when
+// s <= 65.  Everything stays in LDS:
+//   1. R is staged (real, 128 x 128) and the block's window sums are taken
+//      from it exactly as the direct tiles take them (same helpers, same
+//      order), so both paths divide by the same denominators;
+//   2. dim-1 transforms of R, two real columns packed into one complex
+//      128-point line and separated afterwards (65 bins kept);
+//   3. dim-1 transforms of T the same way (ceil(s/2) packed lines);
+//   4. per dim-1 bin (65 lines): dim-2 transforms of both, the product
+//      FFT(R) conj(FFT(T)), its inverse dim-2 transform;
+//   5. inverse dim-1 transforms, two real outputs per complex line;
+//   6. coefficients and the block's first maximum in column-major order.
+// A 128-point line transform is done by 8 lanes of one wave: a 16-point DFT
+// in registers per lane (input stride 8), twiddles, one exchange through
+// the line's own LDS storage, two 8-point DFTs per lane (in-place in the
+// line: the 8 lanes are one wave, whose LDS accesses run in program order).
+// Forward transforms use W = exp(-2 pi i / 128), the inverse ones conj(FFT
+// (conj x)); the 128^2 scale is taken out at the end (a power of two).
+// ---------------------------------------------------------------------------
+constexpr int FFT_P = 128;                      // transform size per axis
+constexpr int FFT_B = 64;                       // lags per block and axis
+constexpr int FFT_SMAX = FFT_P - FFT_B + 1;     // largest template side (65)
+constexpr int FFT_BINS = FFT_P / 2 + 1;         // dim-1 bins kept (65)
+constexpr int FFT_LP = FFT_P + 1;               // complex line pitch of the spectrum
+constexpr int FFT_THREADS = 576;                // 72 groups of 8 lanes >= 65 lines
+constexpr int FFT_XF = FFT_P * 2 * ((FFT_SMAX + 1) / 2) + 2 * FFT_BINS * FFT_SMAX;  // floats
+constexpr int FFT_YF = 2 * FFT_BINS * FFT_LP;   // floats
+static_assert(FFT_XF >= FFT_P * FFT_P, "X holds the real region");
+static_assert(FFT_YF >= 2 * FFT_B * FFT_P, "Y holds the window sums");
+
+__device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 c_mulconj(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
+}
+__device__ __forceinline__ float2 c_conj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 c_mi(float2 a) { return make_float2(a.y, -a.x); }  // -i a
+
+// forward 4-point DFT in place
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  const float2 t0 = c_add(a0, a2), t1 = c_sub(a0, a2), t2 = c_add(a1, a3), t3 = c_mi(c_sub(a1, a3));
+  a0 = c_add(t0, t2);
+  a2 = c_sub(t0, t2);
+  a1 = c_add(t1, t3);
+  a3 = c_sub(t1, t3);
+}
+
+// forward 8-point DFT in place, natural order
+__device__ __forceinline__ void dft8(float2 (&v)[8]) {
+  constexpr float R = 0.70710678118654752f;
+  dft4(v[0], v[2], v[4], v[6]);  // E[k] at v[0], v[2], v[4], v[6]
+  dft4(v[1], v[3], v[5], v[7]);  // O[k] at v[1], v[3], v[5], v[7]
+  const float2 o1 = c_mul(v[3], make_float2(R, -R));
+  const float2 o2 = c_mi(v[5]);
+  const float2 o3 = c_mul(v[7], make_float2(-R, -R));
+  const float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6], o0 = v[1];
+  v[0] = c_add(e0, o0);
+  v[4] = c_sub(e0, o0);
+  v[1] = c_add(e1, o1);
+  v[5] = c_sub(e1, o1);
+  v[2] = c_add(e2, o2);
+  v[6] = c_sub(e2, o2);
+  v[3] = c_add(e3, o3);
+  v[7] = c_sub(e3, o3);
+}
+
+// forward 16-point DFT, x[n] in v[n]; X[k1 + 4 k2] returned in v[4 k1 + k2]
+__device__ __forceinline__ void dft16(float2 (&v)[16]) {
+  constexpr float C1 = 0.92387953251128676f, S1 = 0.38268343236508977f;
+  constexpr float R = 0.70710678118654752f;
+#pragma unroll
+  for (int n2 = 0; n2 < 4; n2++) dft4(v[n2], v[4 + n2], v[8 + n2], v[12 + n2]);
+  // v[n2 + 4 k1] *= W16^(n2 k1)
+  v[1 + 4 * 1] = c_mul(v[1 + 4 * 1], make_float2(C1, -S1));
+  v[1 + 4 * 2] = c_mul(v[1 + 4 * 2], make_float2(R, -R));
+  v[1 + 4 * 3] = c_mul(v[1 + 4 * 3], make_float2(S1, -C1));
+  v[2 + 4 * 1] = c_mul(v[2 + 4 * 1], make_float2(R, -R));
+  v[2 + 4 * 2] = c_mi(v[2 + 4 * 2]);
+  v[2 + 4 * 3] = c_mul(v[2 + 4 * 3], make_float2(-R, -R));
+  v[3 + 4 * 1] = c_mul(v[3 + 4 * 1], make_float2(S1, -C1));
+  v[3 + 4 * 2] = c_mul(v[3 + 4 * 2], make_float2(-R, -R));
+  v[3 + 4 * 3] = c_mul(v[3 + 4 * 3], make_float2(-C1, S1));
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) dft4(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+}
+
+// One 128-point forward transform by the 8 lanes g = 0..7 of a group.  In:
+// v[n1] = x[8 n1 + g].  ex: the line's 128 complex LDS slots (overwritten).
+// Out: o[8 j + k2] = X[(2 g + j) + 16 k2], j = 0, 1, k2 = 0..7.
+__device__ __forceinline__ void fft128(float2 (&v)[16], float2* ex, int g, const float2* tw,
+                                       float2 (&o)[16]) {
+  dft16(v);
+  // y[k1] = X16[k1] W128^(g k1) -> ex[8 k1 + g]; X16[k1 + 4 k2] is in v[4 k1 + k2]
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+      const int k1 = a + 4 * b;
+      ex[8 * k1 + g] = c_mul(v[4 * a + b], tw[g * k1]);
+    }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    float2 z[8];
+#pragma unroll
+    for (int n2 = 0; n2 < 8; n2++) z[n2] = ex[8 * (2 * g + j) + n2];
+    dft8(z);
+#pragma unroll
+    for (int k2 = 0; k2 < 8; k2++) o[8 * j + k2] = z[k2];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// output slot m of fft128 -> frequency index
+__device__ __forceinline__ int fft_out_index(int g, int m) { return (2 * g + m / 8) + 16 * (m % 8); }
+
+// Dim-1 transforms of real columns held in X (column c at X[c * 128 + i]):
+// the pair (2l, 2l+1) is one complex line, transformed in its own storage;
+// the spectra are separated and bins 0..64 written to out[k * ld + 2l (+1)].
+// Columns >= ncols read as zero.
+__device__ __forceinline__ void dim1_real_pair(float* X, int l, int g, int ncols, float2* out,
+                                               int ld, const float2* tw) {
+  float* ca = X + (2 * l) * FFT_P;
+  float* cb = ca + FFT_P;
+  const bool hasb = 2 * l + 1 < ncols;
+  float2 v[16], o[16];
+#pragma unroll
+  for (int n1 = 0; n1 < 16; n1++) {
+    const int i = 8 * n1 + g;
+    v[n1] = make_float2(ca[i], hasb ? cb[i] : 0.0f);
+  }
+  float2* ex = reinterpret_cast<float2*>(ca);  // the pair's 256 floats
+  fft128(v, ex, g, tw, o);
+#pragma unroll
+  for (int m = 0; m < 16; m++) ex[fft_out_index(g, m)] = o[m];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = -i (Z[k] - conj Z[-k]) / 2
+  for (int k = g; k < FFT_BINS; k += 8) {
+    const float2 zk = ex[k], zn = c_conj(ex[(FFT_P - k) & (FFT_P - 1)]);
+    const float2 A = make_float2(0.5f * (zk.x + zn.x), 0.5f * (zk.y + zn.y));
+    const float2 B = c_mi(make_float2(0.5f * (zk.x - zn.x), 0.5f * (zk.y - zn.y)));
+    out[k * ld + 2 * l] = A;
+    if (hasb) out[k * ld + 2 * l + 1] = B;
+  }
+}
+
+__global__ __launch_bounds__(FFT_THREADS) void search_fft_kernel(SearchArgs A,
+                                                                 const float* __restrict__ tmpl,
+                                                                 const float* __restrict__ g) {
+  extern __shared__ float smem[];
+  __shared__ float bestv[FFT_THREADS];
+  __shared__ int besti[FFT_THREADS];
+  float* X = smem;                                          // FFT_XF floats
+  float2* Y = reinterpret_cast<float2*>(smem + FFT_XF);    // FFT_BINS x FFT_LP
+  float2* tw = Y + FFT_BINS * FFT_LP;                       // W128^m, m < 128
+  float* den = reinterpret_cast<float*>(tw + FFT_P);        // per lag of the block
+  const int tid = threadIdx.x, grp = tid / 8, gl = tid % 8;
+  for (int m = tid; m < FFT_P; m += FFT_THREADS) {
+    double sn, cs;
+    sincospi(2.0 * m / FFT_P, &sn, &cs);
+    tw[m] = make_float2((float)cs, (float)-sn);
+  }
+  const int total = num_points(A) * A.ntiles;
+  const int s = A.ws + 1;
+  const int M = A.M;
+  const int SW = FFT_B + s - 1;  // region rows the window sums use
+  const float inv_n = 1.0f / (float)(s * s);
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int t = item / A.ntiles, tile = item - t * A.ntiles;
+    const PointPrm P = A.prm[t];
+    const int K1 = P.L + P.R + 3, K2 = P.U + P.D + 3;
+    const int W1 = s + P.L + P.R, W2 = s + P.U + P.D;
+    const int p0 = (tile % A.nt1) * FFT_B, q0 = (tile / A.nt1) * FFT_B;
+    float my_best = -1e30f;
+    int my_idx = 0x7fffffff;
+    __syncthreads();  // the previous item's reads of X, Y and best are done
+    if (P.valid && p0 < K1 && q0 < K2) {
+      const int g_r0 = P.ii - P.L - 1, g_c0 = P.jj - P.U - 1;  // 0-based window (0,0)
+      // 1. region R(i, j) = window(p0 - 1 + i, q0 - 1 + j) at X[j * 128 + i]; the
+      //    columns past the block's reach are zero (as in the direct tiles)
+      for (int e = tid; e < FFT_P * FFT_P; e += FFT_THREADS) {
+        const int i = e % FFT_P, j = e / FFT_P;
+        const int wr = p0 - 1 + i, wc = q0 - 1 + j;
+        float v = 0.0f;
+        if (wr >= 0 && wr < W1 && wc >= 0 && wc < W2 && j < FFT_B + s - 1)
+          v = g[(int64_t)(g_r0 + wr) + (int64_t)(g_c0 + wc) * M];
+        X[e] = v;
+      }
+      __syncthreads();
+      float* CB = reinterpret_cast<float*>(Y);
+      float* CB2 = CB + FFT_B * SW;
+      for (int task = tid; task < SW * (FFT_B / CB_SEG); task += FFT_THREADS) {
+        const int i = task % SW, seg = task / SW;
+        col_box_sums(X, FFT_P, SW, s, seg * CB_SEG, i, CB, CB2);
+      }
+      __syncthreads();
+      for (int l = tid; l < FFT_B * FFT_B; l += FFT_THREADS)
+        den[l] = lag_den(CB, CB2, SW, s, l / FFT_B, l % FFT_B, inv_n, P.denT);
+      __syncthreads();
+      // 2. dim-1 transforms of R -> Y[k][j]
+      if (grp < FFT_P / 2) dim1_real_pair(X, grp, gl, FFT_P, Y, FFT_LP, tw);
+      __syncthreads();
+      // 3. the template, T(a, b) at X[b * 128 + a] (a < s, zero below), and
+      //    its dim-1 transforms -> Tp[k * s + b] after the pair columns
+      const int npair = (s + 1) / 2;
+      float2* Tp = reinterpret_cast<float2*>(X + FFT_P * 2 * npair);
+      const float* T = tmpl + (int64_t)t * s * A.sp;
+      for (int e = tid; e < FFT_P * s; e += FFT_THREADS) {
+        const int a = e % FFT_P, b = e / FFT_P;
+        X[e] = a < s ? T[a * A.sp + b] : 0.0f;
+      }
+      __syncthreads();
+      if (grp < npair) dim1_real_pair(X, grp, gl, s, Tp, s, tw);
+      __syncthreads();
+      // 4. per dim-1 bin k: dim-2 transforms, product, inverse dim-2 transform
+      if (grp < FFT_BINS) {
+        const int k = grp;
+        float2* line = Y + k * FFT_LP;
+        float2 vr[16], vt[16], oR[16], oT[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; n1++) {
+          const int j = 8 * n1 + gl;
+          vr[n1] = line[j];
+          vt[n1] = j < s ? Tp[k * s + j] : make_float2(0.0f, 0.0f);
+        }
+        fft128(vt, line, gl, tw, oT);
+        fft128(vr, line, gl, tw, oR);
+        // conj of the product into the line in natural order, for the
+        // inverse transform conj(FFT(conj x))
+#pragma unroll
+        for (int m = 0; m < 16; m++) line[fft_out_index(gl, m)] = c_conj(c_mulconj(oR[m], oT[m]));
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int n1 = 0; n1 < 16; n1++) vr[n1] = line[8 * n1 + gl];
+        fft128(vr, line, gl, tw, oR);
+#pragma unroll
+        for (int m = 0; m < 16; m++) line[fft_out_index(gl, m)] = c_conj(oR[m]);
+      }
+      __syncthreads();
+      // 5. inverse dim-1 transforms: columns (2l, 2l+1) from bins 0..64 of
+      //    their Hermitian spectra, real parts to X[2l * 128 + i], imaginary
+      //    parts to X[(2l+1) * 128 + i]
+      if (grp < FFT_P / 2) {
+        const int l = grp;
+        float2 v[16], o[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; n1++) {
+          const int k = 8 * n1 + gl;
+          float2 a, b;
+          if (k < FFT_BINS) {
+            a = Y[k * FFT_LP + 2 * l];
+            b = Y[k * FFT_LP + 2 * l + 1];
+          } else {
+            a = c_conj(Y[(FFT_P - k) * FFT_LP + 2 * l]);
+            b = c_conj(Y[(FFT_P - k) * FFT_LP + 2 * l + 1]);
+          }
+          // z = a + i b, transformed as conj(z)
+          v[n1] = make_float2(a.x - b.y, -(a.y + b.x));
+        }
+        float* ca = X + (2 * l) * FFT_P;
+        fft128(v, reinterpret_cast<float2*>(ca), gl, tw, o);
+#pragma unroll
+        for (int m = 0; m < 16; m++) {
+          const int i = fft_out_index(gl, m);
+          ca[i] = o[m].x;            // real part of conj(o)
+          ca[FFT_P + i] = -o[m].y;   // imaginary part of conj(o)
+        }
+      }
+      __syncthreads();
+      // 6. coefficients and the block's first maximum
+      constexpr float SCALE = 1.0f / (float)(FFT_P * FFT_P);
+      for (int l = tid; l < FFT_B * FFT_B; l += FFT_THREADS) {
+        const int p = l % FFT_B, q = l / FFT_B;
+        const float gam = zncc_of(X[q * FFT_P + p] * SCALE, den[l]);
+        if (p0 + p < K1 && q0 + q < K2) {
+          const int idx = (p0 + p) + (q0 + q) * K1;
+          if (gam > my_best || (gam == my_best && idx < my_idx)) {
+            my_best = gam;
+            my_idx = idx;
+          }
+        }
+      }
+    }
+    bestv[tid] = my_best;
+    besti[tid] = my_idx;
+    __syncthreads();
+    if (tid < 64) {
+      float bv = bestv[tid];
+      int bi = besti[tid];
+      for (int k = tid + 64; k < FFT_THREADS; k += 64) {
+        if (bestv[k] > bv || (bestv[k] == bv && besti[k] < bi)) {
+          bv = bestv[k];
+          bi = besti[k];
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_down(bv, off);
+        const int oi = __shfl_down(bi, off);
+        if (ov > bv || (ov == bv && oi < bi)) {
+          bv = ov;
+          bi = oi;
+        }
+      }
+      if (tid == 0) {
+        A.tbv[item] = bv;
+        A.tbi[item] = bi;
+      }
+    }
   }
 }
 
@@ -622,6 +972,19 @@ void tile_counts(const int reg[4], int* nt1, int* ntiles) {
   *ntiles = *nt1 * ((K2 + KT2 - 1) / KT2);
 }
 
+// FFT blocks per point (never more than the direct tiles: the scratch is
+// sized for those)
+void fft_tile_counts(const int reg[4], int* nt1, int* ntiles) {
+  const int K1 = reg[0] + reg[1] + 3, K2 = reg[2] + reg[3] + 3;
+  *nt1 = (K1 + FFT_B - 1) / FFT_B;
+  *ntiles = *nt1 * ((K2 + FFT_B - 1) / FFT_B);
+}
+
+constexpr size_t FFT_SMEM = sizeof(float) * ((size_t)FFT_XF + FFT_YF + 2 * FFT_P + FFT_B * FFT_B);
+
+// the FFT tiles take a lag window when asked for and the template fits
+bool use_fft(int method, int ws) { return method == 1 && ws + 1 <= FFT_SMAX; }
+
 int pad_len(int ws) { return (ws + 1 + RB - 1) / RB * RB; }
 
 size_t up256(size_t v) { return (v + 255) / 256 * 256; }
@@ -641,11 +1004,17 @@ void carve(SearchArgs& A, void* scratch, int maxpts) {
 // when that fits in QWIN_LDS_MAX bytes
 constexpr size_t QWIN_LDS_MAX = 150 * 1024;
 
-hipError_t run(const SearchArgs& A, int maxpts, hipStream_t s, double* q = nullptr) {
+hipError_t run(SearchArgs A, int maxpts, hipStream_t s, double* q = nullptr) {
   const size_t smem = aldic::search_smem_bytes(A.ws);
   if (smem > 160 * 1024) return hipErrorInvalidValue;
-  // dynamic LDS beyond the 64 KB default is asked for, per launch size
-  if (smem > 64 * 1024) {
+  if (A.fft) {
+    fft_tile_counts(A.reg, &A.nt1, &A.ntiles);
+    const hipError_t e = hipFuncSetAttribute((const void*)search_fft_kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)FFT_SMEM);
+    if (e != hipSuccess) return e;
+  } else if (smem > 64 * 1024) {
+    // dynamic LDS beyond the 64 KB default is asked for, per launch size
     const hipError_t e = hipFuncSetAttribute((const void*)search_tile_kernel,
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)smem);
@@ -675,7 +1044,11 @@ hipError_t run(const SearchArgs& A, int maxpts, hipStream_t s, double* q = nullp
       search_qfactor_kernel<false><<<grid, 256, qsm, s>>>(A, q);
   }
   const int64_t items = (int64_t)maxpts * A.ntiles;
-  search_tile_kernel<<<(int)std::min<int64_t>(items, 4096), 256, smem, s>>>(A, A.tmpl, A.g);
+  if (A.fft)
+    search_fft_kernel<<<(int)std::min<int64_t>(items, 2048), FFT_THREADS, FFT_SMEM, s>>>(
+        A, A.tmpl, A.g);
+  else
+    search_tile_kernel<<<(int)std::min<int64_t>(items, 4096), 256, smem, s>>>(A, A.tmpl, A.g);
   search_final_kernel<<<std::min((maxpts + 3) / 4, 1024), 256, 0, s>>>(A);
   return hipGetLastError();
 }
@@ -701,7 +1074,7 @@ size_t search_scratch_bytes(int maxpts, int ws, const int reg[4]) {
 hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
                                const double* coords, int n, const int* list, const int* count,
                                int max_count, int ws, const int reg[4], void* scratch, float* U0,
-                               float* F0, hipStream_t s) {
+                               float* F0, hipStream_t s, int method) {
   SearchArgs A{};
   A.f = f;
   A.g = g;
@@ -711,6 +1084,7 @@ hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
   A.sp = pad_len(ws);
   for (int k = 0; k < 4; k++) A.reg[k] = reg[k];
   tile_counts(reg, &A.nt1, &A.ntiles);
+  A.fft = use_fft(method, ws) ? 1 : 0;
   A.list = list;
   A.lcount = count;
   A.ncoords = coords;
@@ -725,7 +1099,7 @@ hipError_t launch_search_points(const float* f, const float* g, int M, int N, in
                                 const int gridx[2], const int gridy[2], int ws, int step,
                                 const int reg[4], const int* points, const int* count,
                                 void* scratch, double* x, double* y, double* u, double* v,
-                                double* ccmax, int* ipr, int* ipc, hipStream_t s) {
+                                double* ccmax, int* ipr, int* ipc, hipStream_t s, int method) {
   SearchArgs A{};
   A.f = f;
   A.g = g;
@@ -742,6 +1116,7 @@ hipError_t launch_search_points(const float* f, const float* g, int M, int N, in
   A.step = step;
   for (int k = 0; k < 4; k++) A.reg[k] = reg[k];
   tile_counts(reg, &A.nt1, &A.ntiles);
+  A.fft = use_fft(method, ws) ? 1 : 0;
   A.glist = points;
   A.lcount = count;
   A.x = x;
@@ -759,7 +1134,7 @@ hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, i
                          const int gridx[2], const int gridy[2], int ws, int step,
                          const int reg[4], void* scratch, double* x, double* y, double* u,
                          double* v, double* ccmax, int* ipr, int* ipc, hipStream_t s,
-                         double* q) {
+                         double* q, int method) {
   SearchArgs A{};
   A.f = f;
   A.g = g;
@@ -776,6 +1151,7 @@ hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, i
   A.step = step;
   for (int k = 0; k < 4; k++) A.reg[k] = reg[k];
   tile_counts(reg, &A.nt1, &A.ntiles);
+  A.fft = use_fft(method, ws) ? 1 : 0;
   A.x = x;
   A.y = y;
   A.u = u;

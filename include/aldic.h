@@ -95,6 +95,20 @@ int aldic_ctx_capture_end(aldic_ctx* ctx, aldic_graph** out);
 int aldic_graph_launch(aldic_graph* graph, aldic_ctx* ctx);
 int aldic_graph_destroy(aldic_graph* graph);
 
+/* How the integer searches on this context (every aldic_integer_search*
+ * and aldic_search_* entry) find the integer ZNCC peak; results follow
+ * funIntegerSearch.m:154-169 either way.  ALDIC_SEARCH_DIRECT (default):
+ * direct sums per lag.  ALDIC_SEARCH_FFT: the numerators of each 64 x 64
+ * block of lags as one 128 x 128 FFT cross-correlation in LDS (MATLAB's
+ * normxcorr2 also works in the frequency domain when that is faster), for
+ * template sides up to 65 px (larger ones stay direct); the denominators and
+ * the 3 x 3 sub-pixel fit are the direct ones.  The FP32 numerators differ
+ * from the direct sums by rounding, so a near-tie for the peak can resolve
+ * differently. */
+#define ALDIC_SEARCH_DIRECT 0
+#define ALDIC_SEARCH_FFT 1
+int aldic_ctx_set_search_method(aldic_ctx* ctx, int32_t method);
+
 int aldic_ctx_set_timing(aldic_ctx* ctx, int32_t enable);
 int aldic_ctx_stage_times(aldic_ctx* ctx, double* ms, int32_t* launches, int32_t reset);
 
