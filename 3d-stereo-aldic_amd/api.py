@@ -58,11 +58,13 @@ class Context:
         check(lib().aldic_ctx_synchronize(self.ptr))
 
     def set_search_method(self, method):
-        """Integer-search peak finding on this context: "direct" (default)
-        or "fft" (aldic_ctx_set_search_method)."""
-        modes = {"direct": 0, "fft": 1}
+        """Integer-search peak finding on this context: "auto" (default:
+        FFT lag blocks for whole grids of at least 1024 points with lag
+        windows of at least 32 x 32), "direct" or "fft"
+        (aldic_ctx_set_search_method)."""
+        modes = {"direct": 0, "fft": 1, "auto": 2}
         if method not in modes:
-            raise ValueError(f"search method {method!r}: 'direct' or 'fft'")
+            raise ValueError(f"search method {method!r}: 'auto', 'direct' or 'fft'")
         check(lib().aldic_ctx_set_search_method(self.ptr, modes[method]))
         self.search_method = method
         return self

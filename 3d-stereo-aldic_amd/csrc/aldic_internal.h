@@ -58,8 +58,8 @@ struct aldic_ctx {
   // capture is open: staging buffers must not move while it is non-zero
   // (shared with the graphs so either may be destroyed first)
   std::shared_ptr<std::atomic<int>> pinned = std::make_shared<std::atomic<int>>(0);
-  // integer search lag tiles: ALDIC_SEARCH_DIRECT or ALDIC_SEARCH_FFT
-  int search_method = 0;
+  // integer search lag tiles: ALDIC_SEARCH_AUTO, _DIRECT or _FFT
+  int search_method = 2;
 };
 
 struct aldic_graph {

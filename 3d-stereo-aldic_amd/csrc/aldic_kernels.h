@@ -101,7 +101,8 @@ size_t search_scratch_bytes(int maxpts, int ws, const int reg[4]);
 // q (may be null): raw (PCE, PPE) per grid point, 2 doubles, point order
 // ci + cj * nX (compute_qFactor of the full normxcorr2 map).  method: 0
 // direct lag tiles, 1 FFT lag blocks where the template side is <= 65
-// (direct otherwise); the sub-pixel fit is direct either way.
+// (direct otherwise), 2 FFT blocks where they are the faster ones; the
+// sub-pixel fit is direct either way.
 hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, int nY,
                          const int gridx[2], const int gridy[2], int ws, int step,
                          const int reg[4], void* scratch, double* x, double* y, double* u,

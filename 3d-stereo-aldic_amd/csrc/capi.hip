@@ -279,8 +279,8 @@ extern "C" int aldic_ctx_set_timing(aldic_ctx* ctx, int32_t enable) {
 
 extern "C" int aldic_ctx_set_search_method(aldic_ctx* ctx, int32_t method) {
   if (!ctx) return fail(ALDIC_EINVAL, "NULL context");
-  if (method != ALDIC_SEARCH_DIRECT && method != ALDIC_SEARCH_FFT)
-    return fail(ALDIC_EINVAL, "search method must be ALDIC_SEARCH_DIRECT or ALDIC_SEARCH_FFT");
+  if (method != ALDIC_SEARCH_DIRECT && method != ALDIC_SEARCH_FFT && method != ALDIC_SEARCH_AUTO)
+    return fail(ALDIC_EINVAL, "search method must be ALDIC_SEARCH_AUTO, _DIRECT or _FFT");
   ctx->search_method = method;
   return ALDIC_OK;
 }

@@ -183,10 +183,10 @@ __device__ __forceinline__ void tail_fmas(const float* Ta, const float (&win)[2 
   }
 }
 
-// Window sums of a lag tile, shared by the direct and the FFT tiles so that
-// both give the same denominators: CB(i, q) = sum_{b<s} sub(i, q+b) for
-// q = qa .. qa + CB_SEG - 1 (sub(i, j) at sub[j * ld + i]), by a running
-// sum restarted every CB_SEG lags; CB is stored with row length SW.
+// Window sums of a lag tile, shared by the direct and the FFT tiles:
+// CB(i, q) = sum_{b<s} sub(i, q+b) for q = qa .. qa + CB_SEG - 1 (sub(i, j)
+// at sub[j * ld + i]), by a running sum restarted every CB_SEG lags; CB is
+// stored with row length SW.
 constexpr int CB_SEG = 16;
 
 __device__ __forceinline__ void col_box_sums(const float* sub, int ld, int SW, int s, int qa,
@@ -208,6 +208,11 @@ __device__ __forceinline__ void col_box_sums(const float* sub, int ld, int SW, i
   }
 }
 
+__device__ __forceinline__ float den_of(float sa, float sa2, float inv_n, float denT) {
+  const float varA = sa2 - sa * sa * inv_n;
+  return (varA > 1e-6f * sa2 && varA > 0.0f) ? denT * sqrtf(varA) : 0.0f;
+}
+
 // normxcorr2's denominator at tile lag (p, q) from the column sums:
 // denT * sqrt(var) of the s x s window, or 0 for a window of (numerically)
 // zero variance (its coefficient is 0)
@@ -218,8 +223,7 @@ __device__ __forceinline__ float lag_den(const float* CB, const float* CB2, int 
     sa += CB[q * SW + p + a2];
     sa2 += CB2[q * SW + p + a2];
   }
-  const float varA = sa2 - sa * sa * inv_n;
-  return (varA > 1e-6f * sa2 && varA > 0.0f) ? denT * sqrtf(varA) : 0.0f;
+  return den_of(sa, sa2, inv_n, denT);
 }
 
 // the coefficient; one outside [-1, 1] beyond FP32 round-off is 0
@@ -391,12 +395,17 @@ constexpr int FFT_P = 128;                      // transform size per axis
 constexpr int FFT_B = 64;                       // lags per block and axis
 constexpr int FFT_SMAX = FFT_P - FFT_B + 1;     // largest template side (65)
 constexpr int FFT_BINS = FFT_P / 2 + 1;         // dim-1 bins kept (65)
-constexpr int FFT_LP = FFT_P + 1;               // complex line pitch of the spectrum
+// Line pitches: the 8 line groups of a wave sit 16 floats apart modulo 64
+// LDS banks (a real column pair and a spectrum line both span 272 floats).
+constexpr int FFT_CS = FFT_P + 8;               // real column pitch in X (floats)
+constexpr int FFT_LP = FFT_P + 8;               // complex line pitch of the spectrum
 constexpr int FFT_THREADS = 576;                // 72 groups of 8 lanes >= 65 lines
-constexpr int FFT_XF = FFT_P * 2 * ((FFT_SMAX + 1) / 2) + 2 * FFT_BINS * FFT_SMAX;  // floats
+constexpr int FFT_CBP = FFT_P + 1;              // window-sum row pitch (>= 64 + s - 1)
+// floats, a multiple of 4 so that Y starts 16-byte aligned
+constexpr int FFT_XF = (FFT_CS * 2 * ((FFT_SMAX + 1) / 2) + 2 * FFT_BINS * FFT_SMAX + 3) / 4 * 4;
 constexpr int FFT_YF = 2 * FFT_BINS * FFT_LP;   // floats
-static_assert(FFT_XF >= FFT_P * FFT_P, "X holds the real region");
-static_assert(FFT_YF >= 2 * FFT_B * FFT_P, "Y holds the window sums");
+static_assert(FFT_XF >= FFT_CS * FFT_P, "X holds the real region");
+static_assert(FFT_YF >= 2 * FFT_B * FFT_CBP, "Y holds the window sums");
 
 __device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -457,9 +466,14 @@ __device__ __forceinline__ void dft16(float2 (&v)[16]) {
   for (int k1 = 0; k1 < 4; k1++) dft4(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
 }
 
+// slot of y[k1][n] (k1 < 16, n < 8) in a line's exchange: rows of 8, the
+// neighbouring slots of rows 4..7 and 12..15 swapped, so that the 8 lanes
+// reading one column (k1 = g + 8j) share fewer banks
+__device__ __forceinline__ int ex_slot(int k1, int n) { return 8 * k1 + (n ^ ((k1 >> 2) & 1)); }
+
 // One 128-point forward transform by the 8 lanes g = 0..7 of a group.  In:
 // v[n1] = x[8 n1 + g].  ex: the line's 128 complex LDS slots (overwritten).
-// Out: o[8 j + k2] = X[(2 g + j) + 16 k2], j = 0, 1, k2 = 0..7.
+// Out: o[8 j + k2] = X[(g + 8 j) + 16 k2], j = 0, 1, k2 = 0..7.
 __device__ __forceinline__ void fft128(float2 (&v)[16], float2* ex, int g, const float2* tw,
                                        float2 (&o)[16]) {
   dft16(v);
@@ -469,7 +483,7 @@ __device__ __forceinline__ void fft128(float2 (&v)[16], float2* ex, int g, const
 #pragma unroll
     for (int b = 0; b < 4; b++) {
       const int k1 = a + 4 * b;
-      ex[8 * k1 + g] = c_mul(v[4 * a + b], tw[g * k1]);
+      ex[ex_slot(k1, g)] = c_mul(v[4 * a + b], tw[g * k1]);
     }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -478,7 +492,7 @@ __device__ __forceinline__ void fft128(float2 (&v)[16], float2* ex, int g, const
   for (int j = 0; j < 2; j++) {
     float2 z[8];
 #pragma unroll
-    for (int n2 = 0; n2 < 8; n2++) z[n2] = ex[8 * (2 * g + j) + n2];
+    for (int n2 = 0; n2 < 8; n2++) z[n2] = ex[ex_slot(g + 8 * j, n2)];
     dft8(z);
 #pragma unroll
     for (int k2 = 0; k2 < 8; k2++) o[8 * j + k2] = z[k2];
@@ -489,16 +503,16 @@ __device__ __forceinline__ void fft128(float2 (&v)[16], float2* ex, int g, const
 }
 
 // output slot m of fft128 -> frequency index
-__device__ __forceinline__ int fft_out_index(int g, int m) { return (2 * g + m / 8) + 16 * (m % 8); }
+__device__ __forceinline__ int fft_out_index(int g, int m) { return (g + 8 * (m / 8)) + 16 * (m % 8); }
 
-// Dim-1 transforms of real columns held in X (column c at X[c * 128 + i]):
+// Dim-1 transforms of real columns held in X (column c at X[c * FFT_CS + i]):
 // the pair (2l, 2l+1) is one complex line, transformed in its own storage;
 // the spectra are separated and bins 0..64 written to out[k * ld + 2l (+1)].
 // Columns >= ncols read as zero.
 __device__ __forceinline__ void dim1_real_pair(float* X, int l, int g, int ncols, float2* out,
                                                int ld, const float2* tw) {
-  float* ca = X + (2 * l) * FFT_P;
-  float* cb = ca + FFT_P;
+  float* ca = X + (2 * l) * FFT_CS;
+  float* cb = ca + FFT_CS;
   const bool hasb = 2 * l + 1 < ncols;
   float2 v[16], o[16];
 #pragma unroll
@@ -506,7 +520,7 @@ __device__ __forceinline__ void dim1_real_pair(float* X, int l, int g, int ncols
     const int i = 8 * n1 + g;
     v[n1] = make_float2(ca[i], hasb ? cb[i] : 0.0f);
   }
-  float2* ex = reinterpret_cast<float2*>(ca);  // the pair's 256 floats
+  float2* ex = reinterpret_cast<float2*>(ca);  // the pair's first 256 floats
   fft128(v, ex, g, tw, o);
 #pragma unroll
   for (int m = 0; m < 16; m++) ex[fft_out_index(g, m)] = o[m];
@@ -526,9 +540,10 @@ __device__ __forceinline__ void dim1_real_pair(float* X, int l, int g, int ncols
 __global__ __launch_bounds__(FFT_THREADS) void search_fft_kernel(SearchArgs A,
                                                                  const float* __restrict__ tmpl,
                                                                  const float* __restrict__ g) {
-  extern __shared__ float smem[];
+  extern __shared__ float4 smem4[];  // 16-byte aligned
   __shared__ float bestv[FFT_THREADS];
   __shared__ int besti[FFT_THREADS];
+  float* smem = reinterpret_cast<float*>(smem4);
   float* X = smem;                                          // FFT_XF floats
   float2* Y = reinterpret_cast<float2*>(smem + FFT_XF);    // FFT_BINS x FFT_LP
   float2* tw = Y + FFT_BINS * FFT_LP;                       // W128^m, m < 128
@@ -555,38 +570,56 @@ __global__ __launch_bounds__(FFT_THREADS) void search_fft_kernel(SearchArgs A,
     __syncthreads();  // the previous item's reads of X, Y and best are done
     if (P.valid && p0 < K1 && q0 < K2) {
       const int g_r0 = P.ii - P.L - 1, g_c0 = P.jj - P.U - 1;  // 0-based window (0,0)
-      // 1. region R(i, j) = window(p0 - 1 + i, q0 - 1 + j) at X[j * 128 + i]; the
-      //    columns past the block's reach are zero (as in the direct tiles)
+      // 1. region R(i, j) = window(p0 - 1 + i, q0 - 1 + j) at X[j * FFT_CS + i];
+      //    the columns past the block's reach are zero (as in the direct tiles)
       for (int e = tid; e < FFT_P * FFT_P; e += FFT_THREADS) {
         const int i = e % FFT_P, j = e / FFT_P;
         const int wr = p0 - 1 + i, wc = q0 - 1 + j;
         float v = 0.0f;
         if (wr >= 0 && wr < W1 && wc >= 0 && wc < W2 && j < FFT_B + s - 1)
           v = g[(int64_t)(g_r0 + wr) + (int64_t)(g_c0 + wc) * M];
-        X[e] = v;
+        X[j * FFT_CS + i] = v;
       }
       __syncthreads();
+      // window sums, rows of FFT_CBP floats (odd: the sliding pass below
+      // reads 8 rows at once)
       float* CB = reinterpret_cast<float*>(Y);
-      float* CB2 = CB + FFT_B * SW;
+      float* CB2 = CB + FFT_B * FFT_CBP;
       for (int task = tid; task < SW * (FFT_B / CB_SEG); task += FFT_THREADS) {
         const int i = task % SW, seg = task / SW;
-        col_box_sums(X, FFT_P, SW, s, seg * CB_SEG, i, CB, CB2);
+        col_box_sums(X, FFT_CS, FFT_CBP, s, seg * CB_SEG, i, CB, CB2);
       }
       __syncthreads();
-      for (int l = tid; l < FFT_B * FFT_B; l += FFT_THREADS)
-        den[l] = lag_den(CB, CB2, SW, s, l / FFT_B, l % FFT_B, inv_n, P.denT);
+      // denominators by window sums slid along dim 1: task (q, c) takes
+      // lags p = 8c .. 8c + 7 of column q
+      for (int task = tid; task < FFT_B * (FFT_B / 8); task += FFT_THREADS) {
+        const int c = task % (FFT_B / 8), q = task / (FFT_B / 8), pa = 8 * c;
+        const float* r1 = CB + q * FFT_CBP;
+        const float* r2 = CB2 + q * FFT_CBP;
+        float sa = 0.0f, sa2 = 0.0f;
+        for (int a2 = 0; a2 < s; a2++) {
+          sa += r1[pa + a2];
+          sa2 += r2[pa + a2];
+        }
+        for (int k = 0;; k++) {
+          den[q * FFT_B + pa + k] = den_of(sa, sa2, inv_n, P.denT);
+          if (k == 7) break;
+          sa += r1[pa + k + s] - r1[pa + k];
+          sa2 += r2[pa + k + s] - r2[pa + k];
+        }
+      }
       __syncthreads();
       // 2. dim-1 transforms of R -> Y[k][j]
       if (grp < FFT_P / 2) dim1_real_pair(X, grp, gl, FFT_P, Y, FFT_LP, tw);
       __syncthreads();
-      // 3. the template, T(a, b) at X[b * 128 + a] (a < s, zero below), and
+      // 3. the template, T(a, b) at X[b * FFT_CS + a] (a < s, zero below), and
       //    its dim-1 transforms -> Tp[k * s + b] after the pair columns
       const int npair = (s + 1) / 2;
-      float2* Tp = reinterpret_cast<float2*>(X + FFT_P * 2 * npair);
+      float2* Tp = reinterpret_cast<float2*>(X + FFT_CS * 2 * npair);
       const float* T = tmpl + (int64_t)t * s * A.sp;
       for (int e = tid; e < FFT_P * s; e += FFT_THREADS) {
         const int a = e % FFT_P, b = e / FFT_P;
-        X[e] = a < s ? T[a * A.sp + b] : 0.0f;
+        X[b * FFT_CS + a] = a < s ? T[a * A.sp + b] : 0.0f;
       }
       __syncthreads();
       if (grp < npair) dim1_real_pair(X, grp, gl, s, Tp, s, tw);
@@ -619,8 +652,8 @@ __global__ __launch_bounds__(FFT_THREADS) void search_fft_kernel(SearchArgs A,
       }
       __syncthreads();
       // 5. inverse dim-1 transforms: columns (2l, 2l+1) from bins 0..64 of
-      //    their Hermitian spectra, real parts to X[2l * 128 + i], imaginary
-      //    parts to X[(2l+1) * 128 + i]
+      //    their Hermitian spectra, real parts to X[2l * FFT_CS + i],
+      //    imaginary parts to X[(2l+1) * FFT_CS + i]
       if (grp < FFT_P / 2) {
         const int l = grp;
         float2 v[16], o[16];
@@ -638,13 +671,13 @@ __global__ __launch_bounds__(FFT_THREADS) void search_fft_kernel(SearchArgs A,
           // z = a + i b, transformed as conj(z)
           v[n1] = make_float2(a.x - b.y, -(a.y + b.x));
         }
-        float* ca = X + (2 * l) * FFT_P;
+        float* ca = X + (2 * l) * FFT_CS;
         fft128(v, reinterpret_cast<float2*>(ca), gl, tw, o);
 #pragma unroll
         for (int m = 0; m < 16; m++) {
           const int i = fft_out_index(gl, m);
-          ca[i] = o[m].x;            // real part of conj(o)
-          ca[FFT_P + i] = -o[m].y;   // imaginary part of conj(o)
+          ca[i] = o[m].x;             // real part of conj(o)
+          ca[FFT_CS + i] = -o[m].y;   // imaginary part of conj(o)
         }
       }
       __syncthreads();
@@ -652,7 +685,7 @@ __global__ __launch_bounds__(FFT_THREADS) void search_fft_kernel(SearchArgs A,
       constexpr float SCALE = 1.0f / (float)(FFT_P * FFT_P);
       for (int l = tid; l < FFT_B * FFT_B; l += FFT_THREADS) {
         const int p = l % FFT_B, q = l / FFT_B;
-        const float gam = zncc_of(X[q * FFT_P + p] * SCALE, den[l]);
+        const float gam = zncc_of(X[q * FFT_CS + p] * SCALE, den[l]);
         if (p0 + p < K1 && q0 + q < K2) {
           const int idx = (p0 + p) + (q0 + q) * K1;
           if (gam > my_best || (gam == my_best && idx < my_idx)) {
@@ -982,8 +1015,22 @@ void fft_tile_counts(const int reg[4], int* nt1, int* ntiles) {
 
 constexpr size_t FFT_SMEM = sizeof(float) * ((size_t)FFT_XF + FFT_YF + 2 * FFT_P + FFT_B * FFT_B);
 
-// the FFT tiles take a lag window when asked for and the template fits
-bool use_fft(int method, int ws) { return method == 1 && ws + 1 <= FFT_SMAX; }
+// The FFT tiles take a lag window when asked for and the template fits.
+// Method 2 (automatic) takes them where they are the faster ones: a whole
+// grid of at least FFT_AUTO_POINTS points with lag windows of at least
+// 32 x 32 (measured on MI355X: 2.0x faster than the direct tiles at
+// 103 x 103 lags with 33 px templates, 7.6x with 65 px, 2.3x slower at
+// 23 x 23).  A block holds most of a CU's LDS, so a small or listed search
+// that runs beside other kernels (a fallback, a sparse prior) keeps the
+// direct tiles, which share CUs.
+constexpr int FFT_AUTO_POINTS = 1024;
+
+bool use_fft(int method, int ws, const int reg[4], int points, bool listed) {
+  if (ws + 1 > FFT_SMAX) return false;
+  if (method == 1) return true;
+  return method == 2 && !listed && points >= FFT_AUTO_POINTS && reg[0] + reg[1] + 3 >= 32 &&
+         reg[2] + reg[3] + 3 >= 32;
+}
 
 int pad_len(int ws) { return (ws + 1 + RB - 1) / RB * RB; }
 
@@ -1084,7 +1131,7 @@ hipError_t launch_search_nodes(const float* f, const float* g, int M, int N,
   A.sp = pad_len(ws);
   for (int k = 0; k < 4; k++) A.reg[k] = reg[k];
   tile_counts(reg, &A.nt1, &A.ntiles);
-  A.fft = use_fft(method, ws) ? 1 : 0;
+  A.fft = use_fft(method, ws, reg, max_count, true) ? 1 : 0;
   A.list = list;
   A.lcount = count;
   A.ncoords = coords;
@@ -1116,7 +1163,7 @@ hipError_t launch_search_points(const float* f, const float* g, int M, int N, in
   A.step = step;
   for (int k = 0; k < 4; k++) A.reg[k] = reg[k];
   tile_counts(reg, &A.nt1, &A.ntiles);
-  A.fft = use_fft(method, ws) ? 1 : 0;
+  A.fft = use_fft(method, ws, reg, nX * nY, true) ? 1 : 0;
   A.glist = points;
   A.lcount = count;
   A.x = x;
@@ -1151,7 +1198,7 @@ hipError_t launch_search(const float* f, const float* g, int M, int N, int nX, i
   A.step = step;
   for (int k = 0; k < 4; k++) A.reg[k] = reg[k];
   tile_counts(reg, &A.nt1, &A.ntiles);
-  A.fft = use_fft(method, ws) ? 1 : 0;
+  A.fft = use_fft(method, ws, reg, nX * nY, false) ? 1 : 0;
   A.x = x;
   A.y = y;
   A.u = u;

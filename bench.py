@@ -15,11 +15,13 @@ One step = the whole local matching pass over one pair with inputs resident
 in HBM:
 
   search start:  integer search (ZNCC, radius 50, 33 px templates on a
-                 32 px grid) -> initial guess of every node from the search
-                 grid -> first-order IC-GN of every node (tol 1e-3, <= 101
-                 evaluations);
+                 32 px grid; numerators by 128 x 128 FFT blocks, the
+                 library's default for a grid this size) -> initial guess of
+                 every node from the search grid -> first-order IC-GN of
+                 every node (tol 1e-3, <= 101 evaluations);
   learned start: sparse integer search (same templates and radius, 256 px
-                 grid) -> coarse prior at every node -> CNN on the 48 x 48
+                 grid; direct sums, the default for a search this small) ->
+                 coarse prior at every node -> CNN on the 48 x 48
                  patch pair at the node and at node + round(prior) -> [F0 U0]
                  -> IC-GN (at most 31 evaluations) -> gate (more than 30
                  evaluations or ZNCC < 0.9) -> the search start of the failed
@@ -796,7 +798,7 @@ def main():
                                 f"(0.01 spots/px^2, seed 5), 25 px disparity + 15 px Gaussian "
                                 f"bulge (sigma 600 px), hole r=300, {n} nodes with 8-64 px "
                                 f"subsets; learned start (CNN on a 256 px sparse-search prior, "
-                                f"gate + search fallback) vs ZNCC search start (r=50, "
+                                f"gate + search fallback) vs FFT ZNCC search start (r=50, "
                                 f"{H['meta']['grid'][0]}x{H['meta']['grid'][1]} grid); IC-GN "
                                 f"tol 1e-3"),
                    "init": "learned", "stop": f"step size < {TOL_DP} px", "nodes": n},

@@ -97,16 +97,21 @@ int aldic_graph_destroy(aldic_graph* graph);
 
 /* How the integer searches on this context (every aldic_integer_search*
  * and aldic_search_* entry) find the integer ZNCC peak; results follow
- * funIntegerSearch.m:154-169 either way.  ALDIC_SEARCH_DIRECT (default):
- * direct sums per lag.  ALDIC_SEARCH_FFT: the numerators of each 64 x 64
- * block of lags as one 128 x 128 FFT cross-correlation in LDS (MATLAB's
- * normxcorr2 also works in the frequency domain when that is faster), for
- * template sides up to 65 px (larger ones stay direct); the denominators and
- * the 3 x 3 sub-pixel fit are the direct ones.  The FP32 numerators differ
- * from the direct sums by rounding, so a near-tie for the peak can resolve
- * differently. */
+ * funIntegerSearch.m:154-169 either way.  ALDIC_SEARCH_DIRECT: direct sums
+ * per lag.  ALDIC_SEARCH_FFT: the numerators of each 64 x 64 block of lags
+ * as one 128 x 128 FFT cross-correlation in LDS (MATLAB's normxcorr2 also
+ * works in the frequency domain when that is faster), for template sides up
+ * to 65 px (larger ones stay direct); the window sums and the 3 x 3
+ * sub-pixel fit are direct.  ALDIC_SEARCH_AUTO (default): FFT blocks for a
+ * whole grid of at least 1024 points with lag windows of at least 32 x 32,
+ * direct sums otherwise (an FFT block holds most of a CU's LDS; listed and
+ * small searches that run beside other kernels keep the direct tiles, which
+ * share CUs).  The FP32
+ * numerators differ from the direct sums by rounding, so a near-tie for the
+ * peak can resolve differently. */
 #define ALDIC_SEARCH_DIRECT 0
 #define ALDIC_SEARCH_FFT 1
+#define ALDIC_SEARCH_AUTO 2
 int aldic_ctx_set_search_method(aldic_ctx* ctx, int32_t method);
 
 int aldic_ctx_set_timing(aldic_ctx* ctx, int32_t enable);

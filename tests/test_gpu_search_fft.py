@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def ctxs(pkg):
     api = pkg.api
-    return api.Context(0).set_search_method("fft"), api.Context(0)
+    return api.Context(0).set_search_method("fft"), api.Context(0).set_search_method("direct")
 
 
 def _search(pkg, ctx, f, g, region, gx, gy, ws, step):
@@ -110,3 +110,22 @@ def test_fft_search_node_list(pkg, ctxs):
     assert np.array_equal(out[0], out[1], equal_nan=True)
     err = np.abs(U[0::2] - 4.0 - 0.002 * (coords[:, 0] - M / 2))
     assert np.median(err[np.isfinite(err)]) < 0.1
+
+
+def test_auto_is_the_default(pkg, ctxs):
+    """A new context searches with ALDIC_SEARCH_AUTO: FFT blocks for a grid
+    of 1600 points with 103 x 103 lag windows, direct sums for 23 x 23 ones
+    and for a grid of fewer than 1024 points (bit-identical outputs to the
+    explicit methods)."""
+    fctx, dctx = ctxs
+    actx = pkg.api.Context(0)
+    f, g, _, _ = c1_pair()
+    for region, step, want in (([50, 50], 12, fctx), ([10, 10], 12, dctx), ([50, 50], 16, dctx)):
+        a = _search(pkg, actx, f, g, region, [1, 512], [1, 512], 16, step)
+        b = _search(pkg, want, f, g, region, [1, 512], [1, 512], 16, step)
+        print(f"auto, r {region[0]}, step {step}: {a[2].size} points")
+        for x, y in zip(a[:4] + a[5:], b[:4] + b[5:]):
+            assert np.array_equal(x, y, equal_nan=True)
+        assert np.array_equal(a[4]["max"], b[4]["max"], equal_nan=True)
+    with pytest.raises(ValueError):
+        actx.set_search_method("spatial")
