@@ -393,6 +393,58 @@ def global_step_mode(pkg, args):
 
 
 
+def search_mode(pkg, args):
+    """The integer search alone, direct lag tiles against FFT lag blocks, on
+    the headline image size (4096 x 3000, a smoothed random field and its
+    (25, -3) px shift; normalised): per configuration (template side, grid
+    step, radius) the device time per search over the whole grid (5
+    searches after one warmup, stage timer) and the number of points whose
+    integer peaks differ between the methods.  Prints one JSON line."""
+    api = pkg.api
+    from scipy.ndimage import gaussian_filter
+    M, N = 4096, 3000
+    rng = np.random.default_rng(0)
+    base = gaussian_filter(rng.normal(size=(M + 60, N + 60)), 1.2)
+    f = np.asfortranarray(base[30:-30, 30:-30])
+    g = np.asfortranarray(np.roll(base, (-25, 3), (0, 1))[30:-30, 30:-30])
+    f = (f - f.mean()) / f.std()
+    g = (g - g.mean()) / g.std()
+    out = {"workload": "4096x3000 smoothed random field, (25, -3) px shift"}
+    for ws, step, r in ((32, 32, 50), (64, 32, 50), (32, 32, 20), (16, 16, 10)):
+        nX = len(range(1, M - ws - step + 1, step))
+        nY = len(range(1, N - ws - step + 1, step))
+        res, peaks = {}, {}
+        for method in ("direct", "fft"):
+            ctx = api.Context(0).set_search_method(method)
+            DA = api.DeviceArray
+            d_f = DA.from_host(ctx, f, np.float32)
+            d_g = DA.from_host(ctx, g, np.float32)
+            o = [DA(ctx, nX * nY, np.float64) for _ in range(5)]
+            pk = [DA(ctx, nX * nY, np.int32) for _ in range(2)]
+
+            def run():
+                api.integer_search_dev(ctx, d_f.ptr, d_g.ptr, M, N, [1, M], [1, N], ws, step,
+                                       [r, r, r, r], *[a.ptr for a in o], *[a.ptr for a in pk])
+            run()
+            ctx.synchronize()
+            api.set_timing(ctx, True)
+            api.stage_times(ctx, reset=True)
+            for _ in range(5):
+                run()
+            ms, cnt = api.stage_times(ctx, reset=True)["search"]
+            res[method] = round(ms / cnt, 3)
+            peaks[method] = (pk[0].to_host(), pk[1].to_host())
+            ctx.close()
+        diff = int(np.sum((peaks["direct"][0] != peaks["fft"][0]) |
+                          (peaks["direct"][1] != peaks["fft"][1])))
+        out[f"side{ws + 1}_step{step}_r{r}"] = {"points": nX * nY, "lags": (2 * r + 3) ** 2,
+                                                "direct_ms": res["direct"], "fft_ms": res["fft"],
+                                                "speedup": round(res["direct"] / res["fft"], 2),
+                                                "peaks_differ": diff}
+        progress(f"search side {ws + 1}, r {r}: direct {res['direct']} ms, fft {res['fft']} ms")
+    print(json.dumps(out), flush=True)
+
+
 RULES = {"reference": (0, 0.0), "step": (1, TOL_DP)}
 
 
@@ -715,6 +767,9 @@ def main():
     ap.add_argument("--train-samples", type=int, default=2_000_000)
     ap.add_argument("--workloads", default="bulge4096x3000,tilt2048",
                     help="comma-separated; the first is the headline")
+    ap.add_argument("--search", action="store_true",
+                    help="diagnostic: the integer search alone, direct tiles against FFT blocks, "
+                         "at the headline image size, and exit (no metric line)")
     ap.add_argument("--global-step", action="store_true",
                     help="diagnostic: time the rows around the local step on the 2048 x 2048 "
                          "workload (quadtree mesh, Subpb2, NaN fill, raw decode) and exit "
@@ -727,6 +782,9 @@ def main():
         sys.exit(2)
 
     pkg = ge.load_package()
+    if args.search:
+        search_mode(pkg, args)
+        return
     if args.global_step:
         global_step_mode(pkg, args)
         return
