@@ -244,11 +244,13 @@ class MosaicSource:
 
 
 def train(net: Network, samples=2_000_000, batch=256, lr=2e-3, lr_min=5e-5, seed=4,
-          tiles=64, log_every=50, log=None, time_limit=None):
+          tiles=64, log_every=50, log=None, time_limit=None, trans=12.0):
     """Adam over fresh mosaics until `samples` labelled pairs have been seen
-    (cosine learning-rate decay).  Returns a history dict."""
+    (cosine learning-rate decay).  trans: the tiles' translations are drawn
+    within +-trans px (the range of the residual the network must predict
+    around the rounded prior).  Returns a history dict."""
     ctx = net.ctx
-    src = MosaicSource(ctx, tiles=tiles, seed=seed)
+    src = MosaicSource(ctx, tiles=tiles, seed=seed, trans=trans)
     d_loss = DeviceArray(ctx, 1, np.float32)
     hist = dict(step=[], loss=[], label_rms_u=[], kept=0, total=0)
     steps_total = max(1, samples // batch)
@@ -298,10 +300,11 @@ def from_bf16_bits(b):
     return (np.asarray(b, dtype=np.uint32) << 16).view(np.float32)
 
 
-def evaluate(net: Network, seed=99, tiles=32):
-    """Prediction error on a held-out mosaic: RMS of (u, v) in px and of the
-    gradients, against the IC-GN labels and against the true warps."""
-    src = MosaicSource(net.ctx, tiles=tiles, seed=seed)
+def evaluate(net: Network, seed=99, tiles=32, trans=12.0):
+    """Prediction error on a held-out mosaic (translations within +-trans
+    px): RMS of (u, v) in px and of the gradients, against the IC-GN
+    labels."""
+    src = MosaicSource(net.ctx, tiles=tiles, seed=seed, trans=trans)
     st = src.make(0)
     n = st["kept"]
     out = DeviceArray(net.ctx, 6 * max(n, 1), np.float32)

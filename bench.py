@@ -31,7 +31,10 @@ Each start runs with two IC-GN stopping rules: the reference's relative
 ||b|| rule (parity) and the step-size rule (the update moves the subset by
 less than 1e-3 px; aldic_plan_set_stopping).  The network is trained first
 (not timed) on the device, on 2M patch pairs labelled by IC-GN on seeded
-speckle mosaics.
+speckle mosaics, with tile translations within +-2 px: the network predicts
+the residual around the rounded sparse-search prior, whose interpolation
+error on these fields is below 1 px (a larger residual goes through the
+gate and the search fallback).
 
 Converged = status 1..100, ZNCC >= 0.9 and not a bad point by the
 reference's rule (ConvIt > 99 or > max(mean + std, 6) over the good ones,
@@ -765,6 +768,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-alm", action="store_true", help="skip the incremental-sequence frame")
     ap.add_argument("--train-samples", type=int, default=2_000_000)
+    ap.add_argument("--train-trans", type=float, default=2.0,
+                    help="training translations within +- this many px: the residual the "
+                         "network predicts around the rounded prior (the prior's bilinear "
+                         "error on these smooth fields is below 1 px)")
     ap.add_argument("--workloads", default="bulge4096x3000,tilt2048",
                     help="comma-separated; the first is the headline")
     ap.add_argument("--search", action="store_true",
@@ -794,8 +801,8 @@ def main():
     tctx = pkg.api.Context(0)
     net = learned.Network(tctx, max_batch=256, seed=0)
     hist = learned.train(net, samples=args.train_samples, batch=256, lr=2e-3, tiles=64, seed=4,
-                         log_every=2000, log=progress)
-    ev = learned.evaluate(net, seed=12345, tiles=32)
+                         log_every=2000, log=progress, trans=args.train_trans)
+    ev = learned.evaluate(net, seed=12345, tiles=32, trans=args.train_trans)
     params = net.get_params()
     net.close()
     tctx.close()
@@ -803,6 +810,7 @@ def main():
              "kept_frac": round(hist["kept"] / max(hist["total"], 1), 5),
              "seconds": round(time.perf_counter() - t0, 1),
              "final_loss": round(hist["loss"][-1], 5) if hist["loss"] else None,
+             "translation_range_px": args.train_trans,
              "heldout_rms_u_px": round(ev["rms_u_px"], 4), "heldout_rms_F": round(ev["rms_F"], 5)}
     progress(f"network trained ({train['seconds']} s)")
     names = [w for w in args.workloads.split(",") if w]
