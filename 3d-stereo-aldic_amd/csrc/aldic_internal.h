@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <string>
 #include <utility>
@@ -125,6 +126,20 @@ inline hipEvent_t take_event(aldic_ctx* ctx) {
   return e;
 }
 
+// Names of the stages in profiler traces (roctx ranges, "aldic:<stage>").
+inline const char* stage_name(int id) {
+  static const char* const names[ALDIC_NSTAGES] = {
+      "aldic:search",        "aldic:grid_to_nodes",  "aldic:icgn_c0",        "aldic:icgn_c1",
+      "aldic:icgn_c2",       "aldic:icgn_c3",        "aldic:icgn_generic",   "aldic:icgn_handed_on",
+      "aldic:subpb1_c0",     "aldic:subpb1_c1",      "aldic:subpb1_c2",      "aldic:subpb1_c3",
+      "aldic:subpb1_generic", "aldic:subpb1_handed_on", "aldic:cnn",          "aldic:fallback",
+      "aldic:subpb2"};
+  return id >= 0 && id < ALDIC_NSTAGES ? names[id] : "aldic:stage";
+}
+
+// A stage's launches: a roctx range on the host thread (shown by
+// rocprofv3 --marker-trace; a no-op without a tool) and, with timing on, a
+// pair of events on the stream.
 struct StageTimer {
   aldic_ctx* ctx;
   int id;
@@ -132,12 +147,14 @@ struct StageTimer {
   hipStream_t st;
   StageTimer(aldic_ctx* c, int stage, hipStream_t on = nullptr)
       : ctx(c), id(stage), st(on ? on : c->stream) {
+    roctxRangePushA(stage_name(stage));
     if (!ctx->timing) return;
     a = take_event(ctx);
     b = take_event(ctx);
     if (a) (void)hipEventRecord(a, st);
   }
   ~StageTimer() {
+    roctxRangePop();
     if (!ctx->timing || !a || !b) return;
     (void)hipEventRecord(b, st);
     ctx->pending.push_back({id, {a, b}});
