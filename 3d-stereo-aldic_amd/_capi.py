@@ -23,6 +23,24 @@ _i64 = C.c_int64
 _vp = C.c_void_p
 _pp = C.POINTER(C.c_void_p)
 
+class AldicParams(C.Structure):
+    """aldic_params: the DICpara fields the local matching path reads."""
+    _fields_ = [
+        ("winsize", C.c_int32), ("winstepsize", C.c_int32), ("winsizeMin", C.c_int32),
+        ("gridx", C.c_int64 * 2), ("gridy", C.c_int64 * 2),
+        ("StereoSearchDistance", C.c_int32 * 4), ("nStereoSearchDistance", C.c_int32),
+        ("TemporalSearchDistance", C.c_int32 * 4), ("nTemporalSearchDistance", C.c_int32),
+        ("NewFFTSearchDistance", C.c_int32 * 4), ("nNewFFTSearchDistance", C.c_int32),
+        ("fixSearchDistanceOrNot", C.c_int32), ("InitFFTSearchMethod", C.c_int32),
+        ("DICIncOrNot", C.c_int32), ("UseGlobal", C.c_int32), ("GaussPtOrder", C.c_int32),
+        ("ClusterNo", C.c_int32),
+        ("tol_stereo", C.c_double), ("tol_temporal", C.c_double),
+        ("mu", C.c_double), ("beta", C.c_double), ("alpha", C.c_double), ("tol2", C.c_double),
+    ]
+
+
+_prm = C.POINTER(AldicParams)
+
 # Every symbol of include/aldic.h with (restype, argtypes).
 SIGNATURES = {
     "aldic_last_error": (C.c_char_p, []),
@@ -36,6 +54,9 @@ SIGNATURES = {
     "aldic_memcpy_dtoh": (C.c_int, [_vp, _vp, _vp, _i64]),
     "aldic_ctx_set_timing": (C.c_int, [_vp, C.c_int32]),
     "aldic_ctx_set_search_method": (C.c_int, [_vp, C.c_int32]),
+    "aldic_params_default": (C.c_int, [_prm]),
+    "aldic_params_validate": (C.c_int, [_prm]),
+    "aldic_params_search_region": (C.c_int, [_prm, C.c_int32, C.POINTER(_i64)]),
     "aldic_ctx_record": (C.c_int, [_vp, C.c_int32]),
     "aldic_ctx_capture_begin": (C.c_int, [_vp]),
     "aldic_ctx_capture_end": (C.c_int, [_vp, _pp]),

@@ -672,6 +672,56 @@ def funIntegerSearch(f, g, tempSizeOfSearchRegion, gridx, gridy, winsize, winste
     return out["x"], out["y"], out["u"], out["v"], cc
 
 
+def params_from_dicpara(DICpara):
+    """An aldic_params struct from a DICpara dict: defaults
+    (aldic_params_default, setDICparaDefaults.m) overridden by the fields the
+    dict holds; empty fields stay empty."""
+    from ._capi import AldicParams
+    p = AldicParams()
+    check(lib().aldic_params_default(C.byref(p)))
+    for key in ("winsize", "winstepsize", "winsizeMin", "fixSearchDistanceOrNot",
+                "InitFFTSearchMethod", "DICIncOrNot", "UseGlobal", "GaussPtOrder", "ClusterNo"):
+        v = DICpara.get(key)
+        if v is not None and np.size(v) > 0:
+            setattr(p, key, int(np.asarray(v).reshape(-1)[0]))
+    for key in ("mu", "beta", "alpha"):
+        v = DICpara.get(key)
+        if v is not None and np.size(v) > 0:
+            setattr(p, key, float(np.asarray(v).reshape(-1)[0]))
+    roi = DICpara.get("gridxyROIRange") or {}
+    for key in ("gridx", "gridy"):
+        v = roi.get(key)
+        if v is not None and np.size(v) >= 2:
+            v = np.asarray(v).reshape(-1)
+            getattr(p, key)[0], getattr(p, key)[1] = int(v[0]), int(v[-1])
+    for key in ("StereoSearchDistance", "TemporalSearchDistance", "NewFFTSearchDistance"):
+        v = _preset(DICpara, key)
+        if v is None:
+            continue
+        v = np.asarray(v).reshape(-1)
+        if v.size not in (1, 2, 4):
+            raise ValueError(f"{key} must be empty, a scalar, [x y] or [L R U D]")
+        for k in range(v.size):
+            getattr(p, key)[k] = int(v[k])
+        setattr(p, "n" + key, int(v.size))
+    return p
+
+
+def validate_dicpara(DICpara):
+    """validateDICpara.m's rules on the path's fields (aldic_params_validate);
+    raises AldicError with the reference's message."""
+    check(lib().aldic_params_validate(C.byref(params_from_dicpara(DICpara))))
+
+
+def search_region(DICpara, ImgSeqNum):
+    """[L R U D] of IntegerSearchQuadtree.m:52-90 and funIntegerSearch.m:12
+    (aldic_params_search_region)."""
+    region = (C.c_int64 * 4)()
+    check(lib().aldic_params_search_region(C.byref(params_from_dicpara(DICpara)), int(ImgSeqNum),
+                                           region))
+    return [int(v) for v in region]
+
+
 def _preset(DICpara, key):
     """DICpara.(key) when the field exists and is not empty (isfield and
     ~isempty), else None."""

@@ -26,7 +26,7 @@ FLAGS = [
 ]
 
 SOURCES = ["capi.hip", "cnn.hip", "cnn_net.hip", "fem.hip", "icgn.hip", "interp.hip", "natural.cpp",
-           "prep.hip", "search.hip"]
+           "params.cpp", "prep.hip", "search.hip"]
 
 
 def _compile(src: str) -> str:

@@ -95,6 +95,35 @@ int aldic_ctx_capture_end(aldic_ctx* ctx, aldic_graph** out);
 int aldic_graph_launch(aldic_graph* graph, aldic_ctx* ctx);
 int aldic_graph_destroy(aldic_graph* graph);
 
+/* The DICpara fields the local matching path reads, as a plain struct
+ * (setDICparaDefaults.m:18-56; the drivers' tolerances StereoMatch_STAQ.m:71,
+ * TemporalMatch_quadtree_ST1.m:190; the ALM constants ibid. :266-298, :333).
+ * A search distance holds n = 0 (empty, []), 1, 2 or 4 values.  winsize and
+ * winstepsize 0 mean unset ([]).  beta 0 means "derive": 1e-3 *
+ * winstepsize^2 * mu (TemporalMatch_quadtree_ST1.m:298).  The GUI, plotting,
+ * strain and smoothing fields are not on this path. */
+typedef struct {
+  int32_t winsize, winstepsize, winsizeMin;
+  int64_t gridx[2], gridy[2];
+  int32_t StereoSearchDistance[4], nStereoSearchDistance;
+  int32_t TemporalSearchDistance[4], nTemporalSearchDistance;
+  int32_t NewFFTSearchDistance[4], nNewFFTSearchDistance;
+  int32_t fixSearchDistanceOrNot;
+  int32_t InitFFTSearchMethod; /* 1: whole-field search, the one built */
+  int32_t DICIncOrNot, UseGlobal, GaussPtOrder, ClusterNo;
+  double tol_stereo, tol_temporal;
+  double mu, beta, alpha, tol2;
+} aldic_params;
+/* setDICparaDefaults.m's values (winsize, winstepsize unset). */
+int aldic_params_default(aldic_params* p);
+/* validateDICpara.m's rules on these fields; ALDIC_EINVAL with the
+ * reference's message when one fails. */
+int aldic_params_validate(const aldic_params* p);
+/* The search region [L R U D] of IntegerSearchQuadtree.m:52-90 and the
+ * repmat of funIntegerSearch.m:12 for ImgSeqNum (-1 stereo, >= 2 temporal);
+ * ALDIC_EINVAL where the reference would prompt with input(). */
+int aldic_params_search_region(const aldic_params* p, int32_t ImgSeqNum, int64_t region[4]);
+
 /* How the integer searches on this context (every aldic_integer_search*
  * and aldic_search_* entry) find the integer ZNCC peak; results follow
  * funIntegerSearch.m:154-169 either way.  ALDIC_SEARCH_DIRECT: direct sums
