@@ -821,6 +821,12 @@ class Plan:
         relative ||b|| rule, 1 the step-size rule ||dP|| < tol_dp px."""
         check(lib().aldic_plan_set_stopping(self.ptr, int(rule), float(tol_dp)))
 
+    def set_persistent(self, on: bool = True):
+        """Run the four fast size classes as one persistent launch fed by a
+        size-sorted work queue (aldic_plan_set_persistent)."""
+        check(lib().aldic_plan_set_persistent(self.ptr, 1 if on else 0))
+        return self
+
     def close(self):
         if getattr(self, "ptr", None):
             lib().aldic_plan_destroy(self.ptr)

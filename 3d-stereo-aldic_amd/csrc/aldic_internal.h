@@ -111,6 +111,10 @@ struct aldic_plan {
   // IC-GN stopping rule of the plan's solves (aldic_plan_set_stopping)
   int stop_rule = 0;
   float tol_dp = 0.0f;
+  // the fast classes as one persistent launch (aldic_plan_set_persistent),
+  // with its two work counters
+  int persistent = 0;
+  int* pqueue = nullptr;
 };
 
 // Stage timing: a pair of events around each launch of a stage on the

@@ -49,6 +49,17 @@ struct SolveArgs {
 };
 
 namespace aldic {
+// The persistent launch over the fast size classes: per class c < 4 its
+// node list, count and whether every subset has the class's full side;
+// queue: two zeroed ints (the block and the wave work counters).
+struct PersistArgs {
+  SolveArgs a;
+  const int* list[4];
+  int cnt[4];
+  int full[4];
+  int* queue;
+};
+hipError_t launch_persistent(int mode, const PersistArgs& P, int grid, hipStream_t s);
 int size_class(int S1, int S2);
 int class_side(int cls);  // the largest subset side of a fast-kernel class
 hipError_t launch_fast(int mode, int cls, int var, const SolveArgs& a, hipStream_t s);

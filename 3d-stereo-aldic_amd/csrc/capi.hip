@@ -152,6 +152,7 @@ int build_plan(aldic_ctx* ctx, const int32_t* winsize, int64_t n, int per_node, 
     if (p->cnt[c] > 0) HIP_TRY(hipMalloc(&p->sub_lists[c], sizeof(int) * p->cnt[c]));
   HIP_TRY(hipMalloc(&p->sub_lists[4], sizeof(int) * std::max<int64_t>(n, 1)));
   HIP_TRY(hipMalloc(&p->sub_cnt, sizeof(int) * 6));
+  HIP_TRY(hipMalloc(&p->pqueue, sizeof(int) * 2));
   return ALDIC_OK;
 }
 
@@ -165,6 +166,7 @@ void free_plan(aldic_plan* p) {
   if (p->defer_list) (void)hipFree(p->defer_list);
   if (p->defer_cnt2) (void)hipFree(p->defer_cnt2);
   if (p->defer_list2) (void)hipFree(p->defer_list2);
+  if (p->pqueue) (void)hipFree(p->pqueue);
 }
 
 // Launch sequence shared by IC-GN (mode 0) and Subpb1 (mode 1).
@@ -189,6 +191,48 @@ int largest_class(const int cnt[5]) {
   return big;
 }
 
+// The persistent variant: classes 0..3 in one launch on the context stream
+// (timed as the 65 px class's stage), then the workgroup kernel over the
+// > 65 px class and over the subsets the fast classes handed on.
+int run_solver_persistent(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
+  hipStream_t s = ctx->stream;
+  const int base = mode == 0 ? ALDIC_STAGE_ICGN_C0 : ALDIC_STAGE_SUBPB1_C0;
+  zero_ints<<<1, 64, 0, s>>>(p->defer_cnt, 1);
+  zero_ints<<<1, 64, 0, s>>>(p->pqueue, 2);
+  HIP_TRY(hipGetLastError());
+  a.defer_cnt = p->defer_cnt;
+  a.defer_list = p->defer_list;
+  aldic::PersistArgs P{};
+  P.a = a;
+  int64_t blocks = 0, waves = 0;
+  for (int c = 0; c < 4; c++) {
+    P.list[c] = p->lists[c];
+    P.cnt[c] = p->cnt[c];
+    P.full[c] = p->full[c];
+  }
+  P.queue = p->pqueue;
+  blocks = (int64_t)p->cnt[3] + p->cnt[2];
+  waves = (int64_t)p->cnt[1] + p->cnt[0];
+  // two resident blocks per CU; no more blocks than there is work for
+  int dev_cus = 0;
+  (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  const int64_t resident = 2 * (int64_t)std::max(dev_cus, 1);
+  const int grid = (int)std::min<int64_t>(resident, std::max<int64_t>(blocks, (waves + 3) / 4));
+  {
+    StageTimer t(ctx, base + 3);
+    HIP_TRY(aldic::launch_persistent(mode, P, grid, s));
+  }
+  if (p->cnt[4] > 0) {
+    StageTimer t(ctx, base + 4);
+    const int g4 = std::min(p->cnt[4], 2048);
+    HIP_TRY(aldic::launch_generic(mode, a, p->lists[4], nullptr, p->cnt[4], g4, s));
+  }
+  StageTimer t(ctx, base + 5);
+  const int g5 = (int)std::min<int64_t>(std::max<int64_t>(blocks + waves, 1), 1024);
+  HIP_TRY(aldic::launch_generic(mode, a, p->defer_list, p->defer_cnt, (int)p->n, g5, s));
+  return ALDIC_OK;
+}
+
 int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
   hipStream_t s = ctx->stream;
   RC_TRY(side_stream(ctx));
@@ -197,6 +241,7 @@ int run_solver(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_plan* p) {
     a.stop_rule = p->stop_rule;
     a.tol_dp = p->tol_dp;
   }
+  if (p->persistent) return run_solver_persistent(ctx, mode, a, p);
   const int base = mode == 0 ? ALDIC_STAGE_ICGN_C0 : ALDIC_STAGE_SUBPB1_C0;
   zero_ints<<<1, 64, 0, s>>>(p->defer_cnt, 1);
   zero_ints<<<1, 64, 0, s>>>(p->defer_cnt2, 1);
@@ -1117,6 +1162,13 @@ extern "C" int aldic_plan_set_variant(aldic_plan* plan, int32_t solver, int32_t 
       variant >= aldic::num_variants())
     return fail(ALDIC_EINVAL, "plan_set_variant: out of range");
   plan->var[solver][cls] = variant;
+  return ALDIC_OK;
+}
+
+extern "C" int aldic_plan_set_persistent(aldic_plan* plan, int32_t on) {
+  if (!plan || (on != 0 && on != 1))
+    return fail(ALDIC_EINVAL, "plan_set_persistent: on must be 0 or 1");
+  plan->persistent = on;
   return ALDIC_OK;
 }
 

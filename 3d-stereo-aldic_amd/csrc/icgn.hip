@@ -800,6 +800,76 @@ __global__ __launch_bounds__(fast_block<NW>(), WALK && NW > 1 ? 2 : 1) void fast
 }
 
 // ---------------------------------------------------------------------------
+// Persistent kernel: every fast size class in one launch.  The grid is what
+// stays resident (two 256-thread blocks per CU); a size-sorted work queue
+// feeds it: blocks take whole-block items (65 px, then 33 px subsets, four
+// waves each) from one counter while any are left, then each wave takes
+// single-wave items (17 px, then 11 px subsets) from a second counter, so
+// the small subsets fill the tail of the large ones.  Each subset runs the
+// same per-subset code as the per-class launches (bit-identical results)
+// and leaves its loop when it converges; masked and odd subsets are handed
+// on to the workgroup kernel as there.
+// ---------------------------------------------------------------------------
+constexpr int PK_TS3 = 84, PK_TS2 = 48, PK_TS1 = 32, PK_TS0 = 24;
+constexpr int PK_TILE = PK_TS3 * PK_TS3 + PK_TS3 + 1;  // floats: the largest block tile
+static_assert(4 * (PK_TS1 * PK_TS1 + PK_TS1 + 1) <= PK_TILE, "four wave tiles fit the block's");
+static_assert(PK_TS2 * PK_TS2 + PK_TS2 + 1 <= PK_TILE, "the 33 px block tile fits");
+
+template <int MODE, int BS>
+__global__ __launch_bounds__(256, 2) void persistent_kernel(aldic::PersistArgs P) {
+  __shared__ float tile[PK_TILE];
+  __shared__ float red[4 * 32];
+  __shared__ int item_s;
+  if (P.a.skip && *P.a.skip) return;
+  const int n3 = P.cnt[3], n2 = P.cnt[2], n1 = P.cnt[1], n0 = P.cnt[0];
+  SolveArgs a = P.a;
+  // whole-block items: classes 3 and 2
+  for (;;) {
+    if (threadIdx.x == 0) item_s = atomicAdd(&P.queue[0], 1);
+    __syncthreads();
+    const int it = item_s;
+    __syncthreads();
+    if (it >= n3 + n2) break;
+    if (it < n3) {
+      a.list = P.list[3];
+      if (P.full[3])
+        fast_subset<MODE, 4, 17, PK_TS3, 65, true, BS>(a, it, tile, red);
+      else
+        fast_subset<MODE, 4, 17, PK_TS3, 0, true, BS>(a, it, tile, red);
+    } else {
+      a.list = P.list[2];
+      if (P.full[2])
+        fast_subset<MODE, 4, 5, PK_TS2, 33, true, BS>(a, it - n3, tile, red);
+      else
+        fast_subset<MODE, 4, 5, PK_TS2, 0, true, BS>(a, it - n3, tile, red);
+    }
+    __syncthreads();  // the block's tile is free for the next item
+  }
+  // single-wave items: classes 1 and 0, a tile per wave
+  const int w = threadIdx.x / ALDIC_WAVE;
+  float* wt = tile + w * (PK_TS1 * PK_TS1 + PK_TS1 + 1);
+  for (;;) {
+    int it = 0;
+    if (threadIdx.x % ALDIC_WAVE == 0) it = atomicAdd(&P.queue[1], 1);
+    it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+    if (it >= n1 + n0) break;
+    if (it < n1) {
+      a.list = P.list[1];
+      if (P.full[1])
+        fast_subset<MODE, 1, 5, PK_TS1, 17, true, BS>(a, it, wt, red);
+      else
+        fast_subset<MODE, 1, 5, PK_TS1, 0, true, BS>(a, it, wt, red);
+    } else {
+      a.list = P.list[0];
+      if (P.full[0])
+        fast_subset<MODE, 1, 2, PK_TS0, 11, true, BS>(a, it - n1, wt, red);
+      else
+        fast_subset<MODE, 1, 2, PK_TS0, 0, true, BS>(a, it - n1, wt, red);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Generic kernel: any size, full mask semantics.  One workgroup per subset.
 // ---------------------------------------------------------------------------
 constexpr int GEN_THREADS = 256;
@@ -1484,6 +1554,22 @@ static hipError_t launch_fast_mode(int cls, int var, const SolveArgs& a, hipStre
 hipError_t launch_fast(int mode, int cls, int var, const SolveArgs& a, hipStream_t s) {
   return mode == 0 ? launch_fast_mode<MODE_ICGN>(cls, var, a, s)
                    : launch_fast_mode<MODE_SUBPB1>(cls, var, a, s);
+}
+
+hipError_t launch_persistent(int mode, const PersistArgs& P, int grid, hipStream_t s) {
+  static_assert(CLS_TS[0] == PK_TS0 && CLS_TS[1] == PK_TS1 && CLS_TS[2] == PK_TS2 &&
+                    CLS_TS[3] == PK_TS3,
+                "the persistent kernel's tiles are the classes'");
+  if (grid <= 0) return hipSuccess;
+  if (mode == 0 && P.a.basis)
+    persistent_kernel<MODE_ICGN, 1><<<grid, 256, 0, s>>>(P);
+  else if (mode == 0)
+    persistent_kernel<MODE_ICGN, 0><<<grid, 256, 0, s>>>(P);
+  else if (P.a.basis)
+    persistent_kernel<MODE_SUBPB1, 1><<<grid, 256, 0, s>>>(P);
+  else
+    persistent_kernel<MODE_SUBPB1, 0><<<grid, 256, 0, s>>>(P);
+  return hipGetLastError();
 }
 
 hipError_t launch_generic(int mode, const SolveArgs& a, const int* list, const int* count,

@@ -615,6 +615,29 @@ def run_workload(pkg, W, params, args, headline):
                 "fp32": {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4)},
                 "pass": "learned start, first IC-GN pass (<= 31 evaluations), step-size rule"}
+        # the same whole solve (every size class, the workgroup passes) as
+        # one launch per class on two streams (autotuned shapes) and as the
+        # persistent launch fed by the size-sorted queue: wall time per
+        # solve over ROOFLINE_REPS solves, the stream drained on both sides
+        plan_p = api.Plan(W["ws"], n, ctx=ctx).set_persistent(True)
+        plan_p.set_stopping(*RULES["step"])
+        solve = {}
+        for name, pl in (("per_class", plan), ("persistent", plan_p)):
+            def one():
+                api.local_icgn_f0_dev(im, pl, d_coords.ptr, bb["U0"].ptr, bb["F0"].ptr,
+                                      d_ws.ptr, 1e-3, bb["U"].ptr, bb["F"].ptr,
+                                      d_conv=bb["conv"].ptr, d_zncc=bb["z"].ptr, max_iter=30)
+            one()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(ROOFLINE_REPS):
+                one()
+            ctx.synchronize()
+            solve[name] = round((time.perf_counter() - t0) / ROOFLINE_REPS * 1e3, 4)
+        plan_p.close()
+        roof["whole_solve_ms"] = solve
+        progress(f"whole IC-GN solve: per class {solve['per_class']} ms, "
+                 f"persistent {solve['persistent']} ms")
     inet.close()
     for pl in (plan, plan2, plan3):
         pl.close()

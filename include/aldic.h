@@ -300,6 +300,15 @@ int aldic_plan_set_variant(aldic_plan* plan, int32_t solver, int32_t cls, int32_
  * number of evaluations, 101 if the cap comes first.  The ZNCC output is
  * that of the last evaluated warp, before the last update. */
 int aldic_plan_set_stopping(aldic_plan* plan, int32_t rule, double tol_dp);
+/* on = 1: the plan's solves (aldic_local_icgn*_dev, aldic_subpb1_dev) run
+ * the four fast size classes as one persistent launch fed by a size-sorted
+ * work queue instead of one launch per class on two streams.  Its launch
+ * shapes are fixed (four waves per 65 and per 33 px subset, one wave per 17
+ * and per 11 px subset: variants 0, 0, 2, 0 of classes 0..3; the plan's
+ * autotuned ones are not used); with the same shapes the per-class launches
+ * give bit-identical results.  No counterpart in the reference (its
+ * parfor). */
+int aldic_plan_set_persistent(aldic_plan* plan, int32_t on);
 
 /* Same as aldic_local_icgn_dev with a starting guess of the gradients:
  * P0 = [F0 U0] instead of [0 0 0 0 U0] (funICGNQuadtree.m:41).  d_F0 is 4n
