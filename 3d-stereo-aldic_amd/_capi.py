@@ -55,6 +55,8 @@ SIGNATURES = {
     "aldic_ctx_set_timing": (C.c_int, [_vp, C.c_int32]),
     "aldic_ctx_set_search_method": (C.c_int, [_vp, C.c_int32]),
     "aldic_plan_set_persistent": (C.c_int, [_vp, C.c_int32]),
+    "aldic_field_to_nodes_dev": (C.c_int, [_vp, _vp, _vp, _i64, _i64, C.c_double, C.c_double,
+                                           C.c_double, _vp, _i64, _vp, _vp]),
     "aldic_params_default": (C.c_int, [_prm]),
     "aldic_params_validate": (C.c_int, [_prm]),
     "aldic_params_search_region": (C.c_int, [_prm, C.c_int32, C.POINTER(_i64)]),

@@ -899,6 +899,14 @@ def bspline_prefilter_dev(ctx: Context, d_in, d_out, M, N):
     check(lib().aldic_bspline_prefilter_dev(ctx.ptr, d_in, d_out, int(M), int(N)))
 
 
+def field_to_nodes_dev(ctx: Context, d_Ug, d_Fg, nX, nY, x0, y0, step, d_coords, n, d_U0,
+                       d_F0=None):
+    """A warp field on a regular grid (point ci + cj*nX) -> its bilinear
+    interpolant at the nodes (aldic_field_to_nodes_dev)."""
+    check(lib().aldic_field_to_nodes_dev(ctx.ptr, d_Ug, d_Fg, int(nX), int(nY), float(x0),
+                                         float(y0), float(step), d_coords, int(n), d_U0, d_F0))
+
+
 def grid_to_nodes_dev(ctx: Context, d_u, d_v, nX, nY, x0, y0, step, d_coords, n, d_U0):
     check(lib().aldic_grid_to_nodes_dev(ctx.ptr, d_u, d_v, int(nX), int(nY), float(x0), float(y0),
                                         float(step), d_coords, int(n), d_U0))

@@ -131,6 +131,11 @@ hipError_t launch_search_points(const float* f, const float* g, int M, int N, in
                                 void* scratch, double* x, double* y, double* u, double* v,
                                 double* ccmax, int* ipr, int* ipc, hipStream_t s,
                                 int method = 0);
+// a warp field on a regular grid (point ci + cj*nX; U 2 and F 4 floats per
+// point) -> the bilinear interpolant at every node (F0 may be null)
+hipError_t launch_field_to_nodes(const float* Ug, const float* Fg, int nX, int nY, double x0,
+                                 double y0, double step, const double* coords, int n, float* U0,
+                                 float* F0, hipStream_t s);
 hipError_t launch_grid_to_listed_nodes(const double* gu, const double* gv, int nX, int nY,
                                        double x0, double y0, double step, const double* coords,
                                        int n, const int* list, const int* count, float* U0,

@@ -1461,6 +1461,21 @@ extern "C" int aldic_grid_to_nodes_dev(aldic_ctx* ctx, const double* d_u, const 
   return ALDIC_OK;
 }
 
+extern "C" int aldic_field_to_nodes_dev(aldic_ctx* ctx, const float* d_Ug, const float* d_Fg,
+                                        int64_t nX, int64_t nY, double x0, double y0, double step,
+                                        const double* d_coords, int64_t n, float* d_U0,
+                                        float* d_F0) {
+  if (!ctx || !d_Ug || !d_coords || !d_U0 || (d_F0 && !d_Fg))
+    return fail(ALDIC_EINVAL, "field_to_nodes: NULL argument");
+  if (nX < 1 || nY < 1 || n < 0 || step <= 0 || n > 0x7fffffff || nX * nY > 0x7fffffff)
+    return fail(ALDIC_EINVAL, "field_to_nodes: bad sizes");
+  RC_TRY(set_device(ctx));
+  StageTimer t(ctx, ALDIC_STAGE_GRID2NODES);
+  HIP_TRY(aldic::launch_field_to_nodes(d_Ug, d_Fg, (int)nX, (int)nY, x0, y0, step, d_coords,
+                                       (int)n, d_U0, d_F0, ctx->stream));
+  return ALDIC_OK;
+}
+
 // The same search with its per-point scratch in the file-scope array
 // `scratch` of this GPU instead of a context buffer: no allocation at all.
 extern "C" int aldic_integer_search_noalloc_dev(aldic_ctx* ctx, const float* d_f,

@@ -382,6 +382,35 @@ __global__ __launch_bounds__(256) void grid_to_nodes_kernel(const double* __rest
     for (int q = 0; q < 4; q++) F0[4 * k + q] = 0.0f;
 }
 
+// A warp field held at the points of a regular grid (point t = ci + cj*nX
+// at (x0 + ci*step, y0 + cj*step); Ug 2 and Fg 4 floats per point, the
+// solvers' interleaved layout) -> the bilinear interpolant at every node,
+// held constant beyond the outer points.
+__global__ __launch_bounds__(256) void field_to_nodes_kernel(const float* __restrict__ Ug,
+                                                             const float* __restrict__ Fg,
+                                                             int nX, int nY, double x0, double y0,
+                                                             double step,
+                                                             const double* __restrict__ coords,
+                                                             int n, float* __restrict__ U0,
+                                                             float* __restrict__ F0) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int ix, iy;
+  double tx, ty;
+  grid_cell(coords, n, k, nX, nY, x0, y0, step, &ix, &iy, &tx, &ty);
+  const int ix1 = min(ix + 1, nX - 1), iy1 = min(iy + 1, nY - 1);
+  const int64_t t00 = ix + (int64_t)iy * nX, t10 = ix1 + (int64_t)iy * nX;
+  const int64_t t01 = ix + (int64_t)iy1 * nX, t11 = ix1 + (int64_t)iy1 * nX;
+  const double w00 = (1 - tx) * (1 - ty), w10 = tx * (1 - ty), w01 = (1 - tx) * ty, w11 = tx * ty;
+  for (int q = 0; q < 2; q++)
+    U0[2 * k + q] = (float)(w00 * Ug[2 * t00 + q] + w10 * Ug[2 * t10 + q] +
+                            w01 * Ug[2 * t01 + q] + w11 * Ug[2 * t11 + q]);
+  if (F0 && Fg)
+    for (int q = 0; q < 4; q++)
+      F0[4 * k + q] = (float)(w00 * Fg[4 * t00 + q] + w10 * Fg[4 * t10 + q] +
+                              w01 * Fg[4 * t01 + q] + w11 * Fg[4 * t11 + q]);
+}
+
 // flags of the grid nodes the listed nodes interpolate from
 __global__ void mark_cells_kernel(int nX, int nY, double x0, double y0, double step,
                                   const double* __restrict__ coords, int n,
@@ -419,6 +448,15 @@ hipError_t launch_grid_to_nodes(const double* gu, const double* gv, int nX, int 
   if (n <= 0) return hipSuccess;
   grid_to_nodes_kernel<<<(n + 255) / 256, 256, 0, s>>>(gu, gv, nX, nY, x0, y0, step, coords, n,
                                                        nullptr, nullptr, U0, nullptr);
+  return hipGetLastError();
+}
+
+hipError_t launch_field_to_nodes(const float* Ug, const float* Fg, int nX, int nY, double x0,
+                                 double y0, double step, const double* coords, int n, float* U0,
+                                 float* F0, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  field_to_nodes_kernel<<<(n + 255) / 256, 256, 0, s>>>(Ug, Fg, nX, nY, x0, y0, step, coords, n,
+                                                        U0, F0);
   return hipGetLastError();
 }
 

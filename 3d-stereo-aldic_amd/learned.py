@@ -26,8 +26,8 @@ import time
 import numpy as np
 
 from ._capi import check, lib
-from .api import (Context, DeviceArray, Images, Plan, default_context, gate_dev,
-                  grid_to_nodes_dev, integer_search_dev, local_icgn_f0_dev,
+from .api import (Context, DeviceArray, Images, Plan, default_context, field_to_nodes_dev,
+                  gate_dev, grid_to_nodes_dev, integer_search_dev, local_icgn_f0_dev,
                   local_icgn_list_dev, search_grid_nodes_dev)
 
 PATCH = 48
@@ -390,14 +390,20 @@ class LearnedInit:
     context while IC-GN and the gate of frame k-1 run on ctx_icgn and the
     fallback of frame k-2 on ctx_fb (each solver context with its own plan
     of the same nodes).  The per-frame buffers are rotated over three sets;
-    outputs(k) returns frame k's (U, F, convIt, ZNCC) device arrays."""
+    outputs(k) returns frame k's (U, F, convIt, ZNCC) device arrays.
+
+    net_step (px, optional): the network runs on a regular grid of that
+    spacing over the nodes' bounding box instead of at every node, and its
+    [F0 U0] reach the nodes by bilinear interpolation
+    (aldic_field_to_nodes_dev) -- fewer network passes where the nodes are
+    denser than the field's variation."""
 
     NSETS = 3
 
     def __init__(self, net: Network, images: Images, d_f, d_g, plan: Plan, d_coords, n,
                  d_winsize, prior_grid, fallback_grid, region=(50, 50, 50, 50), max_iter=30,
                  min_zncc=0.9, tol=1e-3, ctx_fb: Context | None = None, plan_fb: Plan | None = None,
-                 ctx_icgn: Context | None = None):
+                 ctx_icgn: Context | None = None, net_step=None):
         ctx = images.ctx
         self.ctx = ctx
         self.ctx_icgn = ctx_icgn or ctx
@@ -422,9 +428,29 @@ class LearnedInit:
         self.g_ipr = DeviceArray(ctx, npnt, np.int32)
         self.g_ipc = DeviceArray(ctx, npnt, np.int32)
         nn = self.n
+        # the network's points: the nodes, or a regular grid over them
+        self.net_step = None if net_step is None else float(net_step)
+        self.n_net, self.d_net_coords = nn, d_coords
+        if self.net_step is not None:
+            xy = self._node_box()
+            xs = np.arange(xy[0], xy[1] + self.net_step, self.net_step)
+            ys = np.arange(xy[2], xy[3] + self.net_step, self.net_step)
+            self.nXn, self.nYn = len(xs), len(ys)
+            X, Y = np.meshgrid(xs, ys, indexing="xy")  # point ci + cj*nX
+            gc = np.concatenate([X.reshape(-1), Y.reshape(-1)]).astype(np.float64)
+            self.n_net = self.nXn * self.nYn
+            self.x0n, self.y0n = float(xs[0]), float(ys[0])
+            self.d_net_coords = DeviceArray.from_host(ctx, gc, np.float64)
         self.bufs = []
         for _ in range(self.NSETS):
-            self.bufs.append(dict(
+            if self.net_step is not None:
+                ng = self.n_net
+                grid_bufs = dict(prior_g=DeviceArray(ctx, 2 * ng, np.float32),
+                                 U0_g=DeviceArray(ctx, 2 * ng, np.float32),
+                                 F0_g=DeviceArray(ctx, 4 * ng, np.float32))
+            else:
+                grid_bufs = {}
+            self.bufs.append(dict(**grid_bufs,
                 prior=DeviceArray(ctx, 2 * nn, np.float32), U0=DeviceArray(ctx, 2 * nn, np.float32),
                 F0=DeviceArray(ctx, 4 * nn, np.float32), list=DeviceArray(ctx, max(nn, 1), np.int32),
                 count=DeviceArray(ctx, 1, np.int32), U=DeviceArray(ctx, 2 * nn, np.float32),
@@ -434,6 +460,13 @@ class LearnedInit:
     def outputs(self, k):
         b = self.bufs[k % self.NSETS]
         return b["U"], b["F"], b["conv"], b["z"]
+
+    def _node_box(self):
+        """(xmin, xmax, ymin, ymax) of the nodes (read once from the device)."""
+        c = np.empty(2 * self.n, dtype=np.float64)
+        check(lib().aldic_memcpy_dtoh(self.ctx.ptr, c.ctypes.data, self.d_coords, c.nbytes))
+        x, y = c[:self.n], c[self.n:]
+        return float(x.min()), float(x.max()), float(y.min()), float(y.max())
 
     # ---- the three stages of one frame, each on its own context ----
     def _stage_net(self, sb):
@@ -445,10 +478,20 @@ class LearnedInit:
                            self.g_ipc.ptr)
         x0 = self.gx[0] + self.pws / 2.0
         y0 = self.gy[0] + self.pws / 2.0
+        if self.net_step is None:
+            grid_to_nodes_dev(A, self.g["u"].ptr, self.g["v"].ptr, self.nX, self.nY, x0, y0,
+                              self.pstep, self.d_coords, self.n, b["prior"].ptr)
+            # 2. network -> [F0 U0]
+            self.net.predict_dev(im, self.d_coords, self.n, b["prior"].ptr, b["U0"].ptr,
+                                 b["F0"].ptr)
+            return
+        # 2'. the network on its grid, [F0 U0] interpolated to the nodes
         grid_to_nodes_dev(A, self.g["u"].ptr, self.g["v"].ptr, self.nX, self.nY, x0, y0,
-                          self.pstep, self.d_coords, self.n, b["prior"].ptr)
-        # 2. network -> [F0 U0]
-        self.net.predict_dev(im, self.d_coords, self.n, b["prior"].ptr, b["U0"].ptr, b["F0"].ptr)
+                          self.pstep, self.d_net_coords.ptr, self.n_net, b["prior_g"].ptr)
+        self.net.predict_dev(im, self.d_net_coords.ptr, self.n_net, b["prior_g"].ptr,
+                             b["U0_g"].ptr, b["F0_g"].ptr)
+        field_to_nodes_dev(A, b["U0_g"].ptr, b["F0_g"].ptr, self.nXn, self.nYn, self.x0n, self.y0n,
+                           self.net_step, self.d_coords, self.n, b["U0"].ptr, b["F0"].ptr)
 
     def _stage_icgn(self, sb):
         B, im, b = self.ctx_icgn, self.images, self.bufs[sb]
@@ -513,3 +556,14 @@ class LearnedInit:
 
     def fallback_count(self, k=0):
         return int(self.bufs[k % self.NSETS]["count"].to_host()[0])
+
+    def close(self):
+        """Free the device buffers (the caller synchronises first)."""
+        for b in self.bufs:
+            for a in b.values():
+                a.free()
+        self.bufs = []
+        for a in (*self.g.values(), self.g_ipr, self.g_ipc):
+            a.free()
+        if self.net_step is not None:
+            self.d_net_coords.free()

@@ -21,8 +21,10 @@ in HBM:
                  every node (tol 1e-3, <= 101 evaluations);
   learned start: sparse integer search (same templates and radius, 256 px
                  grid; direct sums, the default for a search this small) ->
-                 coarse prior at every node -> CNN on the 48 x 48
-                 patch pair at the node and at node + round(prior) -> [F0 U0]
+                 coarse prior on a 32 px grid over the nodes (--net-step)
+                 -> CNN on the 48 x 48 patch pair at each grid point and at
+                 point + round(prior) -> [F0 U0] bilinear to every node (the
+                 fields vary over tens of px, the nodes are 8 px apart)
                  -> IC-GN (at most 31 evaluations) -> gate (more than 30
                  evaluations or ZNCC < 0.9) -> the search start of the failed
                  nodes (only the grid points they read) -> IC-GN rerun.
@@ -43,7 +45,9 @@ subsets/s of the learned start with the step-size rule on the headline
 workload; the other starts, rules and the second workload sit beside it,
 with one frame of the incremental sequence (temporal IC-GN and the two
 augmented-Lagrangian iterations replayed as one HIP graph, `seq2048_alm`),
-each with its RMS error against the known field.  The roofline is the
+each with its RMS error against the known field; the learned start with
+the network at every node is timed beside the grid (`network_at_every_node`).
+The roofline is the
 dominant kernel of that step (its HIP-event time, algorithmic bytes); on
 rank 0 the FP64 CPU restatement (oracle, OpenMP) runs the search-start path
 with the reference's rule on the same pair.
@@ -520,7 +524,7 @@ def run_workload(pkg, W, params, args, headline):
                              prior_grid=([1, M], [1, N], sws, 256),
                              fallback_grid=([1, M], [1, N], sws, sst), region=(r, r, r, r),
                              max_iter=30, min_zncc=0.9, tol=1e-3, ctx_fb=ctx2, plan_fb=plan2,
-                             ctx_icgn=ctx3)
+                             ctx_icgn=ctx3, net_step=args.net_step or None)
     out = {}
     U_search_ref = None
     for rule, (code, tol_dp) in RULES.items():
@@ -543,6 +547,28 @@ def run_workload(pkg, W, params, args, headline):
         progress(f"{W['name']} learned start, {rule} rule: {dt * 1e3:.3f} ms, "
                  f"{acc['ok'].sum()} converged, rms {acc['rms']:.4f} px, fallback "
                  f"{li.fallback_count(last)}")
+    if args.net_step:
+        # ---- the network at every node instead of on its grid, step-size
+        # rule: what the grid saves, and that the start is as good ----
+        li_node = learned.LearnedInit(inet, im, d_f.ptr, d_g.ptr, plan3, d_coords.ptr, n, d_ws.ptr,
+                                      prior_grid=([1, M], [1, N], sws, 256),
+                                      fallback_grid=([1, M], [1, N], sws, sst),
+                                      region=(r, r, r, r), max_iter=30, min_zncc=0.9, tol=1e-3,
+                                      ctx_fb=ctx2, plan_fb=plan2, ctx_icgn=ctx3)
+        for pl in (plan2, plan3):
+            pl.set_stopping(*RULES["step"])
+        dt, st, last = timed([ctx, ctx2, ctx3], api, li_node.run, args.steps, args.warmup)
+        lU, lF, lconv, lz = li_node.outputs(last)
+        acc = accuracy(api, W, lU.to_host().astype(np.float64), lconv.to_host(), lz.to_host(),
+                       truth_mean)
+        out[("learned", "step_per_node")] = dict(dt=dt, stages=st, acc=acc,
+                                                 fallback=li_node.fallback_count(last))
+        progress(f"{W['name']} learned start at every node, step rule: {dt * 1e3:.3f} ms, "
+                 f"{acc['ok'].sum()} converged, rms {acc['rms']:.4f} px, fallback "
+                 f"{li_node.fallback_count(last)}")
+        for c in (ctx, ctx2, ctx3):
+            c.synchronize()
+        li_node.close()
     # ---- the interpolating cubic B-spline option (not the reference's
     # interpolant; no parity with it): both starts under the step-size rule.
     # Each frame needs its coefficient plane: the prefilter is timed on its
@@ -791,6 +817,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-alm", action="store_true", help="skip the incremental-sequence frame")
     ap.add_argument("--train-samples", type=int, default=2_000_000)
+    ap.add_argument("--net-step", type=float, default=32.0,
+                    help="run the network on a regular grid of this spacing (px) and "
+                         "interpolate its [F0 U0] to the nodes; 0: at every node.  The "
+                         "per-node network is timed beside it under the step-size rule")
     ap.add_argument("--train-trans", type=float, default=2.0,
                     help="training translations within +- this many px: the residual the "
                          "network predicts around the rounded prior (the prior's bilinear "
@@ -834,6 +864,7 @@ def main():
              "seconds": round(time.perf_counter() - t0, 1),
              "final_loss": round(hist["loss"][-1], 5) if hist["loss"] else None,
              "translation_range_px": args.train_trans,
+             "grid_step_px": args.net_step or None,
              "heldout_rms_u_px": round(ev["rms_u_px"], 4), "heldout_rms_F": round(ev["rms_F"], 5)}
     progress(f"network trained ({train['seconds']} s)")
     names = [w for w in args.workloads.split(",") if w]
@@ -870,6 +901,8 @@ def main():
         return blk
 
     spec = H["W"]["spec"]
+    net_where = (f"on a {args.net_step:g} px grid interpolated to the nodes" if args.net_step
+                 else "at every node")
     line = {
         "metric": "converged IC-GN subsets/s at <=0.01 px RMS, learned vs FFT init, 1 MI355X",
         "value": round(float(value), 1),
@@ -887,7 +920,7 @@ def main():
                                 f"(0.01 spots/px^2, seed 5), 25 px disparity + 15 px Gaussian "
                                 f"bulge (sigma 600 px), hole r=300, {n} nodes with 8-64 px "
                                 f"subsets; learned start (CNN on a 256 px sparse-search prior, "
-                                f"gate + search fallback) vs FFT ZNCC search start (r=50, "
+                                f"{net_where}, gate + search fallback) vs FFT ZNCC search start (r=50, "
                                 f"{H['meta']['grid'][0]}x{H['meta']['grid'][1]} grid); IC-GN "
                                 f"tol 1e-3"),
                    "init": "learned", "stop": f"step size < {TOL_DP} px", "nodes": n},
@@ -911,6 +944,13 @@ def main():
         "prefilter_ms": round(bsp["prefilter_dt"] * 1e3, 4),
         "rms_px": round(bsp["acc"]["rms"], 5), "rms_ok": bool(bsp["acc"]["rms"] <= 0.01),
         "learned_vs_search": line[head_name]["learned_vs_search_step_bspline_rule"]}
+    if ("learned", "step_per_node") in H["out"]:
+        pn = H["out"][("learned", "step_per_node")]
+        line["network_at_every_node"] = {
+            "note": "the same step with the network at every node instead of on its grid",
+            "value": round(float(pn["acc"]["ok"].sum() / pn["dt"]), 1),
+            "ms_per_step": round(pn["dt"] * 1e3, 4), "rms_px": round(pn["acc"]["rms"], 5),
+            "fallback": pn["fallback"], "grid_speedup": round(pn["dt"] / headd["dt"], 3)}
     if not args.no_alm:
         progress("incremental-sequence frame: temporal IC-GN + ALM graph")
         line["seq2048_alm"] = alm_block(pkg, args)

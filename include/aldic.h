@@ -352,6 +352,15 @@ int aldic_integer_search(aldic_ctx* ctx, const double* f, const double* g, int64
 int aldic_grid_to_nodes_dev(aldic_ctx* ctx, const double* d_u, const double* d_v, int64_t nX,
                             int64_t nY, double x0, double y0, double step,
                             const double* d_coords, int64_t n, float* d_U0);
+/* A warp field held on a regular grid (point t = ci + cj*nX at (x0 +
+ * ci*step, y0 + cj*step), FP32 device arrays in the solvers' layout: d_Ug
+ * 2 and d_Fg 4 values per point) -> its bilinear interpolant at every node
+ * (d_U0 2n, d_F0 4n, may be NULL), constant beyond the outer points.  The
+ * learned start uses it to run the network on a grid coarser than the
+ * nodes.  No counterpart in the reference. */
+int aldic_field_to_nodes_dev(aldic_ctx* ctx, const float* d_Ug, const float* d_Fg, int64_t nX,
+                             int64_t nY, double x0, double y0, double step,
+                             const double* d_coords, int64_t n, float* d_U0, float* d_F0);
 
 /* Device-resident integer search on FP32 planes d_f, d_g (M x N); outputs
  * are device arrays of nX*nY entries (nY x nX column-major).  The per-point
