@@ -818,7 +818,10 @@ class Plan:
 
     def set_stopping(self, rule: int, tol_dp: float = 0.0):
         """IC-GN stopping rule of this plan's solves: 0 the reference's
-        relative ||b|| rule, 1 the step-size rule ||dP|| < tol_dp px."""
+        relative ||b|| rule, 1 the step-size rule ||dP|| < tol_dp px, 2 its
+        a-posteriori form (also converged once the contraction observed
+        over the last two updates predicts a distance to the fixed point
+        below tol_dp px)."""
         check(lib().aldic_plan_set_stopping(self.ptr, int(rule), float(tol_dp)))
 
     def set_persistent(self, on: bool = True):

@@ -53,14 +53,18 @@ __device__ __forceinline__ int64_t mround(double v) { return (int64_t)round(v); 
 // coefficients a_ij = rot90(BC*Q*BC',2) (ba_interp2_spline.cpp:310-325,
 // func/BicubicBsplineBuild.m:13-16) factor as sum_rc Q[r][c] w_r(dx) w_c(dy)
 // with 56*w_k(t) the cubics below (taps -1, 0, +1, +2).
-// The same weights times 56 (three FMA-class operations each).  A value
+// The same weights times 56.  With m = t (1 - t) the cubics factor as
+// 56 w_-1 = m (24 t - 33), 56 w_+2 = m (-24 t - 9), 56 w_+1 = 56 t +
+// m (64 t - 11) and 56 w_0 = 56 + 42 m - 56 w_+1 (the four sum to 56): ten
+// FMA-class operations for the four instead of twelve by Horner.  A value
 // built from them is 3136 times the interpolant, which the zero-normalised
 // residual does not see.
 __device__ __forceinline__ void spline_weights56(float t, float w[4]) {
-  w[0] = t * fmaf(t, fmaf(-24.0f, t, 57.0f), -33.0f);
-  w[1] = fmaf(t, fmaf(t, fmaf(64.0f, t, -117.0f), -3.0f), 56.0f);
-  w[2] = t * fmaf(t, fmaf(-64.0f, t, 75.0f), 45.0f);
-  w[3] = t * fmaf(t, fmaf(24.0f, t, -15.0f), -9.0f);
+  const float m = fmaf(-t, t, t);
+  w[0] = m * fmaf(24.0f, t, -33.0f);
+  w[3] = m * fmaf(-24.0f, t, -9.0f);
+  w[2] = fmaf(m, fmaf(64.0f, t, -11.0f), 56.0f * t);
+  w[1] = fmaf(42.0f, m, 56.0f) - w[2];
 }
 
 __device__ __forceinline__ void spline_weights(float t, float w[4]) {

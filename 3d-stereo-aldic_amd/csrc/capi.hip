@@ -1173,12 +1173,13 @@ extern "C" int aldic_plan_set_persistent(aldic_plan* plan, int32_t on) {
 }
 
 extern "C" int aldic_plan_set_stopping(aldic_plan* plan, int32_t rule, double tol_dp) {
-  if (!plan || rule < 0 || rule > 1)
-    return fail(ALDIC_EINVAL, "plan_set_stopping: rule must be 0 (reference) or 1 (step size)");
-  if (rule == 1 && !(tol_dp > 0.0 && tol_dp < 1e3))
+  if (!plan || rule < 0 || rule > 2)
+    return fail(ALDIC_EINVAL,
+                "plan_set_stopping: rule must be 0 (reference), 1 (step size) or 2 (a-posteriori)");
+  if (rule >= 1 && !(tol_dp > 0.0 && tol_dp < 1e3))
     return fail(ALDIC_EINVAL, "plan_set_stopping: the step-size threshold must be positive");
   plan->stop_rule = rule;
-  plan->tol_dp = rule == 1 ? (float)tol_dp : 0.0f;
+  plan->tol_dp = rule >= 1 ? (float)tol_dp : 0.0f;
   return ALDIC_OK;
 }
 

@@ -186,10 +186,22 @@ __device__ __forceinline__ bool ic_update(float P[6], const float d[6]) {
 // of squares of the centre shift and the gradient terms at the half widths
 // (hx, hy): sqrt(du^2 + dv^2 + (du/dx^2 + dv/dx^2) hx^2 + (du/dy^2 +
 // dv/dy^2) hy^2) < tol_dp px.
-__device__ __forceinline__ bool dp_small(const float d[6], float hx, float hy, float tol_dp) {
-  const float m = d[4] * d[4] + d[5] * d[5] + (d[0] * d[0] + d[1] * d[1]) * hx * hx +
-                  (d[2] * d[2] + d[3] * d[3]) * hy * hy;
-  return m < tol_dp * tol_dp;
+//
+// rule 1: converged when that measure m_k of the k-th update is below tol_dp.
+// rule 2 (a-posteriori): also converged when the iteration contracts (m_k <=
+// m_{k-1} / 2) and the distance to its fixed point that the observed ratio q
+// = m_k / m_{k-1} predicts, m_k q / (1 - q), is below tol_dp, i.e. m_k^2 <
+// tol_dp (m_{k-1} - m_k).  *m_prev carries m_{k-1} (negative before the
+// first update).
+__device__ __forceinline__ bool dp_small(int rule, const float d[6], float hx, float hy,
+                                         float tol_dp, float* m_prev) {
+  const float m2 = d[4] * d[4] + d[5] * d[5] + (d[0] * d[0] + d[1] * d[1]) * hx * hx +
+                   (d[2] * d[2] + d[3] * d[3]) * hy * hy;
+  if (m2 < tol_dp * tol_dp) return true;
+  if (rule != 2) return false;
+  const float m = __builtin_amdgcn_sqrtf(m2), mp = *m_prev;
+  *m_prev = m;
+  return mp > 0.0f && 2.0f * m <= mp && m2 < tol_dp * (mp - m);
 }
 
 // ---------------------------------------------------------------------------
@@ -356,27 +368,46 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   // their sums need no lane mask; the slots of the last, partial round are
   // masked and their offsets set to the centre so every gather stays inside
   // the tile.
+  // All loads of the subset are issued before any of them is used (one
+  // memory latency for the whole subset): each slot reads at a 32-bit
+  // offset from the window's first pixel, which an empty slot reads too and
+  // then discards.
   const int kfull = NS / NT;
   float pf[PPL], pgx[PPL], pgy[PPL], pdx[PPL], pdy[PPL], pg[PPL];
   int zf = 0, nanf = 0, tinyf = 0;
+  {
+    uint32_t off[PPL];
 #pragma unroll
-  for (int k = 0; k < PPL; k++) {
-    const int p = tid + NT * k;
-    const int aa = p % S1, bb = p / S1;
-    const bool v = p < NS;
-    pdx[k] = v ? (float)(aa - hx) : 0.0f;
-    pdy[k] = v ? (float)(bb - hy) : 0.0f;
-    pf[k] = pgx[k] = pgy[k] = 0.0f;
-    if (v && in_img) {
-      const int64_t idx = (int64_t)(x0 - hx - 1 + aa) + (int64_t)(y0 - hy - 1 + bb) * M;
-      pf[k] = a.fm[idx];
+    for (int k = 0; k < PPL; k++) {
+      const int p = tid + NT * k;
+      const int aa = p % S1, bb = p / S1;
+      const bool v = p < NS;
+      pdx[k] = v ? (float)(aa - hx) : 0.0f;
+      pdy[k] = v ? (float)(bb - hy) : 0.0f;
+      off[k] = v ? (uint32_t)(aa + bb * M) : 0u;
+      pf[k] = pgx[k] = pgy[k] = 0.0f;
+    }
+    if (in_img) {  // uniform
+      const int64_t w0 = (int64_t)(x0 - hx - 1) + (int64_t)(y0 - hy - 1) * M;
+      const float* fb = a.fm + w0;
+#pragma unroll
+      for (int k = 0; k < PPL; k++) pf[k] = fb[off[k]];
       if (in_df) {
-        pgx[k] = a.fx[idx];
-        pgy[k] = a.fy[idx];
+        const float *xb = a.fx + w0, *yb = a.fy + w0;
+#pragma unroll
+        for (int k = 0; k < PPL; k++) {
+          pgx[k] = xb[off[k]];
+          pgy[k] = yb[off[k]];
+        }
       }
-      zf += (pf[k] == 0.0f);
-      tinyf |= (fabsf(pf[k]) <= 1e-10f);
-      nanf |= (pf[k] != pf[k]);
+#pragma unroll
+      for (int k = 0; k < PPL; k++) {
+        if (k >= kfull && tid + NT * k >= NS) pf[k] = pgx[k] = pgy[k] = 0.0f;
+        const bool v = k < kfull || tid + NT * k < NS;
+        zf += (v && pf[k] == 0.0f);
+        tinyf |= (v && fabsf(pf[k]) <= 1e-10f);
+        nanf |= (pf[k] != pf[k]);
+      }
     }
   }
   {
@@ -518,8 +549,9 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   int step = 0;
   // step-size rule (IC-GN only): converged once the applied update moves
   // the subset by less than a.tol_dp px (centre and corners, see dp_small)
-  const bool dp_rule = MODE == MODE_ICGN && a.stop_rule == 1;
+  const bool dp_rule = MODE == MODE_ICGN && a.stop_rule >= 1;
   bool dp_conv = false;
+  float m_prev = -1.0f;
   int tx0 = -1000000, ty0 = -1000000;  // tile origin (0-based image index)
   float last_zncc = __builtin_nanf("");
   bool err = false;
@@ -748,7 +780,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     if (!ic_update(P, dP)) break;
 #pragma unroll
     for (int k = 0; k < 6; k++) P[k] = uni(P[k]);
-    if (dp_rule && dp_small(dP, (float)hx, (float)hy, a.tol_dp)) {
+    if (dp_rule && dp_small(a.stop_rule, dP, (float)hx, (float)hy, a.tol_dp, &m_prev)) {
       dp_conv = true;
       break;
     }
@@ -1193,8 +1225,9 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
       const int maxit = a.maxit > 0 ? a.maxit : 100;
       float normNew = 1.0f, normAbs = 1.0f, normInit = 0.0f;
       int step = 0;
-      const bool dp_rule = MODE == MODE_ICGN && a.stop_rule == 1;
+      const bool dp_rule = MODE == MODE_ICGN && a.stop_rule >= 1;
       bool dp_conv = false;
+      float m_prev = -1.0f;
       bool have_bwg = false, err = false, defer_oob = false, have_f8 = false;
       float sum_bwg = 0.0f;
       const int ws0 = wx0;
@@ -1414,7 +1447,8 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
         }
         if (!ic_update(P, dP)) break;
         __syncthreads();
-        if (dp_rule && dp_small(dP, 0.5f * (float)(S1 - 1), 0.5f * (float)(S2 - 1), a.tol_dp)) {
+        if (dp_rule && dp_small(a.stop_rule, dP, 0.5f * (float)(S1 - 1), 0.5f * (float)(S2 - 1),
+                                a.tol_dp, &m_prev)) {
           dp_conv = true;
           break;
         }

@@ -29,9 +29,12 @@ in HBM:
                  evaluations or ZNCC < 0.9) -> the search start of the failed
                  nodes (only the grid points they read) -> IC-GN rerun.
 
-Each start runs with two IC-GN stopping rules: the reference's relative
-||b|| rule (parity) and the step-size rule (the update moves the subset by
-less than 1e-3 px; aldic_plan_set_stopping).  The network is trained first
+Each start runs with three IC-GN stopping rules: the reference's relative
+||b|| rule (parity), the step-size rule (the update moves the subset by
+less than 1e-3 px) and its a-posteriori form (also converged once the
+contraction of the last two updates predicts a distance to the fixed point
+below 1e-3 px; aldic_plan_set_stopping), each with its distance from the
+reference rule's answers.  The network is trained first
 (not timed) on the device, on 2M patch pairs labelled by IC-GN on seeded
 speckle mosaics, with tile translations within +-2 px: the network predicts
 the residual around the rounded sparse-search prior, whose interpolation
@@ -41,7 +44,7 @@ gate and the search fallback).
 Converged = status 1..100, ZNCC >= 0.9 and not a bad point by the
 reference's rule (ConvIt > 99 or > max(mean + std, 6) over the good ones,
 LocalICGNQuadtree.m:167-192).  The JSON line's value is the converged
-subsets/s of the learned start with the step-size rule on the headline
+subsets/s of the learned start with the a-posteriori rule on the headline
 workload; the other starts, rules and the second workload sit beside it,
 with one frame of the incremental sequence (temporal IC-GN and the two
 augmented-Lagrangian iterations replayed as one HIP graph, `seq2048_alm`),
@@ -452,7 +455,8 @@ def search_mode(pkg, args):
     print(json.dumps(out), flush=True)
 
 
-RULES = {"reference": (0, 0.0), "step": (1, TOL_DP)}
+RULES = {"reference": (0, 0.0), "step": (1, TOL_DP), "posterior": (2, TOL_DP)}
+HEAD_RULE = "posterior"  # the rule of the headline value and the side runs
 
 
 def run_workload(pkg, W, params, args, headline):
@@ -534,21 +538,22 @@ def run_workload(pkg, W, params, args, headline):
         b = sets[last % 2]
         U = b["U"].to_host().astype(np.float64)
         acc = accuracy(api, W, U, b["conv"].to_host(), b["z"].to_host(), truth_mean)
-        out[("search", rule)] = dict(dt=dt, stages=st, acc=acc)
+        out[("search", rule)] = dict(dt=dt, stages=st, acc=acc, U=U)
         if rule == "reference":
             U_search_ref = U
         progress(f"{W['name']} search start, {rule} rule: {dt * 1e3:.3f} ms, "
                  f"{acc['ok'].sum()} converged, rms {acc['rms']:.4f} px")
         dt, st, last = timed([ctx, ctx2, ctx3], api, li.run, args.steps, args.warmup)
         lU, lF, lconv, lz = li.outputs(last)
-        acc = accuracy(api, W, lU.to_host().astype(np.float64), lconv.to_host(), lz.to_host(),
-                       truth_mean)
-        out[("learned", rule)] = dict(dt=dt, stages=st, acc=acc, fallback=li.fallback_count(last))
+        Ul = lU.to_host().astype(np.float64)
+        acc = accuracy(api, W, Ul, lconv.to_host(), lz.to_host(), truth_mean)
+        out[("learned", rule)] = dict(dt=dt, stages=st, acc=acc, fallback=li.fallback_count(last),
+                                      U=Ul)
         progress(f"{W['name']} learned start, {rule} rule: {dt * 1e3:.3f} ms, "
                  f"{acc['ok'].sum()} converged, rms {acc['rms']:.4f} px, fallback "
                  f"{li.fallback_count(last)}")
     if args.net_step:
-        # ---- the network at every node instead of on its grid, step-size
+        # ---- the network at every node instead of on its grid, headline
         # rule: what the grid saves, and that the start is as good ----
         li_node = learned.LearnedInit(inet, im, d_f.ptr, d_g.ptr, plan3, d_coords.ptr, n, d_ws.ptr,
                                       prior_grid=([1, M], [1, N], sws, 256),
@@ -556,21 +561,21 @@ def run_workload(pkg, W, params, args, headline):
                                       region=(r, r, r, r), max_iter=30, min_zncc=0.9, tol=1e-3,
                                       ctx_fb=ctx2, plan_fb=plan2, ctx_icgn=ctx3)
         for pl in (plan2, plan3):
-            pl.set_stopping(*RULES["step"])
+            pl.set_stopping(*RULES[HEAD_RULE])
         dt, st, last = timed([ctx, ctx2, ctx3], api, li_node.run, args.steps, args.warmup)
         lU, lF, lconv, lz = li_node.outputs(last)
         acc = accuracy(api, W, lU.to_host().astype(np.float64), lconv.to_host(), lz.to_host(),
                        truth_mean)
-        out[("learned", "step_per_node")] = dict(dt=dt, stages=st, acc=acc,
+        out[("learned", "posterior_per_node")] = dict(dt=dt, stages=st, acc=acc,
                                                  fallback=li_node.fallback_count(last))
-        progress(f"{W['name']} learned start at every node, step rule: {dt * 1e3:.3f} ms, "
+        progress(f"{W['name']} learned start at every node, a-posteriori rule: {dt * 1e3:.3f} ms, "
                  f"{acc['ok'].sum()} converged, rms {acc['rms']:.4f} px, fallback "
                  f"{li_node.fallback_count(last)}")
         for c in (ctx, ctx2, ctx3):
             c.synchronize()
         li_node.close()
     # ---- the interpolating cubic B-spline option (not the reference's
-    # interpolant; no parity with it): both starts under the step-size rule.
+    # interpolant; no parity with it): both starts under the headline rule.
     # Each frame needs its coefficient plane: the prefilter is timed on its
     # own and added to the step ----
     d_gc = DA(ctx, M * N, np.float32)
@@ -579,31 +584,31 @@ def run_workload(pkg, W, params, args, headline):
     d_gc.free()
     im.set_interp("bspline")
     for pl in (plan2, plan3):
-        pl.set_stopping(*RULES["step"])
+        pl.set_stopping(*RULES[HEAD_RULE])
     dt, st, last = timed([ctx, ctx2], api, step_search, args.steps, args.warmup)
     b = sets[last % 2]
     acc = accuracy(api, W, b["U"].to_host().astype(np.float64), b["conv"].to_host(),
                    b["z"].to_host(), truth_mean)
-    out[("search", "step_bspline")] = dict(dt=dt + dtp, stages=st, acc=acc, prefilter_dt=dtp)
-    progress(f"{W['name']} search start, step rule, B-spline: {(dt + dtp) * 1e3:.3f} ms "
+    out[("search", "posterior_bspline")] = dict(dt=dt + dtp, stages=st, acc=acc, prefilter_dt=dtp)
+    progress(f"{W['name']} search start, a-posteriori rule, B-spline: {(dt + dtp) * 1e3:.3f} ms "
              f"(prefilter {dtp * 1e3:.3f} ms), rms {acc['rms']:.4f} px")
     dt, st, last = timed([ctx, ctx2, ctx3], api, li.run, args.steps, args.warmup)
     lU, lF, lconv, lz = li.outputs(last)
     acc = accuracy(api, W, lU.to_host().astype(np.float64), lconv.to_host(), lz.to_host(),
                    truth_mean)
-    out[("learned", "step_bspline")] = dict(dt=dt + dtp, stages=st, acc=acc, prefilter_dt=dtp,
+    out[("learned", "posterior_bspline")] = dict(dt=dt + dtp, stages=st, acc=acc, prefilter_dt=dtp,
                                            fallback=li.fallback_count(last))
-    progress(f"{W['name']} learned start, step rule, B-spline: {(dt + dtp) * 1e3:.3f} ms, "
+    progress(f"{W['name']} learned start, a-posteriori rule, B-spline: {(dt + dtp) * 1e3:.3f} ms, "
              f"rms {acc['rms']:.4f} px, fallback {li.fallback_count(last)}")
     im.set_interp("local")
     for c in (ctx, ctx2, ctx3):
         c.synchronize()
     # ---- roofline of the headline step's dominant kernel: the learned
-    # start's first IC-GN pass (step-size rule), ROOFLINE_REPS launches
+    # start's first IC-GN pass (a-posteriori rule), ROOFLINE_REPS launches
     # alone on one stream, HIP events on that stream ----
     roof = None
     if headline:
-        plan.set_stopping(*RULES["step"])
+        plan.set_stopping(*RULES[HEAD_RULE])
         li._stage_net(0)
         ctx.synchronize()
         bb = li.bufs[0]
@@ -640,13 +645,14 @@ def run_workload(pkg, W, params, args, headline):
                 "algorithmic_bytes": nbytes, "subsets": units, "mean_evaluations": round(evals, 3),
                 "fp32": {"achieved": round(tflops, 2), "peak": FP32_PEAK_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 4)},
-                "pass": "learned start, first IC-GN pass (<= 31 evaluations), step-size rule"}
+                "pass": "learned start, first IC-GN pass (<= 31 evaluations), a-posteriori "
+                        "step-size rule"}
         # the same whole solve (every size class, the workgroup passes) as
         # one launch per class on two streams (autotuned shapes) and as the
         # persistent launch fed by the size-sorted queue: wall time per
         # solve over ROOFLINE_REPS solves, the stream drained on both sides
         plan_p = api.Plan(W["ws"], n, ctx=ctx).set_persistent(True)
-        plan_p.set_stopping(*RULES["step"])
+        plan_p.set_stopping(*RULES[HEAD_RULE])
         solve = {}
         for name, pl in (("per_class", plan), ("persistent", plan_p)):
             def one():
@@ -820,7 +826,7 @@ def main():
     ap.add_argument("--net-step", type=float, default=32.0,
                     help="run the network on a regular grid of this spacing (px) and "
                          "interpolate its [F0 U0] to the nodes; 0: at every node.  The "
-                         "per-node network is timed beside it under the step-size rule")
+                         "per-node network is timed beside it under the a-posteriori rule")
     ap.add_argument("--train-trans", type=float, default=2.0,
                     help="training translations within +- this many px: the residual the "
                          "network predicts around the rounded prior (the prior's bilinear "
@@ -881,7 +887,7 @@ def main():
     head_name = names[0]
     H = results[head_name]
     n = H["meta"]["nodes"]
-    headd = H["out"][("learned", "step")]
+    headd = H["out"][("learned", HEAD_RULE)]
     value = headd["acc"]["ok"].sum() / headd["dt"]
 
     def block(res):
@@ -889,7 +895,16 @@ def main():
         blk = {}
         for (start, rule), d in res["out"].items():
             blk[f"{start}_start_{rule}_rule"] = summary(d, nn, args.steps)
-        for rule in (*RULES, "step_bspline"):
+            ref = res["out"].get((start, "reference"))
+            if rule in RULES and rule != "reference" and ref is not None:
+                # how far the rule's answer lies from the reference rule's
+                # on the nodes both count as converged (px, per node |dU|)
+                both = d["acc"]["ok"] & ref["acc"]["ok"]
+                du = np.abs(d["U"] - ref["U"]).reshape(-1, 2)[both].max(axis=1)
+                blk[f"{start}_start_{rule}_rule"]["du_vs_reference_rule_px"] = {
+                    "nodes": int(both.sum()), "mean": float(du.mean()),
+                    "p99": float(np.percentile(du, 99)), "max": float(du.max())}
+        for rule in (*RULES, "posterior_bspline"):
             a = res["out"][("learned", rule)]
             b = res["out"][("search", rule)]
             blk[f"learned_vs_search_{rule}_rule"] = round(
@@ -923,7 +938,8 @@ def main():
                                 f"{net_where}, gate + search fallback) vs FFT ZNCC search start (r=50, "
                                 f"{H['meta']['grid'][0]}x{H['meta']['grid'][1]} grid); IC-GN "
                                 f"tol 1e-3"),
-                   "init": "learned", "stop": f"step size < {TOL_DP} px", "nodes": n},
+                   "init": "learned", "stop": f"a-posteriori step size < {TOL_DP} px",
+                   "nodes": n},
         "converged": int(headd["acc"]["ok"].sum()),
         "converged_frac": round(float(headd["acc"]["ok"].sum()) / n, 5),
         "rms_px": round(headd["acc"]["rms"], 5),
@@ -934,18 +950,18 @@ def main():
     line[head_name] = block(H)
     for name in names[1:]:
         line[name] = block(results[name])
-    line["learned_vs_fft"] = line[head_name]["learned_vs_search_step_rule"]
-    bsp = H["out"][("learned", "step_bspline")]
+    line["learned_vs_fft"] = line[head_name][f"learned_vs_search_{HEAD_RULE}_rule"]
+    bsp = H["out"][("learned", "posterior_bspline")]
     line["interp_bspline"] = {
         "note": "option, not the reference's interpolant: the interpolating cubic B-spline on "
-                "prefiltered coefficients; learned start, step-size rule, prefilter included",
+                "prefiltered coefficients; learned start, a-posteriori rule, prefilter included",
         "value": round(float(bsp["acc"]["ok"].sum() / bsp["dt"]), 1),
         "ms_per_step": round(bsp["dt"] * 1e3, 4),
         "prefilter_ms": round(bsp["prefilter_dt"] * 1e3, 4),
         "rms_px": round(bsp["acc"]["rms"], 5), "rms_ok": bool(bsp["acc"]["rms"] <= 0.01),
-        "learned_vs_search": line[head_name]["learned_vs_search_step_bspline_rule"]}
-    if ("learned", "step_per_node") in H["out"]:
-        pn = H["out"][("learned", "step_per_node")]
+        "learned_vs_search": line[head_name]["learned_vs_search_posterior_bspline_rule"]}
+    if ("learned", "posterior_per_node") in H["out"]:
+        pn = H["out"][("learned", "posterior_per_node")]
         line["network_at_every_node"] = {
             "note": "the same step with the network at every node instead of on its grid",
             "value": round(float(pn["acc"]["ok"].sum() / pn["dt"]), 1),

@@ -297,8 +297,15 @@ int aldic_plan_set_variant(aldic_plan* plan, int32_t solver, int32_t cls, int32_
  * and the subset is converged when it moved the subset by less than tol_dp
  * px, sqrt(du^2 + dv^2 + (du/dx^2 + dv/dx^2) hx^2 + (du/dy^2 + dv/dy^2)
  * hy^2) < tol_dp with (hx, hy) the subset's half widths; the status is the
- * number of evaluations, 101 if the cap comes first.  The ZNCC output is
- * that of the last evaluated warp, before the last update. */
+ * number of evaluations, 101 if the cap comes first.  rule 2: the
+ * a-posteriori form of rule 1, no counterpart in the reference either:
+ * with m_k that measure of the k-th update, converged when m_k < tol_dp or
+ * when the iteration contracts (m_k <= m_{k-1} / 2) and the distance to its
+ * fixed point predicted from the observed ratio q = m_k / m_{k-1},
+ * m_k q / (1 - q), is below tol_dp (m_k^2 < tol_dp (m_{k-1} - m_k)): it
+ * saves the evaluation rule 1 spends confirming a step it could predict.
+ * The ZNCC output is that of the last evaluated warp, before the last
+ * update. */
 int aldic_plan_set_stopping(aldic_plan* plan, int32_t rule, double tol_dp);
 /* on = 1: the plan's solves (aldic_local_icgn*_dev, aldic_subpb1_dev) run
  * the four fast size classes as one persistent launch fed by a size-sorted

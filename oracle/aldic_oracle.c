@@ -667,8 +667,9 @@ int orc_icgn_subset_ex(const orc_images* im, int64_t x0, int64_t y0, int winsize
 
     double normNew = 1.0, normAbs = 1.0, normInit = 0.0;
     int step = 0;
-    const int dp_rule = (stop_rule == 1);
+    const int dp_rule = (stop_rule == 1 || stop_rule == 2);
     int dp_conv = 0;
+    double m_prev = -1.0; /* measure of the previous update (rule 2) */
     int have_bwg = 0;
     int64_t sum_bwg = 0;
     double last_zncc = NAN;
@@ -795,6 +796,17 @@ int orc_icgn_subset_ex(const orc_images* im, int64_t x0, int64_t y0, int winsize
         if (m < tol_dp * tol_dp) {
           dp_conv = 1;
           break;
+        }
+        if (stop_rule == 2) {
+          /* the a-posteriori form: the iteration contracts by q = m_k /
+           * m_{k-1} <= 1/2 and the distance to its fixed point that q
+           * predicts, m_k q / (1 - q), is below tol_dp */
+          const double mk = sqrt(m), mp = m_prev;
+          m_prev = mk;
+          if (mp > 0.0 && 2.0 * mk <= mp && m < tol_dp * (mp - mk)) {
+            dp_conv = 1;
+            break;
+          }
         }
       }
     }

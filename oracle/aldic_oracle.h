@@ -80,7 +80,9 @@ int orc_icgn_subset(const orc_images* im, int64_t x0, int64_t y0, int winsize,
  * reference's cap is 100, so at most maxit + 1 evaluations); stop_rule 0
  * is the reference's relative ||b|| rule, 1 the step-size rule (the
  * applied update moves the subset by less than tol_dp px, the same measure
- * as the library's aldic_plan_set_stopping); F0 (NULL = zeros) starts P at
+ * as the library's aldic_plan_set_stopping), 2 its a-posteriori form (also
+ * converged when m_k <= m_{k-1} / 2 and m_k^2 < tol_dp (m_{k-1} - m_k), m_k
+ * the measure of the k-th update); F0 (NULL = zeros) starts P at
  * [F0 U0].  Used to read the warp after k updates and to check the learned
  * starting point's IC-GN; statuses as above. */
 int orc_icgn_subset_ex(const orc_images* im, int64_t x0, int64_t y0, int winsize,

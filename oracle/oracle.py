@@ -202,7 +202,9 @@ def local_icgn(im: Images, coords, U0, winsize, tol, nthreads=1, F0=None, max_it
     F0 (4n, optional) starts each node at P0 = [F0 U0]; max_iter (>= 0)
     runs the loop while k <= max_iter (None: the reference's 100), i.e. at
     most max_iter + 1 evaluations; stop_rule 1 replaces the reference's
-    relative ||b|| rule by the step-size rule ||dP|| < tol_dp px.
+    relative ||b|| rule by the step-size rule ||dP|| < tol_dp px, stop_rule 2
+    by its a-posteriori form (also converged when the last two updates'
+    measures satisfy m_k <= m_{k-1} / 2 and m_k^2 < tol_dp (m_{k-1} - m_k)).
     Returns U (2n), F (4n), H (n x 21), convIt (n), markHole (n), zncc (n)."""
     coords = _F(coords)
     n = coords.shape[0]

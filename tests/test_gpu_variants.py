@@ -294,20 +294,21 @@ def test_subpb1_mask_branch(pkg, orc):
     im.close()
 
 
-@pytest.mark.parametrize("tol_dp", [1e-3, 1e-2])
-def test_step_size_rule(pkg, orc, images, tol_dp):
-    """The plan's step-size stopping rule (aldic_plan_set_stopping, rule 1)
-    on every size class and the workgroup kernel (> 65 px), from the search
-    start and from a gradient start, against the oracle run with the same
-    rule; switching the plan back gives the reference rule's results."""
+@pytest.mark.parametrize("rule,tol_dp", [(1, 1e-3), (1, 1e-2), (2, 1e-3)])
+def test_step_size_rule(pkg, orc, images, rule, tol_dp):
+    """The plan's step-size stopping rules (aldic_plan_set_stopping, rule 1
+    and its a-posteriori form, rule 2) on every size class and the
+    workgroup kernel (> 65 px), from the search start, against the oracle
+    run with the same rule; switching the plan back gives the reference
+    rule's results."""
     api = pkg.api
     ctx = images.ctx
     f, g, coords, U0, ws = _icgn_case([8, 10, 16, 30, 32, 48, 64, 80], 120, 600)
     n = len(ws)
     d = _Dev(api, ctx, coords, U0, ws)
     plan = api.Plan(ws, n, ctx=ctx)
-    plan.set_stopping(1, tol_dp)
-    want = orc.local_icgn(orc.Images(f, g), coords, U0, ws, 1e-3, nthreads=8, stop_rule=1,
+    plan.set_stopping(rule, tol_dp)
+    want = orc.local_icgn(orc.Images(f, g), coords, U0, ws, 1e-3, nthreads=8, stop_rule=rule,
                           tol_dp=tol_dp)
     api.local_icgn_dev(images, plan, d.coords.ptr, d.U0.ptr, d.ws.ptr, 1e-3, *d.outs())
     got = d.host()
@@ -316,7 +317,7 @@ def test_step_size_rule(pkg, orc, images, tol_dp):
     du = np.abs(got[0] - want[0]).reshape(-1, 2)[ok].max()
     same = ok & (conv == convo)
     dF = np.abs(got[1] - want[1]).reshape(-1, 4)[same].max()
-    _say(f"step-size rule {tol_dp}: {ok.sum()} of {n} converged in both, counts differ "
+    _say(f"stopping rule {rule}, {tol_dp}: {ok.sum()} of {n} converged in both, counts differ "
          f"{np.sum(conv != convo)}, mean evaluations {conv[ok].mean():.2f} (oracle "
          f"{convo[ok].mean():.2f}), max |dU| {du:.2e}, max |dF| same count {dF:.2e}")
     assert ok.sum() >= 0.9 * n and np.mean(conv != convo) <= 0.05

@@ -641,6 +641,61 @@ def test_icgn_step_size_rule(orc):
         assert cr[0] != s or np.abs(Ur - U).max() > 1e-9
 
 
+def _update_measures(orc, im, c, U0, wsa, rule, tol_dp, s, F, U):
+    """The measures m_1..m_s of the updates of a rule-`rule` run that
+    stopped after s evaluations with warp [F U]: the warps after j updates
+    come from capped runs, and dP_j from W(P_j) = W(P_{j-1}) W(dP_j)^-1."""
+    Ps = [np.zeros(6)]
+    for j in range(1, s):
+        Uj, Fj, _, cj, _, _ = orc.local_icgn(im, c, U0, wsa, 1e-3, max_iter=j - 1,
+                                             stop_rule=rule, tol_dp=tol_dp)
+        assert cj[0] == 101
+        Ps.append(np.concatenate([Fj, Uj]))
+    Ps.append(np.concatenate([F, U]))
+    h = int(wsa[0]) // 2
+    moves = []
+    for j in range(1, s + 1):
+        Wd = np.linalg.solve(_warp_matrix(Ps[j]), _warp_matrix(Ps[j - 1]))
+        d = np.array([Wd[0, 0] - 1, Wd[1, 0], Wd[0, 1], Wd[1, 1] - 1, Wd[0, 2], Wd[1, 2]])
+        moves.append(np.sqrt(d[4] ** 2 + d[5] ** 2 + (d[0] ** 2 + d[1] ** 2) * h * h +
+                             (d[2] ** 2 + d[3] ** 2) * h * h))
+    return moves
+
+
+def test_icgn_a_posteriori_rule(orc):
+    """Stop rule 2 (no counterpart in the reference): converged at the first
+    evaluation k whose update measure m_k is below tol_dp, or whose last two
+    measures contract (m_k <= m_{k-1} / 2) with a predicted distance to the
+    fixed point m_k q / (1 - q) below tol_dp, q = m_k / m_{k-1}.  The
+    measures are recovered from capped runs as for rule 1; the rule never
+    takes more evaluations than rule 1, takes fewer on some subset, and its
+    warp stays within tol_dp (centre) of a tightly converged one."""
+    f, g, _, _ = small_pair(shift=(0.45, -0.3), strain=0.003)
+    im = orc.Images(f, g)
+    tol_dp = 1e-3
+    fewer = 0
+    for (x0, y0, ws) in ((70, 64, 20), (90, 80, 30), (100, 70, 24), (80, 90, 28)):
+        c = np.array([[float(x0), float(y0)]])
+        wsa = np.array([ws], np.int32)
+        U0 = np.array([0.0, 0.0])
+        U, F, H, conv, hole, z = orc.local_icgn(im, c, U0, wsa, 1e-3, stop_rule=2, tol_dp=tol_dp)
+        s = int(conv[0])
+        assert 1 < s <= 100, s
+        m = _update_measures(orc, im, c, U0, wsa, 2, tol_dp, s, F, U)
+
+        def done(k):  # 0-based evaluation index
+            if m[k] < tol_dp:
+                return True
+            return k >= 1 and 2 * m[k] <= m[k - 1] and m[k] ** 2 < tol_dp * (m[k - 1] - m[k])
+        assert done(s - 1) and not any(done(k) for k in range(s - 1)), m
+        _, _, _, c1, _, _ = orc.local_icgn(im, c, U0, wsa, 1e-3, stop_rule=1, tol_dp=tol_dp)
+        assert s <= c1[0]
+        fewer += int(s < c1[0])
+        Ut, _, _, ct, _, _ = orc.local_icgn(im, c, U0, wsa, 1e-3, stop_rule=1, tol_dp=1e-9)
+        assert ct[0] < 101 and np.abs(Ut - U).max() < tol_dp
+    assert fewer >= 1
+
+
 def test_normalize_roi_moments_and_the_host_version(orc, pkg):
     """funNormalizeImg.m:22-40: (Img - mean) / std over the ROI (clamped to
     the image), applied to the whole image.  Pinned by the ROI's moments
