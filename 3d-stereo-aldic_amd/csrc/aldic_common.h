@@ -8,6 +8,26 @@
 
 #define ALDIC_WAVE 64
 
+// Two FP16 values in one 32-bit register.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// a * h + c with h the low (fma_lo) or high (fma_hi) FP16 half of a packed
+// register, as one mixed-precision FMA (v_fma_mix_f32: the half is widened
+// exactly and the FMA rounds once, as an FP32 FMA would).  Written as an
+// instruction so that the compiler keeps the half packed instead of
+// holding a widened FP32 copy in a register of its own.
+__device__ __forceinline__ float fma_lo(float a, half2_t h, float c) {
+  float d;
+  __asm__("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,1,0]" : "=v"(d) : "v"(a), "v"(h), "v"(c));
+  return d;
+}
+__device__ __forceinline__ float fma_hi(float a, half2_t h, float c) {
+  float d;
+  __asm__("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,1,0] op_sel_hi:[0,1,0]"
+          : "=v"(d) : "v"(a), "v"(h), "v"(c));
+  return d;
+}
+
 // Status codes shared with the reference (funICGNQuadtree.m:425-444).
 #define ST_ERR (-1)      // MATLAB would throw inside the subset
 #define ST_NOCONV 101    // not converged / NaN / warped out of the image

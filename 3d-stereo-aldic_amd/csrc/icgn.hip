@@ -373,7 +373,11 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   // offset from the window's first pixel, which an empty slot reads too and
   // then discards.
   const int kfull = NS / NT;
-  float pf[PPL], pgx[PPL], pgy[PPL], pdx[PPL], pdy[PPL], pg[PPL];
+  // pd holds the pixel's offsets from the centre (dx, dy) as two halves
+  // of one register: integers of magnitude <= 32 are exact in FP16, and the
+  // mixed-precision FMA reads either half as an FP32 operand
+  float pf[PPL], pgx[PPL], pgy[PPL], pg[PPL];
+  half2_t pd[PPL];
   int zf = 0, nanf = 0, tinyf = 0;
   {
     uint32_t off[PPL];
@@ -382,8 +386,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       const int p = tid + NT * k;
       const int aa = p % S1, bb = p / S1;
       const bool v = p < NS;
-      pdx[k] = v ? (float)(aa - hx) : 0.0f;
-      pdy[k] = v ? (float)(bb - hy) : 0.0f;
+      pd[k] = half2_t{(_Float16)(v ? aa - hx : 0), (_Float16)(v ? bb - hy : 0)};
       off[k] = v ? (uint32_t)(aa + bb * M) : 0u;
       pf[k] = pgx[k] = pgy[k] = 0.0f;
     }
@@ -503,8 +506,9 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     for (int k = 0; k < 21; k++) h21[k] = 0.0f;
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-      const float sd[6] = {pdx[k] * pgx[k], pdx[k] * pgy[k], pdy[k] * pgx[k],
-                           pdy[k] * pgy[k], pgx[k], pgy[k]};
+      const float pdx = (float)pd[k].x, pdy = (float)pd[k].y;
+      const float sd[6] = {pdx * pgx[k], pdx * pgy[k], pdy * pgx[k], pdy * pgy[k], pgx[k],
+                           pgy[k]};
       int t = 0;
 #pragma unroll
       for (int c = 0; c < 6; c++)
@@ -647,8 +651,8 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     float s1[2] = {0.0f, 0.0f};  // sum, sum of squares
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-      const float tx = fmaf(A11, pdx[k], fmaf(A12, pdy[k], cx));
-      const float ty = fmaf(A21, pdx[k], fmaf(A22, pdy[k], cy));
+      const float tx = fma_lo(A11, pd[k], fma_hi(A12, pd[k], cx));
+      const float ty = fma_lo(A21, pd[k], fma_hi(A22, pd[k], cy));
       // tx, ty >= 2 inside the tile (the bounding-box test above leaves one
       // pixel to spare), where fract(t) = t - floor(t) exactly and the
       // truncating conversion is the floor; the 4 x 4 cell starts one pixel
@@ -720,10 +724,10 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       for (int k = 0; k < PPL; k++) {
         const float r = fmaf(-ig, pg[k], pf[k] + c2);
         const float rx = r * pgx[k], ry = r * pgy[k];
-        v[0] = fmaf(pdx[k], rx, v[0]);
-        v[1] = fmaf(pdx[k], ry, v[1]);
-        v[2] = fmaf(pdy[k], rx, v[2]);
-        v[3] = fmaf(pdy[k], ry, v[3]);
+        v[0] = fma_lo(rx, pd[k], v[0]);
+        v[1] = fma_lo(ry, pd[k], v[1]);
+        v[2] = fma_hi(rx, pd[k], v[2]);
+        v[3] = fma_hi(ry, pd[k], v[3]);
         v[4] += rx;
         v[5] += ry;
         if (k < kfull)
@@ -807,9 +811,17 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
 // walks with a grid stride; otherwise the grid covers the list and every
 // group takes one entry (a separate instance: the loop costs the one-entry
 // kernel registers; a walking multi-wave one is held to two waves per
-// SIMD).
+// SIMD).  The one-entry instance of the 65 px class at four waves per
+// subset and 17 pixels per lane is held to three waves per SIMD (168
+// registers: three subsets per CU), which its packed pixel offsets leave
+// room for without spills.
+template <int NW, int PPL, bool WALK>
+constexpr int fast_waves_per_simd() {
+  return (NW == 4 && PPL == 17 && !WALK) ? 3 : (WALK && NW > 1 ? 2 : 1);
+}
+
 template <int MODE, int NW, int PPL, int TS, int FS, bool WALK, int BS>
-__global__ __launch_bounds__(fast_block<NW>(), WALK && NW > 1 ? 2 : 1) void fast_kernel(
+__global__ __launch_bounds__(fast_block<NW>(), (fast_waves_per_simd<NW, PPL, WALK>())) void fast_kernel(
     SolveArgs a) {
   constexpr int NT = ALDIC_WAVE * NW;
   constexpr int SUBS = fast_block<NW>() / NT;
