@@ -207,25 +207,51 @@ __device__ __forceinline__ bool dp_small(int rule, const float d[6], float hx, f
 // ---------------------------------------------------------------------------
 // Group reductions: NW waves cooperate on one subset.
 // ---------------------------------------------------------------------------
+// NW > 1: each value is summed within the 16-lane rows (four DPP adds),
+// the NW * 4 row sums go to LDS, the first K lanes of the block add them up
+// and every lane reads the K totals back: four vector operations per value
+// and wave instead of the eleven of a full wave sum, plus three barriers.
+// red holds group_red_floats<NW>() floats.
+constexpr int GROUP_KMAX = 21;
+template <int NW>
+constexpr int group_red_floats() {
+  return NW * 4 * GROUP_KMAX + GROUP_KMAX;
+}
+
+__device__ __forceinline__ float row_sum16(float v) {
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x124, 0xF, 0xF, false));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x128, 0xF, 0xF, false));
+  return v;
+}
+
 template <int NW, int K>
 __device__ __forceinline__ void group_sum(float (&v)[K], float* red) {
+  static_assert(K <= GROUP_KMAX, "group_sum holds at most GROUP_KMAX values");
+  if constexpr (NW == 1) {
 #pragma unroll
-  for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
-  if constexpr (NW > 1) {
-    const int w = threadIdx.x / ALDIC_WAVE, l = threadIdx.x % ALDIC_WAVE;
-    __syncthreads();
-    if (l == 0) {
+    for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
+  } else {
+    constexpr int R = NW * 4;  // 16-lane rows of the group
+    const int t = threadIdx.x % (ALDIC_WAVE * NW);
 #pragma unroll
-      for (int k = 0; k < K; k++) red[w * K + k] = v[k];
+    for (int k = 0; k < K; k++) v[k] = row_sum16(v[k]);
+    __syncthreads();  // the previous sum's totals have been read
+    if ((t & 15) == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) red[(t >> 4) * K + k] = v[k];
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
+    if (t < K) {
       float s = 0.0f;
 #pragma unroll
-      for (int q = 0; q < NW; q++) s += red[q * K + k];
-      v[k] = s;
+      for (int q = 0; q < R; q++) s += red[q * K + t];
+      red[R * K + t] = s;
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = red[R * K + k];
   }
 }
 
@@ -369,9 +395,11 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   // masked and their offsets set to the centre so every gather stays inside
   // the tile.
   // All loads of the subset are issued before any of them is used (one
-  // memory latency for the whole subset): each slot reads at a 32-bit
+  // memory latency for the whole subset): each slot reads at a 32-bit byte
   // offset from the window's first pixel, which an empty slot reads too and
-  // then discards.
+  // then discards.  A window that leaves the image is read from the
+  // nearest position inside it: such a subset only needs its hole flag,
+  // which is set for it whatever it reads, and the throw of :146-147.
   const int kfull = NS / NT;
   // pd holds the pixel's offsets from the centre (dx, dy) as two halves
   // of one register: integers of magnitude <= 32 are exact in FP16, and the
@@ -380,37 +408,32 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   half2_t pd[PPL];
   int zf = 0, nanf = 0, tinyf = 0;
   {
-    uint32_t off[PPL];
+    uint32_t boff[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
       const int p = tid + NT * k;
       const int aa = p % S1, bb = p / S1;
       const bool v = p < NS;
       pd[k] = half2_t{(_Float16)(v ? aa - hx : 0), (_Float16)(v ? bb - hy : 0)};
-      off[k] = v ? (uint32_t)(aa + bb * M) : 0u;
-      pf[k] = pgx[k] = pgy[k] = 0.0f;
+      boff[k] = v ? 4u * (uint32_t)(aa + bb * M) : 0u;
     }
-    if (in_img) {  // uniform
-      const int64_t w0 = (int64_t)(x0 - hx - 1) + (int64_t)(y0 - hy - 1) * M;
-      const float* fb = a.fm + w0;
+    const int wx0 = min(max(x0 - hx - 1, 0), M - S1), wy0 = min(max(y0 - hy - 1, 0), N - S2);
+    const int64_t w0 = (int64_t)wx0 + (int64_t)wy0 * M;
+    const char *fb = (const char*)(a.fm + w0), *xb = (const char*)(a.fx + w0),
+               *yb = (const char*)(a.fy + w0);
 #pragma unroll
-      for (int k = 0; k < PPL; k++) pf[k] = fb[off[k]];
-      if (in_df) {
-        const float *xb = a.fx + w0, *yb = a.fy + w0;
+    for (int k = 0; k < PPL; k++) {
+      pf[k] = *(const float*)(fb + boff[k]);
+      pgx[k] = *(const float*)(xb + boff[k]);
+      pgy[k] = *(const float*)(yb + boff[k]);
+    }
 #pragma unroll
-        for (int k = 0; k < PPL; k++) {
-          pgx[k] = xb[off[k]];
-          pgy[k] = yb[off[k]];
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < PPL; k++) {
-        if (k >= kfull && tid + NT * k >= NS) pf[k] = pgx[k] = pgy[k] = 0.0f;
-        const bool v = k < kfull || tid + NT * k < NS;
-        zf += (v && pf[k] == 0.0f);
-        tinyf |= (v && fabsf(pf[k]) <= 1e-10f);
-        nanf |= (pf[k] != pf[k]);
-      }
+    for (int k = 0; k < PPL; k++) {
+      if (k >= kfull && tid + NT * k >= NS) pf[k] = pgx[k] = pgy[k] = 0.0f;
+      const bool v = k < kfull || tid + NT * k < NS;
+      zf += (v && pf[k] == 0.0f);
+      tinyf |= (v && fabsf(pf[k]) <= 1e-10f);
+      nanf |= (pf[k] != pf[k]);
     }
   }
   {
@@ -827,7 +850,7 @@ __global__ __launch_bounds__(fast_block<NW>(), (fast_waves_per_simd<NW, PPL, WAL
   constexpr int SUBS = fast_block<NW>() / NT;
   static_assert(NW == 1 || SUBS == 1, "multi-wave subsets own their block");
   __shared__ float tiles[SUBS][TS * TS + TS + 1];
-  __shared__ float red[NW * 32];
+  __shared__ float red[group_red_floats<NW>()];
   if (a.skip && *a.skip) return;  // uniform: the whole launch is skipped
   const int sub = threadIdx.x / NT;
   // a device count is clamped to the list's capacity a.cnt
@@ -862,7 +885,7 @@ static_assert(PK_TS2 * PK_TS2 + PK_TS2 + 1 <= PK_TILE, "the 33 px block tile fit
 template <int MODE, int BS>
 __global__ __launch_bounds__(256, 2) void persistent_kernel(aldic::PersistArgs P) {
   __shared__ float tile[PK_TILE];
-  __shared__ float red[4 * 32];
+  __shared__ float red[group_red_floats<4>()];
   __shared__ int item_s;
   if (P.a.skip && *P.a.skip) return;
   const int n3 = P.cnt[3], n2 = P.cnt[2], n1 = P.cnt[1], n0 = P.cnt[0];
@@ -1058,7 +1081,7 @@ __global__ __launch_bounds__(GEN_THREADS) void generic_kernel(SolveArgs a, const
   __shared__ uint8_t stmp[GEN_MAXPIX];  // logical image fed to bwselect
   __shared__ uint8_t sfsel[GEN_MAXPIX]; // Subpb1: f selection after its mask block
   __shared__ uint8_t sbwf8[GEN_MAXPIX]; // 8-connected component of the f mask (on first use)
-  __shared__ float red[4 * 32];
+  __shared__ float red[group_red_floats<GEN_THREADS / ALDIC_WAVE>()];
   if (a.skip && *a.skip) return;  // uniform: the whole launch is skipped
   const int total = count ? min(*count, fixed_count) : fixed_count;
   const int M = a.M, N = a.N;
