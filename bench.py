@@ -76,11 +76,11 @@ FP32_PEAK_TFLOPS = 157.3
 ROOFLINE_REPS = 5
 TOL_DP = 1e-3  # step-size rule threshold (px)
 # HBM bytes per launch of the dominant IC-GN kernel (65 px class, learned
-# start, step-size rule, headline workload), rocprofv3 --pmc FETCH_SIZE /
+# start, a-posteriori rule, headline workload), rocprofv3 --pmc FETCH_SIZE /
 # WRITE_SIZE in KiB per dispatch, each scaled by the same counter's reading
-# on a device-to-device copy of known size (FETCH x 1.9996, WRITE x 1);
+# on a device-to-device copy of known size (FETCH x 1.9995, WRITE x 1);
 # measured with this bench (profiles/icgn_c3_bulge4096_hbm_traffic.md)
-ICGN_C3_TRAFFIC = (3367983.1 * 1.9996 + 21790.4) * 1024.0
+ICGN_C3_TRAFFIC = (3414294.1 * 1.9995 + 23553.8) * 1024.0
 
 
 def load(name, rel):
@@ -638,7 +638,8 @@ def run_workload(pkg, W, params, args, headline):
         ms, nbytes, flops, units, evals = kern[dom]
         achieved = nbytes / (ms * 1e-3) / 1e9
         tflops = flops / (ms * 1e-3) / 1e12
-        roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+        roof = {"bound": "hbm", "limiter": "VALU issue (the fp32 entry)", "kernel": dom,
+                "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": ICGN_C3_TRAFFIC if dom == "icgn_c3" else None,
                 "kernel_ms": round(ms, 4), "launches": ROOFLINE_REPS,
