@@ -212,10 +212,20 @@ __device__ __forceinline__ bool dp_small(int rule, const float d[6], float hx, f
 // and every lane reads the K totals back: four vector operations per value
 // and wave instead of the eleven of a full wave sum, plus three barriers.
 // red holds group_red_floats<NW>() floats.
-constexpr int GROUP_KMAX = 21;
+constexpr int GROUP_KMAX = 22;
 template <int NW>
 constexpr int group_red_floats() {
   return NW * 4 * GROUP_KMAX + GROUP_KMAX;
+}
+
+// The fast kernel's group sums alternate between two such areas (rb
+// toggles at every call, across the subsets a group walks), so that a sum
+// never writes the area the previous one is still being read from and its
+// guarding barrier can go: the area of call i is written again by call
+// i + 2 only, after the barriers of call i + 1.
+template <int NW>
+constexpr int fast_red_floats() {
+  return 2 * group_red_floats<NW>();
 }
 
 __device__ __forceinline__ float row_sum16(float v) {
@@ -226,7 +236,7 @@ __device__ __forceinline__ float row_sum16(float v) {
   return v;
 }
 
-template <int NW, int K>
+template <int NW, int K, bool GUARD = true>
 __device__ __forceinline__ void group_sum(float (&v)[K], float* red) {
   static_assert(K <= GROUP_KMAX, "group_sum holds at most GROUP_KMAX values");
   if constexpr (NW == 1) {
@@ -237,7 +247,7 @@ __device__ __forceinline__ void group_sum(float (&v)[K], float* red) {
     const int t = threadIdx.x % (ALDIC_WAVE * NW);
 #pragma unroll
     for (int k = 0; k < K; k++) v[k] = row_sum16(v[k]);
-    __syncthreads();  // the previous sum's totals have been read
+    if constexpr (GUARD) __syncthreads();  // the previous sum's totals have been read
     if ((t & 15) == 0) {
 #pragma unroll
       for (int k = 0; k < K; k++) red[(t >> 4) * K + k] = v[k];
@@ -252,6 +262,16 @@ __device__ __forceinline__ void group_sum(float (&v)[K], float* red) {
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; k++) v[k] = red[R * K + k];
+  }
+}
+
+template <int NW, int K>
+__device__ __forceinline__ void group_sum_db(float (&v)[K], float* red, int& rb) {
+  if constexpr (NW == 1) {
+    group_sum<1, K>(v, red);
+  } else {
+    rb ^= 1;
+    group_sum<NW, K, false>(v, red + rb * group_red_floats<NW>());
   }
 }
 
@@ -325,7 +345,7 @@ constexpr int fast_block() {
 // the pixel layout, the partial slot and the divisions are compile-time.
 template <int MODE, int NW, int PPL, int TS, int FS, bool WALK, int BS>
 __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float* tileb,
-                                            float* red) {
+                                            float* red, int& rb) {
   constexpr int NT = ALDIC_WAVE * NW;
   int tid = threadIdx.x % NT;
   // in a list walk the lane index is made opaque to the optimiser, so that
@@ -436,12 +456,22 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       nanf |= (pf[k] != pf[k]);
     }
   }
+  // the mask counts and the first moments of f in one sum (:119-120,
+  // :434: every pixel counts; empty slots hold 0)
+  float sumf, sabs;
   {
-    float v[3] = {(float)zf, (float)nanf, (float)tinyf};
-    group_sum<NW, 3>(v, red);
+    float v[5] = {(float)zf, (float)nanf, (float)tinyf, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < PPL; k++) {
+      v[3] += pf[k];
+      v[4] += fabsf(pf[k]);
+    }
+    group_sum_db<NW, 5>(v, red, rb);
     zf = (int)v[0];
     nanf = (int)v[1];
     tinyf = (int)v[2];
+    sumf = v[3];
+    sabs = v[4];
   }
   // markCoordHoleStrainOrNot (LocalICGNQuadtree.m:75-86)
   if (MODE == MODE_ICGN && a.hole && tid == 0)
@@ -486,47 +516,26 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     return;
   }
 
-  // ---- moments of f (:119-120) and sum |f| (:434); every pixel counts ----
-  float meanf, bottomf, sabs;
-  {
-    float v[2] = {0.0f, 0.0f};
+  // ---- second moment of f (:119-120) and the Hessian, in one sum ----
+  const float meanf = sumf / (float)NS;
+  float sqf = 0.0f;
 #pragma unroll
-    for (int k = 0; k < PPL; k++) {
-      v[0] += pf[k];  // empty slots hold 0
-      v[1] += fabsf(pf[k]);
-    }
-    group_sum<NW, 2>(v, red);
-    meanf = v[0] / (float)NS;
-    sabs = v[1];
-    float w[1] = {0.0f};
-#pragma unroll
-    for (int k = 0; k < PPL; k++) {
-      const float d = pf[k] - meanf;
-      if (k < kfull)
-        w[0] += d * d;
-      else
-        w[0] += (tid + NT * k < NS) ? d * d : 0.0f;
-    }
-    group_sum<NW, 1>(w, red);
-    bottomf = (NS > 1) ? sqrtf(w[0]) : 0.0f;
+  for (int k = 0; k < PPL; k++) {
+    const float d = pf[k] - meanf;
+    if (k < kfull)
+      sqf += d * d;
+    else
+      sqf += (tid + NT * k < NS) ? d * d : 0.0f;
   }
-  // fc = (f - meanf) / bottomf, the reference half of the residual
-  {
-    const float ibf = 1.0f / bottomf;
-#pragma unroll
-    for (int k = 0; k < PPL; k++) {
-      const float fc = (pf[k] - meanf) * ibf;
-      pf[k] = (k < kfull || tid + NT * k < NS) ? fc : 0.0f;
-    }
-  }
-
-  // ---- Hessian ----
   float h21[21];
   float L[21];
   float H2[4] = {0, 0, 0, 0};
+  float bottomf;
   if (MODE == MODE_ICGN) {
+    float hv[22];
+    hv[21] = sqf;
 #pragma unroll
-    for (int k = 0; k < 21; k++) h21[k] = 0.0f;
+    for (int k = 0; k < 21; k++) hv[k] = 0.0f;
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
       const float pdx = (float)pd[k].x, pdy = (float)pd[k].y;
@@ -536,11 +545,12 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
 #pragma unroll
       for (int c = 0; c < 6; c++)
 #pragma unroll
-        for (int r = c; r < 6; r++) h21[t++] += sd[r] * sd[c];
+        for (int r = c; r < 6; r++) hv[t++] += sd[r] * sd[c];
     }
-    group_sum<NW, 21>(h21, red);
+    group_sum_db<NW, 22>(hv, red, rb);
 #pragma unroll
-    for (int k = 0; k < 21; k++) h21[k] = uni(h21[k]);
+    for (int k = 0; k < 21; k++) h21[k] = uni(hv[k]);
+    bottomf = (NS > 1) ? sqrtf(hv[21]) : 0.0f;
     const bool spd = chol6(h21, L);
     if (wave_any(!spd)) {
       // an indefinite H takes MATLAB's LU fallback: the generic kernel
@@ -554,19 +564,29 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
 #pragma unroll
     for (int k = 0; k < 21; k++) L[k] = uni(L[k]);
   } else {
-    float v[3] = {0.0f, 0.0f, 0.0f};
+    float v[4] = {0.0f, 0.0f, 0.0f, sqf};
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
       v[0] += pgx[k] * pgx[k];
       v[1] += pgx[k] * pgy[k];
       v[2] += pgy[k] * pgy[k];
     }
-    group_sum<NW, 3>(v, red);
+    group_sum_db<NW, 4>(v, red, rb);
+    bottomf = (NS > 1) ? sqrtf(v[3]) : 0.0f;
     const float s = 2.0f / (bottomf * bottomf);
     H2[0] = v[0] * s + a.mu;
     H2[1] = v[1] * s;
     H2[2] = v[1] * s;
     H2[3] = v[2] * s + a.mu;
+  }
+  // fc = (f - meanf) / bottomf, the reference half of the residual
+  {
+    const float ibf = 1.0f / bottomf;
+#pragma unroll
+    for (int k = 0; k < PPL; k++) {
+      const float fc = (pf[k] - meanf) * ibf;
+      pf[k] = (k < kfull || tid + NT * k < NS) ? fc : 0.0f;
+    }
   }
 
   // ---- Gauss-Newton loop (:135-422) ----
@@ -715,7 +735,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     }
     {
       float v[3] = {s1[0], s1[1], (float)tiny};
-      group_sum<NW, 3>(v, red);
+      group_sum_db<NW, 3>(v, red, rb);
       s1[0] = v[0];
       s1[1] = v[1];
       tiny = (int)v[2];
@@ -758,7 +778,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
         else
           v[6] += (tid + NT * k < NS) ? r * r : 0.0f;
       }
-      group_sum<NW, 7>(v, red);
+      group_sum_db<NW, 7>(v, red, rb);
       float nb = 0.0f;
 #pragma unroll
       for (int k = 0; k < 6; k++) {
@@ -779,7 +799,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
         else
           v[2] += (tid + NT * k < NS) ? r * r : 0.0f;
       }
-      group_sum<NW, 3>(v, red);
+      group_sum_db<NW, 3>(v, red, rb);
       const float s = 2.0f / (bottomf * bottomf);
       bvec[0] = bottomf * v[0] * s + a.mu * (P[4] - a.UOld[2 * node] - a.vdual[2 * node]);
       bvec[1] = bottomf * v[1] * s + a.mu * (P[5] - a.UOld[2 * node + 1] - a.vdual[2 * node + 1]);
@@ -850,19 +870,20 @@ __global__ __launch_bounds__(fast_block<NW>(), (fast_waves_per_simd<NW, PPL, WAL
   constexpr int SUBS = fast_block<NW>() / NT;
   static_assert(NW == 1 || SUBS == 1, "multi-wave subsets own their block");
   __shared__ float tiles[SUBS][TS * TS + TS + 1];
-  __shared__ float red[group_red_floats<NW>()];
+  __shared__ float red[fast_red_floats<NW>()];
   if (a.skip && *a.skip) return;  // uniform: the whole launch is skipped
   const int sub = threadIdx.x / NT;
+  int rb = 0;  // the group's reduction area (alternates per sum)
   // a device count is clamped to the list's capacity a.cnt
   const int cnt = a.dcnt ? min(*a.dcnt, a.cnt) : a.cnt;
   // uniform per subset group (SUBS == 1 when NW > 1); the group's barriers
   // and the reduction's barrier keep one entry's LDS use apart from the next
   if constexpr (WALK) {
     for (int slot = blockIdx.x * SUBS + sub; slot < cnt; slot += gridDim.x * SUBS)
-      fast_subset<MODE, NW, PPL, TS, FS, true, BS>(a, slot, tiles[sub], red);
+      fast_subset<MODE, NW, PPL, TS, FS, true, BS>(a, slot, tiles[sub], red, rb);
   } else {
     const int slot = blockIdx.x * SUBS + sub;
-    if (slot < cnt) fast_subset<MODE, NW, PPL, TS, FS, false, BS>(a, slot, tiles[sub], red);
+    if (slot < cnt) fast_subset<MODE, NW, PPL, TS, FS, false, BS>(a, slot, tiles[sub], red, rb);
   }
 }
 
@@ -885,8 +906,9 @@ static_assert(PK_TS2 * PK_TS2 + PK_TS2 + 1 <= PK_TILE, "the 33 px block tile fit
 template <int MODE, int BS>
 __global__ __launch_bounds__(256, 2) void persistent_kernel(aldic::PersistArgs P) {
   __shared__ float tile[PK_TILE];
-  __shared__ float red[group_red_floats<4>()];
+  __shared__ float red[fast_red_floats<4>()];
   __shared__ int item_s;
+  int rb = 0;  // the block's reduction area (alternates per sum)
   if (P.a.skip && *P.a.skip) return;
   const int n3 = P.cnt[3], n2 = P.cnt[2], n1 = P.cnt[1], n0 = P.cnt[0];
   SolveArgs a = P.a;
@@ -900,15 +922,15 @@ __global__ __launch_bounds__(256, 2) void persistent_kernel(aldic::PersistArgs P
     if (it < n3) {
       a.list = P.list[3];
       if (P.full[3])
-        fast_subset<MODE, 4, 17, PK_TS3, 65, true, BS>(a, it, tile, red);
+        fast_subset<MODE, 4, 17, PK_TS3, 65, true, BS>(a, it, tile, red, rb);
       else
-        fast_subset<MODE, 4, 17, PK_TS3, 0, true, BS>(a, it, tile, red);
+        fast_subset<MODE, 4, 17, PK_TS3, 0, true, BS>(a, it, tile, red, rb);
     } else {
       a.list = P.list[2];
       if (P.full[2])
-        fast_subset<MODE, 4, 5, PK_TS2, 33, true, BS>(a, it - n3, tile, red);
+        fast_subset<MODE, 4, 5, PK_TS2, 33, true, BS>(a, it - n3, tile, red, rb);
       else
-        fast_subset<MODE, 4, 5, PK_TS2, 0, true, BS>(a, it - n3, tile, red);
+        fast_subset<MODE, 4, 5, PK_TS2, 0, true, BS>(a, it - n3, tile, red, rb);
     }
     __syncthreads();  // the block's tile is free for the next item
   }
@@ -923,15 +945,15 @@ __global__ __launch_bounds__(256, 2) void persistent_kernel(aldic::PersistArgs P
     if (it < n1) {
       a.list = P.list[1];
       if (P.full[1])
-        fast_subset<MODE, 1, 5, PK_TS1, 17, true, BS>(a, it, wt, red);
+        fast_subset<MODE, 1, 5, PK_TS1, 17, true, BS>(a, it, wt, red, rb);
       else
-        fast_subset<MODE, 1, 5, PK_TS1, 0, true, BS>(a, it, wt, red);
+        fast_subset<MODE, 1, 5, PK_TS1, 0, true, BS>(a, it, wt, red, rb);
     } else {
       a.list = P.list[0];
       if (P.full[0])
-        fast_subset<MODE, 1, 2, PK_TS0, 11, true, BS>(a, it - n1, wt, red);
+        fast_subset<MODE, 1, 2, PK_TS0, 11, true, BS>(a, it - n1, wt, red, rb);
       else
-        fast_subset<MODE, 1, 2, PK_TS0, 0, true, BS>(a, it - n1, wt, red);
+        fast_subset<MODE, 1, 2, PK_TS0, 0, true, BS>(a, it - n1, wt, red, rb);
     }
   }
 }
