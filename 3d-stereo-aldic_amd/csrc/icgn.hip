@@ -212,7 +212,7 @@ __device__ __forceinline__ bool dp_small(int rule, const float d[6], float hx, f
 // and every lane reads the K totals back: four vector operations per value
 // and wave instead of the eleven of a full wave sum, plus three barriers.
 // red holds group_red_floats<NW>() floats.
-constexpr int GROUP_KMAX = 22;
+constexpr int GROUP_KMAX = 34;
 template <int NW>
 constexpr int group_red_floats() {
   return NW * 4 * GROUP_KMAX + GROUP_KMAX;
@@ -347,6 +347,10 @@ template <int MODE, int NW, int PPL, int TS, int FS, bool WALK, int BS>
 __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float* tileb,
                                             float* red, int& rb) {
   constexpr int NT = ALDIC_WAVE * NW;
+  // IC-GN evaluates b in one pass over the gathered g (no g kept per pixel)
+  // in the 65 px class's launch shapes; smaller subsets keep the two-pass
+  // residual, whose rounding their gradients need for 1e-5 agreement
+  constexpr bool OP = MODE == MODE_ICGN && NT * PPL >= 2048;
   int tid = threadIdx.x % NT;
   // in a list walk the lane index is made opaque to the optimiser, so that
   // nothing derived from it is hoisted out of the walk and kept live across
@@ -530,12 +534,14 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   float h21[21];
   float L[21];
   float H2[4] = {0, 0, 0, 0};
+  float SF[6], S6[6];
   float bottomf;
   if (MODE == MODE_ICGN) {
-    float hv[22];
-    hv[21] = sqf;
+    constexpr int NH = OP ? 34 : 22;
+    float hv[NH];
 #pragma unroll
-    for (int k = 0; k < 21; k++) hv[k] = 0.0f;
+    for (int k = 0; k < NH; k++) hv[k] = 0.0f;
+    hv[21] = sqf;
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
       const float pdx = (float)pd[k].x, pdy = (float)pd[k].y;
@@ -546,11 +552,27 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       for (int c = 0; c < 6; c++)
 #pragma unroll
         for (int r = c; r < 6; r++) hv[t++] += sd[r] * sd[c];
+      if constexpr (OP) {
+        const float d = pf[k] - meanf;  // empty slots: sd = 0
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+          hv[22 + j] = fmaf(sd[j], d, hv[22 + j]);
+          hv[28 + j] += sd[j];
+        }
+      }
     }
-    group_sum_db<NW, 22>(hv, red, rb);
+    group_sum_db<NW, NH>(hv, red, rb);
 #pragma unroll
     for (int k = 0; k < 21; k++) h21[k] = uni(hv[k]);
     bottomf = (NS > 1) ? sqrtf(hv[21]) : 0.0f;
+    if constexpr (OP) {
+      const float ibf = 1.0f / bottomf;
+#pragma unroll
+      for (int j = 0; j < 6; j++) {
+        SF[j] = uni(hv[22 + j] * ibf);
+        S6[j] = uni(hv[28 + j]);
+      }
+    }
     const bool spd = chol6(h21, L);
     if (wave_any(!spd)) {
       // an indefinite H takes MATLAB's LU fallback: the generic kernel
@@ -690,8 +712,9 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     // (|g| <= 1e-10, :272-273) is scaled with it.
     constexpr float TINY = 1e-10f * (BS ? 36.0f : 3136.0f);
     const float cx = (float)(x0 - 1 - tx0) + P[4], cy = (float)(y0 - 1 - ty0) + P[5];
-    int tiny = 0;
     float s1[2] = {0.0f, 0.0f};  // sum, sum of squares
+    float gmin = 3.0e38f;        // smallest |g| of the lane's pixels
+    float bg[7] = {0, 0, 0, 0, 0, 0, 0};  // IC-GN: sum sd_j g (6), sum fc g
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
       const float tx = fma_lo(A11, pd[k], fma_hi(A12, pd[k], cx));
@@ -718,23 +741,45 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
         const float rsum = wr[0] * row[0] + wr[1] * row[1] + wr[2] * row[2] + wr[3] * row[3];
         acc = fmaf(wc[c], rsum, acc);
       }
-      pg[k] = acc;
       // k < kfull is uniform (a scalar branch); the partial slots mask by
       // selects, never by a divergent branch
       if (k < kfull) {
         s1[0] += acc;
         s1[1] = fmaf(acc, acc, s1[1]);
-        tiny |= (fabsf(acc) <= TINY);
+        gmin = fminf(gmin, fabsf(acc));
       } else {
         const bool in = tid + NT * k < NS;
         const float av = in ? acc : 0.0f;
         s1[0] += av;
         s1[1] = fmaf(av, av, s1[1]);
-        tiny |= in & (fabsf(acc) <= TINY);
+        gmin = fminf(gmin, in ? fabsf(acc) : 3.0e38f);
+      }
+      if constexpr (OP) {
+        // empty slots carry zero gradients and fc: they add nothing
+        const float t = acc * pgx[k], u = acc * pgy[k];
+        bg[0] = fma_lo(t, pd[k], bg[0]);
+        bg[1] = fma_lo(u, pd[k], bg[1]);
+        bg[2] = fma_hi(t, pd[k], bg[2]);
+        bg[3] = fma_hi(u, pd[k], bg[3]);
+        bg[4] += t;
+        bg[5] += u;
+        bg[6] = fmaf(pf[k], acc, bg[6]);
+      } else {
+        pg[k] = acc;
       }
     }
-    {
-      float v[3] = {s1[0], s1[1], (float)tiny};
+    int tiny;
+    if constexpr (OP) {
+      float v[10] = {s1[0], s1[1], gmin <= TINY ? 1.0f : 0.0f,
+                     bg[0], bg[1], bg[2], bg[3], bg[4], bg[5], bg[6]};
+      group_sum_db<NW, 10>(v, red, rb);
+      s1[0] = v[0];
+      s1[1] = v[1];
+      tiny = (int)v[2];
+#pragma unroll
+      for (int j = 0; j < 7; j++) bg[j] = v[3 + j];
+    } else {
+      float v[3] = {s1[0], s1[1], gmin <= TINY ? 1.0f : 0.0f};
       group_sum_db<NW, 3>(v, red, rb);
       s1[0] = v[0];
       s1[1] = v[1];
@@ -757,11 +802,25 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     const float ig = __builtin_amdgcn_rcpf(bottomg);
     const float c2 = meang * ig;
     // residual r = (f - mean f)/bottom f - (g - mean g)/bottom g and b
-    // (:365-374); empty slots carry zero gradients and contribute to b
-    // nothing, only their r^2 is masked
+    // (:365-374)
     float bnorm;
     float bvec[6];
-    if (MODE == MODE_ICGN) {
+    if constexpr (OP) {
+      // one pass: b_j = bottomf (SF_j + c2 S_j - ig sum sd_j g) with the
+      // subset constants SF_j = sum sd_j fc and S_j = sum sd_j; with gc =
+      // ig (g - mean g), sum gc = 0 and sum gc^2 = 1, and as sum fc = 0 and
+      // sum fc^2 = 1, sum r^2 = 2 - 2 ig sum fc g
+      float nb = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        bvec[k] = bottomf * fmaf(-ig, bg[k], fmaf(c2, S6[k], SF[k]));
+        nb += bvec[k] * bvec[k];
+      }
+      bnorm = __builtin_amdgcn_sqrtf(nb);
+      last_zncc = ig * bg[6];
+    } else if constexpr (MODE == MODE_ICGN) {
+      // two passes: r per pixel from the kept g; empty slots carry zero
+      // gradients and contribute to b nothing, only their r^2 is masked
       float v[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
       for (int k = 0; k < PPL; k++) {
