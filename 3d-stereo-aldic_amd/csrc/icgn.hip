@@ -298,10 +298,25 @@ __device__ __forceinline__ void write_icgn(const SolveArgs& a, int node, int st,
   if (a.zncc) a.zncc[node] = zn;
 }
 
+// Every output but H; H is zeroed when zero_h (the fast kernel writes a
+// subset's H at setup and zeroes it again only when the subset throws).
 __device__ __forceinline__ void write_icgn_noh(const SolveArgs& a, int node, int st,
-                                               const float (&P)[6], float zn) {
-  const float z[21] = {0};
-  write_icgn(a, node, st, P, z, false, zn);
+                                               const float (&P)[6], float zn,
+                                               bool zero_h = true) {
+  a.conv[node] = st;
+  const bool err = (st == ST_ERR);
+  const float nan = __builtin_nanf("");
+  a.U[2 * node] = err ? nan : P[4];
+  a.U[2 * node + 1] = err ? nan : P[5];
+  if (a.F) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) a.F[4 * node + k] = err ? nan : P[k];
+  }
+  if (a.H21 && zero_h) {
+#pragma unroll
+    for (int k = 0; k < 21; k++) a.H21[(int64_t)k * a.n + node] = 0.0f;
+  }
+  if (a.zncc) a.zncc[node] = zn;
 }
 
 __device__ __forceinline__ void write_subpb1(const SolveArgs& a, int node, int st,
@@ -438,7 +453,8 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       const int p = tid + NT * k;
       const int aa = p % S1, bb = p / S1;
       const bool v = p < NS;
-      pd[k] = half2_t{(_Float16)(v ? aa - hx : 0), (_Float16)(v ? bb - hy : 0)};
+      pd[k] = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(v ? (float)(aa - hx) : 0.0f,
+                                                                     v ? (float)(bb - hy) : 0.0f));
       boff[k] = v ? 4u * (uint32_t)(aa + bb * M) : 0u;
     }
     const int wx0 = min(max(x0 - hx - 1, 0), M - S1), wy0 = min(max(y0 - hy - 1, 0), N - S2);
@@ -544,8 +560,8 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     hv[21] = sqf;
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-      const float pdx = (float)pd[k].x, pdy = (float)pd[k].y;
-      const float sd[6] = {pdx * pgx[k], pdx * pgy[k], pdy * pgx[k], pdy * pgy[k], pgx[k],
+      const float sd[6] = {fma_lo(pgx[k], pd[k], 0.0f), fma_lo(pgy[k], pd[k], 0.0f),
+                           fma_hi(pgx[k], pd[k], 0.0f), fma_hi(pgy[k], pd[k], 0.0f), pgx[k],
                            pgy[k]};
       int t = 0;
 #pragma unroll
@@ -564,6 +580,12 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     group_sum_db<NW, NH>(hv, red, rb);
 #pragma unroll
     for (int k = 0; k < 21; k++) h21[k] = uni(hv[k]);
+    // H goes out now (a later throw overwrites it with zeros), so that its
+    // 21 values are not held in registers through the iterations
+    if (a.H21 && tid == 0) {
+#pragma unroll
+      for (int k = 0; k < 21; k++) a.H21[(int64_t)k * a.n + node] = h21[k];
+    }
     bottomf = (NS > 1) ? sqrtf(hv[21]) : 0.0f;
     if constexpr (OP) {
       const float ibf = 1.0f / bottomf;
@@ -902,7 +924,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   }
   if (tid == 0) {
     if (MODE == MODE_ICGN)
-      write_icgn(a, node, st, P, h21, true, last_zncc);
+      write_icgn_noh(a, node, st, P, last_zncc, st == ST_ERR);
     else
       write_subpb1(a, node, st, P);
   }
