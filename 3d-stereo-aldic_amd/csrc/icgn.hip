@@ -476,6 +476,90 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       nanf |= (pf[k] != pf[k]);
     }
   }
+  // ---- the deformed image's tile ----
+  int tx0 = -1000000, ty0 = -1000000;  // tile origin (0-based image index)
+  // corner offsets of the subset for the tile bounding box
+  const float cdx[2] = {(float)(-hx), (float)(hx)};
+  const float cdy[2] = {(float)(-hy), (float)(hy)};
+  const float lo_u = 3.0f - (float)x0, hi_u = (float)(M - 2 - x0);
+  const float lo_v = 3.0f - (float)y0, hi_v = (float)(N - 2 - y0);
+  // The warped subset's bounding box: 0 inside the image and the tile's
+  // reach (lox..hiy set), 1 outside the image (the bounds test of
+  // :154-157), 2 a NaN warp, 3 larger than the tile.  The warp (:149-152) is
+  // affine, so its extremes over the subset are at the four corners.
+  auto warp_box = [&](int& lox, int& hix, int& loy, int& hiy) -> int {
+    const float A11 = 1.0f + P[0], A12 = P[2], A21 = P[1], A22 = 1.0f + P[3];
+    float umin = 1e30f, umax = -1e30f, vmin = 1e30f, vmax = -1e30f;
+    bool nanw = false;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const float cu = (A11 * cdx[i] + A12 * cdy[j]) + P[4];
+        const float cv = (A21 * cdx[i] + A22 * cdy[j]) + P[5];
+        nanw |= (cu != cu) | (cv != cv);
+        umin = fminf(umin, cu);
+        umax = fmaxf(umax, cu);
+        vmin = fminf(vmin, cv);
+        vmax = fmaxf(vmax, cv);
+      }
+    if (!nanw && (umin < lo_u || umax > hi_u || vmin < lo_v || vmax > hi_v)) return 1;
+    if (nanw) return 2;
+    lox = (int)floorf((float)(x0 - 1) + umin) - 2;
+    hix = (int)floorf((float)(x0 - 1) + umax) + 3;
+    loy = (int)floorf((float)(y0 - 1) + vmin) - 2;
+    hiy = (int)floorf((float)(y0 - 1) + vmax) + 3;
+    return (hix - lox + 1 > TS || hiy - loy + 1 > TS) ? 3 : 0;
+  };
+  // Stage the TS x TS tile at (tx0, ty0) into LDS (the callers order it
+  // against the gathers).
+  auto stage_tile = [&]() {
+    // thread t < CG * TS owns tile row t % TS and the columns t / TS +
+    // CG k: each load is the launch-uniform column base plus the thread's
+    // fixed offset, each LDS store a fixed offset from the thread's slot,
+    // so the staging needs no per-element index arithmetic.  The loads go
+    // out in groups of TG before their stores (one memory latency a group;
+    // one or two groups), at 32-bit byte offsets from the tile's first
+    // pixel (a column of the tile is at most TS * M * 4 bytes from it).
+    constexpr int CG = NT / TS;                // column groups
+    constexpr int KC = (TS + CG - 1) / CG;     // columns per thread
+    constexpr int TG = KC <= 16 ? KC : (KC + 1) / 2;
+    static_assert(CG >= 1, "a tile row per thread");
+    if (tid < CG * TS) {
+      const int rr = tid % TS, cg = tid / TS;
+      const char* gb = (const char*)(a.g + (int64_t)tx0 + (int64_t)ty0 * M);
+      const uint32_t off0 = 4u * (uint32_t)(rr + cg * M), cstep = 4u * (uint32_t)(CG * M);
+      float* tt = tile + rr + cg * TS;
+#pragma unroll 1
+      for (int k0 = 0; k0 < KC; k0 += TG) {
+        float tv[TG];
+#pragma unroll
+        for (int k = 0; k < TG; k++) {
+          const bool in = k0 + k < KC && cg + CG * (k0 + k) < TS;
+          tv[k] = in ? *(const float*)(gb + (off0 + (uint32_t)(k0 + k) * cstep)) : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < TG; k++) {
+          const bool in = k0 + k < KC && cg + CG * (k0 + k) < TS;
+          if (in) tt[CG * (k0 + k) * TS] = tv[k];
+        }
+      }
+    }
+  };
+  // The first warp's tile is staged now, so that its loads overlap the
+  // setup below; the loop stages again only when the warp leaves it.
+  // Nothing reads the tile before the setup's group sums, whose barriers
+  // order these stores before the first gather.
+  {
+    int lox, hix, loy, hiy;
+    if (warp_box(lox, hix, loy, hiy) == 0) {
+      tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
+      ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
+      stage_tile();
+      if constexpr (NW == 1) __builtin_amdgcn_wave_barrier();
+    }
+  }
+
   // the mask counts and the first moments of f in one sum (:119-120,
   // :434: every pixel counts; empty slots hold 0)
   float sumf, sabs;
@@ -643,46 +727,23 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   const bool dp_rule = MODE == MODE_ICGN && a.stop_rule >= 1;
   bool dp_conv = false;
   float m_prev = -1.0f;
-  int tx0 = -1000000, ty0 = -1000000;  // tile origin (0-based image index)
   float last_zncc = __builtin_nanf("");
   bool err = false;
-  // corner offsets of the subset for the tile bounding box
-  const float cdx[2] = {(float)(-hx), (float)(hx)};
-  const float cdy[2] = {(float)(-hy), (float)(hy)};
-  const float lo_u = 3.0f - (float)x0, hi_u = (float)(M - 2 - x0);
-  const float lo_v = 3.0f - (float)y0, hi_v = (float)(N - 2 - y0);
 
   while (step <= maxit && (dp_rule || (normNew > tol && normAbs > tol))) {
     step++;
     const float A11 = 1.0f + P[0], A12 = P[2], A21 = P[1], A22 = 1.0f + P[3];
-    // warp (:149-152) in displacement-relative form.  The warp is affine,
-    // so its extremes over the subset are at the four corners: the bounds
-    // test of :154-157 and the tile's bounding box need only the corners.
-    float umin = 1e30f, umax = -1e30f, vmin = 1e30f, vmax = -1e30f;
-    bool nanw = false;
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const float cu = (A11 * cdx[i] + A12 * cdy[j]) + P[4];
-        const float cv = (A21 * cdx[i] + A22 * cdy[j]) + P[5];
-        nanw |= (cu != cu) | (cv != cv);
-        umin = fminf(umin, cu);
-        umax = fmaxf(umax, cu);
-        vmin = fminf(vmin, cv);
-        vmax = fmaxf(vmax, cv);
-      }
-    if (!nanw && (umin < lo_u || umax > hi_u || vmin < lo_v || vmax > hi_v)) {
+    int lox, hix, loy, hiy;
+    const int box = warp_box(lox, hix, loy, hiy);
+    if (box == 1) {
       normNew = 1e6f;
       break;
     }
-    if (nanw) {  // interpolating at NaN gives NaN g and logical(NaN) throws
+    if (box == 2) {  // interpolating at NaN gives NaN g and logical(NaN) throws
       err = true;
       break;
     }
-    const int lox = (int)floorf((float)(x0 - 1) + umin) - 2, hix = (int)floorf((float)(x0 - 1) + umax) + 3;
-    const int loy = (int)floorf((float)(y0 - 1) + vmin) - 2, hiy = (int)floorf((float)(y0 - 1) + vmax) + 3;
-    if (hix - lox + 1 > TS || hiy - loy + 1 > TS) {
+    if (box == 3) {
       // the warp outgrew the tile: the generic kernel takes the subset
       if (tid == 0) {
         const int q = atomicAdd(a.defer_cnt, 1);
@@ -694,34 +755,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
       ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
       if constexpr (NW > 1) __syncthreads();
-      // thread t < CG * TS owns tile row t % TS and the columns t / TS +
-      // CG k: each load is the launch-uniform column base plus the thread's
-      // fixed offset, each LDS store a fixed offset from the thread's slot,
-      // so the staging needs no per-element index arithmetic.  The loads go
-      // out in groups of TG before their stores (one memory latency a group).
-      constexpr int CG = NT / TS;                // column groups
-      constexpr int KC = (TS + CG - 1) / CG;     // columns per thread
-      constexpr int TG = 8;
-      static_assert(CG >= 1, "a tile row per thread");
-      if (tid < CG * TS) {
-        const int rr = tid % TS, cg = tid / TS;
-        const float* gt = a.g + (int64_t)(tx0 + rr) + (int64_t)(ty0 + cg) * M;
-        float* tt = tile + rr + cg * TS;
-#pragma unroll 1
-        for (int k0 = 0; k0 < KC; k0 += TG) {
-          float tv[TG];
-#pragma unroll
-          for (int k = 0; k < TG; k++) {
-            const bool in = k0 + k < KC && cg + CG * (k0 + k) < TS;
-            tv[k] = in ? gt[(int64_t)(CG * (k0 + k)) * M] : 0.0f;
-          }
-#pragma unroll
-          for (int k = 0; k < TG; k++) {
-            const bool in = k0 + k < KC && cg + CG * (k0 + k) < TS;
-            if (in) tt[CG * (k0 + k) * TS] = tv[k];
-          }
-        }
-      }
+      stage_tile();
       if constexpr (NW > 1)
         __syncthreads();
       else
