@@ -526,6 +526,106 @@ __global__ void cg_dir_kernel(int64_t n, const unsigned char* __restrict__ kind,
   }
 }
 
+// The whole CG iteration in one workgroup, for meshes of up to
+// CG_SMALL_N nodes: the chunked launches above cost a few microseconds of
+// dispatch per kernel (three per iteration, and the asynchronous solve
+// launches every chunk its limit allows), while one CU runs such a mesh's
+// SpMV in about as long.  Same iteration as cg_spmv/update/dir (the
+// Jacobi-preconditioned CG on the free rows, both components, stopped by
+// the same per-component test after each update, or the limit, or a NaN);
+// every block sum is a fixed tree, so the result is deterministic.  Starts
+// from what cg_init left (x, r, z, p and the (r . z), (r . r) and norm
+// scalars); leaves the iteration count in ctr[2] and the done flag set.
+constexpr int CG_BT = 1024;        // threads of the one workgroup
+constexpr int64_t CG_SMALL_N = 8192;
+
+template <int NV>
+__device__ __forceinline__ void block_sum_cg(double (&v)[NV], double (*sh)[CG_BT / ALDIC_WAVE]) {
+  const int lane = threadIdx.x % ALDIC_WAVE, wv = threadIdx.x / ALDIC_WAVE;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double t = v[k];
+    for (int o = ALDIC_WAVE / 2; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if (lane == 0) sh[k][wv] = t;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double t = 0.0;
+    for (int w = 0; w < CG_BT / ALDIC_WAVE; w++) t += sh[k][w];
+    v[k] = t;
+  }
+  __syncthreads();  // every thread has read sh before the next sum writes it
+}
+
+__global__ __launch_bounds__(CG_BT) void cg_block_kernel(
+    int64_t n, const int* __restrict__ rowptr, const int* __restrict__ col,
+    const double* __restrict__ val, const unsigned char* __restrict__ kind,
+    double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, double* p,
+    double* __restrict__ q, const double* __restrict__ dinv, int* ctr, double* sc, double tol2,
+    int max_iter) {
+  __shared__ double sh[4][CG_BT / ALDIC_WAVE];
+  if (ctr[1]) return;  // a skipped solve
+  double rz[2] = {sc[0], sc[1]};
+  const double nx = sc[6], ny = sc[7];
+  const double2* p2 = reinterpret_cast<const double2*>(p);
+  int it = 0;
+  while (true) {
+    // q = A p and (p . q) per component
+    double pq[2] = {0.0, 0.0};
+    for (int64_t i = threadIdx.x; i < n; i += CG_BT) {
+      if (kind[i] != 1) continue;
+      double qx = 0.0, qy = 0.0;
+      for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+        const double2 pj = p2[col[k]];  // p is zero off the free rows
+        const double a = val[k];
+        qx += a * pj.x;
+        qy += a * pj.y;
+      }
+      q[2 * i] = qx;
+      q[2 * i + 1] = qy;
+      pq[0] += p[2 * i] * qx;
+      pq[1] += p[2 * i + 1] * qy;
+    }
+    block_sum_cg<2>(pq, sh);
+    // x += a p, r -= a q, z = D^-1 r, (r . z) and (r . r)
+    const double al[2] = {pq[0] != 0.0 ? rz[0] / pq[0] : 0.0, pq[1] != 0.0 ? rz[1] / pq[1] : 0.0};
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = threadIdx.x; i < n; i += CG_BT) {
+      if (kind[i] != 1) continue;
+#pragma unroll
+      for (int c = 0; c < 2; c++) {
+        x[2 * i + c] += al[c] * p[2 * i + c];
+        const double rr = r[2 * i + c] - al[c] * q[2 * i + c];
+        r[2 * i + c] = rr;
+        const double zz = rr * dinv[i];
+        z[2 * i + c] = zz;
+        v[c] += rr * zz;
+        v[2 + c] += rr * rr;
+      }
+    }
+    block_sum_cg<4>(v, sh);
+    it++;
+    const bool conv = v[2] <= tol2 * nx && v[3] <= tol2 * ny;
+    if (conv || it >= max_iter || !(v[2] == v[2]) || !(v[3] == v[3])) break;
+    // p = z + (rz_new / rz) p
+    const double bx = v[0] != 0.0 && rz[0] != 0.0 ? v[0] / rz[0] : 0.0;
+    const double by = v[1] != 0.0 && rz[1] != 0.0 ? v[1] / rz[1] : 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += CG_BT) {
+      if (kind[i] != 1) continue;
+      p[2 * i] = z[2 * i] + bx * p[2 * i];
+      p[2 * i + 1] = z[2 * i + 1] + by * p[2 * i + 1];
+    }
+    rz[0] = v[0];
+    rz[1] = v[1];
+    __syncthreads();  // the next SpMV reads every row's new p
+  }
+  if (threadIdx.x == 0) {
+    ctr[2] = it;
+    ctr[1] = 1;
+  }
+}
+
 // the iteration count of the last solve, for the caller's device buffer
 __global__ void iters_out_kernel(const int* ctr, int* out) { *out = ctr[2]; }
 
@@ -585,6 +685,11 @@ void launch_cg_chunk(aldic_fem* f, hipStream_t s, double tol2, int max_iter) {
                                                max_iter);
     cg_dir_kernel<<<f->nblk, FEM_T, 0, s>>>(n, f->kind, f->z, f->p, f->sc, f->ctr, par);
   }
+}
+
+void launch_cg_block(aldic_fem* f, hipStream_t s, double tol2, int max_iter) {
+  cg_block_kernel<<<1, CG_BT, 0, s>>>(f->n, f->rowptr, f->col, f->val, f->kind, f->x, f->r, f->z,
+                                       f->p, f->q, f->dinv, f->ctr, f->sc, tol2, max_iter);
 }
 
 // The chunk as a graph: recorded on the caller's stream the first time (or
@@ -849,12 +954,14 @@ extern "C" int aldic_subpb2_dev(aldic_ctx* ctx, aldic_fem* f, double beta, doubl
   hipStream_t s = ctx->stream;
   StageTimer t(ctx, ALDIC_STAGE_SUBPB2);
   subpb2_setup(f, beta, mu, alpha, winstepsize, d_U, d_F, s);
-  const bool graph = cg_chunk_graph(f, s, CG_TOL2, max_iter);
+  const bool small = f->n <= CG_SMALL_N;
+  if (small) launch_cg_block(f, s, CG_TOL2, max_iter);
+  const bool graph = !small && cg_chunk_graph(f, s, CG_TOL2, max_iter);
   int done = 0, iters = 0;
   while (!done) {
     if (graph)
       HIP_TRY(hipGraphLaunch(f->cg_exec, s));
-    else
+    else if (!small)
       launch_cg_chunk(f, s, CG_TOL2, max_iter);
     int h[4];
     HIP_TRY(hipMemcpyAsync(h, f->ctr, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -903,12 +1010,16 @@ int aldic::subpb2_async(aldic_ctx* ctx, aldic_fem* f, double beta, double mu, do
   // every chunk the limit may need; those after convergence return at once.
   // Under a caller's capture the chunks are recorded as plain launches (the
   // mesh's own chunk graph is for eager calls).
-  const bool graph = !capturing && cg_chunk_graph(f, s, CG_TOL2, max_iter);
-  for (int done = 0; done < max_iter; done += CG_CHUNK) {
-    if (graph)
-      HIP_TRY(hipGraphLaunch(f->cg_exec, s));
-    else
-      launch_cg_chunk(f, s, CG_TOL2, max_iter);
+  if (f->n <= CG_SMALL_N) {
+    launch_cg_block(f, s, CG_TOL2, max_iter);
+  } else {
+    const bool graph = !capturing && cg_chunk_graph(f, s, CG_TOL2, max_iter);
+    for (int done = 0; done < max_iter; done += CG_CHUNK) {
+      if (graph)
+        HIP_TRY(hipGraphLaunch(f->cg_exec, s));
+      else
+        launch_cg_chunk(f, s, CG_TOL2, max_iter);
+    }
   }
   out_kernel<<<f->nblk, FEM_T, 0, s>>>(f->n, f->kind, f->x, d_U, d_Uhat);
   if (d_iterations) iters_out_kernel<<<1, 1, 0, s>>>(f->ctr, d_iterations);
