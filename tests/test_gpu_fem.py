@@ -89,11 +89,10 @@ def test_subpb2_larger_mesh(pkg):
 
 
 def test_subpb2_iteration_limit_and_replays(pkg):
-    """The CG chunks are replayed from a recorded graph that is rebuilt when
-    the iteration limit changes: a limit of 3 stops after exactly 3
-    iterations, twice with the same result, the default limit then gives
-    the converged result of the host entry point, and a limit of 5 after
-    it stops at 5."""
+    """Iteration limits on a small mesh (the one-workgroup CG): a limit of 3
+    stops after exactly 3 iterations, twice with the same result, the
+    default limit then gives the converged result of the host entry point,
+    and a limit of 5 after it stops at 5."""
     from aldic_amd import api
     coords, el, bnd = quadtree_mesh(9, 7, 4.0, refine=[(2, 2), (5, 3), (6, 5)])
     U, F = _fields(coords, 11)
@@ -164,4 +163,35 @@ def test_subpb2_async_graph_replays(pkg):
     assert dIt.to_host()[0] == 5
     with pytest.raises(pkg.AldicError):
         mesh.solve_dev_async(ctx, *args, dU.ptr, dF.ptr, dA.ptr, 0.0, 8.0, max_iter=0)
+    mesh.close()
+
+
+def test_subpb2_chunked_path_on_a_large_mesh(pkg):
+    """Meshes of more than 8192 nodes take the chunked multi-block CG (the
+    smaller ones above run the one-workgroup kernel): the solve matches the
+    oracle to 1e-9, the asynchronous solve equals the synchronous one bit
+    for bit with the same count, and an iteration limit stops it there."""
+    from aldic_amd import api
+    refine = [(i, j) for i in range(10, 90, 7) for j in range(5, 80, 6)]
+    coords, el, bnd = quadtree_mesh(100, 85, 4.0, refine=refine)
+    n = len(coords)
+    assert n > 8192
+    U, F = _fields(coords, 31)
+    ctx = pkg.default_context()
+    mesh = pkg.FEMesh(coords, el, bnd, None, 2)
+    DA = api.DeviceArray
+    dU = DA.from_host(ctx, U, np.float64)
+    dF = DA.from_host(ctx, F, np.float64)
+    dS = DA(ctx, 2 * n, np.float64)
+    dA = DA(ctx, 2 * n, np.float64)
+    dIt = DA(ctx, 1, np.int32)
+    args = (1e-3, 1e-3)
+    it = mesh.solve_dev(ctx, *args, dU.ptr, dF.ptr, dS.ptr, 0.0, 8.0)
+    want = fo.subpb2(coords, el, bnd, None, 2, *args, U, F, 0.0, 8.0)
+    print(f"{n} nodes, {it} CG iterations")
+    assert np.max(np.abs(dS.to_host() - want)) / np.max(np.abs(U)) < 1e-9
+    mesh.solve_dev_async(ctx, *args, dU.ptr, dF.ptr, dA.ptr, 0.0, 8.0, max_iter=512,
+                         d_iterations=dIt.ptr)
+    assert dIt.to_host()[0] == it and np.array_equal(dA.to_host(), dS.to_host())
+    assert mesh.solve_dev(ctx, *args, dU.ptr, dF.ptr, dA.ptr, 0.0, 8.0, max_iter=5) == 5
     mesh.close()
