@@ -42,12 +42,16 @@ __device__ __forceinline__ int tri(int r, int c) {
 }
 
 __device__ __forceinline__ bool chol6(const float h[21], float L[21]) {
+  // straight-line: a non-positive pivot clears ok and is replaced by 1 so
+  // the remaining columns stay finite (the caller drops L when !ok)
+  bool ok = true;
 #pragma unroll
   for (int j = 0; j < 6; j++) {
     float d = h[tri(j, j)];
 #pragma unroll
     for (int k = 0; k < j; k++) d -= L[tri(j, k)] * L[tri(j, k)];
-    if (!(d > 0.0f)) return false;
+    ok = ok && (d > 0.0f);
+    d = (d > 0.0f) ? d : 1.0f;
     // hardware square root and reciprocal (1 ulp): no denormal rescaling
     // or division fix-up sequences in the uniform setup
     const float ljj = __builtin_amdgcn_sqrtf(d);
@@ -61,7 +65,7 @@ __device__ __forceinline__ bool chol6(const float h[21], float L[21]) {
       L[tri(i, j)] = s * inv;
     }
   }
-  return true;
+  return ok;
 }
 
 // Forward and back substitution; the diagonal is used through its
@@ -670,9 +674,9 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
 #pragma unroll
       for (int k = 0; k < 21; k++) a.H21[(int64_t)k * a.n + node] = h21[k];
     }
-    bottomf = (NS > 1) ? sqrtf(hv[21]) : 0.0f;
+    bottomf = (NS > 1) ? __builtin_amdgcn_sqrtf(hv[21]) : 0.0f;
     if constexpr (OP) {
-      const float ibf = 1.0f / bottomf;
+      const float ibf = __builtin_amdgcn_rcpf(bottomf);
 #pragma unroll
       for (int j = 0; j < 6; j++) {
         SF[j] = uni(hv[22 + j] * ibf);
@@ -700,7 +704,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       v[2] += pgy[k] * pgy[k];
     }
     group_sum_db<NW, 4>(v, red, rb);
-    bottomf = (NS > 1) ? sqrtf(v[3]) : 0.0f;
+    bottomf = (NS > 1) ? __builtin_amdgcn_sqrtf(v[3]) : 0.0f;
     const float s = 2.0f / (bottomf * bottomf);
     H2[0] = v[0] * s + a.mu;
     H2[1] = v[1] * s;
@@ -709,7 +713,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   }
   // fc = (f - meanf) / bottomf, the reference half of the residual
   {
-    const float ibf = 1.0f / bottomf;
+    const float ibf = __builtin_amdgcn_rcpf(bottomf);
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
       const float fc = (pf[k] - meanf) * ibf;
