@@ -211,15 +211,16 @@ __device__ __forceinline__ bool dp_small(int rule, const float d[6], float hx, f
 // ---------------------------------------------------------------------------
 // Group reductions: NW waves cooperate on one subset.
 // ---------------------------------------------------------------------------
-// NW > 1: each value is summed within the 16-lane rows (four DPP adds),
-// the NW * 4 row sums go to LDS, the first K lanes of the block add them up
-// and every lane reads the K totals back: four vector operations per value
-// and wave instead of the eleven of a full wave sum, plus three barriers.
+// NW > 1: each value is summed over 8-lane groups (three DPP adds: a quad
+// and the quad four lanes along its 16-lane row), the NW * 8 partial sums
+// (lanes 0 and 8 of every row) go to LDS, the first K lanes of the block
+// add them up and every lane reads the K totals back: three vector
+// operations per value and wave instead of the eleven of a full wave sum.
 // red holds group_red_floats<NW>() floats.
 constexpr int GROUP_KMAX = 34;
 template <int NW>
 constexpr int group_red_floats() {
-  return NW * 4 * GROUP_KMAX + GROUP_KMAX;
+  return NW * 8 * GROUP_KMAX + GROUP_KMAX;
 }
 
 // The fast kernel's group sums alternate between two such areas (rb
@@ -232,11 +233,17 @@ constexpr int fast_red_floats() {
   return 2 * group_red_floats<NW>();
 }
 
-__device__ __forceinline__ float row_sum16(float v) {
-  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
-  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, false));
-  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x124, 0xF, 0xF, false));
-  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x128, 0xF, 0xF, false));
+// Lanes 0-3 of each 16-lane row end with the sum of lanes 0-7, lanes 8-11
+// with that of lanes 8-15 (every DPP source lane is valid: bound_ctrl is
+// moot and lets the add take the DPP operand directly).
+__device__ __forceinline__ float half_row_sum(float v) {
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x4E, 0xF, 0xF, true));
+  v += __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x124, 0xF, 0xF, true));
+  // the sum is complete here, with every lane active: otherwise the last
+  // add sinks into the writers' branch and can no longer take the DPP
+  // operand directly
+  __asm__ volatile("" : "+v"(v));
   return v;
 }
 
@@ -247,14 +254,14 @@ __device__ __forceinline__ void group_sum(float (&v)[K], float* red) {
 #pragma unroll
     for (int k = 0; k < K; k++) v[k] = wave_sum(v[k]);
   } else {
-    constexpr int R = NW * 4;  // 16-lane rows of the group
+    constexpr int R = NW * 8;  // 8-lane partial sums of the group
     const int t = threadIdx.x % (ALDIC_WAVE * NW);
 #pragma unroll
-    for (int k = 0; k < K; k++) v[k] = row_sum16(v[k]);
+    for (int k = 0; k < K; k++) v[k] = half_row_sum(v[k]);
     if constexpr (GUARD) __syncthreads();  // the previous sum's totals have been read
-    if ((t & 15) == 0) {
+    if ((t & 7) == 0) {
 #pragma unroll
-      for (int k = 0; k < K; k++) red[(t >> 4) * K + k] = v[k];
+      for (int k = 0; k < K; k++) red[(t >> 3) * K + k] = v[k];
     }
     __syncthreads();
     if (t < K) {
