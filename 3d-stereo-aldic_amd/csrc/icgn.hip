@@ -20,6 +20,7 @@
 // from scratch by the generic kernel, so the result is the reference's
 // whatever path produced it.
 #include <algorithm>
+#include <type_traits>
 
 #include "aldic_common.h"
 #include "aldic_kernels.h"
@@ -524,17 +525,18 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   };
   // Stage the TS x TS tile at (tx0, ty0) into LDS (the callers order it
   // against the gathers).
-  auto stage_tile = [&]() {
+  auto stage_tile = [&](auto one_group) {
     // thread t < CG * TS owns tile row t % TS and the columns t / TS +
     // CG k: each load is the launch-uniform column base plus the thread's
     // fixed offset, each LDS store a fixed offset from the thread's slot,
     // so the staging needs no per-element index arithmetic.  The loads go
-    // out in groups of TG before their stores (one memory latency a group;
-    // one or two groups), at 32-bit byte offsets from the tile's first
-    // pixel (a column of the tile is at most TS * M * 4 bytes from it).
+    // out in groups of TG before their stores (one memory latency a group:
+    // one group at setup, where the registers are free, up to two in the
+    // loop), at 32-bit byte offsets from the tile's first pixel (a column
+    // of the tile is at most TS * M * 4 bytes from it).
     constexpr int CG = NT / TS;                // column groups
     constexpr int KC = (TS + CG - 1) / CG;     // columns per thread
-    constexpr int TG = KC <= 16 ? KC : (KC + 1) / 2;
+    constexpr int TG = (decltype(one_group)::value || KC <= 16) ? KC : (KC + 1) / 2;
     static_assert(CG >= 1, "a tile row per thread");
     if (tid < CG * TS) {
       const int rr = tid % TS, cg = tid / TS;
@@ -566,7 +568,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     if (warp_box(lox, hix, loy, hiy) == 0) {
       tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
       ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
-      stage_tile();
+      stage_tile(std::true_type{});  // one group: its loads fly with the subset's
       if constexpr (NW == 1) __builtin_amdgcn_wave_barrier();
     }
   }
@@ -766,7 +768,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
       ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
       if constexpr (NW > 1) __syncthreads();
-      stage_tile();
+      stage_tile(std::false_type{});
       if constexpr (NW > 1)
         __syncthreads();
       else
