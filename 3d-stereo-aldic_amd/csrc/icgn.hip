@@ -378,6 +378,15 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   // in the 65 px class's launch shapes; smaller subsets keep the two-pass
   // residual, whose rounding their gradients need for 1e-5 agreement
   constexpr bool OP = MODE == MODE_ICGN && NT * PPL >= 2048;
+  // Column layout of the 65 x 65 subset on four waves (COL): slot k < 16 of
+  // lane l in wave w holds the pixel (dx, dy) = (l - 32, 4 (k - 8) + w), so
+  // a lane's dx is fixed and its dy steps by 4 from slot to slot; slot 16
+  // holds row dy = 32 (wave 0) or column dx = 32 (waves 1-2, 65 pixels).
+  // Sums of a lane's pixels then factor into dx times sums over the slot
+  // index, and the warp advances by one FMA per slot.  The partial slot is
+  // valid for tid < 129, as in the raster layout (p = tid + 256 k).
+  constexpr bool COL = OP && FS == 65 && NT == 256 && PPL == 17;
+  constexpr int KM = COL ? 16 : 0;  // slots in the column layout
   int tid = threadIdx.x % NT;
   // in a list walk the lane index is made opaque to the optimiser, so that
   // nothing derived from it is hoisted out of the walk and kept live across
@@ -458,13 +467,30 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   float pf[PPL], pgx[PPL], pgy[PPL], pg[PPL];
   half2_t pd[PPL];
   int zf = 0, nanf = 0, tinyf = 0;
+  // column layout: the lane's column offset and the wave's row phase
+  const int lane = tid & (ALDIC_WAVE - 1);
+  const int wv = __builtin_amdgcn_readfirstlane(tid / ALDIC_WAVE);
   {
     uint32_t boff[PPL];
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-      const int p = tid + NT * k;
-      const int aa = p % S1, bb = p / S1;
-      const bool v = p < NS;
+      if (k < KM) {
+        boff[k] = 4u * (uint32_t)(lane + (4 * k + wv) * M);
+        continue;
+      }
+      int aa, bb;
+      bool v;
+      if constexpr (COL) {  // the edge slot
+        const int j = (wv - 1) * ALDIC_WAVE + lane;
+        v = wv == 0 || j < S2;
+        aa = wv == 0 ? lane : S1 - 1;
+        bb = wv == 0 ? S2 - 1 : j;
+      } else {
+        const int p = tid + NT * k;
+        aa = p % S1;
+        bb = p / S1;
+        v = p < NS;
+      }
       pd[k] = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(v ? (float)(aa - hx) : 0.0f,
                                                                      v ? (float)(bb - hy) : 0.0f));
       boff[k] = v ? 4u * (uint32_t)(aa + bb * M) : 0u;
@@ -655,8 +681,74 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
 #pragma unroll
     for (int k = 0; k < NH; k++) hv[k] = 0.0f;
     hv[21] = sqf;
+    if constexpr (COL) {
+      // sd = (fx dx, fy dx, fx dy, fy dy, fx, fy) = g_i q_a with g = (fx,
+      // fy), q = (dx, dy, 1): each H entry is a moment of a gradient product
+      // g_i g_j over q_a q_b.  Over the column slots dx is the lane's and dy
+      // = 4 kk + w (kk = k - 8), so the lane sums each product, its kk- and
+      // kk^2-weighted sums (A), the same of g_i (f - mean f) and of g_i (E,
+      // G), and forms the moments once.
+      float A[9], E[4], G[4];
 #pragma unroll
-    for (int k = 0; k < PPL; k++) {
+      for (int j = 0; j < 9; j++) A[j] = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 4; j++) E[j] = G[j] = 0.0f;
+#pragma unroll
+      for (int k = 0; k < KM; k++) {
+        const float kk = (float)(k - 8), k2 = kk * kk;
+        const float gx = pgx[k], gy = pgy[k];
+        const float gg[3] = {gx * gx, gx * gy, gy * gy};
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          A[3 * j] += gg[j];
+          A[3 * j + 1] = fmaf(kk, gg[j], A[3 * j + 1]);
+          A[3 * j + 2] = fmaf(k2, gg[j], A[3 * j + 2]);
+        }
+        const float d = pf[k] - meanf;
+        const float ex = gx * d, ey = gy * d;
+        E[0] += ex;
+        E[1] = fmaf(kk, ex, E[1]);
+        E[2] += ey;
+        E[3] = fmaf(kk, ey, E[3]);
+        G[0] += gx;
+        G[1] = fmaf(kk, gx, G[1]);
+        G[2] += gy;
+        G[3] = fmaf(kk, gy, G[3]);
+      }
+      const float dx = (float)(lane - hx), w = (float)wv;
+      // moments over q: index 0 dx^2, 1 dx dy, 2 dy^2, 3 dx, 4 dy, 5 1
+      float mom[3][6];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        const float a0 = A[3 * j], a1 = A[3 * j + 1], a2 = A[3 * j + 2];
+        const float my = fmaf(4.0f, a1, w * a0);                          // sum gg dy
+        mom[j][2] = fmaf(16.0f, a2, w * fmaf(8.0f, a1, w * a0));          // sum gg dy^2
+        mom[j][3] = dx * a0;
+        mom[j][0] = dx * mom[j][3];
+        mom[j][1] = dx * my;
+        mom[j][4] = my;
+        mom[j][5] = a0;
+      }
+      // the q pair of sd indices a, b (0 dx, 1 dy, 2 one)
+      constexpr int QQ[3][3] = {{0, 1, 3}, {1, 2, 4}, {3, 4, 5}};
+      int t = 0;
+#pragma unroll
+      for (int c = 0; c < 6; c++)
+#pragma unroll
+        for (int r = c; r < 6; r++, t++) hv[t] = mom[(r & 1) + (c & 1)][QQ[r >> 1][c >> 1]];
+#pragma unroll
+      for (int i = 0; i < 2; i++) {
+        const float e0 = E[2 * i], e1 = E[2 * i + 1], g0 = G[2 * i], g1 = G[2 * i + 1];
+        hv[22 + i] = dx * e0;
+        hv[24 + i] = fmaf(4.0f, e1, w * e0);
+        hv[26 + i] = e0;
+        hv[28 + i] = dx * g0;
+        hv[30 + i] = fmaf(4.0f, g1, w * g0);
+        hv[32 + i] = g0;
+      }
+    }
+#pragma unroll
+    for (int k = KM; k < PPL; k++) {
       const float sd[6] = {fma_lo(pgx[k], pd[k], 0.0f), fma_lo(pgy[k], pd[k], 0.0f),
                            fma_hi(pgx[k], pd[k], 0.0f), fma_hi(pgy[k], pd[k], 0.0f), pgx[k],
                            pgy[k]};
@@ -784,10 +876,30 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     float s1[2] = {0.0f, 0.0f};  // sum, sum of squares
     float gmin = 3.0e38f;        // smallest |g| of the lane's pixels
     float bg[7] = {0, 0, 0, 0, 0, 0, 0};  // IC-GN: sum sd_j g (6), sum fc g
+    // column slots: the lane's first position and the step of 4 rows, and
+    // sum g fx, sum g fy and their kk-weighted sums (kk = k - 8)
+    float bx = 0.0f, by = 0.0f, s4x = 0.0f, s4y = 0.0f;
+    float cs[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if constexpr (COL) {
+      const float dx = (float)(lane - hx), w = (float)wv;
+      bx = fmaf(A11, dx, fmaf(A12, w, cx));
+      by = fmaf(A21, dx, fmaf(A22, w, cy));
+      s4x = 4.0f * A12;
+      s4y = 4.0f * A22;
+      // per-lane copies: with the slot constant in a scalar register, the
+      // per-slot FMA takes them as its vector operand
+      __asm__ volatile("" : "+v"(s4x), "+v"(s4y));
+    }
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-      const float tx = fma_lo(A11, pd[k], fma_hi(A12, pd[k], cx));
-      const float ty = fma_lo(A21, pd[k], fma_hi(A22, pd[k], cy));
+      float tx, ty;
+      if (k < KM) {
+        tx = fmaf((float)(k - 8), s4x, bx);
+        ty = fmaf((float)(k - 8), s4y, by);
+      } else {
+        tx = fma_lo(A11, pd[k], fma_hi(A12, pd[k], cx));
+        ty = fma_lo(A21, pd[k], fma_hi(A22, pd[k], cy));
+      }
       // tx, ty >= 2 inside the tile (the bounding-box test above leaves one
       // pixel to spare), where fract(t) = t - floor(t) exactly and the
       // truncating conversion is the floor; the 4 x 4 cell starts one pixel
@@ -823,7 +935,15 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
         s1[1] = fmaf(av, av, s1[1]);
         gmin = fminf(gmin, in ? fabsf(acc) : 3.0e38f);
       }
-      if constexpr (OP) {
+      if (k < KM) {
+        const float kk = (float)(k - 8);
+        const float t = acc * pgx[k], u = acc * pgy[k];
+        cs[0] += t;
+        cs[1] = fmaf(kk, t, cs[1]);
+        cs[2] += u;
+        cs[3] = fmaf(kk, u, cs[3]);
+        bg[6] = fmaf(pf[k], acc, bg[6]);
+      } else if constexpr (OP) {
         // empty slots carry zero gradients and fc: they add nothing
         const float t = acc * pgx[k], u = acc * pgy[k];
         bg[0] = fma_lo(t, pd[k], bg[0]);
@@ -836,6 +956,18 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       } else {
         pg[k] = acc;
       }
+    }
+    if constexpr (COL) {
+      // sum over the column slots of sd_j g: dx sum g fx, dx sum g fy,
+      // sum g fx dy = 4 (sum kk g fx) + w sum g fx, the same for fy, sum g fx,
+      // sum g fy
+      const float dx = (float)(lane - hx), w = (float)wv;
+      bg[0] = fmaf(dx, cs[0], bg[0]);
+      bg[1] = fmaf(dx, cs[2], bg[1]);
+      bg[2] += fmaf(4.0f, cs[1], w * cs[0]);
+      bg[3] += fmaf(4.0f, cs[3], w * cs[2]);
+      bg[4] += cs[0];
+      bg[5] += cs[2];
     }
     int tiny;
     if constexpr (OP) {
@@ -984,8 +1116,9 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
 // kernel registers; a walking multi-wave one is held to two waves per
 // SIMD).  The one-entry instance of the 65 px class at four waves per
 // subset and 17 pixels per lane is held to three waves per SIMD (168
-// registers: three subsets per CU), which its packed pixel offsets leave
-// room for without spills.
+// registers: three subsets per CU); its column layout needs 138.  At four
+// waves per SIMD (128 registers, a few spills) the setup is faster and an
+// evaluation slower, for the same time at 2.4 evaluations.
 template <int NW, int PPL, bool WALK>
 constexpr int fast_waves_per_simd() {
   return (NW == 4 && PPL == 17 && !WALK) ? 3 : (WALK && NW > 1 ? 2 : 1);
