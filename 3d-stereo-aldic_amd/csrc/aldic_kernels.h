@@ -59,6 +59,9 @@ struct PersistArgs {
   int full[4];
   int* queue;
 };
+// 256-thread blocks of the persistent launch resident per CU (its launch
+// bound): the grid is this many blocks per CU, or fewer when there is less work
+constexpr int PERSIST_BLOCKS_PER_CU = 3;
 hipError_t launch_persistent(int mode, const PersistArgs& P, int grid, hipStream_t s);
 int size_class(int S1, int S2);
 int class_side(int cls);  // the largest subset side of a fast-kernel class

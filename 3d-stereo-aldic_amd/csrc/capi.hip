@@ -213,10 +213,10 @@ int run_solver_persistent(aldic_ctx* ctx, int mode, SolveArgs a, const aldic_pla
   P.queue = p->pqueue;
   blocks = (int64_t)p->cnt[3] + p->cnt[2];
   waves = (int64_t)p->cnt[1] + p->cnt[0];
-  // two resident blocks per CU; no more blocks than there is work for
+  // the resident blocks; no more blocks than there is work for
   int dev_cus = 0;
   (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
-  const int64_t resident = 2 * (int64_t)std::max(dev_cus, 1);
+  const int64_t resident = aldic::PERSIST_BLOCKS_PER_CU * (int64_t)std::max(dev_cus, 1);
   const int grid = (int)std::min<int64_t>(resident, std::max<int64_t>(blocks, (waves + 3) / 4));
   {
     StageTimer t(ctx, base + 3);

@@ -1150,7 +1150,7 @@ __global__ __launch_bounds__(fast_block<NW>(), (fast_waves_per_simd<NW, PPL, WAL
 
 // ---------------------------------------------------------------------------
 // Persistent kernel: every fast size class in one launch.  The grid is what
-// stays resident (two 256-thread blocks per CU); a size-sorted work queue
+// stays resident (three 256-thread blocks per CU); a size-sorted work queue
 // feeds it: blocks take whole-block items (65 px, then 33 px subsets, four
 // waves each) from one counter while any are left, then each wave takes
 // single-wave items (17 px, then 11 px subsets) from a second counter, so
@@ -1165,7 +1165,7 @@ static_assert(4 * (PK_TS1 * PK_TS1 + PK_TS1 + 1) <= PK_TILE, "four wave tiles fi
 static_assert(PK_TS2 * PK_TS2 + PK_TS2 + 1 <= PK_TILE, "the 33 px block tile fits");
 
 template <int MODE, int BS>
-__global__ __launch_bounds__(256, 2) void persistent_kernel(aldic::PersistArgs P) {
+__global__ __launch_bounds__(256, aldic::PERSIST_BLOCKS_PER_CU) void persistent_kernel(aldic::PersistArgs P) {
   __shared__ float tile[PK_TILE];
   __shared__ float red[fast_red_floats<4>()];
   __shared__ int item_s;
