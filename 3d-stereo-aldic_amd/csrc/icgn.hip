@@ -467,6 +467,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   float pf[PPL], pgx[PPL], pgy[PPL], pg[PPL];
   half2_t pd[PPL];
   int zf = 0, nanf = 0, tinyf = 0;
+  float fmin = 3.0e38f;  // the smallest |f| of the lane's pixels
   // column layout: the lane's column offset and the wave's row phase
   const int lane = tid & (ALDIC_WAVE - 1);
   const int wv = __builtin_amdgcn_readfirstlane(tid / ALDIC_WAVE);
@@ -508,10 +509,10 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
       if (k >= kfull && tid + NT * k >= NS) pf[k] = pgx[k] = pgy[k] = 0.0f;
-      const bool v = k < kfull || tid + NT * k < NS;
-      zf += (v && pf[k] == 0.0f);
-      tinyf |= (v && fabsf(pf[k]) <= 1e-10f);
-      nanf |= (pf[k] != pf[k]);
+      if (k < kfull)
+        fmin = fminf(fmin, fabsf(pf[k]));
+      else
+        fmin = fminf(fmin, tid + NT * k < NS ? fabsf(pf[k]) : 3.0e38f);
     }
   }
   // ---- the deformed image's tile ----
@@ -599,22 +600,31 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     }
   }
 
-  // the mask counts and the first moments of f in one sum (:119-120,
-  // :434: every pixel counts; empty slots hold 0)
+  // the first moments of f (:119-120, :434: every pixel counts; empty
+  // slots hold 0) and whether any pixel has |f| <= 1e-10, in one sum.  A NaN
+  // pixel makes sum |f| NaN (fminf passes over it); zeros, the reference's
+  // mask, are tiny, so they are counted only when a tiny pixel exists.
   float sumf, sabs;
   {
-    float v[5] = {(float)zf, (float)nanf, (float)tinyf, 0.0f, 0.0f};
+    float v[3] = {fmin <= 1e-10f ? 1.0f : 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-      v[3] += pf[k];
-      v[4] += fabsf(pf[k]);
+      v[1] += pf[k];
+      v[2] += fabsf(pf[k]);
     }
-    group_sum_db<NW, 5>(v, red, rb);
-    zf = (int)v[0];
-    nanf = (int)v[1];
-    tinyf = (int)v[2];
-    sumf = v[3];
-    sabs = v[4];
+    group_sum_db<NW, 3>(v, red, rb);
+    tinyf = v[0] > 0.0f;
+    sumf = v[1];
+    sabs = v[2];
+    nanf = sabs != sabs;
+    if (tinyf) {  // uniform over the group
+      float z[1] = {0.0f};
+#pragma unroll
+      for (int k = 0; k < PPL; k++)
+        z[0] += ((k < kfull || tid + NT * k < NS) && pf[k] == 0.0f) ? 1.0f : 0.0f;
+      group_sum_db<NW, 1>(z, red, rb);
+      zf = (int)z[0];
+    }
   }
   // markCoordHoleStrainOrNot (LocalICGNQuadtree.m:75-86)
   if (MODE == MODE_ICGN && a.hole && tid == 0)
