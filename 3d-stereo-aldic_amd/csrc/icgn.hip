@@ -506,14 +506,6 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
       pgx[k] = *(const float*)(xb + boff[k]);
       pgy[k] = *(const float*)(yb + boff[k]);
     }
-#pragma unroll
-    for (int k = 0; k < PPL; k++) {
-      if (k >= kfull && tid + NT * k >= NS) pf[k] = pgx[k] = pgy[k] = 0.0f;
-      if (k < kfull)
-        fmin = fminf(fmin, fabsf(pf[k]));
-      else
-        fmin = fminf(fmin, tid + NT * k < NS ? fabsf(pf[k]) : 3.0e38f);
-    }
   }
   // ---- the deformed image's tile ----
   int tx0 = -1000000, ty0 = -1000000;  // tile origin (0-based image index)
@@ -589,15 +581,63 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   // The first warp's tile is staged now, so that its loads overlap the
   // setup below; the loop stages again only when the warp leaves it.
   // Nothing reads the tile before the setup's group sums, whose barriers
-  // order these stores before the first gather.
+  // order these stores before the first gather.  In the column layout the
+  // tile's values stay in registers until the Hessian's sum (DEFER): the
+  // setup's sums then wait for the subset's loads only, and the tile's
+  // loads complete behind their arithmetic.
+  constexpr bool DEFER = COL;
+  constexpr int TCG = NT / TS, TKC = (TS + TCG - 1) / TCG;  // the staging's shape
+  float tv0[DEFER ? TKC : 1];
+  bool held = false;
+  auto store_held = [&]() {
+    if (DEFER && held && tid < TCG * TS) {
+      float* tt = tile + tid % TS + (tid / TS) * TS;
+#pragma unroll
+      for (int k = 0; k < TKC; k++)
+        if (tid / TS + TCG * k < TS) tt[TCG * k * TS] = tv0[k];
+    }
+  };
   {
-    int lox, hix, loy, hiy;
-    if (warp_box(lox, hix, loy, hiy) == 0) {
+    int lox = 0, hix = 0, loy = 0, hiy = 0;
+    const int box0 = warp_box(lox, hix, loy, hiy);
+    if constexpr (DEFER) {
+      // The loads go out on every lane and every path, from a clamped
+      // origin and clamped columns (kept or not), so that no branch joins
+      // paths with different loads in flight and the sums' waits count
+      // the subset's loads only.
+      const int ox = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
+      const int oy = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
+      held = box0 == 0;
+      if (held) {
+        tx0 = ox;
+        ty0 = oy;
+      }
+      __builtin_amdgcn_sched_barrier(0);  // after the subset's loads
+      // (the lanes past the staging's, tid >= TCG * TS, read the last
+      // column group's pixels and store nothing)
+      static_assert(TCG - 1 + TCG * (TKC - 1) < TS, "every column group's columns lie in the tile");
+      const int rr = tid % TS, cg = min(tid / TS, TCG - 1);
+      const char* gb = (const char*)(a.g + (int64_t)ox + (int64_t)oy * M);
+      const uint32_t off0 = 4u * (uint32_t)(rr + cg * M), cstep = 4u * (uint32_t)(TCG * M);
+#pragma unroll
+      for (int k = 0; k < TKC; k++) tv0[k] = *(const float*)(gb + (off0 + (uint32_t)k * cstep));
+    } else if (box0 == 0) {
       tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
       ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
-      stage_tile(std::true_type{});  // one group: its loads fly with the subset's
-      if constexpr (NW == 1) __builtin_amdgcn_wave_barrier();
+      {
+        stage_tile(std::true_type{});  // one group: its loads fly with the subset's
+        if constexpr (NW == 1) __builtin_amdgcn_wave_barrier();
+      }
     }
+  }
+  // empty slots: f and its gradients 0; the smallest |f| of the lane
+#pragma unroll
+  for (int k = 0; k < PPL; k++) {
+    if (k >= kfull && tid + NT * k >= NS) pf[k] = pgx[k] = pgy[k] = 0.0f;
+    if (k < kfull)
+      fmin = fminf(fmin, fabsf(pf[k]));
+    else
+      fmin = fminf(fmin, tid + NT * k < NS ? fabsf(pf[k]) : 3.0e38f);
   }
 
   // the first moments of f (:119-120, :434: every pixel counts; empty
@@ -776,6 +816,7 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
         }
       }
     }
+    store_held();  // ordered before the first gather by this sum's barriers
     group_sum_db<NW, NH>(hv, red, rb);
 #pragma unroll
     for (int k = 0; k < 21; k++) h21[k] = uni(hv[k]);
