@@ -943,13 +943,6 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     }
 #pragma unroll
     for (int k = 0; k < PPL; k++) {
-      // B-spline option: scheduling regions of two slots.  Over the whole
-      // unrolled loop the scheduler has given this instance a
-      // minimum-register order in which every tap read waits for the one
-      // before it (the same destination registers): per evaluation 1.07 ->
-      // 1.49 ms.  Two slots a region keep four reads in flight.
-      if constexpr (BS > 0)
-        if (k > 0 && k % 2 == 0) __builtin_amdgcn_sched_barrier(0);
       float tx, ty;
       if (k < KM) {
         tx = fmaf((float)(k - 8), s4x, bx);
@@ -973,10 +966,23 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
         spline_weights56(dyf, wc);
       }
       const float* base = tileb + __umul24(iy, TS) + ix;
+      // The 16 taps are read into registers of their own before the
+      // combine.  Read inside the combine's expressions, the compiler has
+      // given several instances (the B-spline option, the persistent
+      // launch) a minimum-register order in which every read waits for the
+      // one before it (one destination pair for all the loop's reads; the
+      // B-spline instance took 1.49 instead of 1.07 ms per evaluation);
+      // this form keeps the eight two-tap reads of a pixel in flight
+      // together in every instance.
+      float tp[16];
+#pragma unroll
+      for (int c = 0; c < 4; c++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) tp[4 * c + r] = base[c * TS + r];
       float acc = 0.0f;
 #pragma unroll
       for (int c = 0; c < 4; c++) {
-        const float* row = base + c * TS;
+        const float* row = tp + 4 * c;
         const float rsum = wr[0] * row[0] + wr[1] * row[1] + wr[2] * row[2] + wr[3] * row[3];
         acc = fmaf(wc[c], rsum, acc);
       }
