@@ -518,9 +518,11 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   // reach (lox..hiy set), 1 outside the image (the bounds test of
   // :154-157), 2 a NaN warp, 3 larger than the tile.  The warp (:149-152) is
   // affine, so its extremes over the subset are at the four corners.
+  // The box's corner extremes (umin, umax, vmin, vmax) of the last test.
+  float umin = 0.0f, umax = 0.0f, vmin = 0.0f, vmax = 0.0f;
   auto warp_box = [&](int& lox, int& hix, int& loy, int& hiy) -> int {
     const float A11 = 1.0f + P[0], A12 = P[2], A21 = P[1], A22 = 1.0f + P[3];
-    float umin = 1e30f, umax = -1e30f, vmin = 1e30f, vmax = -1e30f;
+    umin = 1e30f, umax = -1e30f, vmin = 1e30f, vmax = -1e30f;
     bool nanw = false;
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -886,36 +888,58 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
   float last_zncc = __builtin_nanf("");
   bool err = false;
 
+  // The bounding-box test is skipped while the corners cannot have left
+  // the tile or the image: slack is how far they may move from where the
+  // last test found them (at the warp Pb), and an affine warp's corners
+  // move by at most |du| + |dA11| hx + |dA12| hy (and the same for v).
+  // A non-finite warp never passes the comparison, so it is always tested.
+  float slack = -1.0f, Pb[6] = {0, 0, 0, 0, 0, 0};
   while (step <= maxit && (dp_rule || (normNew > tol && normAbs > tol))) {
     step++;
     const float A11 = 1.0f + P[0], A12 = P[2], A21 = P[1], A22 = 1.0f + P[3];
-    int lox, hix, loy, hiy;
-    const int box = warp_box(lox, hix, loy, hiy);
-    if (box == 1) {
-      normNew = 1e6f;
-      break;
-    }
-    if (box == 2) {  // interpolating at NaN gives NaN g and logical(NaN) throws
-      err = true;
-      break;
-    }
-    if (box == 3) {
-      // the warp outgrew the tile: the generic kernel takes the subset
-      if (tid == 0) {
-        const int q = atomicAdd(a.defer_cnt, 1);
-        a.defer_list[q] = node;
+    const float mu = fabsf(P[4] - Pb[4]) + fabsf(P[0] - Pb[0]) * (float)hx + fabsf(P[2] - Pb[2]) * (float)hy;
+    const float mv = fabsf(P[5] - Pb[5]) + fabsf(P[1] - Pb[1]) * (float)hx + fabsf(P[3] - Pb[3]) * (float)hy;
+    // (uniform: P, Pb and slack are the same on every lane)
+    if (!__builtin_amdgcn_readfirstlane(fmaxf(mu, mv) < slack)) {
+      int lox, hix, loy, hiy;
+      const int box = warp_box(lox, hix, loy, hiy);
+      if (box == 1) {
+        normNew = 1e6f;
+        break;
       }
-      return;
-    }
-    if (lox < tx0 || hix > tx0 + TS - 1 || loy < ty0 || hiy > ty0 + TS - 1) {
-      tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
-      ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
-      if constexpr (NW > 1) __syncthreads();
-      stage_tile(std::false_type{});
-      if constexpr (NW > 1)
-        __syncthreads();
-      else
-        __builtin_amdgcn_wave_barrier();
+      if (box == 2) {  // interpolating at NaN gives NaN g and logical(NaN) throws
+        err = true;
+        break;
+      }
+      if (box == 3) {
+        // the warp outgrew the tile: the generic kernel takes the subset
+        if (tid == 0) {
+          const int q = atomicAdd(a.defer_cnt, 1);
+          a.defer_list[q] = node;
+        }
+        return;
+      }
+      if (lox < tx0 || hix > tx0 + TS - 1 || loy < ty0 || hiy > ty0 + TS - 1) {
+        tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
+        ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
+        if constexpr (NW > 1) __syncthreads();
+        stage_tile(std::false_type{});
+        if constexpr (NW > 1)
+          __syncthreads();
+        else
+          __builtin_amdgcn_wave_barrier();
+      }
+      // the room left: the image's bounds test (lo_u ... hi_v) and the
+      // tile's reach (floor(x0 - 1 + u) - 2 >= tx0 holds once u >= tx0 -
+      // x0 + 4; floor(x0 - 1 + u) + 3 <= tx0 + TS - 1 once u <= tx0 + TS -
+      // 3 - x0), less a margin for the rounding of the corners
+      const float su = fminf(umin - fmaxf(lo_u, (float)(tx0 - x0 + 4)),
+                             fminf(hi_u, (float)(tx0 + TS - 3 - x0)) - umax);
+      const float sv = fminf(vmin - fmaxf(lo_v, (float)(ty0 - y0 + 4)),
+                             fminf(hi_v, (float)(ty0 + TS - 3 - y0)) - vmax);
+      slack = uni(fminf(su, sv) - 1e-3f);
+#pragma unroll
+      for (int k = 0; k < 6; k++) Pb[k] = uni(P[k]);
     }
     // gather + interpolate.  The value is kept 3136 times the interpolant
     // (unscaled 56x weights on both axes; 36x for the B-spline option): the
