@@ -90,7 +90,7 @@ TOL_DP = 1e-3  # step-size rule threshold (px)
 # WRITE_SIZE in KiB per dispatch, each scaled by the same counter's reading
 # on a device-to-device copy of known size (FETCH x 1.9994, WRITE x 1);
 # measured with this bench (profiles/icgn_c3_bulge4096_hbm_traffic.md)
-ICGN_C3_TRAFFIC = (3591208.1 * 1.9994 + 22797.8) * 1024.0
+ICGN_C3_TRAFFIC = (3471772.8 * 1.9994 + 22939.0) * 1024.0
 
 
 def load(name, rel):
