@@ -603,8 +603,8 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     int lox = 0, hix = 0, loy = 0, hiy = 0;
     const int box0 = warp_box(lox, hix, loy, hiy);
     if constexpr (DEFER) {
-      // The loads go out on every lane and every path, from a clamped
-      // origin and clamped columns (kept or not), so that no branch joins
+      // The loads go out on every lane and every path, from the clamped
+      // origin (whether or not the tile is kept), so that no branch joins
       // paths with different loads in flight and the sums' waits count
       // the subset's loads only.
       const int ox = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
@@ -626,10 +626,8 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
     } else if (box0 == 0) {
       tx0 = min(max((lox + hix) / 2 - TS / 2, 0), M - TS);
       ty0 = min(max((loy + hiy) / 2 - TS / 2, 0), N - TS);
-      {
-        stage_tile(std::true_type{});  // one group: its loads fly with the subset's
-        if constexpr (NW == 1) __builtin_amdgcn_wave_barrier();
-      }
+      stage_tile(std::true_type{});  // one group: its loads fly with the subset's
+      if constexpr (NW == 1) __builtin_amdgcn_wave_barrier();
     }
   }
   // empty slots: f and its gradients 0; the smallest |f| of the lane
