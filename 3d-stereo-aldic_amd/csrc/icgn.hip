@@ -1202,9 +1202,9 @@ __device__ __forceinline__ void fast_subset(const SolveArgs& a, int slot, float*
 // kernel registers; a walking multi-wave one is held to two waves per
 // SIMD).  The one-entry instance of the 65 px class at four waves per
 // subset and 17 pixels per lane is held to three waves per SIMD (168
-// registers: three subsets per CU); its column layout needs 138.  At four
-// waves per SIMD (128 registers, a few spills) the setup is faster and an
-// evaluation slower, for the same time at 2.4 evaluations.
+// registers: three subsets per CU); its column layout needs about 150.  At
+// four waves per SIMD (128 registers, a few spills) the setup was faster
+// and an evaluation slower, for the same time at 2.4 evaluations.
 template <int NW, int PPL, bool WALK>
 constexpr int fast_waves_per_simd() {
   return (NW == 4 && PPL == 17 && !WALK) ? 3 : (WALK && NW > 1 ? 2 : 1);
