@@ -76,7 +76,11 @@ sys.path.insert(0, ROOT)
 # frame's network waits behind this frame's fallback, which waits for this
 # frame's IC-GN (4.91 against 5.10 ms per step on the headline workload).
 # A larger setting of the environment is kept.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+try:
+    _hw_queues = int(os.environ.get("GPU_MAX_HW_QUEUES", "") or 4)
+except ValueError:
+    _hw_queues = 4
+if _hw_queues < 8:
     os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
 import __graft_entry__ as ge  # noqa: E402
